@@ -1,0 +1,228 @@
+/*
+ * eprem_hip.h -- C ABI of the MI355X energetic-particle update.
+ *
+ * This is the drop-in boundary for EPREM's per-stream particle update.  A C
+ * host that owns the cube-shell grid, node motion, MHD background and output
+ * calls these entry points in place of the reference's internal functions:
+ *
+ *   ep_step            replaces updateEnergeticParticles()
+ *                          src/energeticParticles.h:51, called at src/eprem.c:671
+ *                      (which runs, per EP substep, updateStreamValues
+ *                       src/simCore.c:325, seedPopulationCheck
+ *                       src/energeticParticlesBoundary.c:28, GetStreamList
+ *                       src/energeticParticles.c:2147, DiffuseStreamData :2237,
+ *                       AdiabaticFocusing :1533 and AdiabaticChange :805),
+ *                      and the MPI stream gather/scatter src/simCore.c:177-318,
+ *                      which disappears because every stream is resident.
+ *   ep_set_nodes       replaces the Node_t fields the operators read
+ *                          (src/cubeShellStruct.h:51-93)
+ *   ep_upload_f /      replace direct access to eParts[]
+ *   ep_download_f          (src/global.h:61, allocated src/global.c:112)
+ *   ep_mas_set_level   replaces the MAS shared-memory windows filled by
+ *                          masGetInterpData() (src/readMAS.c:645-980)
+ *   ep_interp_mhd      replaces the per-node masGetNode()/masTriLinear()
+ *                          loop inside updateMhd() (src/flow.c:48-77,
+ *                          src/masInterp.c:30-68, :539-741, :1007-1065)
+ *
+ * Conventions
+ *   - Every call returns EP_OK (0) or a negative EP_ERR_* code.  The
+ *     reference has no return codes: it panics (src/error.c:39-49).  The
+ *     host is expected to print ep_last_error() and abort, exactly like
+ *     panic() does.
+ *   - All physical quantities are in the reference's code units: lengths of
+ *     node positions in units of rScale AU, times in TAU = AU/c, speeds in c,
+ *     B in MHD_B_NORM, energies in m_p c^2 (src/baseTypes.h:50-115).
+ *   - One species only (num_species = 1).  The reference's multi-species
+ *     path indexes the per-energy egrid[] with idx_se(species, energy)
+ *     (src/energeticParticles.c:1137-1138, src/energeticParticlesTypes.c:58-61),
+ *     which reads past the array for species > 0, so there is no defined
+ *     multi-species behaviour to reproduce.
+ *   - Host arrays stay owned by the host.  The library owns its device
+ *     buffers, allocated once in ep_init / ep_mas_set_level; ep_step does
+ *     not allocate.
+ *   - Single-threaded host calls, one HIP stream, no callbacks.
+ */
+#ifndef EPREM_HIP_H
+#define EPREM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Error codes (mirror the reference's panic sites)                     */
+/* ------------------------------------------------------------------ */
+#define EP_OK                 0
+#define EP_ERR_INVALID_ARG   -1  /* bad dims/params/pointers            */
+#define EP_ERR_NOT_INIT      -2  /* call before ep_init                 */
+#define EP_ERR_HIP           -3  /* HIP runtime error                   */
+#define EP_ERR_NAN           -4  /* checkNaN  src/error.c:62-80          */
+#define EP_ERR_INF           -5  /* checkInf  src/error.c:84-100         */
+#define EP_ERR_DS_UNDERFLOW  -6  /* src/energeticParticles.c:2594-2595   */
+#define EP_ERR_UNSUPPORTED   -7  /* option outside the implemented set  */
+#define EP_ERR_STREAM_SHORT  -8  /* fewer than 3 kept nodes on a stream */
+#define EP_ERR_NO_DEVICE     -9  /* no usable GPU                       */
+
+/* Layout selectors for ep_upload_f / ep_download_f. */
+#define EP_LAYOUT_REFERENCE   0  /* f[stream][node][energy][mu]  (idx_frcsspem
+                                    of src/global.h:61 with species=1 and
+                                    the buffer shell removed)          */
+
+/* Operator flags for ep_step (AND-ed with the params' use* switches). */
+#define EP_OP_STREAM      0x1u   /* DiffuseStreamData (+GetStreamList)  */
+#define EP_OP_FOCUS       0x2u   /* AdiabaticFocusing                   */
+#define EP_OP_CHANGE      0x4u   /* AdiabaticChange                     */
+#define EP_OP_SEEDCHECK   0x8u   /* seedPopulationCheck                 */
+#define EP_OP_ALL         0xFu
+
+/* ------------------------------------------------------------------ */
+/* Dimensions and parameters                                           */
+/* ------------------------------------------------------------------ */
+typedef struct ep_dims {
+  int32_t num_streams;   /* NUM_STREAMS = 6 * rows * cols                */
+  int32_t num_nodes;     /* TOTAL_NUM_SHELLS: max nodes per stream        */
+  int32_t num_energies;  /* NUM_ESTEPS                                    */
+  int32_t num_mu;        /* NUM_MUSTEPS (2..32)                           */
+} ep_dims;
+
+/* Runtime parameters, already converted to code units the way
+ * setRuntimeConstants() does (src/configuration.c:324-348). */
+typedef struct ep_params {
+  double rScale;             /* config.rScale  [AU]                        */
+  double tDel;               /* config.tDel    [TAU] (after /DAY)          */
+  int32_t numEpSteps;        /* config.numEpSteps                          */
+  int32_t mfpType;           /* 1: r-power, 2: B-power  (meanFreePath.c)   */
+  double lamo;               /* config.lamo                                */
+  double mfpRadialPower;     /* config.mfpRadialPower                      */
+  double mfpBPower;          /* config.mfpBPower                           */
+  double mhdBsAu;            /* config.mhdBAu / MHD_B_NORM                 */
+  double rigidityPower;      /* config.rigidityPower                       */
+  double eMin, eMax;         /* [MeV/nucleon]                              */
+  double mass, charge;       /* species 0: config.mass[0], config.charge[0]*/
+  double dsh_min;            /* config.dsh_min (dsh_hel_min not used: the
+                                helio-coupled phase is out of scope)       */
+  double dampCooling;        /* config.dampCooling                         */
+  double boundaryFunctAmplitude, boundaryFunctXi, boundaryFunctGamma,
+         boundaryFunctBeta, boundaryFunctEcutoff, boundaryFunctR0;
+  int32_t useParallelDiffusion, useAdiabaticFocus, useAdiabaticChange,
+          checkSeedPopulation, useEPBoundary, useBoundaryFunction;
+  int32_t adiabaticFocusAlg;  /* 1 = upwind (the only one on the GPU)      */
+  int32_t adiabaticChangeAlg; /* 1 = upwind (the only one on the GPU)      */
+} ep_params;
+
+/* Per-node inputs, each array [num_streams][num_nodes] (node fastest).
+ * x,y,z,rmag: node position (Node_t.r, .rmag); bmag: Node_t.mhdBmag;
+ * dlnb, dlnn, dupar: Node_t.mhdDlnB/.mhdDlnN/.mhdDuPar (set by updateMhd). */
+typedef struct ep_node_soa {
+  const double *x, *y, *z, *rmag;
+  const double *bmag, *dlnb, *dlnn, *dupar;
+} ep_node_soa;
+
+/* Per-global-step diagnostics (src/energeticParticles.c:149-156, :943-945,
+ * :1668-1670, reduced at src/eprem.c:694-716) plus error location. */
+typedef struct ep_diag {
+  int32_t max_subcycles_change;    /* maxsubcycles_energychange           */
+  int32_t max_subcycles_focusing;  /* maxsubcycles_focusing               */
+  int32_t max_nsteps_stream;       /* largest DiffuseStreamData nsteps     */
+  int32_t err_code;                /* EP_OK or the first error found      */
+  double  min_tau;                 /* min_tau                             */
+  int32_t err_stream, err_node, err_energy, err_mu;
+  int64_t cell_updates;            /* active cells x EP substeps          */
+  double  device_ms;               /* HIP-event time of the whole step    */
+  double  stream_ms, focus_ms, change_ms, prep_ms; /* per-operator time   */
+} ep_diag;
+
+/* ------------------------------------------------------------------ */
+/* Entry points                                                        */
+/* ------------------------------------------------------------------ */
+
+/* Allocate device state for dims, build the energy/mu grids
+ * (initEnergeticParticlesGrids, src/energeticParticlesInit.c:36-100) and
+ * set every f to DBL_MIN like that routine does. */
+int ep_init(const ep_dims *dims, const ep_params *params);
+
+/* Replace the parameters (dims unchanged). */
+int ep_set_params(const ep_params *params);
+
+/* Copy the energy/mu grids built by ep_init back to the host; any pointer
+ * may be NULL.  Arrays have num_energies / num_mu entries. */
+int ep_get_grids(double *egrid, double *vgrid, double *mugrid,
+                 double *rigidity, double *dlnp, double *dmu);
+
+/* Upload node data.  n_active (may be NULL = all num_nodes) gives the
+ * number of nodes of each stream (2..num_nodes); nodes past it are
+ * ignored.  The reference has the same count on every stream
+ * (TOTAL_NUM_SHELLS); per-stream counts serve batched workloads. */
+int ep_set_nodes(const ep_node_soa *nodes, const int32_t *n_active);
+
+/* Copy f between host and device (layout: EP_LAYOUT_*).  The host array
+ * covers num_streams*num_nodes*num_energies*num_mu doubles. */
+int ep_upload_f(const double *f, int32_t layout);
+int ep_download_f(double *f, int32_t layout);
+
+/* Fill f with the seed population sepSeedFunction(E, rmag) at every node
+ * (initEnergeticParticles, src/energeticParticlesInit.c:106-147). */
+int ep_init_seed_population(void);
+
+/* One global step: numEpSteps EP substeps of dt = tDel/numEpSteps, all on
+ * the device (updateEnergeticParticles, src/energeticParticles.c:69-203).
+ * op_flags (EP_OP_*) select operators on top of the params' switches.
+ * diag may be NULL.  Returns the first device error (NaN, Inf, ds
+ * underflow) with its location in diag. */
+int ep_step(uint32_t op_flags, ep_diag *diag);
+
+/* Node values the step computes for the host (updateStreamValues,
+ * src/simCore.c:325-435): per node ds, mhdBmagPlus, mhdBmagMinus. */
+int ep_get_node_outputs(double *ds, double *bmag_plus, double *bmag_minus);
+
+/* ---------------- MAS trilinear interpolation ---------------------- */
+
+/* One MAS time level.  fields[7] = {Bp, Bt, Br, Vp, Vt, Vr, D} in MAS
+ * units, each (phi, theta, r) with r fastest, i.e. Vindex(r,t,p) =
+ * r + nr*t + nr*nt*p (src/masInterp.c:185).  Field k has its own 1-D
+ * meshes: meshes[3k+0] = r [nr_k] (in Rs), meshes[3k+1] = theta [nt_k],
+ * meshes[3k+2] = phi [np_k]; dims[3k+0..2] = nr_k, nt_k, np_k.
+ * level is 0 or 1 (the two time levels blended by s_cor). */
+int ep_mas_set_level(int32_t level, const float *const fields[7],
+                     const float *const meshes[21], const int32_t dims[21]);
+
+/* Interpolation constants (src/masInterp.c:41, :706). */
+typedef struct ep_mas_params {
+  double s_cor;          /* time blend factor (readMAS.c:681-708)          */
+  double phi_offset;     /* phiOffset                                      */
+  double mas_radial_max; /* config.masRadialMax [Rs]                        */
+  double mas_vmin;       /* config.masVmin [cm/s]                           */
+} ep_mas_params;
+
+/* Per-node previous MHD state used by the outer fallback masWind()
+ * (src/masInterp.c:1007-1065); arrays [num_streams][num_nodes]. */
+typedef struct ep_mas_prev {
+  const double *rold_x, *rold_y, *rold_z;
+  const double *br, *btheta, *bphi, *vr, *vtheta, *vphi, *density;
+} ep_mas_prev;
+
+/* Output: per node B (r, theta, phi), V (r, theta, phi) and density in
+ * code units, each [num_streams][num_nodes]. */
+typedef struct ep_mas_out {
+  double *br, *btheta, *bphi, *vr, *vtheta, *vphi, *density;
+} ep_mas_out;
+
+/* Interpolate both MAS levels onto the node positions last given to
+ * ep_set_nodes (masGetNode for every node).  prev may be NULL when every
+ * node lies inside mas_radial_max.  out receives host copies (may be
+ * NULL: results stay on the device).  *device_ms, if given, receives the
+ * kernel time. */
+int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev,
+                  ep_mas_out *out, double *device_ms);
+
+/* ---------------- misc --------------------------------------------- */
+int ep_last_error(char *buf, size_t len);
+int ep_finalize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EPREM_HIP_H */

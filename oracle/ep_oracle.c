@@ -1,0 +1,1202 @@
+/*
+ * ep_oracle.c -- FP64 CPU oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * Plain-C restatement of the reference hot path, used only by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The product
+ * never links it.  Build: oracle/Makefile (gcc -O2 -std=c99
+ * -ffp-contract=off -fopenmp).  Each function cites the reference lines it
+ * follows; the arithmetic keeps the reference's operation order so that it
+ * reproduces the reference's rounding.
+ */
+#define _GNU_SOURCE
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "ep_oracle.h"
+
+/* Physical constants (src/baseTypes.h:50-115). */
+#define OR_PI 3.1415926535897932
+#define OR_TWO_PI 6.283185307179586
+#define OR_VERYSMALL 1.0e-33
+#define OR_MP 1.6726e-24
+#define OR_MEV 1.6022e-6
+#define OR_GEV 1.6022e-3
+#define OR_C 2.99792458e10
+#define OR_MZERO (OR_MP * OR_C * OR_C / OR_GEV)
+#define OR_AU 1.495978707e13
+#define OR_RSUN 6.96e10
+#define OR_RSAU (OR_RSUN / OR_AU)
+#define OR_TAU (OR_AU / OR_C)
+#define OR_DAY (OR_TAU / (24.0 * 60.0 * 60.0))
+#define OR_MHD_DENSITY_NORM 1.0
+#define OR_MHD_B_NORM (sqrt(OR_MP * OR_MHD_DENSITY_NORM) * OR_C)
+#define OR_MAS_TIME_NORM 1445.87003080685
+#define OR_MAS_LENGTH_NORM 6.96e10
+#define OR_MAS_RHO_NORM 1.6726e-16
+#define OR_MAS_V_CONVERT (OR_MAS_LENGTH_NORM / OR_MAS_TIME_NORM) / OR_C
+#define OR_MAS_RHO_CONVERT (OR_MAS_RHO_NORM / OR_MP) / OR_MHD_DENSITY_NORM
+#define OR_MAS_B_CONVERT \
+  sqrt(4.0 * OR_PI * OR_MAS_RHO_NORM) * OR_MAS_LENGTH_NORM / OR_MAS_TIME_NORM / OR_MHD_B_NORM
+
+/* ------------------------------------------------------------------ */
+/* Grids  (src/energeticParticlesInit.c:36-100)                        */
+/* ------------------------------------------------------------------ */
+void or_build_grids(const ep_params *p, int E, int M, double *egrid,
+                    double *vgrid, double *vgrid_i, double *pgrid,
+                    double *rigidity, double *mugrid, double *dlnp_out,
+                    double *dmu_out) {
+  const double one = 1.0, two = 2.0, half = 0.5;
+  double kin, lnpmin, lnpmax, dlnp, dmu, pp;
+  int e, m;
+  kin = p->eMin * OR_MEV / (OR_MP * OR_C * OR_C);
+  lnpmin = log(sqrt(kin * kin + two * kin));
+  kin = p->eMax * OR_MEV / (OR_MP * OR_C * OR_C);
+  lnpmax = log(sqrt(kin * kin + two * kin));
+  dlnp = (lnpmax - lnpmin) / (E - 1.0);
+  for (e = 0; e < E; e++) {
+    pp = exp(lnpmin + e * dlnp);
+    if (pgrid) pgrid[e] = pp;
+    egrid[e] = sqrt(one + pp * pp) - one;
+    vgrid[e] = pp / sqrt(one + pp * pp);
+    vgrid_i[e] = sqrt(one + pp * pp) / pp;
+  }
+  for (e = 0; e < E; e++) {
+    pp = exp(lnpmin + e * dlnp);
+    rigidity[e] = (p->mass / p->charge) * pp * OR_MZERO;
+    rigidity[e] = pow(rigidity[e], p->rigidityPower);
+  }
+  dmu = two / (one * M);
+  for (m = 0; m < M; m++) {
+    double mup = -one + (m + one) * dmu;
+    double mum = -one + m * dmu;
+    mugrid[m] = half * (mup + mum);
+  }
+  *dlnp_out = dlnp;
+  *dmu_out = dmu;
+}
+
+/* sepSeedFunction (src/energeticParticlesBoundary.c:67-116). */
+double or_seed(const ep_params *p, double energy, double r) {
+  const double two = 2.0;
+  const double E1 = 1.0;
+  const double efac = OR_MEV / (OR_MP * OR_C * OR_C);
+  double normJ0 = p->boundaryFunctAmplitude * (OR_MP * OR_C) / (OR_MHD_DENSITY_NORM * OR_MEV);
+  double normRadius = p->boundaryFunctR0 / p->rScale;
+  double normE1 = E1 * efac;
+  double normE0 = p->boundaryFunctEcutoff * efac;
+  double radialTerm = pow((r / normRadius), -p->boundaryFunctBeta);
+  double powerLawTerm = pow((energy / normE1), -p->boundaryFunctGamma);
+  double expTerm = exp(-(energy / normE0));
+  double normJHe = normJ0 * radialTerm * powerLawTerm * expTerm;
+  double normJ = normJHe / p->boundaryFunctXi;
+  double normf = normJ / (two * energy);
+  if (normf < DBL_MIN) normf = DBL_MIN;
+  return normf;
+}
+
+/* meanFreePath (src/meanFreePath.c:13-50). */
+double or_mfp(const ep_params *p, double rig, double rmag, double bmag) {
+  if (p->mfpType == 2)
+    return rig * p->lamo * pow(p->mhdBsAu / bmag, p->mfpBPower);
+  return rig * p->lamo * pow(rmag * p->rScale, p->mfpRadialPower);
+}
+
+/* ------------------------------------------------------------------ */
+/* Shared grid context                                                 */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int E, M;
+  double *egrid, *vgrid, *vgrid_i, *rigidity, *mugrid;
+  double dlnp, dmu;
+} or_grid;
+
+static void grid_alloc(or_grid *g, const ep_params *p, int E, int M) {
+  g->E = E;
+  g->M = M;
+  g->egrid = (double *)malloc(sizeof(double) * E);
+  g->vgrid = (double *)malloc(sizeof(double) * E);
+  g->vgrid_i = (double *)malloc(sizeof(double) * E);
+  g->rigidity = (double *)malloc(sizeof(double) * E);
+  g->mugrid = (double *)malloc(sizeof(double) * M);
+  or_build_grids(p, E, M, g->egrid, g->vgrid, g->vgrid_i, NULL, g->rigidity,
+                 g->mugrid, &g->dlnp, &g->dmu);
+}
+
+static void grid_free(or_grid *g) {
+  free(g->egrid);
+  free(g->vgrid);
+  free(g->vgrid_i);
+  free(g->rigidity);
+  free(g->mugrid);
+}
+
+/* Error bookkeeping: the first panic in the reference's own traversal
+ * order wins.  Keys order events like the reference loops do. */
+typedef struct {
+  int code;
+  unsigned long long key;
+  int stream, node, energy, mu;
+} or_err;
+
+static void err_note(or_err *e, int code, unsigned long long key, int s,
+                     int n, int en, int m) {
+  if (e->code == EP_OK || key < e->key) {
+    e->code = code;
+    e->key = key;
+    e->stream = s;
+    e->node = n;
+    e->energy = en;
+    e->mu = m;
+  }
+}
+
+/* key layout: [substep 8][group 4][stream 16][op 2][a 16][b 8][c 8]
+ * (a,b,c follow the reference loop nesting of the failing operator) */
+static unsigned long long mkkey(int sub, int group, int s, int op, int a,
+                                int b, int c) {
+  unsigned long long k = 0;
+  k |= (unsigned long long)(sub > 255 ? 255 : sub) << 56;
+  k |= (unsigned long long)(group & 0xF) << 52;
+  k |= (unsigned long long)(s & 0xFFFF) << 36;
+  k |= (unsigned long long)(op & 0x3) << 34;
+  k |= (unsigned long long)(a & 0xFFFF) << 18;
+  k |= (unsigned long long)(b & 0xFF) << 10;
+  k |= (unsigned long long)(c & 0x3FF);
+  return k;
+}
+
+/* ------------------------------------------------------------------ */
+/* updateStreamValues (src/simCore.c:325-435)                          */
+/* ------------------------------------------------------------------ */
+static void update_stream_values(const ep_params *p, int n, const double *x,
+                                 const double *y, const double *z,
+                                 const double *bmag, double *ds,
+                                 double *bplus, double *bminus) {
+  const double rs = p->rScale;
+  int k;
+  for (k = 0; k < n; k++) {
+    if (k == 0) {
+      double px = x[k + 1] * rs, py = y[k + 1] * rs, pz = z[k + 1] * rs;
+      double cx = x[k] * rs, cy = y[k] * rs, cz = z[k] * rs;
+      ds[k] = sqrt((px - cx) * (px - cx) + (py - cy) * (py - cy) + (pz - cz) * (pz - cz));
+      bminus[k] = bmag[k];
+      bplus[k] = bmag[k + 1];
+    } else if (k == n - 1) {
+      double cx = x[k] * rs, cy = y[k] * rs, cz = z[k] * rs;
+      double mx = x[k - 1] * rs, my = y[k - 1] * rs, mz = z[k - 1] * rs;
+      ds[k] = sqrt((cx - mx) * (cx - mx) + (cy - my) * (cy - my) + (cz - mz) * (cz - mz));
+      bminus[k] = bmag[k - 1];
+      bplus[k] = bmag[k];
+    } else {
+      double px = x[k + 1] * rs, py = y[k + 1] * rs, pz = z[k + 1] * rs;
+      double cx = x[k] * rs, cy = y[k] * rs, cz = z[k] * rs;
+      double mx = x[k - 1] * rs, my = y[k - 1] * rs, mz = z[k - 1] * rs;
+      double dsp = sqrt((px - cx) * (px - cx) + (py - cy) * (py - cy) + (pz - cz) * (pz - cz));
+      double dsm = sqrt((cx - mx) * (cx - mx) + (cy - my) * (cy - my) + (cz - mz) * (cz - mz));
+      ds[k] = 0.5 * (dsp + dsm);
+      bminus[k] = bmag[k - 1];
+      bplus[k] = bmag[k + 1];
+    }
+  }
+}
+
+/* seedPopulationCheck (src/energeticParticlesBoundary.c:28-60). */
+static void seed_population_check(const ep_params *p, const or_grid *g,
+                                  int n, const double *rmag, double *f) {
+  int k, e, m;
+  for (k = 0; k < n; k++)
+    for (e = 0; e < g->E; e++) {
+      double sd = or_seed(p, g->egrid[e], rmag[k]);
+      double *fr = f + ((size_t)k * g->E + e) * g->M;
+      for (m = 0; m < g->M; m++)
+        if (fr[m] < sd) fr[m] = sd;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* GetStreamList + FindSegmentLengths                                  */
+/*   (src/energeticParticles.c:2147-2234, :2493-2601)                  */
+/* ------------------------------------------------------------------ */
+static double dist3(const ep_params *p, const double *x, const double *y,
+                    const double *z, int a, int b) {
+  const double rs = p->rScale;
+  double ax = x[a] * rs, ay = y[a] * rs, az = z[a] * rs;
+  double bx = x[b] * rs, by = y[b] * rs, bz = z[b] * rs;
+  return sqrt((ax - bx) * (ax - bx) + (ay - by) * (ay - by) + (az - bz) * (az - bz));
+}
+
+int or_get_stream_list(const ep_params *p, int n, const double *x,
+                       const double *y, const double *z, int *list,
+                       int *ref, double *dsl, double *dsil, double *ds_min) {
+  int shell, size = 0, down = 0, up, sl;
+  const double one = 1.0, half = 0.5;
+  double dmin;
+  for (shell = 0; shell < n; shell++) {
+    double dsh;
+    if (shell < n - 1) {
+      up = shell + 1;
+    } else {
+      up = shell;
+      down = shell - 1;
+    }
+    /* r1 = streamGrid[up]*rScale, r0 = streamGrid[down]*rScale */
+    dsh = dist3(p, x, y, z, up, down);
+    if ((dsh > p->dsh_min) || (shell == n - 1)) {
+      list[size++] = shell;
+      ref[shell] = shell;
+      down = shell + 1;
+    } else {
+      int sp = shell - 1;
+      ref[shell] = shell + 1;
+      while ((sp >= 0) && (ref[sp] != sp)) {
+        ref[sp] = ref[sp + 1];
+        sp--;
+      }
+    }
+  }
+  dmin = 10000.0 * p->rScale;
+  for (sl = 0; sl < size; sl++) {
+    int ts = list[sl];
+    if (sl == 0) {
+      double d = dist3(p, x, y, z, list[sl + 1], ts);
+      if (d < dmin) dmin = d;
+      dsl[sl] = d;
+    } else if (sl == size - 1) {
+      double d = dist3(p, x, y, z, ts, list[sl - 1]);
+      if (d < dmin) dmin = d;
+      dsl[sl] = d;
+    } else {
+      double d1 = dist3(p, x, y, z, list[sl + 1], ts);
+      double d9 = dist3(p, x, y, z, ts, list[sl - 1]);
+      if (d1 < dmin) dmin = d1;
+      if (d9 < dmin) dmin = d9;
+      dsl[sl] = half * (d1 + d9);
+    }
+    if (dmin <= 0.0) {
+      *ds_min = dmin;
+      return EP_ERR_DS_UNDERFLOW;
+    }
+    dsil[sl] = one / dsl[sl];
+  }
+  *ds_min = dmin;
+  return size;
+}
+
+/* ------------------------------------------------------------------ */
+/* DiffuseStreamData (src/energeticParticles.c:2237-2490)              */
+/* ------------------------------------------------------------------ */
+static void diffuse_stream(const ep_params *p, const or_grid *g, int s,
+                           int sub, int n, const double *rmag,
+                           const double *bmag, const int *list,
+                           const int *ref, int size, const double *dsil,
+                           double dsMin, double dt, double *f, int *max_nsteps,
+                           or_err *err) {
+  const int E = g->E, M = g->M;
+  const double one = 1.0;
+  const double NUM_MUSTEPS_I = one / M;
+  double *f_old = (double *)malloc(sizeof(double) * size * M);
+  double *f_new = (double *)malloc(sizeof(double) * size * M);
+  double *dsm = (double *)malloc(sizeof(double) * size);
+  double *q = (double *)malloc(sizeof(double) * size);
+  double *iso_vec = (double *)malloc(sizeof(double) * size);
+  double *del_fac = (double *)malloc(sizeof(double) * M);
+  double seed3[3];
+  int e, m, k, step, shell;
+  for (e = 0; e < E; e++) {
+    double v = g->vgrid[e], vi = g->vgrid_i[e];
+    double dtMin = 0.4 * dsMin * vi;
+    int nsteps = (int)floor(dt / dtMin + one);
+    double dtProp = dt / (one * nsteps);
+    double dtProp_i = one / dtProp;
+    if (nsteps > *max_nsteps) *max_nsteps = nsteps;
+    for (k = 0; k < size; k++) {
+      int sh = list[k];
+      double tau = vi * or_mfp(p, g->rigidity[e], rmag[sh], bmag[sh]);
+      if (dtProp > tau)
+        dsm[k] = dsil[k] * tau * dtProp_i;
+      else
+        dsm[k] = dsil[k] * one;
+      q[k] = exp(-dtProp / tau);
+    }
+    for (k = 0; k < 3; k++) seed3[k] = or_seed(p, g->egrid[e], rmag[list[k]]);
+    for (m = 0; m < M; m++) del_fac[m] = v * dtProp * g->mugrid[m];
+    for (m = 0; m < M; m++)
+      for (k = 0; k < size; k++) f_old[size * m + k] = f[((size_t)list[k] * E + e) * M + m];
+    for (step = 0; step < nsteps; step++) {
+      if ((p->useEPBoundary > 0) && (p->useBoundaryFunction > 0))
+        for (m = 0; m < M; m++)
+          for (k = 0; k < 3; k++) f_old[size * m + k] = seed3[k];
+      for (m = 0; m < M; m++) {
+        double df = del_fac[m];
+        double *fo = f_old + size * m, *fn = f_new + size * m;
+        if (g->mugrid[m] >= 0.0) {
+          fn[0] = fo[0];
+          for (k = 1; k < size; k++) fn[k] = fo[k] + dsm[k] * df * (fo[k - 1] - fo[k]);
+        } else {
+          fn[size - 1] = fo[size - 1];
+          for (k = 0; k < size - 1; k++) fn[k] = fo[k] + dsm[k] * df * (fo[k] - fo[k + 1]);
+        }
+      }
+      for (k = 0; k < size; k++) iso_vec[k] = 0.0;
+      for (m = 0; m < M; m++)
+        for (k = 0; k < size; k++) iso_vec[k] = iso_vec[k] + f_new[size * m + k];
+      for (m = 0; m < M; m++)
+        for (k = 0; k < size; k++) {
+          double iso = NUM_MUSTEPS_I * iso_vec[k];
+          f_old[size * m + k] = iso + (f_new[size * m + k] - iso) * q[k];
+        }
+    }
+    for (m = 0; m < M; m++)
+      for (k = 0; k < size; k++) f[((size_t)list[k] * E + e) * M + m] = f_old[size * m + k];
+    for (m = 0; m < M; m++)
+      for (shell = 0; shell < n; shell++) {
+        double *cell = &f[((size_t)shell * E + e) * M + m];
+        if (shell != ref[shell]) *cell = f[((size_t)ref[shell] * E + e) * M + m];
+        if (isnan(*cell))
+          err_note(err, EP_ERR_NAN, mkkey(sub, 0, s, 1, e, m, shell), s, shell, e, m);
+        else if (isinf(*cell))
+          err_note(err, EP_ERR_INF, mkkey(sub, 0, s, 1, e, m, shell), s, shell, e, m);
+        if (*cell < DBL_MIN) *cell = DBL_MIN;
+      }
+  }
+  free(f_old);
+  free(f_new);
+  free(dsm);
+  free(q);
+  free(iso_vec);
+  free(del_fac);
+}
+
+/* ------------------------------------------------------------------ */
+/* AdiabaticFocusing + _Operator_Upwind                                */
+/*   (src/energeticParticles.c:1533-1803, :1806-1896)                  */
+/* ------------------------------------------------------------------ */
+static void focus_operator_upwind(int E, int M, double dmu, double *f1,
+                                  const double *f, const double *v_avg,
+                                  double *flux) {
+  const double upwind = 1.0;
+  const double dmu_i = 1.0 / dmu;
+  int e, m;
+  /* flux / v_avg are indexed [e][mu] with stride M+1 (idx_spemp1) */
+  for (e = 0; e < E; e++)
+    for (m = 1; m < M; m++) {
+      double va = v_avg[e * (M + 1) + m];
+      double cce = copysign(upwind, va);
+      flux[e * (M + 1) + m] =
+          va * 0.5 * ((1.0 - cce) * f[e * M + m] + (1.0 + cce) * f[e * M + m - 1]);
+    }
+  for (e = 0; e < E; e++)
+    for (m = 1; m < M - 1; m++)
+      f1[e * M + m] = (flux[e * (M + 1) + m + 1] - flux[e * (M + 1) + m]) * dmu_i;
+  for (e = 0; e < E; e++) {
+    double vl = v_avg[e * (M + 1) + 1];
+    double vr = v_avg[e * (M + 1) + M - 1];
+    if (vl <= 0)
+      f1[e * M + 0] = vl * f[e * M + 1] * dmu_i;
+    else
+      f1[e * M + 0] = vl * f[e * M + 0] * dmu_i;
+    if (vr >= 0)
+      f1[e * M + M - 1] = vr * (-f[e * M + M - 2]) * dmu_i;
+    else
+      f1[e * M + M - 1] = vr * (-f[e * M + M - 1]) * dmu_i;
+  }
+}
+
+/* One node: returns N_subcycles (0 if vel_abs_max == 0). */
+static int focus_node(const ep_params *p, const or_grid *g, double dt,
+                      double ds_node, double bplus, double bminus,
+                      double dlnn, double dlnb, double dupar, double *fn,
+                      double *f, double *f1, double *v_avg, double *flux) {
+  const int E = g->E, M = g->M;
+  const double safety_factor = 0.9, half = 0.5, one = 1.0, two = 2.0, three = 3.0;
+  double dt_full = dt * p->numEpSteps;
+  double DuParDt = dupar / dt_full;
+  double b = two * (dlnn / dt_full) - three * (dlnb / dt_full);
+  double dlnBds, vel_abs_max = 0.0, dt_stable, dt_subcycle;
+  int e, m, s, N;
+  if ((bplus == 0.0) || (bminus == 0.0))
+    dlnBds = 0.0;
+  else
+    dlnBds = (log(bplus) - log(bminus)) / (two * ds_node + DBL_MIN);
+  for (e = 0; e < E; e++) {
+    double a = -g->vgrid[e] * dlnBds - (two / g->vgrid[e]) * DuParDt;
+    double vel_prev = 0.0;
+    for (m = 0; m < M; m++) {
+      double muval = g->mugrid[m];
+      double vel = half * (one - muval * muval) * (a + muval * b);
+      f[e * M + m] = fn[e * M + m];
+      if (m > 0) {
+        double va = half * (vel + vel_prev);
+        v_avg[e * (M + 1) + m] = va;
+        if (fabs(va) > vel_abs_max) vel_abs_max = fabs(va);
+      }
+      vel_prev = vel;
+    }
+    v_avg[e * (M + 1) + 0] = v_avg[e * (M + 1) + 1];
+    v_avg[e * (M + 1) + M] = v_avg[e * (M + 1) + M - 1];
+  }
+  dt_stable = safety_factor * g->dmu / vel_abs_max;
+  N = (int)ceil(dt / dt_stable);
+  dt_subcycle = dt / N;
+  for (s = 0; s < N; s++) {
+    focus_operator_upwind(E, M, g->dmu, f1, f, v_avg, flux);
+    for (e = 0; e < E * M; e++) f[e] = f[e] - dt_subcycle * f1[e];
+  }
+  for (e = 0; e < E * M; e++) fn[e] = f[e];
+  return N;
+}
+
+/* ------------------------------------------------------------------ */
+/* AdiabaticChange + _Operator_Upwind                                  */
+/*   (src/energeticParticles.c:805-1077, :1081-1176)                   */
+/* ------------------------------------------------------------------ */
+static void change_operator_upwind(const ep_params *p, const or_grid *g,
+                                   double rmag, double *f1, const double *f,
+                                   const double *v_avg, double *flux) {
+  const int E = g->E, M = g->M;
+  const double upwind = 1.0;
+  const double dlnp_i = 1.0 / g->dlnp;
+  double bc_val0, bc_val1;
+  int e, m;
+  /* v_avg / flux are [e][mu] with e in 0..E (idx_spep1m) */
+  for (e = 1; e < E; e++)
+    for (m = 0; m < M; m++) {
+      double va = v_avg[e * M + m];
+      double cce = copysign(upwind, va);
+      flux[e * M + m] = va * 0.5 * ((1.0 - cce) * f[e * M + m] + (1.0 + cce) * f[(e - 1) * M + m]);
+    }
+  for (e = 1; e < E - 1; e++)
+    for (m = 0; m < M; m++)
+      f1[e * M + m] = (flux[(e + 1) * M + m] - flux[e * M + m]) * dlnp_i;
+  bc_val0 = log(or_seed(p, g->egrid[0], rmag));
+  bc_val1 = log(or_seed(p, g->egrid[E - 1], rmag));
+  for (m = 0; m < M; m++) {
+    double vl = v_avg[1 * M + m];
+    double vr = v_avg[(E - 1) * M + m];
+    if (vl <= 0)
+      f1[0 * M + m] = vl * (f[1 * M + m] - f[0 * M + m]) * dlnp_i;
+    else
+      f1[0 * M + m] = vl * (f[0 * M + m] - bc_val0) * dlnp_i;
+    if (vr >= 0)
+      f1[(E - 1) * M + m] = vr * (f[(E - 1) * M + m] - f[(E - 2) * M + m]) * dlnp_i;
+    else
+      f1[(E - 1) * M + m] = vr * (bc_val1 - f[(E - 1) * M + m]) * dlnp_i;
+  }
+}
+
+static int change_node(const ep_params *p, const or_grid *g, double dt,
+                       double rmag, double dlnn, double dlnb, double dupar,
+                       double *fn, double *f, double *f1, double *v_avg,
+                       double *vel, double *flux) {
+  const int E = g->E, M = g->M;
+  const double safety_factor = 0.9, half = 0.5, one = 1.0;
+  double dt_full = dt * p->numEpSteps;
+  double DlnBDt = dlnb / dt_full;
+  double DlnNDt = dlnn / dt_full;
+  double cool_damp_fac = (DlnNDt < 0) ? 1.0 - p->dampCooling : 1.0;
+  double DuParDt = dupar / dt_full;
+  double vel_abs_max = 0.0, dt_stable, dt_subcycle;
+  int e, m, s, N;
+  for (e = 0; e < E; e++)
+    for (m = 0; m < M; m++) {
+      double muval = g->mugrid[m], a, b;
+      f[e * M + m] = log(fn[e * M + m]);
+      a = -muval * DuParDt;
+      b = muval * muval * (DlnNDt - DlnBDt) + half * (one - muval * muval) * DlnBDt;
+      vel[e * M + m] = cool_damp_fac * (a * g->vgrid_i[e] + b);
+      if (fabs(vel[e * M + m]) > vel_abs_max) vel_abs_max = fabs(vel[e * M + m]);
+    }
+  for (e = 0; e < E - 1; e++)
+    for (m = 0; m < M; m++)
+      v_avg[(e + 1) * M + m] = half * (vel[(e + 1) * M + m] + vel[e * M + m]);
+  for (m = 0; m < M; m++) {
+    v_avg[0 * M + m] = v_avg[1 * M + m];
+    v_avg[E * M + m] = v_avg[(E - 1) * M + m];
+  }
+  dt_stable = safety_factor * g->dlnp / vel_abs_max;
+  N = (int)ceil(dt / dt_stable);
+  dt_subcycle = dt / N;
+  for (s = 0; s < N; s++) {
+    change_operator_upwind(p, g, rmag, f1, f, v_avg, flux);
+    for (e = 0; e < E * M; e++) f[e] = f[e] - dt_subcycle * f1[e];
+  }
+  for (e = 0; e < E * M; e++) fn[e] = exp(f[e]);
+  return N;
+}
+
+/* ------------------------------------------------------------------ */
+/* updateEnergeticParticles (src/energeticParticles.c:69-203)          */
+/* ------------------------------------------------------------------ */
+int or_update_ep(const ep_dims *d, const ep_params *p, const int *n_active,
+                 const double *x, const double *y, const double *z,
+                 const double *rmag, const double *bmag, const double *dlnb,
+                 const double *dlnn, const double *dupar, double *f,
+                 double *ds_out, double *bplus_out, double *bminus_out,
+                 uint32_t op_flags, int nthreads, ep_diag *diag) {
+  const int S = d->num_streams, N = d->num_nodes, E = d->num_energies, M = d->num_mu;
+  const size_t SN = (size_t)S * N;
+  const size_t cells_node = (size_t)E * M;
+  or_grid g;
+  or_err err;
+  double *ds = (double *)malloc(sizeof(double) * SN);
+  double *bplus = (double *)malloc(sizeof(double) * SN);
+  double *bminus = (double *)malloc(sizeof(double) * SN);
+  double dt = p->tDel / (1.0 * p->numEpSteps);
+  int max_focus = 0, max_change = 0, max_nsteps = 0, step;
+  double min_tau = DBL_MAX;
+  const int do_stream = (p->useParallelDiffusion > 0) && (op_flags & EP_OP_STREAM);
+  const int do_focus = (p->useAdiabaticFocus > 0) && (op_flags & EP_OP_FOCUS);
+  const int do_change = (p->useAdiabaticChange > 0) && (op_flags & EP_OP_CHANGE);
+  const int do_seed = (p->checkSeedPopulation > 0) && (op_flags & EP_OP_SEEDCHECK);
+  long long cells = 0;
+  int s;
+  memset(&err, 0, sizeof(err));
+  grid_alloc(&g, p, E, M);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  for (s = 0; s < S; s++) cells += (long long)(n_active ? n_active[s] : N) * E * M;
+
+  for (step = 0; step < p->numEpSteps; step++) {
+    /* ---- stream stage (per stream; the MPI gather/scatter of
+       src/simCore.c:177-318 is a no-op on resident data) ---- */
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : max_nsteps)
+    for (s = 0; s < S; s++) {
+      const int n = n_active ? n_active[s] : N;
+      const size_t o = (size_t)s * N;
+      double *fs = f + o * cells_node;
+      or_err lerr;
+      memset(&lerr, 0, sizeof(lerr));
+      update_stream_values(p, n, x + o, y + o, z + o, bmag + o, ds + o, bplus + o, bminus + o);
+      if (do_seed) seed_population_check(p, &g, n, rmag + o, fs);
+      if (do_stream) {
+        int *list = (int *)malloc(sizeof(int) * n);
+        int *ref = (int *)malloc(sizeof(int) * n);
+        double *dsl = (double *)malloc(sizeof(double) * n);
+        double *dsil = (double *)malloc(sizeof(double) * n);
+        double dsMin;
+        int size = or_get_stream_list(p, n, x + o, y + o, z + o, list, ref, dsl, dsil, &dsMin);
+        int mx = 0;
+        if (size == EP_ERR_DS_UNDERFLOW) {
+          err_note(&lerr, EP_ERR_DS_UNDERFLOW, mkkey(step, 0, s, 0, 0, 0, 0), s, -1, -1, -1);
+        } else if (size < 3 && (p->useEPBoundary > 0) && (p->useBoundaryFunction > 0)) {
+          err_note(&lerr, EP_ERR_STREAM_SHORT, mkkey(step, 0, s, 0, 0, 0, 0), s, -1, -1, -1);
+        } else {
+          diffuse_stream(p, &g, s, step, n, rmag + o, bmag + o, list, ref, size, dsil, dsMin, dt,
+                         fs, &mx, &lerr);
+        }
+        if (mx > max_nsteps) max_nsteps = mx;
+        free(list);
+        free(ref);
+        free(dsl);
+        free(dsil);
+      }
+      if (lerr.code != EP_OK) {
+#pragma omp critical
+        err_note(&err, lerr.code, lerr.key, lerr.stream, lerr.node, lerr.energy, lerr.mu);
+      }
+    }
+    if (err.code != EP_OK) break;
+
+    /* ---- node stage: focusing then change per stream (skip node 0,
+       innerComputeShell, src/mpiInit.c:105-109) ---- */
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : max_focus, max_change) reduction(min : min_tau)
+    for (s = 0; s < S; s++) {
+      const int n = n_active ? n_active[s] : N;
+      const size_t o = (size_t)s * N;
+      double *fs = f + o * cells_node;
+      double *ft = (double *)malloc(sizeof(double) * cells_node);
+      double *f1 = (double *)malloc(sizeof(double) * cells_node);
+      double *vel = (double *)malloc(sizeof(double) * cells_node);
+      double *v_avg = (double *)malloc(sizeof(double) * (E + 1) * (M + 1));
+      double *flux = (double *)malloc(sizeof(double) * (E + 1) * (M + 1));
+      or_err lerr;
+      int k, e, m;
+      memset(&lerr, 0, sizeof(lerr));
+      for (e = 0; e < E; e++)
+        for (k = 1; k < n; k++) {
+          double tau = or_mfp(p, g.rigidity[e], rmag[o + k], bmag[o + k]) * g.vgrid_i[e];
+          if (tau < min_tau) min_tau = tau;
+        }
+      if (do_focus)
+        for (k = 1; k < n; k++) {
+          double *fn = fs + (size_t)k * cells_node;
+          int nsub = focus_node(p, &g, dt, ds[o + k], bplus[o + k], bminus[o + k], dlnn[o + k],
+                                dlnb[o + k], dupar[o + k], fn, ft, f1, v_avg, flux);
+          if (nsub > max_focus) max_focus = nsub;
+          for (e = 0; e < E; e++)
+            for (m = 0; m < M; m++) {
+              double *c = fn + e * M + m;
+              if (isnan(*c))
+                err_note(&lerr, EP_ERR_NAN, mkkey(step, 1, s, 0, k, e, m), s, k, e, m);
+              else if (isinf(*c))
+                err_note(&lerr, EP_ERR_INF, mkkey(step, 1, s, 0, k, e, m), s, k, e, m);
+              if (*c < DBL_MIN) *c = DBL_MIN;
+            }
+        }
+      if (do_change)
+        for (k = 1; k < n; k++) {
+          double *fn = fs + (size_t)k * cells_node;
+          int nsub = change_node(p, &g, dt, rmag[o + k], dlnn[o + k], dlnb[o + k], dupar[o + k],
+                                 fn, ft, f1, v_avg, vel, flux);
+          if (nsub > max_change) max_change = nsub;
+          for (e = 0; e < E; e++)
+            for (m = 0; m < M; m++) {
+              double *c = fn + e * M + m;
+              if (isnan(*c))
+                err_note(&lerr, EP_ERR_NAN, mkkey(step, 1, s, 1, k, e, m), s, k, e, m);
+              else if (isinf(*c))
+                err_note(&lerr, EP_ERR_INF, mkkey(step, 1, s, 1, k, e, m), s, k, e, m);
+              if (*c < DBL_MIN) *c = DBL_MIN;
+            }
+        }
+      if (lerr.code != EP_OK) {
+#pragma omp critical
+        err_note(&err, lerr.code, lerr.key, lerr.stream, lerr.node, lerr.energy, lerr.mu);
+      }
+      free(ft);
+      free(f1);
+      free(vel);
+      free(v_avg);
+      free(flux);
+    }
+    if (err.code != EP_OK) break;
+  }
+  if (ds_out) memcpy(ds_out, ds, sizeof(double) * SN);
+  if (bplus_out) memcpy(bplus_out, bplus, sizeof(double) * SN);
+  if (bminus_out) memcpy(bminus_out, bminus, sizeof(double) * SN);
+  if (diag) {
+    memset(diag, 0, sizeof(*diag));
+    diag->max_subcycles_change = max_change;
+    diag->max_subcycles_focusing = max_focus;
+    diag->max_nsteps_stream = max_nsteps;
+    diag->min_tau = min_tau;
+    diag->err_code = err.code;
+    diag->err_stream = err.code ? err.stream : -1;
+    diag->err_node = err.code ? err.node : -1;
+    diag->err_energy = err.code ? err.energy : -1;
+    diag->err_mu = err.code ? err.mu : -1;
+    diag->cell_updates = cells * p->numEpSteps;
+  }
+  free(ds);
+  free(bplus);
+  free(bminus);
+  grid_free(&g);
+  return err.code;
+}
+
+/* ------------------------------------------------------------------ */
+/* MAS trilinear interpolation (src/masInterp.c)                       */
+/* ------------------------------------------------------------------ */
+double or_mas_bsearch(const float *A, double key, int *imin, int *imax,
+                      int lower, int upper) {
+  int imid;
+  while (upper != (lower + 1)) {
+    imid = (lower + upper) / 2;
+    if (A[imid] < key)
+      lower = imid;
+    else
+      upper = imid;
+  }
+  *imin = lower;
+  *imax = upper;
+  if (A[*imin] > key) return 0.0;
+  if (A[*imax] < key) return 1.0;
+  return ((key - A[lower]) / (A[upper] - A[lower]));
+}
+
+/* masInterpolate (src/masInterp.c:177-200). */
+static double mas_interpolate(const float *V, int r0, int r1, int t0, int t1,
+                              int p0, int p1, double rd, double td, double pd,
+                              int nr, int nt) {
+#define OR_VIDX(r, t, p) ((size_t)(r) + (size_t)nr * (t) + (size_t)nr * nt * (p))
+  double c00 = V[OR_VIDX(r0, t0, p0)] * (1.0 - rd) + V[OR_VIDX(r1, t0, p0)] * rd;
+  double c10 = V[OR_VIDX(r0, t0, p1)] * (1.0 - rd) + V[OR_VIDX(r1, t0, p1)] * rd;
+  double c01 = V[OR_VIDX(r0, t1, p0)] * (1.0 - rd) + V[OR_VIDX(r1, t1, p0)] * rd;
+  double c11 = V[OR_VIDX(r0, t1, p1)] * (1.0 - rd) + V[OR_VIDX(r1, t1, p1)] * rd;
+  double c0 = c00 * (1.0 - pd) + c10 * pd;
+  double c1 = c01 * (1.0 - pd) + c11 * pd;
+#undef OR_VIDX
+  return (c0 * (1.0 - td) + c1 * td);
+}
+
+/* One field on one time level at (rr [Rs], th, ph). */
+static double mas_field(const float *V, const float *const meshes[21],
+                        const int *dims, int k, double rr, double th,
+                        double ph) {
+  int r0, r1, t0, t1, p0, p1;
+  const int nr = dims[3 * k], nt = dims[3 * k + 1], np = dims[3 * k + 2];
+  double rd = or_mas_bsearch(meshes[3 * k + 0], rr, &r0, &r1, 0, nr - 1);
+  double td = or_mas_bsearch(meshes[3 * k + 1], th, &t0, &t1, 0, nt - 1);
+  double pd = or_mas_bsearch(meshes[3 * k + 2], ph, &p0, &p1, 0, np - 1);
+  (void)np;
+  return mas_interpolate(V, r0, r1, t0, t1, p0, p1, rd, td, pd, nr, nt);
+}
+
+void or_mas_interp(int n, double rScale, const double *x, const double *y,
+                   const double *z, const double *rmag,
+                   const float *const fields0[7], const float *const meshes0[21],
+                   const int *dims0, const float *const fields1[7],
+                   const float *const meshes1[21], const int *dims1,
+                   double s, double phi_offset, double mas_radial_max,
+                   double mas_vmin, const double *rold_x,
+                   const double *rold_y, const double *rold_z,
+                   const double *prev_br, const double *prev_bt,
+                   const double *prev_bp, const double *prev_vr,
+                   const double *prev_vt, const double *prev_vp,
+                   const double *prev_d, double *br, double *bt, double *bp,
+                   double *vr, double *vt, double *vp, double *dens) {
+  int i;
+  for (i = 0; i < n; i++) {
+    /* cartToSphPosAu (src/geometry.c:131-160, :35-45) */
+    double ax = x[i] * rScale, ay = y[i] * rScale, az = z[i] * rScale;
+    double pr = sqrt(ax * ax + ay * ay + az * az);
+    double pth = acos(az / pr);
+    double pph = atan2(ay, ax);
+    while (pph < 0.0) pph += (2.0 * OR_PI);
+    if (pph < 0.0)
+      while (pph < 0.0) pph += 2 * OR_PI;
+    if (pph > 2 * OR_PI)
+      while (pph > 2 * OR_PI) pph -= 2 * OR_PI;
+    if (pr <= (mas_radial_max * OR_RSAU)) {
+      /* masTriLinear (src/masInterp.c:539-741) */
+      double rr = pr / OR_RSAU, th = pth, ph = pph - phi_offset;
+      double v;
+      if (ph < 0.0)
+        while (ph < 0.0) ph += (2.0 * OR_PI);
+      if (ph > (2.0 * OR_PI))
+        while (ph > (2.0 * OR_PI)) ph -= (2.0 * OR_PI);
+      bp[i] = ((1.0 - s) * mas_field(fields0[0], meshes0, dims0, 0, rr, th, ph) +
+               s * mas_field(fields1[0], meshes1, dims1, 0, rr, th, ph)) * OR_MAS_B_CONVERT;
+      bt[i] = ((1.0 - s) * mas_field(fields0[1], meshes0, dims0, 1, rr, th, ph) +
+               s * mas_field(fields1[1], meshes1, dims1, 1, rr, th, ph)) * OR_MAS_B_CONVERT;
+      br[i] = ((1.0 - s) * mas_field(fields0[2], meshes0, dims0, 2, rr, th, ph) +
+               s * mas_field(fields1[2], meshes1, dims1, 2, rr, th, ph)) * OR_MAS_B_CONVERT;
+      vp[i] = ((1.0 - s) * mas_field(fields0[3], meshes0, dims0, 3, rr, th, ph) +
+               s * mas_field(fields1[3], meshes1, dims1, 3, rr, th, ph)) * OR_MAS_V_CONVERT;
+      vt[i] = ((1.0 - s) * mas_field(fields0[4], meshes0, dims0, 4, rr, th, ph) +
+               s * mas_field(fields1[4], meshes1, dims1, 4, rr, th, ph)) * OR_MAS_V_CONVERT;
+      v = ((1.0 - s) * mas_field(fields0[5], meshes0, dims0, 5, rr, th, ph) +
+           s * mas_field(fields1[5], meshes1, dims1, 5, rr, th, ph)) * OR_MAS_V_CONVERT;
+      if (v < (mas_vmin / OR_C)) v = (mas_vmin / OR_C);
+      vr[i] = v;
+      dens[i] = ((1.0 - s) * mas_field(fields0[6], meshes0, dims0, 6, rr, th, ph) +
+                 s * mas_field(fields1[6], meshes1, dims1, 6, rr, th, ph)) * OR_MAS_RHO_CONVERT;
+    } else {
+      /* masWind (src/masInterp.c:1007-1065), no fake co-rotation */
+      double ox = rold_x[i], oy = rold_y[i], oz = rold_z[i];
+      double rOldmag = sqrt(ox * ox + oy * oy + oz * oz);
+      double oneOverR = rOldmag / rmag[i];
+      br[i] = prev_br[i] * oneOverR * oneOverR;
+      bt[i] = prev_bt[i] * oneOverR;
+      bp[i] = prev_bp[i] * oneOverR;
+      vr[i] = prev_vr[i];
+      vt[i] = prev_vt[i] * oneOverR;
+      vp[i] = prev_vp[i] * oneOverR;
+      dens[i] = prev_d[i] * oneOverR * oneOverR;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Parker-wind host pipeline (src/eprem.c:426-744, masCouple = 0)      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  /* [stream][shell], shell 0 is the spawn template (INNER_SHELL of
+     rank 0); shells 1..N are the active nodes (global node = shell-1) */
+  double *x, *y, *z, *rmag, *zen, *azi;
+  double *rox, *roy, *roz;
+  double *dens, *densOld, *br, *bt, *bp, *bmag, *bmagOld;
+  double *vr, *vt, *vp, *vrOld, *vtOld, *vpOld;
+  double *vvx, *vvy, *vvz;
+  double *dlnb, *dlnn, *dupar;
+  double *f; /* [stream][shell][E][M] */
+} or_state;
+
+#define OR_NF 26
+
+static void state_alloc(or_state *st, size_t nodes, size_t cells) {
+  double **fld[OR_NF] = {&st->x, &st->y, &st->z, &st->rmag, &st->zen, &st->azi,
+                         &st->rox, &st->roy, &st->roz, &st->dens, &st->densOld,
+                         &st->br, &st->bt, &st->bp, &st->bmag, &st->bmagOld,
+                         &st->vr, &st->vt, &st->vp, &st->vrOld, &st->vtOld,
+                         &st->vpOld, &st->vvx, &st->vvy, &st->vvz, &st->dlnb};
+  int i;
+  for (i = 0; i < OR_NF; i++) *fld[i] = (double *)calloc(nodes, sizeof(double));
+  st->dlnn = (double *)calloc(nodes, sizeof(double));
+  st->dupar = (double *)calloc(nodes, sizeof(double));
+  st->f = (double *)calloc(cells, sizeof(double));
+}
+
+static void state_free(or_state *st) {
+  double *fld[OR_NF + 3] = {st->x, st->y, st->z, st->rmag, st->zen, st->azi,
+                            st->rox, st->roy, st->roz, st->dens, st->densOld,
+                            st->br, st->bt, st->bp, st->bmag, st->bmagOld,
+                            st->vr, st->vt, st->vp, st->vrOld, st->vtOld,
+                            st->vpOld, st->vvx, st->vvy, st->vvz, st->dlnb,
+                            st->dlnn, st->dupar, st->f};
+  int i;
+  for (i = 0; i < OR_NF + 3; i++) free(fld[i]);
+}
+
+/* Copy one node's whole record (rippleShellsOut memcpy, src/simCore.c:701-728). */
+static void node_copy(or_state *st, size_t dst, size_t src, size_t cells_node) {
+  double *fld[OR_NF + 2] = {st->x, st->y, st->z, st->rmag, st->zen, st->azi,
+                            st->rox, st->roy, st->roz, st->dens, st->densOld,
+                            st->br, st->bt, st->bp, st->bmag, st->bmagOld,
+                            st->vr, st->vt, st->vp, st->vrOld, st->vtOld,
+                            st->vpOld, st->vvx, st->vvy, st->vvz, st->dlnb,
+                            st->dlnn, st->dupar};
+  int i;
+  for (i = 0; i < OR_NF + 2; i++) fld[i][dst] = fld[i][src];
+  memcpy(st->f + dst * cells_node, st->f + src * cells_node, sizeof(double) * cells_node);
+}
+
+int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads) {
+  const int R = c->rows, C = c->cols, N = c->num_nodes, E = c->num_energies, M = c->num_mu;
+  const int S = 6 * R * C, L = N + 1; /* shells per stream incl. template */
+  const size_t cells_node = (size_t)E * M;
+  or_state st;
+  ep_params p;
+  or_grid g;
+  double tDel, t_global, t_init, simStartTimeDay, simStopTimeDay, azi_sun;
+  double mhdUs, mhdNsAu, mhdBsAu, omega, unifiedOutputTime, epCalcStartTime;
+  int num_loops = 0, rciter = 0, simStarted = 0, epInit = 0, rc = EP_OK;
+  int face, row, col, s, k, e, m;
+  double *ax, *ay, *az, *arm, *abm, *adb, *adn, *adu, *af;
+
+  memset(&p, 0, sizeof(p));
+  p.rScale = c->rScale;
+  p.numEpSteps = c->numEpSteps;
+  p.mfpType = c->mfpType;
+  p.lamo = c->lamo;
+  p.mfpRadialPower = c->mfpRadialPower;
+  p.mfpBPower = c->mfpBPower;
+  p.rigidityPower = c->rigidityPower;
+  p.eMin = c->eMin;
+  p.eMax = c->eMax;
+  p.mass = c->mass;
+  p.charge = c->charge;
+  p.dsh_min = c->dsh_min;
+  p.dampCooling = c->dampCooling;
+  p.boundaryFunctAmplitude = c->boundaryFunctAmplitude;
+  p.boundaryFunctXi = c->boundaryFunctXi;
+  p.boundaryFunctGamma = c->boundaryFunctGamma;
+  p.boundaryFunctBeta = c->boundaryFunctBeta;
+  p.boundaryFunctEcutoff = c->boundaryFunctEcutoff;
+  p.boundaryFunctR0 = c->boundaryFunctR0;
+  p.useParallelDiffusion = c->useParallelDiffusion;
+  p.useAdiabaticFocus = c->useAdiabaticFocus;
+  p.useAdiabaticChange = c->useAdiabaticChange;
+  p.checkSeedPopulation = c->checkSeedPopulation;
+  p.useEPBoundary = c->useEPBoundary;
+  p.useBoundaryFunction = c->useBoundaryFunction;
+  p.adiabaticFocusAlg = 1;
+  p.adiabaticChangeAlg = 1;
+
+  /* setRuntimeConstants (src/configuration.c:324-348) */
+  simStartTimeDay = c->simStartTime / OR_DAY;
+  simStopTimeDay = c->simStopTime / OR_DAY;
+  tDel = c->tDel / OR_DAY;
+  p.tDel = tDel;
+  mhdUs = c->flowMag / OR_C;
+  mhdNsAu = c->mhdDensityAu / OR_MHD_DENSITY_NORM;
+  mhdBsAu = c->mhdBAu / OR_MHD_B_NORM;
+  p.mhdBsAu = mhdBsAu;
+  omega = c->omegaSun;
+  /* timeInitialization, non-MAS branch (src/simCore.c:160-165): the
+     output time uses tDel after the /DAY conversion */
+  unifiedOutputTime = (c->unifiedOutputTime >= 0.0) ? c->unifiedOutputTime
+                                                    : c->simStartTime + 0.5 * tDel * OR_DAY;
+  epCalcStartTime = (c->epCalcStartTime >= 0.0) ? c->epCalcStartTime : c->simStartTime;
+
+  grid_alloc(&g, &p, E, M);
+  state_alloc(&st, (size_t)S * L, (size_t)S * L * cells_node);
+  ax = (double *)malloc(sizeof(double) * S * N);
+  ay = (double *)malloc(sizeof(double) * S * N);
+  az = (double *)malloc(sizeof(double) * S * N);
+  arm = (double *)malloc(sizeof(double) * S * N);
+  abm = (double *)malloc(sizeof(double) * S * N);
+  adb = (double *)malloc(sizeof(double) * S * N);
+  adn = (double *)malloc(sizeof(double) * S * N);
+  adu = (double *)malloc(sizeof(double) * S * N);
+  af = (double *)malloc(sizeof(double) * S * N * cells_node);
+
+  /* simCoreInit (src/simCore.c:56-75) */
+  azi_sun = c->aziSunStart;
+  t_init = N * tDel;
+  t_global = simStartTimeDay - t_init;
+
+  /* gridStructInit: initCubeCoords, initSphereCoords, initCopyAll,
+     initInnerShellValues (src/cubeShellInit.c:46-91, :529-822,
+     :1016-1090, :1164-1236).  Stream s = idx_frc(face,row,col). */
+  {
+    double cw = 1.0 / C, chh = 1.0 / R;
+    for (face = 0; face < 6; face++)
+      for (row = 0; row < R; row++)
+        for (col = 0; col < C; col++) {
+          double cx = 0, cy = 0, cz = 0, rm, zen, azi;
+          /* The reference walks each face accumulating the coordinate by
+             repeated +/- cellWidth (cellHeight); u runs from -0.5 upward
+             with the column, u2 from +0.5 downward, w from +0.5 downward
+             with the row.  Reproduce that accumulation exactly. */
+          double u = -0.5, u2 = 0.5, w = 0.5;
+          size_t i0;
+          int j;
+          u += 0.5 * cw;
+          u2 -= 0.5 * cw;
+          w -= 0.5 * chh;
+          for (j = 0; j < col; j++) {
+            u += cw;
+            u2 -= cw;
+          }
+          for (j = 0; j < row; j++) w -= chh;
+          switch (face) {
+            case 0: cx = u; cy = w; cz = 0.5; break;    /* TOP    */
+            case 1: cx = u2; cy = w; cz = -0.5; break;  /* BOTTOM */
+            case 2: cx = 0.5; cy = u; cz = w; break;    /* FRONT  */
+            case 3: cx = -0.5; cy = u2; cz = w; break;  /* BACK   */
+            case 4: cx = u2; cy = 0.5; cz = w; break;   /* LEFT   */
+            default: cx = u; cy = -0.5; cz = w; break;  /* RIGHT  */
+          }
+          s = (face * R + row) * C + col;
+          rm = sqrt((cx * cx) + (cy * cy) + (cz * cz));
+          cx = cx / rm;
+          cy = cy / rm;
+          cz = cz / rm;
+          rm = 1.0;
+          azi = atan2(cy, cx);
+          zen = acos(cz / rm);
+          i0 = (size_t)s * L;
+          st.rmag[i0] = 1.0;
+          st.azi[i0] = azi;
+          st.zen[i0] = zen;
+          st.z[i0] = st.rmag[i0] * cos(zen);
+          st.x[i0] = st.rmag[i0] * sin(zen) * cos(azi);
+          st.y[i0] = st.rmag[i0] * sin(zen) * sin(azi);
+          for (k = 1; k < L; k++) {
+            size_t i = i0 + k;
+            st.rmag[i] = 1.0 + 0 + k;
+            st.azi[i] = azi;
+            st.zen[i] = zen;
+            st.z[i] = st.rmag[i] * cos(zen);
+            st.x[i] = st.rmag[i] * sin(zen) * cos(azi);
+            st.y[i] = st.rmag[i] * sin(zen) * sin(azi);
+          }
+          for (k = 0; k < L; k++) {
+            size_t i = i0 + k;
+            for (e = 0; e < (int)cells_node; e++) st.f[i * cells_node + e] = DBL_MIN;
+          }
+        }
+  }
+
+  out->num_dumps = 0;
+  out->num_steps = 0;
+  if (out->egrid) memcpy(out->egrid, g.egrid, sizeof(double) * E);
+  if (out->vgrid) memcpy(out->vgrid, g.vgrid, sizeof(double) * E);
+  if (out->mu) memcpy(out->mu, g.mugrid, sizeof(double) * M);
+
+  do {
+    rciter += 1;
+    if (rciter == N + 1 && simStarted == 0) {
+      simStarted = 1;
+      /* initEnergeticParticles (src/energeticParticlesInit.c:106-147) */
+      if (p.useBoundaryFunction > 0 && c->boundaryFunctionInitDomain > 0)
+        for (s = 0; s < S; s++)
+          for (k = 0; k < L; k++)
+            for (e = 0; e < E; e++) {
+              double sd = or_seed(&p, g.egrid[e], st.rmag[(size_t)s * L + k]);
+              for (m = 0; m < M; m++) st.f[((size_t)s * L + k) * cells_node + e * M + m] = sd;
+            }
+      t_global = simStartTimeDay;
+    }
+    if (t_global * OR_DAY >= epCalcStartTime && simStarted == 1) simStarted = 2;
+
+    /* dataDumpIO -> writeObserverDataNetCDF (src/unifiedOutput.c:442-549,
+       :1479-1530) */
+    if ((c->dumpFreq > 0) && (num_loops % c->dumpFreq) == 0 &&
+        (t_global * OR_DAY >= unifiedOutputTime) && out->num_dumps < out->max_dumps) {
+      int dd = out->num_dumps;
+      out->time[dd] = t_global * OR_DAY;
+      for (s = 0; s < S; s++)
+        for (k = 0; k < N; k++) {
+          size_t i = (size_t)s * L + k + 1;
+          size_t o = ((size_t)dd * S + s) * N + k;
+          double zen = acos(st.z[i] / st.rmag[i]);
+          double azi = atan2(st.y[i], st.x[i]);
+          if (azi < 0.0) azi += 2.0 * OR_PI;
+          if (azi > (2.0 * OR_PI)) azi -= 2.0 * OR_PI;
+          out->R[o] = st.rmag[i] * c->rScale;
+          out->T[o] = zen;
+          out->P[o] = azi;
+          out->Br[o] = st.br[i] * (OR_MHD_B_NORM * 1.0e5);
+          out->Bt[o] = st.bt[i] * (OR_MHD_B_NORM * 1.0e5);
+          out->Bp[o] = st.bp[i] * (OR_MHD_B_NORM * 1.0e5);
+          out->Vr[o] = st.vr[i] * (OR_C / 1.0e5);
+          out->Vt[o] = st.vt[i] * (OR_C / 1.0e5);
+          out->Vp[o] = st.vp[i] * (OR_C / 1.0e5);
+          out->Rho[o] = st.dens[i] * OR_MHD_DENSITY_NORM;
+          for (e = 0; e < (int)cells_node; e++)
+            out->dist[o * cells_node + e] = st.f[i * cells_node + e] * (1.0 / 27.0);
+        }
+      out->num_dumps++;
+    }
+
+    /* rotSunAndSpawnShell (src/simCore.c:442-618): single rank, no ring */
+    azi_sun += (omega * tDel);
+    if (azi_sun > OR_PI) azi_sun -= OR_TWO_PI;
+    for (s = 0; s < S; s++) {
+      size_t i0 = (size_t)s * L;
+      for (k = L - 1; k > 0; k--) node_copy(&st, i0 + k, i0 + k - 1, cells_node);
+      /* spawnNewShell (src/simCore.c:629-687) */
+      {
+        size_t i = i0 + 1;
+        double azi = st.azi[i], zen = st.zen[i], rm = st.rmag[i];
+        azi += azi_sun;
+        if (azi > OR_PI) azi -= OR_TWO_PI;
+        st.zen[i] = zen;
+        st.azi[i] = azi;
+        st.x[i] = rm * sin(zen) * cos(azi);
+        st.y[i] = rm * sin(zen) * sin(azi);
+        st.z[i] = rm * cos(zen);
+      }
+    }
+
+    /* moveNodes (src/simCore.c:736-785), delrFlow MHD_DEFAULT
+       (src/flow.c:169-178): active shells 2..N move */
+    for (s = 0; s < S; s++)
+      for (k = 2; k < L; k++) {
+        size_t i = (size_t)s * L + k;
+        double dx = tDel * st.vvx[i] / c->rScale;
+        double dy = tDel * st.vvy[i] / c->rScale;
+        double dz = tDel * st.vvz[i] / c->rScale;
+        double rx = st.x[i], ry = st.y[i], rz = st.z[i];
+        st.rox[i] = rx;
+        st.roy[i] = ry;
+        st.roz[i] = rz;
+        rx += dx;
+        ry += dy;
+        rz += dz;
+        st.x[i] = rx;
+        st.y[i] = ry;
+        st.z[i] = rz;
+        st.rmag[i] = sqrt((rx * rx) + (ry * ry) + (rz * rz));
+      }
+
+    /* updateMhd (src/flow.c:33-154) with the Parker model
+       (mhdDensity :241-275, mhdV :355-403, mhdB :280-350) */
+    for (s = 0; s < S; s++)
+      for (k = 1; k < L; k++) {
+        size_t i = (size_t)s * L + k;
+        double rr, theta, phi;
+        st.bmagOld[i] = st.bmag[i];
+        st.densOld[i] = st.dens[i];
+        st.vrOld[i] = st.vr[i];
+        st.vtOld[i] = st.vt[i];
+        st.vpOld[i] = st.vp[i];
+        rr = st.rmag[i] * c->rScale;
+        st.dens[i] = mhdNsAu / (rr * rr);
+        /* mhdV */
+        theta = acos(st.z[i] / st.rmag[i]);
+        phi = atan2(st.y[i], st.x[i]);
+        st.vr[i] = mhdUs;
+        st.vt[i] = 0.0;
+        st.vp[i] = 0.0;
+        st.vvx[i] = st.vr[i] * sin(theta) * cos(phi) + st.vt[i] * cos(theta) * cos(phi) -
+                    st.vp[i] * sin(phi);
+        st.vvy[i] = st.vr[i] * sin(theta) * sin(phi) + st.vt[i] * cos(theta) * sin(phi) +
+                    st.vp[i] * cos(phi);
+        st.vvz[i] = st.vr[i] * cos(theta) - st.vt[i] * sin(theta);
+        /* mhdB */
+        st.br[i] = mhdBsAu / (rr * rr);
+        st.bt[i] = 0.0;
+        st.bp[i] = -1.0 * rr * st.br[i] * (omega / (st.vr[i] + OR_VERYSMALL)) * sin(theta);
+        st.bmag[i] = sqrt(st.br[i] * st.br[i] + st.bt[i] * st.bt[i] + st.bp[i] * st.bp[i]);
+        if (k == 1) {
+          st.dupar[i] = 0.0;
+          st.dlnb[i] = 0.0;
+          st.dlnn[i] = 0.0;
+        } else {
+          st.dupar[i] = (st.vr[i] - st.vrOld[i]) * st.br[i] + (st.vt[i] - st.vtOld[i]) * st.bt[i] +
+                        (st.vp[i] - st.vpOld[i]) * st.bp[i];
+          st.dupar[i] /= st.bmag[i];
+          st.dlnb[i] = log(st.bmag[i] / st.bmagOld[i]);
+          st.dlnn[i] = log(st.dens[i] / st.densOld[i]);
+        }
+      }
+
+    /* particle computation (src/eprem.c:653-682) */
+    if (t_global * OR_DAY >= epCalcStartTime) {
+      ep_diag dg;
+      if (epInit == 0) {
+        epInit = 1;
+        if (p.useBoundaryFunction > 0 && c->boundaryFunctionInitDomain > 0)
+          for (s = 0; s < S; s++)
+            for (k = 0; k < L; k++)
+              for (e = 0; e < E; e++) {
+                double sd = or_seed(&p, g.egrid[e], st.rmag[(size_t)s * L + k]);
+                for (m = 0; m < M; m++) st.f[((size_t)s * L + k) * cells_node + e * M + m] = sd;
+              }
+      }
+      /* pack the active shells as resident streams */
+      for (s = 0; s < S; s++)
+        for (k = 0; k < N; k++) {
+          size_t i = (size_t)s * L + k + 1, o = (size_t)s * N + k;
+          ax[o] = st.x[i];
+          ay[o] = st.y[i];
+          az[o] = st.z[i];
+          arm[o] = st.rmag[i];
+          abm[o] = st.bmag[i];
+          adb[o] = st.dlnb[i];
+          adn[o] = st.dlnn[i];
+          adu[o] = st.dupar[i];
+          memcpy(af + o * cells_node, st.f + i * cells_node, sizeof(double) * cells_node);
+        }
+      {
+        ep_dims d;
+        d.num_streams = S;
+        d.num_nodes = N;
+        d.num_energies = E;
+        d.num_mu = M;
+        rc = or_update_ep(&d, &p, NULL, ax, ay, az, arm, abm, adb, adn, adu, af, NULL, NULL,
+                          NULL, EP_OP_ALL, nthreads, &dg);
+      }
+      for (s = 0; s < S; s++)
+        for (k = 0; k < N; k++) {
+          size_t i = (size_t)s * L + k + 1, o = (size_t)s * N + k;
+          memcpy(st.f + i * cells_node, af + o * cells_node, sizeof(double) * cells_node);
+        }
+      if (c->seedFunctionTest > 0 && p.useBoundaryFunction > 0 && c->boundaryFunctionInitDomain > 0)
+        for (s = 0; s < S; s++)
+          for (k = 0; k < L; k++)
+            for (e = 0; e < E; e++) {
+              double sd = or_seed(&p, g.egrid[e], st.rmag[(size_t)s * L + k]);
+              for (m = 0; m < M; m++) st.f[((size_t)s * L + k) * cells_node + e * M + m] = sd;
+            }
+      if (out->step_diag && out->num_steps < out->max_steps) out->step_diag[out->num_steps++] = dg;
+      if (rc != EP_OK) break;
+    }
+    t_global += tDel;
+    num_loops++;
+  } while (t_global <= simStopTimeDay);
+
+  free(ax);
+  free(ay);
+  free(az);
+  free(arm);
+  free(abm);
+  free(adb);
+  free(adn);
+  free(adu);
+  free(af);
+  state_free(&st);
+  grid_free(&g);
+  return rc;
+}

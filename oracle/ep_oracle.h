@@ -1,0 +1,130 @@
+/*
+ * ep_oracle.h -- FP64 CPU oracle for the EPREM energetic-particle update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's hot path, used by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg as the checker and the CPU baseline.  The
+ * product (eprem_amd/) never links, loads or calls it.
+ *
+ * Every routine follows the reference's operation order, compiled with
+ * -ffp-contract=off, so that it reproduces the reference arithmetic.  Each
+ * cites the reference file:line it restates.
+ *
+ * Layouts: node arrays are [stream][node]; f is [stream][node][energy][mu]
+ * (the reference's eParts order, src/global.h:61, with species = 1).
+ */
+#ifndef EP_ORACLE_H
+#define EP_ORACLE_H
+
+#include "eprem_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Energy/mu grids, initEnergeticParticlesGrids
+ * (src/energeticParticlesInit.c:36-100). */
+void or_build_grids(const ep_params *p, int E, int M, double *egrid,
+                    double *vgrid, double *vgrid_i, double *pgrid,
+                    double *rigidity, double *mugrid, double *dlnp,
+                    double *dmu);
+
+/* sepSeedFunction (src/energeticParticlesBoundary.c:67-116). */
+double or_seed(const ep_params *p, double energy, double r);
+
+/* meanFreePath (src/meanFreePath.c:13-50); rig = rigidity[e]. */
+double or_mfp(const ep_params *p, double rig, double rmag, double bmag);
+
+/* Whole updateEnergeticParticles() (src/energeticParticles.c:69-203) for
+ * all streams.  n_active may be NULL (all = num_nodes).  ds/bplus/bminus
+ * receive updateStreamValues' per-node output (may be NULL).  nthreads:
+ * OpenMP threads over streams (<=0: library default).  Returns EP_OK or
+ * the first panic's EP_ERR_* code with its location in diag. */
+int or_update_ep(const ep_dims *d, const ep_params *p, const int *n_active,
+                 const double *x, const double *y, const double *z,
+                 const double *rmag, const double *bmag, const double *dlnb,
+                 const double *dlnn, const double *dupar, double *f,
+                 double *ds, double *bplus, double *bminus, uint32_t op_flags,
+                 int nthreads, ep_diag *diag);
+
+/* GetStreamList + FindSegmentLengths (src/energeticParticles.c:2147-2234,
+ * :2493-2601) for one stream of n nodes with positions x,y,z.  Writes
+ * shell_list[<=n], shell_ref[n], ds_list[<=n], ds_i_list[<=n], *ds_min;
+ * returns streamlistSize, or EP_ERR_DS_UNDERFLOW. */
+int or_get_stream_list(const ep_params *p, int n, const double *x,
+                       const double *y, const double *z, int *shell_list,
+                       int *shell_ref, double *ds_list, double *ds_i_list,
+                       double *ds_min);
+
+/* ---------------- MAS trilinear (src/masInterp.c) ----------------- */
+
+/* masTriLinearBinarySearch (src/masInterp.c:117-171). */
+double or_mas_bsearch(const float *A, double key, int *imin, int *imax,
+                      int lower, int upper);
+
+/* masTriLinear + masGetNode + masWind for a batch of nodes
+ * (src/masInterp.c:30-68, :177-200, :539-741, :1007-1065).
+ * fields0/1[7], meshes0/1[21], dims0/1[21] as in ep_mas_set_level.
+ * Positions x,y,z,rmag are in rScale units.  prev_* give the node's
+ * previous state for masWind (may be NULL when all nodes are inside).
+ * Outputs [n] in code units. */
+void or_mas_interp(int n, double rScale, const double *x, const double *y,
+                   const double *z, const double *rmag,
+                   const float *const fields0[7], const float *const meshes0[21],
+                   const int *dims0, const float *const fields1[7],
+                   const float *const meshes1[21], const int *dims1,
+                   double s_cor, double phi_offset, double mas_radial_max,
+                   double mas_vmin, const double *rold_x,
+                   const double *rold_y, const double *rold_z,
+                   const double *prev_br, const double *prev_bt,
+                   const double *prev_bp, const double *prev_vr,
+                   const double *prev_vt, const double *prev_vp,
+                   const double *prev_d, double *br, double *bt, double *bp,
+                   double *vr, double *vt, double *vp, double *dens);
+
+/* ---------------- Parker-wind host pipeline ------------------------ */
+
+/* Inputs of the analytic-wind pipeline (src/eprem.c:426-744 with
+ * masCouple = 0), in input-file units (converted inside the way
+ * setRuntimeConstants does, src/configuration.c:324-348). */
+typedef struct or_sim_config {
+  int rows, cols, num_nodes, num_energies, num_mu;
+  double rScale, flowMag, mhdDensityAu, mhdBAu;
+  double simStartTime, simStopTime, tDel;   /* [days] */
+  int numEpSteps;
+  double aziSunStart, omegaSun;
+  double dsh_min, lamo, mfpRadialPower, mfpBPower, rigidityPower;
+  int mfpType;
+  double eMin, eMax, mass, charge, dampCooling;
+  double boundaryFunctAmplitude, boundaryFunctXi, boundaryFunctGamma,
+         boundaryFunctBeta, boundaryFunctEcutoff, boundaryFunctR0;
+  int useParallelDiffusion, useAdiabaticFocus, useAdiabaticChange,
+      checkSeedPopulation, useEPBoundary, useBoundaryFunction,
+      boundaryFunctionInitDomain, seedFunctionTest, dumpFreq;
+  double unifiedOutputTime;   /* < 0: default simStartTime + tDel/2 days */
+  double epCalcStartTime;     /* < 0: default simStartTime             */
+} or_sim_config;
+
+/* Per-dump stream output (writeObserverDataNetCDF,
+ * src/unifiedOutput.c:442-549), values already multiplied by the
+ * netCDF scale_factor (physical units).  Buffers are sized by the caller
+ * for max_dumps dumps: time[max_dumps], node fields
+ * [max_dumps][S][N] (R, T, P, Br, Bt, Bp, Vr, Vt, Vp, Rho),
+ * dist [max_dumps][S][N][E][M].  Per-step diagnostics go to
+ * step_diag[max_steps] (may be NULL). */
+typedef struct or_sim_output {
+  int max_dumps, num_dumps, max_steps, num_steps;
+  double *time;
+  double *R, *T, *P, *Br, *Bt, *Bp, *Vr, *Vt, *Vp, *Rho;
+  double *dist;
+  double *egrid, *vgrid, *mu;
+  ep_diag *step_diag;
+} or_sim_output;
+
+/* Run the whole analytic-wind simulation.  Returns EP_OK or an error. */
+int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
