@@ -1,0 +1,169 @@
+"""Benchmark of the EP update on one MI355X.
+
+Workload (default, --config c2): 6x8x8 = 384 streams x 2000 nodes x 50
+energies (10 keV - 1 GeV, uniform in ln p) x 30 mu, FP64, a Parker-spiral
+snapshot (400 km/s, B_r = 2.5 nT at 1 AU, 25-day rotation) with seeded
+perturbations, lambda = 0.1 AU (R/1 GV)^(1/3) (r/1 AU), seed spectrum
+E^-2, tDel = 1/64 day, 10 EP substeps per global step.  A "step" is one
+global step (ep_step: all 10 EP substeps of streaming + relaxation,
+focusing and adiabatic change on every node).  The MAS trilinear
+interpolation of a synthetic 300 x 200 x 400 FP32 grid (two time levels,
+Parker solution + Gaussian CME shell) onto every node runs once per step
+and is timed separately.
+
+Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from eprem_amd import config, constants as K, mas_synth, runtime, workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (FMA = 2 flops)
+STREAM_FLOPS_PER_SWEEP = 8   # upwind 4 + iso sum 1 + relaxation 3 (FP64)
+
+CONFIGS = {
+    # rows, cols, N, E, M, numEpSteps
+    "c1": dict(rows=2, cols=2, N=400, E=20, M=20, eps=10),
+    "c2": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10),
+}
+
+
+def make_params(c):
+    return config.default_params(
+        rScale=0.00465047, tDel=1.0 / 64, numEpSteps=c["eps"], eMin=0.01, eMax=1000.0, lamo=0.1,
+        rigidityPower=1.0 / 3.0, mfpRadialPower=1.0, mhdBAu=2.5e-5, boundaryFunctGamma=2.0,
+        boundaryFunctEcutoff=1.0e30)
+
+
+def cpu_baseline(c, p, nodes, budget_s=20.0):
+    """The FP64 oracle (OpenMP over streams) on a bounded stream subset of
+    the same workload, scaled per cell-update."""
+    import oracle
+
+    cores = min(16, os.cpu_count() or 1)
+    E, M = c["E"], c["M"]
+    nsub = max(1, min(cores, 16))
+    sub = {k: np.ascontiguousarray(v[:nsub]) for k, v in nodes.items()}
+    dims = workloads.dims_of(sub, E, M)
+    g = oracle.build_grids(p, E, M)
+    f0 = workloads.seeded_f(p, g, sub, noise=0.0)
+    t0 = time.perf_counter()
+    rc, d, fo, _, _, _ = oracle.update_ep(dims, p, sub, f0, nthreads=cores)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError("oracle failed: %d" % rc)
+    return {"value": d["cell_updates"] / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": "%d of %d streams x %d nodes x %d E x %d mu, 1 global step (%d EP substeps), "
+                      "%.1f s" % (nsub, 6 * c["rows"] * c["cols"], c["N"], E, M, p.numEpSteps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--halo", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mas", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs every stream on one GPU; --gpus %d is not supported "
+              "(the path would shard by stream, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    c = CONFIGS[args.config]
+    p = make_params(c)
+    nodes = workloads.parker_nodes(c["rows"], c["cols"], c["N"], tdel_days=1.0 / 64, flow_cms=400e5,
+                                   b_au=2.5e-5, omega=2 * math.pi * K.TAU / (25 * 86400.0),
+                                   rscale=0.00465047, seed=20261203, perturb=0.15, jitter=0.05)
+    dims = workloads.dims_of(nodes, c["E"], c["M"])
+    sol = runtime.EpSolver(dims, p)
+    if args.block or args.halo:
+        sol.set_tuning(args.block or 256, args.halo or 32)
+    sol.set_nodes(nodes)
+    sol.init_seed_population()
+
+    mas_ms = None
+    if not args.no_mas:
+        lv0, lv1 = mas_synth.levels(seed=20261203)
+        sol.mas_set_level(0, *lv0)
+        sol.mas_set_level(1, *lv1)
+        prev = mas_synth.prev_from_nodes(nodes, p.rScale)
+
+    for _ in range(args.warmup):
+        sol.step()
+        if not args.no_mas:
+            sol.interp_mhd(0.5, 0.0, 30.0, 50.0e5, prev)
+
+    acc = dict(device_ms=0.0, stream_ms=0.0, focus_ms=0.0, change_ms=0.0, prep_ms=0.0,
+               stream_sweeps=0, focus_sweeps=0, change_sweeps=0, stream_launches=0, cell_updates=0)
+    mas_total = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, dg = sol.step()
+        for k in acc:
+            acc[k] += dg[k]
+    t1 = time.perf_counter()
+    if not args.no_mas:
+        for _ in range(args.steps):
+            mas_total += sol.interp_mhd(0.5, 0.0, 30.0, 50.0e5, prev)["device_ms"]
+        mas_ms = mas_total / args.steps
+    wall = t1 - t0
+    value = acc["cell_updates"] / wall
+
+    # dominant kernel: the streaming passes (k_stream_pass)
+    st_s = acc["stream_ms"] / 1e3
+    launches = max(1, acc["stream_launches"])
+    alg_bytes_per_launch = 16.0 * acc["stream_sweeps"] / launches
+    avg_launch_s = st_s / launches
+    achieved = alg_bytes_per_launch / avg_launch_s / 1e9
+    fp64 = STREAM_FLOPS_PER_SWEEP * acc["stream_sweeps"] / st_s / 1e12
+    res = {
+        "metric": "phase-space cell-updates/s (1 GPU), surrogate vs plain at <=1% flux error",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "%s: %d streams x %d nodes x %d energies x %d mu, FP64 plain path, "
+                               "%d EP substeps per step" % (args.config, dims.num_streams, c["N"], c["E"],
+                                                             c["M"], c["eps"]),
+                   "path": "plain FP64 (no surrogate)"},
+        "roofline": {"bound": "hbm", "kernel": "k_stream_pass", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "note": "16 B per cell per Euler sweep; the kernel keeps up to `halo` sweeps "
+                             "on chip, so the binding ceiling is FP64 VALU (see fp64)"},
+        "fp64": {"kernel": "k_stream_pass", "achieved_tflops": fp64, "peak_tflops": FP64_PEAK_TFLOPS,
+                 "frac": fp64 / FP64_PEAK_TFLOPS},
+        "breakdown_ms_per_step": {k: acc[k] / args.steps for k in
+                                  ("device_ms", "prep_ms", "stream_ms", "focus_ms", "change_ms")},
+        "sweeps_per_step": {k: acc[k] // args.steps for k in ("stream_sweeps", "focus_sweeps",
+                                                              "change_sweeps")},
+        "interp_ms": mas_ms,
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(c, p, nodes)
+    sol.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

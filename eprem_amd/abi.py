@@ -106,6 +106,11 @@ class EpDiag(C.Structure):
         ("focus_ms", C.c_double),
         ("change_ms", C.c_double),
         ("prep_ms", C.c_double),
+        ("stream_sweeps", C.c_int64),
+        ("focus_sweeps", C.c_int64),
+        ("change_sweeps", C.c_int64),
+        ("stream_launches", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
     def as_dict(self):
@@ -167,4 +172,6 @@ EXPORTED_SYMBOLS = (
     "ep_interp_mhd",
     "ep_last_error",
     "ep_finalize",
+    "ep_set_tuning",
+    "ep_selftest_dpp",
 )

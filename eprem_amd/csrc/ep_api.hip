@@ -1,0 +1,550 @@
+// ep_api.hip -- the C ABI of include/eprem_hip.h (EP update part).
+//
+// Owns the device state: every buffer is allocated once in ep_init and
+// reused; ep_step launches kernels on one HIP stream and synchronises twice
+// per global step (after the once-per-step preparation, to size the
+// streaming passes, and at the end, to read errors and diagnostics).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ep_internal.h"
+#include "ep_launch.h"
+#include "ep_state.h"
+
+using namespace ep;
+
+namespace ep {
+State g;
+static std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(EP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(x, what)              \
+  do {                               \
+    hipError_t e_ = (x);             \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+template <typename T>
+static int dalloc(T **p, size_t count, const char *what) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T) + 64);
+  if (e != hipSuccess) return hip_fail(e, what);
+  g.allocs.push_back(q);
+  *p = (T *)q;
+  return EP_OK;
+}
+
+static void free_all() {
+  for (void *p : g.allocs) hipFree(p);
+  g.allocs.clear();
+  for (auto &ev : g.events) hipEventDestroy(ev);
+  g.events.clear();
+  if (g.st) hipStreamDestroy(g.st);
+  g.st = nullptr;
+  g.mas_free();
+  g.init = false;
+  g.nodes_set = false;
+}
+
+// Constants of the reference (src/baseTypes.h:50-115).
+static const double MP = 1.6726e-24, MEV = 1.6022e-6, GEV = 1.6022e-3, C = 2.99792458e10;
+static const double MZERO = (MP * C * C / GEV);
+static const double MHD_DENSITY_NORM = 1.0;
+
+// initEnergeticParticlesGrids (src/energeticParticlesInit.c:36-100) and
+// the energy-only factors of sepSeedFunction
+// (src/energeticParticlesBoundary.c:81-99).
+static void build_grids(const ep_params &p, int E, int M) {
+  const double one = 1.0, two = 2.0, half = 0.5;
+  g.egrid.assign(E, 0);
+  g.vgrid.assign(E, 0);
+  g.vgrid_i.assign(E, 0);
+  g.rigidity.assign(E, 0);
+  g.rig_lamo.assign(E, 0);
+  g.plt.assign(E, 0);
+  g.ext.assign(E, 0);
+  g.mugrid.assign(M, 0);
+  double kin = p.eMin * MEV / (MP * C * C);
+  const double lnpmin = log(sqrt(kin * kin + two * kin));
+  kin = p.eMax * MEV / (MP * C * C);
+  const double lnpmax = log(sqrt(kin * kin + two * kin));
+  g.dlnp = (lnpmax - lnpmin) / (E - 1.0);
+  for (int e = 0; e < E; e++) {
+    const double pp = exp(lnpmin + e * g.dlnp);
+    g.egrid[e] = sqrt(one + pp * pp) - one;
+    g.vgrid[e] = pp / sqrt(one + pp * pp);
+    g.vgrid_i[e] = sqrt(one + pp * pp) / pp;
+    double rig = (p.mass / p.charge) * pp * MZERO;
+    rig = pow(rig, p.rigidityPower);
+    g.rigidity[e] = rig;
+    g.rig_lamo[e] = rig * p.lamo;
+  }
+  g.dmu = two / (one * M);
+  g.mpos = M;
+  for (int m = 0; m < M; m++) {
+    const double mup = -one + (m + one) * g.dmu;
+    const double mum = -one + m * g.dmu;
+    g.mugrid[m] = half * (mup + mum);
+    if (g.mugrid[m] >= 0.0 && g.mpos == M) g.mpos = m;
+  }
+  const double efac = MEV / (MP * C * C);
+  const double normE1 = 1.0 * efac;
+  const double normE0 = p.boundaryFunctEcutoff * efac;
+  for (int e = 0; e < E; e++) {
+    g.plt[e] = pow((g.egrid[e] / normE1), -p.boundaryFunctGamma);
+    g.ext[e] = exp(-(g.egrid[e] / normE0));
+  }
+  g.normJ0 = p.boundaryFunctAmplitude * (MP * C) / (MHD_DENSITY_NORM * MEV);
+  g.normRadius = p.boundaryFunctR0 / p.rScale;
+}
+
+static int check_params(const ep_params &p) {
+  if (!(p.rScale > 0) || !(p.tDel > 0) || p.numEpSteps < 1)
+    return fail(EP_ERR_INVALID_ARG, "rScale, tDel must be > 0 and numEpSteps >= 1");
+  if (p.mfpType != 1 && p.mfpType != 2) return fail(EP_ERR_INVALID_ARG, "mfpType must be 1 or 2");
+  if (!(p.eMin > 0) || !(p.eMax >= p.eMin)) return fail(EP_ERR_INVALID_ARG, "bad eMin/eMax");
+  if (p.adiabaticFocusAlg != 1 || p.adiabaticChangeAlg != 1)
+    return fail(EP_ERR_UNSUPPORTED,
+                "only the upwind focusing/change operators (alg 1, the reference default) are "
+                "implemented on the device");
+  return EP_OK;
+}
+
+static int upload_grids() {
+  const int E = g.d.num_energies, M = g.d.num_mu;
+  HIPCHK(hipMemcpy(g.egrid_d, g.egrid.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
+  HIPCHK(hipMemcpy(g.vgrid_d, g.vgrid.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
+  HIPCHK(hipMemcpy(g.vgrid_i_d, g.vgrid_i.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
+  HIPCHK(hipMemcpy(g.rig_lamo_d, g.rig_lamo.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
+  HIPCHK(hipMemcpy(g.plt_d, g.plt.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
+  HIPCHK(hipMemcpy(g.ext_d, g.ext.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
+  HIPCHK(hipMemcpy(g.mugrid_d, g.mugrid.data(), M * 8, hipMemcpyHostToDevice), "grid upload");
+  return EP_OK;
+}
+
+Dev make_dev() {
+  Dev D;
+  memset(&D, 0, sizeof(D));
+  const ep_params &p = g.p;
+  D.S = g.d.num_streams;
+  D.N = g.d.num_nodes;
+  D.Npad = g.Npad;
+  D.E = g.d.num_energies;
+  D.M = g.d.num_mu;
+  D.rScale = p.rScale;
+  D.dt = p.tDel / (1.0 * p.numEpSteps);
+  D.dt_full = D.dt * p.numEpSteps;
+  D.numEpSteps = p.numEpSteps;
+  D.mfpType = p.mfpType;
+  D.useBC = (p.useEPBoundary > 0) && (p.useBoundaryFunction > 0);
+  D.stream_block = g.stream_block;
+  D.stream_halo = g.stream_halo;
+  D.mfpRadialPower = p.mfpRadialPower;
+  D.mfpBPower = p.mfpBPower;
+  D.mhdBsAu = p.mhdBsAu;
+  D.dsh_min = p.dsh_min;
+  D.dampCooling = p.dampCooling;
+  D.normJ0 = g.normJ0;
+  D.normRadius = g.normRadius;
+  D.xi = p.boundaryFunctXi;
+  D.beta = p.boundaryFunctBeta;
+  D.dlnp = g.dlnp;
+  D.dmu = g.dmu;
+  D.mpos = g.mpos;
+  D.egrid = g.egrid_d;
+  D.vgrid = g.vgrid_d;
+  D.vgrid_i = g.vgrid_i_d;
+  D.rig_lamo = g.rig_lamo_d;
+  D.plt = g.plt_d;
+  D.ext = g.ext_d;
+  D.mugrid = g.mugrid_d;
+  D.fA = g.fA;
+  D.fB = g.fB;
+  D.loc = g.loc;
+  D.x = g.nx;
+  D.y = g.ny;
+  D.z = g.nz;
+  D.rmag = g.nrmag;
+  D.bmag = g.nbmag;
+  D.dlnb = g.ndlnb;
+  D.dlnn = g.ndlnn;
+  D.dupar = g.ndupar;
+  D.n_active = g.n_active;
+  D.ds = g.ds;
+  D.bplus = g.bplus;
+  D.bminus = g.bminus;
+  D.mfp_r = g.mfp_r;
+  D.radial = g.radial;
+  D.list = g.list;
+  D.ref = g.ref;
+  D.nkept = g.nkept;
+  D.thin = g.thin;
+  D.dsil = g.dsil;
+  D.dsmin = g.dsmin;
+  D.nsteps = g.nsteps;
+  D.halo = g.halo;
+  D.passes = g.passes;
+  D.chunks = g.chunks;
+  D.dtprop = g.dtprop;
+  D.del_fac = g.del_fac;
+  D.seed3 = g.seed3;
+  D.dsm = g.dsm;
+  D.q = g.q;
+  D.nsub_f = g.nsub_f;
+  D.nsub_c = g.nsub_c;
+  D.dtsub_f = g.dtsub_f;
+  D.dtsub_c = g.dtsub_c;
+  D.f_dlnbds = g.f_dlnbds;
+  D.f_dupar = g.f_dupar;
+  D.f_b = g.f_b;
+  D.c_dupar = g.c_dupar;
+  D.c_dlnn = g.c_dlnn;
+  D.c_dlnb = g.c_dlnb;
+  D.c_damp = g.c_damp;
+  D.c_bc0 = g.c_bc0;
+  D.c_bc1 = g.c_bc1;
+  D.diag = g.diag;
+  return D;
+}
+
+// Decode an error key (layout in ep_internal.h) into the diag fields.
+static int decode_err(unsigned long long key, ep_diag *dg) {
+  const int group = (int)((key >> 53) & 0xF);
+  const int s = (int)((key >> 39) & 0x3FFF);
+  const int op = (int)((key >> 37) & 0x3);
+  const int a = (int)((key >> 23) & 0x3FFF);
+  const int b = (int)((key >> 15) & 0xFF);
+  const int c = (int)((key >> 1) & 0x3FFF);
+  const int inf = (int)(key & 1);
+  int code;
+  dg->err_stream = s;
+  if (group == 0 && op == 0) {
+    code = (a == kErrDsUnderflow) ? EP_ERR_DS_UNDERFLOW
+                                  : (a == kErrStreamShort) ? EP_ERR_STREAM_SHORT : EP_ERR_UNSUPPORTED;
+    dg->err_node = dg->err_energy = dg->err_mu = -1;
+  } else if (group == 0) {
+    code = inf ? EP_ERR_INF : EP_ERR_NAN;
+    dg->err_energy = a;
+    dg->err_mu = b;
+    dg->err_node = c;
+  } else {
+    code = inf ? EP_ERR_INF : EP_ERR_NAN;
+    dg->err_node = a;
+    dg->err_energy = b;
+    dg->err_mu = c;
+  }
+  dg->err_code = code;
+  return code;
+}
+
+static const char *err_text(int code) {
+  switch (code) {
+    case EP_ERR_NAN: return "NaN in the distribution";
+    case EP_ERR_INF: return "Inf in the distribution";
+    case EP_ERR_DS_UNDERFLOW: return "FindSegmentLengths(): Underflow in ds";
+    case EP_ERR_STREAM_SHORT: return "fewer than 3 kept nodes on a stream";
+    case EP_ERR_UNSUPPORTED: return "subcycle count exceeds the supported bound";
+    default: return "error";
+  }
+}
+
+static int ensure_events(int count) {
+  while ((int)g.events.size() < count) {
+    hipEvent_t ev;
+    HIPCHK(hipEventCreate(&ev), "hipEventCreate");
+    g.events.push_back(ev);
+  }
+  return EP_OK;
+}
+
+}  // namespace ep
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+int ep_init(const ep_dims *dims, const ep_params *params) {
+  if (!dims || !params) return fail(EP_ERR_INVALID_ARG, "null dims/params");
+  if (g.init) free_all();
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(EP_ERR_NO_DEVICE, "no HIP device available");
+  const ep_dims d = *dims;
+  if (d.num_streams < 1 || d.num_streams > 16383 || d.num_nodes < 2 || d.num_nodes > 16383 ||
+      d.num_energies < 2 || d.num_energies > 64 || d.num_mu < 2 || d.num_mu > 32)
+    return fail(EP_ERR_INVALID_ARG,
+                "dims out of range (streams 1..16383, nodes 2..16383, energies 2..64, mu 2..32)");
+  int rc = check_params(*params);
+  if (rc) return rc;
+  g.d = d;
+  g.p = *params;
+  g.Npad = (d.num_nodes + 63) / 64 * 64;
+  HIPCHK(hipStreamCreate(&g.st), "hipStreamCreate");
+  const size_t S = d.num_streams, E = d.num_energies, M = d.num_mu, NP = g.Npad;
+  const size_t SN = S * NP, SE = S * E;
+  g.init = true;  // so that free_all() releases partial allocations
+#define AL(ptr, cnt) \
+  if ((rc = dalloc(&(ptr), (cnt), #ptr))) { free_all(); return rc; }
+  AL(g.egrid_d, E) AL(g.vgrid_d, E) AL(g.vgrid_i_d, E) AL(g.rig_lamo_d, E) AL(g.plt_d, E)
+  AL(g.ext_d, E) AL(g.mugrid_d, M)
+  AL(g.fA, SE * M * NP) AL(g.fB, SE * M * NP) AL(g.loc, SE)
+  AL(g.nx, SN) AL(g.ny, SN) AL(g.nz, SN) AL(g.nrmag, SN) AL(g.nbmag, SN) AL(g.ndlnb, SN)
+  AL(g.ndlnn, SN) AL(g.ndupar, SN) AL(g.n_active, S)
+  AL(g.ds, SN) AL(g.bplus, SN) AL(g.bminus, SN) AL(g.mfp_r, SN) AL(g.radial, SN)
+  AL(g.list, SN) AL(g.ref, SN) AL(g.nkept, S) AL(g.thin, S) AL(g.dsil, SN) AL(g.dsmin, S)
+  AL(g.nsteps, SE) AL(g.halo, SE) AL(g.passes, SE) AL(g.chunks, SE) AL(g.dtprop, SE)
+  AL(g.del_fac, SE * M) AL(g.seed3, SE * 3) AL(g.dsm, SE * NP) AL(g.q, SE * NP)
+  AL(g.nsub_f, SN) AL(g.nsub_c, SN) AL(g.dtsub_f, SN) AL(g.dtsub_c, SN) AL(g.f_dlnbds, SN)
+  AL(g.f_dupar, SN) AL(g.f_b, SN) AL(g.c_dupar, SN) AL(g.c_dlnn, SN) AL(g.c_dlnb, SN)
+  AL(g.c_damp, SN) AL(g.c_bc0, SN) AL(g.c_bc1, SN) AL(g.diag, 1)
+#undef AL
+  build_grids(g.p, (int)E, (int)M);
+  if ((rc = upload_grids())) { free_all(); return rc; }
+  // every f starts at DBL_MIN (initEnergeticParticlesGrids), in buffer A
+  HIPCHK(hipMemset(g.loc, 0, SE), "hipMemset");
+  HIPCHK(launch_fill(g.fA, SE * M * NP, kDblMin, g.st), "fill");
+  HIPCHK(launch_fill(g.fB, SE * M * NP, kDblMin, g.st), "fill");
+  std::vector<int32_t> na(S, d.num_nodes);
+  HIPCHK(hipMemcpy(g.n_active, na.data(), S * 4, hipMemcpyHostToDevice), "n_active");
+  g.h_nactive = na;
+  HIPCHK(hipStreamSynchronize(g.st), "init sync");
+  return EP_OK;
+}
+
+int ep_set_params(const ep_params *params) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!params) return fail(EP_ERR_INVALID_ARG, "null params");
+  int rc = check_params(*params);
+  if (rc) return rc;
+  g.p = *params;
+  build_grids(g.p, g.d.num_energies, g.d.num_mu);
+  return upload_grids();
+}
+
+int ep_set_tuning(int32_t stream_block, int32_t stream_halo) {
+  if (stream_block < 64 || stream_block > 512 || stream_block % 64)
+    return fail(EP_ERR_INVALID_ARG, "stream_block must be a multiple of 64 in 64..512");
+  if (stream_halo < 1 || 2 * stream_halo >= stream_block)
+    return fail(EP_ERR_INVALID_ARG, "stream_halo must be >= 1 and < stream_block/2");
+  g.stream_block = stream_block;
+  g.stream_halo = stream_halo;
+  return EP_OK;
+}
+
+int ep_get_grids(double *egrid, double *vgrid, double *mugrid, double *rigidity, double *dlnp,
+                 double *dmu) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  const int E = g.d.num_energies, M = g.d.num_mu;
+  if (egrid) memcpy(egrid, g.egrid.data(), E * 8);
+  if (vgrid) memcpy(vgrid, g.vgrid.data(), E * 8);
+  if (mugrid) memcpy(mugrid, g.mugrid.data(), M * 8);
+  if (rigidity) memcpy(rigidity, g.rigidity.data(), E * 8);
+  if (dlnp) *dlnp = g.dlnp;
+  if (dmu) *dmu = g.dmu;
+  return EP_OK;
+}
+
+int ep_set_nodes(const ep_node_soa *nodes, const int32_t *n_active) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!nodes) return fail(EP_ERR_INVALID_ARG, "null nodes");
+  const double *src[8] = {nodes->x, nodes->y, nodes->z, nodes->rmag,
+                          nodes->bmag, nodes->dlnb, nodes->dlnn, nodes->dupar};
+  double *dst[8] = {g.nx, g.ny, g.nz, g.nrmag, g.nbmag, g.ndlnb, g.ndlnn, g.ndupar};
+  const size_t S = g.d.num_streams, N = g.d.num_nodes;
+  for (int i = 0; i < 8; i++) {
+    if (!src[i]) return fail(EP_ERR_INVALID_ARG, "node array %d is null", i);
+    HIPCHK(hipMemcpy2D(dst[i], g.Npad * 8, src[i], N * 8, N * 8, S, hipMemcpyHostToDevice),
+           "node upload");
+  }
+  std::vector<int32_t> na(S, (int32_t)N);
+  if (n_active) {
+    for (size_t s = 0; s < S; s++) {
+      if (n_active[s] < 2 || n_active[s] > (int32_t)N)
+        return fail(EP_ERR_INVALID_ARG, "n_active[%zu] = %d outside 2..%zu", s, n_active[s], N);
+      na[s] = n_active[s];
+    }
+  }
+  HIPCHK(hipMemcpy(g.n_active, na.data(), S * 4, hipMemcpyHostToDevice), "n_active");
+  g.h_nactive = na;
+  g.nodes_set = true;
+  return EP_OK;
+}
+
+int ep_upload_f(const double *f, int32_t layout) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!f) return fail(EP_ERR_INVALID_ARG, "null f");
+  if (layout != EP_LAYOUT_REFERENCE) return fail(EP_ERR_INVALID_ARG, "unknown layout %d", layout);
+  const size_t cnt = (size_t)g.d.num_streams * g.d.num_nodes * g.d.num_energies * g.d.num_mu;
+  Dev D = make_dev();
+  HIPCHK(hipMemcpy(g.fB, f, cnt * 8, hipMemcpyHostToDevice), "f upload");
+  HIPCHK(launch_from_reference(D, g.fB, g.fA, g.st), "from_reference");
+  HIPCHK(hipMemsetAsync(g.loc, 0, (size_t)g.d.num_streams * g.d.num_energies, g.st), "loc reset");
+  HIPCHK(hipStreamSynchronize(g.st), "upload sync");
+  return EP_OK;
+}
+
+int ep_download_f(double *f, int32_t layout) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!f) return fail(EP_ERR_INVALID_ARG, "null f");
+  if (layout != EP_LAYOUT_REFERENCE) return fail(EP_ERR_INVALID_ARG, "unknown layout %d", layout);
+  const size_t cnt = (size_t)g.d.num_streams * g.d.num_nodes * g.d.num_energies * g.d.num_mu;
+  Dev D = make_dev();
+  HIPCHK(launch_normalize(D, g.st), "normalize");
+  HIPCHK(launch_to_reference(D, g.fA, g.fB, g.st), "to_reference");
+  HIPCHK(hipStreamSynchronize(g.st), "download sync");
+  HIPCHK(hipMemcpy(f, g.fB, cnt * 8, hipMemcpyDeviceToHost), "f download");
+  return EP_OK;
+}
+
+int ep_init_seed_population(void) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
+  Dev D = make_dev();
+  HIPCHK(launch_init_seed(D, g.st), "init_seed");
+  HIPCHK(hipStreamSynchronize(g.st), "init_seed sync");
+  return EP_OK;
+}
+
+int ep_step(uint32_t op_flags, ep_diag *out) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
+  const ep_params &p = g.p;
+  const int do_stream = (p.useParallelDiffusion > 0) && (op_flags & EP_OP_STREAM);
+  const int do_focus = (p.useAdiabaticFocus > 0) && (op_flags & EP_OP_FOCUS);
+  const int do_change = (p.useAdiabaticChange > 0) && (op_flags & EP_OP_CHANGE);
+  const int do_seed = (p.checkSeedPopulation > 0) && (op_flags & EP_OP_SEEDCHECK);
+  const int nsub = p.numEpSteps;
+  int rc = ensure_events(2 + 4 * nsub + 1);
+  if (rc) return rc;
+  Dev D = make_dev();
+  hipStream_t st = g.st;
+  std::vector<hipEvent_t> &ev = g.events;
+  HIPCHK(hipEventRecord(ev[0], st), "event");
+  HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
+  HIPCHK(launch_stream_prep(D, st), "stream_prep");
+  HIPCHK(launch_node_prep(D, st), "node_prep");
+  HIPCHK(hipEventRecord(ev[1], st), "event");
+  DevDiag hd;
+  HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
+  HIPCHK(hipStreamSynchronize(st), "prep sync");
+  ep_diag dg;
+  memset(&dg, 0, sizeof(dg));
+  dg.err_stream = dg.err_node = dg.err_energy = dg.err_mu = -1;
+  if (hd.err_key != ~0ull) {
+    int code = decode_err(hd.err_key, &dg);
+    if (out) *out = dg;
+    return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
+  }
+  for (int sub = 0; sub < nsub; sub++) {
+    hipEvent_t *e4 = &ev[2 + 4 * sub];
+    if (do_seed && !do_stream) HIPCHK(launch_seed_check(D, st), "seed_check");
+    if (do_stream) {
+      for (int pass = 0; pass < hd.max_passes; pass++)
+        HIPCHK(launch_stream_pass(D, pass, do_seed && pass == 0, sub, hd.max_chunks, st), "stream_pass");
+      HIPCHK(launch_stream_finish(D, sub, st), "stream_finish");
+    }
+    HIPCHK(hipEventRecord(e4[0], st), "event");
+    if (do_focus) HIPCHK(launch_focus(D, sub, st), "focus");
+    HIPCHK(hipEventRecord(e4[1], st), "event");
+    if (do_change) HIPCHK(launch_change(D, sub, st), "change");
+    HIPCHK(hipEventRecord(e4[2], st), "event");
+  }
+  HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
+  HIPCHK(hipStreamSynchronize(st), "step sync");
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[2 + 4 * (nsub - 1) + 2]), "elapsed");
+  dg.device_ms = ms;
+  HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]), "elapsed");
+  dg.prep_ms = ms;
+  for (int sub = 0; sub < nsub; sub++) {
+    hipEvent_t start = (sub == 0) ? ev[1] : ev[2 + 4 * (sub - 1) + 2];
+    hipEvent_t *e4 = &ev[2 + 4 * sub];
+    HIPCHK(hipEventElapsedTime(&ms, start, e4[0]), "elapsed");
+    dg.stream_ms += ms;
+    HIPCHK(hipEventElapsedTime(&ms, e4[0], e4[1]), "elapsed");
+    dg.focus_ms += ms;
+    HIPCHK(hipEventElapsedTime(&ms, e4[1], e4[2]), "elapsed");
+    dg.change_ms += ms;
+  }
+  dg.max_subcycles_change = do_change ? hd.max_sub_change : 0;
+  dg.max_subcycles_focusing = do_focus ? hd.max_sub_focus : 0;
+  dg.max_nsteps_stream = do_stream ? hd.max_nsteps : 0;
+  double mt;
+  memcpy(&mt, &hd.min_tau_bits, 8);
+  dg.min_tau = mt;
+  int64_t cells = 0;
+  for (int32_t n : g.h_nactive) cells += (int64_t)n * g.d.num_energies * g.d.num_mu;
+  dg.cell_updates = cells * nsub;
+  dg.stream_sweeps = do_stream ? (int64_t)hd.stream_sweeps * nsub : 0;
+  dg.focus_sweeps = do_focus ? (int64_t)hd.focus_sweeps * nsub : 0;
+  dg.change_sweeps = do_change ? (int64_t)hd.change_sweeps * nsub : 0;
+  dg.stream_launches = do_stream ? hd.max_passes * nsub : 0;
+  int code = EP_OK;
+  if (hd.err_key != ~0ull) code = decode_err(hd.err_key, &dg);
+  if (out) *out = dg;
+  if (code != EP_OK)
+    return fail(code, "%s (stream %d node %d energy %d mu %d)", err_text(code), dg.err_stream,
+                dg.err_node, dg.err_energy, dg.err_mu);
+  return EP_OK;
+}
+
+int ep_get_node_outputs(double *ds, double *bmag_plus, double *bmag_minus) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  const size_t S = g.d.num_streams, N = g.d.num_nodes;
+  double *dst[3] = {ds, bmag_plus, bmag_minus};
+  const double *src[3] = {g.ds, g.bplus, g.bminus};
+  for (int i = 0; i < 3; i++)
+    if (dst[i])
+      HIPCHK(hipMemcpy2D(dst[i], N * 8, src[i], g.Npad * 8, N * 8, S, hipMemcpyDeviceToHost),
+             "node download");
+  return EP_OK;
+}
+
+int ep_last_error(char *buf, size_t len) {
+  if (buf && len) {
+    strncpy(buf, g_err.c_str(), len - 1);
+    buf[len - 1] = 0;
+  }
+  return EP_OK;
+}
+
+int ep_finalize(void) {
+  if (g.init) free_all();
+  return EP_OK;
+}
+
+// Test hook: checks the DPP lane layout of one wave on the device.
+int ep_selftest_dpp(const double *in64, double *shr64, double *shl64) {
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, 3 * 64 * 8), "hipMalloc");
+  hipError_t e = hipMemcpy(d, in64, 64 * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_dpp_probe(d, d + 64, d + 128, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(shr64, d + 64, 64 * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(shl64, d + 128, 64 * 8, hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "dpp selftest");
+  return EP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// ep_internal.h -- device-side state and helpers of the MI355X EP update.
+//
+// Device layout (HBM):
+//   f            [stream][energy][mu][node_pad]   FP64, node fastest, two
+//                buffers A/B; loc[stream*E+energy] says which one holds the
+//                current distribution of that (stream, energy) slab.
+//   node arrays  [stream][node_pad]               FP64 SoA
+//   kept arrays  [stream][energy][kept_index]     FP64 (streaming coefs)
+//   task arrays  [stream][energy]                 per (stream, energy)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eprem_hip.h"
+
+namespace ep {
+
+constexpr double kDblMin = 2.2250738585072014e-308;  // DBL_MIN
+constexpr double kPI = 3.1415926535897932;
+constexpr double kTwoPI = 6.283185307179586;
+constexpr double kMP = 1.6726e-24;
+constexpr double kMEV = 1.6022e-6;
+constexpr double kGEV = 1.6022e-3;
+constexpr double kC = 2.99792458e10;
+constexpr double kAU = 1.495978707e13;
+constexpr double kRSUN = 6.96e10;
+constexpr double kRSAU = kRSUN / kAU;
+constexpr double kMhdDensityNorm = 1.0;
+
+// Largest subcycle count the library accepts for one operator on one node
+// or one (stream, energy).  The reference casts ceil(dt/dt_stable) to int
+// without a check (src/energeticParticles.c:940, :1665, :2316); counts past
+// this bound only arise from degenerate inputs and are reported as errors.
+constexpr int kMaxSubcycles = 1 << 22;
+
+// Error key.  The reference panics at the first bad cell in its own loop
+// order; every failing cell here computes a key that sorts in that order,
+// and the smallest key wins (one 64-bit atomicMin).  Layout, high to low:
+//   [substep 7][group 4][stream 14][op 2][a 14][b 8][c 14][inf 1]
+// group 0 = stream stage (op 0: prep error, a = kind; op 1: streaming,
+// a = energy, b = mu, c = node), group 1 = node stage (op 0: focusing,
+// op 1: change; a = node, b = energy, c = mu).  inf = 1 for Inf, 0 NaN.
+enum PrepErr { kErrDsUnderflow = 1, kErrStreamShort = 2, kErrSubcycles = 3 };
+
+__host__ __device__ inline unsigned long long err_key(int sub, int group, int s, int op, int a,
+                                                      int b, int c, int inf) {
+  unsigned long long k = 0;
+  k |= (unsigned long long)(sub > 127 ? 127 : sub) << 57;
+  k |= (unsigned long long)(group & 0xF) << 53;
+  k |= (unsigned long long)(s & 0x3FFF) << 39;
+  k |= (unsigned long long)(op & 0x3) << 37;
+  k |= (unsigned long long)(a & 0x3FFF) << 23;
+  k |= (unsigned long long)(b & 0xFF) << 15;
+  k |= (unsigned long long)(c & 0x3FFF) << 1;
+  k |= (unsigned long long)(inf & 1);
+  return k;
+}
+
+// Per-step scalar diagnostics kept on the device.
+struct DevDiag {
+  int max_sub_focus, max_sub_change, max_nsteps, max_passes, max_chunks, any_thin;
+  unsigned long long min_tau_bits;  // positive doubles order like their bits
+  unsigned long long err_key;       // ~0ull = no error
+  unsigned long long stream_sweeps, focus_sweeps, change_sweeps;  // per substep
+};
+
+// Everything the kernels need, passed by value.
+struct Dev {
+  // dims
+  int S, N, Npad, E, M;
+  // params (code units)
+  double rScale, dt, dt_full;
+  int numEpSteps, mfpType, useBC, stream_block, stream_halo;
+  double mfpRadialPower, mfpBPower, mhdBsAu, dsh_min, dampCooling;
+  double normJ0, normRadius, xi, beta, dlnp, dmu;
+  int mpos;  // first mu index with mugrid >= 0
+  // grids [E] / [M]
+  const double *egrid, *vgrid, *vgrid_i, *rig_lamo, *plt, *ext, *mugrid;
+  // f
+  double *fA, *fB;
+  uint8_t *loc;  // [S*E]
+  // nodes [S*Npad]
+  const double *x, *y, *z, *rmag, *bmag, *dlnb, *dlnn, *dupar;
+  const int32_t *n_active;  // [S]
+  double *ds, *bplus, *bminus, *mfp_r, *radial;
+  // thinning [S*Npad] / [S]
+  int32_t *list, *ref, *nkept, *thin;
+  double *dsil, *dsmin;
+  // streaming task tables [S*E]
+  int32_t *nsteps, *halo, *passes, *chunks;
+  double *dtprop;
+  double *del_fac;  // [S*E*M]
+  double *seed3;    // [S*E*3]
+  double *dsm, *q;  // [S*E*Npad] by kept index
+  // node-stage coefficients [S*Npad]
+  int32_t *nsub_f, *nsub_c;
+  double *dtsub_f, *dtsub_c, *f_dlnbds, *f_dupar, *f_b;
+  double *c_dupar, *c_dlnn, *c_dlnb, *c_damp, *c_bc0, *c_bc1;
+  DevDiag *diag;
+};
+
+__device__ inline void note_error(const Dev &D, unsigned long long key) {
+  atomicMin(&D.diag->err_key, key);
+}
+
+// sepSeedFunction (src/energeticParticlesBoundary.c:67-116) with the
+// node-only factor radial = pow(r/normRadius, -beta) and the energy-only
+// factors plt[e] = pow(E/E1, -gamma), ext[e] = exp(-E/E0) precomputed; the
+// product keeps the reference's association order.
+__device__ inline double seed_value(const Dev &D, double radial, int e) {
+  double normJHe = D.normJ0 * radial * D.plt[e] * D.ext[e];
+  double normJ = normJHe / D.xi;
+  double normf = normJ / (2.0 * D.egrid[e]);
+  if (normf < kDblMin) normf = kDblMin;
+  return normf;
+}
+
+__device__ inline size_t fidx(const Dev &D, int s, int e, int m, int node) {
+  return (((size_t)s * D.E + e) * D.M + m) * (size_t)D.Npad + node;
+}
+
+__device__ inline double *fbuf(const Dev &D, int which) { return which ? D.fB : D.fA; }
+
+}  // namespace ep

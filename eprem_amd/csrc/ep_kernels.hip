@@ -1,0 +1,918 @@
+// ep_kernels.hip -- CDNA4 kernels of the EP update (FP64 parity path).
+//
+// Arithmetic follows the reference's operation order and is compiled with
+// -ffp-contract=off, so results agree with the FP64 oracle to rounding of
+// the device libm (exp/log/pow); division and sqrt are IEEE-correct.
+//
+// Per global step (node positions are fixed inside it, so everything that
+// depends only on nodes is computed once):
+//   k_stream_prep   updateStreamValues, GetStreamList, FindSegmentLengths,
+//                   mean-free-path and seed radial factors, min tau
+//   k_task_prep     per (stream, energy): nsteps, dtProp, tiling, del_fac
+//   k_stream_coef   per kept node and energy: ds_i multiplier, exp(-dt/tau)
+//   k_node_prep     per node: focusing/change coefficients and subcycles
+// Per EP substep:
+//   k_stream_pass   DiffuseStreamData on register-resident node chunks,
+//                   temporal blocking with halos (one launch per pass)
+//   k_stream_finish alias copies of thinned nodes, buffer bookkeeping
+//   k_focus         AdiabaticFocusing, a lane per node, mu in registers
+//   k_change        AdiabaticChange, a lane per node, energy in registers
+#include <float.h>
+#include <math.h>
+
+#include "ep_internal.h"
+#include "ep_launch.h"
+
+namespace ep {
+
+// ------------------------------------------------------------------------
+// Cross-lane shifts along the stream (DPP wave_shr:1 / wave_shl:1 move a
+// 32-bit value one lane up/down the whole 64-lane wave; lanes without a
+// source keep `old`).
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double dpp_shr1(double v, double old) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32), 0x138,
+                                             0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ double dpp_shl1(double v, double old) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32), 0x130,
+                                             0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Lane-layout probe used by the tests: out[i] = in[i-1] (lane 0: -1),
+// out2[i] = in[i+1] (lane 63: -2) for one wave.
+__global__ void k_dpp_probe(const double *in, double *out, double *out2) {
+  const int i = threadIdx.x;
+  const double v = in[i];
+  out[i] = dpp_shr1(v, -1.0);
+  out2[i] = dpp_shl1(v, -2.0);
+}
+
+__device__ __forceinline__ double dist_scaled(const Dev &D, size_t a, size_t b) {
+  const double rs = D.rScale;
+  const double ax = D.x[a] * rs, ay = D.y[a] * rs, az = D.z[a] * rs;
+  const double bx = D.x[b] * rs, by = D.y[b] * rs, bz = D.z[b] * rs;
+  return sqrt((ax - bx) * (ax - bx) + (ay - by) * (ay - by) + (az - bz) * (az - bz));
+}
+
+__device__ __forceinline__ void atomic_min_pos_double(unsigned long long *addr, double v) {
+  atomicMin(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// Block-wide minimum (256 threads) of non-NaN values.
+__device__ double block_min_256(double v, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) {
+      const double o = sh[t + st];
+      if (o < sh[t]) sh[t] = o;
+    }
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------------
+// k_stream_prep: one 256-thread block per stream.
+//   updateStreamValues      src/simCore.c:325-435
+//   GetStreamList           src/energeticParticles.c:2147-2234
+//   FindSegmentLengths      src/energeticParticles.c:2493-2601
+//   meanFreePath (node part) src/meanFreePath.c:13-50
+//   min_tau diagnostic      src/energeticParticles.c:149-156
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stream_prep(Dev D) {
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int n = D.n_active[s];
+  const size_t o = (size_t)s * D.Npad;
+  __shared__ int s_thin, s_size;
+  __shared__ double sh[256];
+  if (tid == 0) s_thin = 0;
+  __syncthreads();
+  for (int k = tid; k < n; k += blockDim.x) {
+    const size_t i = o + k;
+    if (k == 0) {
+      D.ds[i] = dist_scaled(D, i + 1, i);
+      D.bminus[i] = D.bmag[i];
+      D.bplus[i] = D.bmag[i + 1];
+    } else if (k == n - 1) {
+      D.ds[i] = dist_scaled(D, i, i - 1);
+      D.bminus[i] = D.bmag[i - 1];
+      D.bplus[i] = D.bmag[i];
+    } else {
+      const double dsp = dist_scaled(D, i + 1, i);
+      const double dsm = dist_scaled(D, i, i - 1);
+      D.ds[i] = 0.5 * (dsp + dsm);
+      D.bminus[i] = D.bmag[i - 1];
+      D.bplus[i] = D.bmag[i + 1];
+    }
+    D.mfp_r[i] = (D.mfpType == 2) ? pow(D.mhdBsAu / D.bmag[i], D.mfpBPower)
+                                  : pow(D.rmag[i] * D.rScale, D.mfpRadialPower);
+    D.radial[i] = pow(D.rmag[i] / D.normRadius, -D.beta);
+    // Thinning only happens if some neighbour spacing is <= dsh_min: with
+    // every node kept, GetStreamList's test for shell k measures exactly
+    // |r[k+1] - r[k]|.
+    if (k < n - 1) {
+      const double d = dist_scaled(D, i + 1, i);
+      if (!(d > D.dsh_min)) s_thin = 1;
+    }
+  }
+  __syncthreads();
+  const int thin = s_thin;
+  if (!thin) {
+    for (int k = tid; k < n; k += blockDim.x) {
+      D.list[o + k] = k;
+      D.ref[o + k] = k;
+    }
+    if (tid == 0) s_size = n;
+  } else if (tid == 0) {
+    // Sequential GetStreamList (src/energeticParticles.c:2180-2228).
+    int size = 0, down = 0;
+    for (int shell = 0; shell < n; shell++) {
+      int up;
+      if (shell < n - 1) {
+        up = shell + 1;
+      } else {
+        up = shell;
+        down = shell - 1;
+      }
+      const double dsh = dist_scaled(D, o + up, o + down);
+      if ((dsh > D.dsh_min) || (shell == n - 1)) {
+        D.list[o + size] = shell;
+        size++;
+        D.ref[o + shell] = shell;
+        down = shell + 1;
+      } else {
+        D.ref[o + shell] = shell + 1;
+        int sp = shell - 1;
+        while ((sp >= 0) && (D.ref[o + sp] != sp)) {
+          D.ref[o + sp] = D.ref[o + sp + 1];
+          sp--;
+        }
+      }
+    }
+    s_size = size;
+    atomicMax(&D.diag->any_thin, 1);
+  }
+  __syncthreads();
+  const int size = s_size;
+  if (tid == 0) {
+    D.thin[s] = thin;
+    D.nkept[s] = size;
+  }
+  // FindSegmentLengths over the kept list.
+  double lmin = 10000.0 * D.rScale;
+  if (size >= 2) {
+    for (int sl = tid; sl < size; sl += blockDim.x) {
+      const size_t ts = o + D.list[o + sl];
+      double dsl;
+      if (sl == 0) {
+        const double d = dist_scaled(D, o + D.list[o + sl + 1], ts);
+        if (d < lmin) lmin = d;
+        dsl = d;
+      } else if (sl == size - 1) {
+        const double d = dist_scaled(D, ts, o + D.list[o + sl - 1]);
+        if (d < lmin) lmin = d;
+        dsl = d;
+      } else {
+        const double d1 = dist_scaled(D, o + D.list[o + sl + 1], ts);
+        const double d9 = dist_scaled(D, ts, o + D.list[o + sl - 1]);
+        if (d1 < lmin) lmin = d1;
+        if (d9 < lmin) lmin = d9;
+        dsl = 0.5 * (d1 + d9);
+      }
+      D.dsil[o + sl] = 1.0 / dsl;
+    }
+  }
+  const double dsMin = block_min_256(lmin, sh);
+  if (tid == 0) {
+    D.dsmin[s] = dsMin;
+    if (size < 2 || (size < 3 && D.useBC))
+      note_error(D, err_key(0, 0, s, 0, kErrStreamShort, 0, 0, 0));
+    else if (dsMin <= 0.0)
+      note_error(D, err_key(0, 0, s, 0, kErrDsUnderflow, 0, 0, 0));
+  }
+  // min_tau over the nodes the shell stage processes (1..n-1).
+  double tmin = DBL_MAX;
+  for (int k = 1 + tid; k < n; k += blockDim.x) {
+    const double mr = D.mfp_r[o + k];
+    for (int e = 0; e < D.E; e++) {
+      const double tau = (D.rig_lamo[e] * mr) * D.vgrid_i[e];
+      if (tau < tmin) tmin = tau;
+    }
+  }
+  const double tm = block_min_256(tmin, sh);
+  if (tid == 0 && tm < DBL_MAX) atomic_min_pos_double(&D.diag->min_tau_bits, tm);
+}
+
+// ------------------------------------------------------------------------
+// k_task_prep: one thread per (stream, energy).  DiffuseStreamData's
+// per-energy constants (src/energeticParticles.c:2301-2350) and the
+// temporal-blocking tiling of the streaming passes.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_task_prep(Dev D) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= D.S * D.E) return;
+  const int s = t / D.E, e = t % D.E;
+  const int size = D.nkept[s];
+  const double dsMin = D.dsmin[s];
+  int nsteps = 1;
+  double dtProp = D.dt;
+  if (dsMin > 0.0 && size >= 2) {
+    const double dtMin = 0.4 * dsMin * D.vgrid_i[e];
+    const double xs = D.dt / dtMin + 1.0;
+    if (!(xs < (double)kMaxSubcycles)) {
+      note_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
+    } else {
+      nsteps = (int)floor(xs);
+      dtProp = D.dt / (1.0 * nsteps);
+    }
+  }
+  const int H = D.stream_halo, W = D.stream_block;
+  const int h = nsteps < H ? nsteps : H;
+  const int passes = (nsteps + h - 1) / h;
+  const int interior = W - 2 * h;
+  const int chunks = (size + interior - 1) / interior;
+  D.nsteps[t] = nsteps;
+  D.halo[t] = h;
+  D.passes[t] = passes;
+  D.chunks[t] = chunks;
+  D.dtprop[t] = dtProp;
+  atomicMax(&D.diag->max_nsteps, nsteps);
+  atomicMax(&D.diag->max_passes, passes);
+  atomicMax(&D.diag->max_chunks, chunks);
+  atomicAdd(&D.diag->stream_sweeps, (unsigned long long)nsteps * size * D.M);
+  const double v = D.vgrid[e];
+  for (int m = 0; m < D.M; m++) D.del_fac[(size_t)t * D.M + m] = v * dtProp * D.mugrid[m];
+  const size_t o = (size_t)s * D.Npad;
+  for (int j = 0; j < 3; j++)
+    D.seed3[(size_t)t * 3 + j] = (j < size) ? seed_value(D, D.radial[o + D.list[o + j]], e) : 0.0;
+}
+
+// ------------------------------------------------------------------------
+// k_stream_coef: per (stream, energy, kept index).  ds_i multiplier and
+// the relaxation factor exp(-dtProp/tau) (src/energeticParticles.c:2320-2337).
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stream_coef(Dev D) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y, s = blockIdx.z;
+  if (k >= D.nkept[s]) return;
+  const int t = s * D.E + e;
+  const size_t o = (size_t)s * D.Npad;
+  const int node = D.list[o + k];
+  const double tau = D.vgrid_i[e] * (D.rig_lamo[e] * D.mfp_r[o + node]);
+  const double dtProp = D.dtprop[t];
+  const double dtProp_i = 1.0 / dtProp;
+  const double dsi = D.dsil[o + k];
+  const double dsm = (dtProp > tau) ? dsi * tau * dtProp_i : dsi * 1.0;
+  D.dsm[(size_t)t * D.Npad + k] = dsm;
+  D.q[(size_t)t * D.Npad + k] = exp(-dtProp / tau);
+}
+
+// ------------------------------------------------------------------------
+// k_node_prep: one thread per node 1..n-1 (innerComputeShell skips node 0,
+// src/mpiInit.c:105-109).  Focusing coefficients and subcycle count
+// (src/energeticParticles.c:1581-1674) and adiabatic-change coefficients,
+// subcycle count and inflow boundary values (src/energeticParticles.c:
+// 859-949, :1137-1138).
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_node_prep(Dev D) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  const int n = D.n_active[s];
+  if (k < 1 || k >= n) return;
+  const size_t i = (size_t)s * D.Npad + k;
+  const double dt = D.dt, dt_full = D.dt_full;
+  const double half = 0.5, one = 1.0, two = 2.0, three = 3.0;
+  // ---- focusing
+  {
+    const double DuParDt = D.dupar[i] / dt_full;
+    const double b = two * (D.dlnn[i] / dt_full) - three * (D.dlnb[i] / dt_full);
+    const double bp = D.bplus[i], bm = D.bminus[i];
+    double dlnBds;
+    if ((bp == 0.0) || (bm == 0.0))
+      dlnBds = 0.0;
+    else
+      dlnBds = (log(bp) - log(bm)) / (two * D.ds[i] + kDblMin);
+    double vmax = 0.0;
+    for (int e = 0; e < D.E; e++) {
+      const double v = D.vgrid[e];
+      const double a = -v * dlnBds - (two / v) * DuParDt;
+      double vprev = 0.0;
+      for (int m = 0; m < D.M; m++) {
+        const double mu = D.mugrid[m];
+        const double vel = half * (one - mu * mu) * (a + mu * b);
+        if (m > 0) {
+          const double va = half * (vel + vprev);
+          if (fabs(va) > vmax) vmax = fabs(va);
+        }
+        vprev = vel;
+      }
+    }
+    const double dt_stable = 0.9 * D.dmu / vmax;
+    const double xs = ceil(dt / dt_stable);
+    int nsub = 0;
+    if (!(xs < (double)kMaxSubcycles)) {
+      note_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
+    } else {
+      nsub = (int)xs;
+    }
+    D.nsub_f[i] = nsub;
+    D.dtsub_f[i] = nsub > 0 ? dt / nsub : 0.0;
+    D.f_dlnbds[i] = dlnBds;
+    D.f_dupar[i] = DuParDt;
+    D.f_b[i] = b;
+    atomicMax(&D.diag->max_sub_focus, nsub);
+    atomicAdd(&D.diag->focus_sweeps, (unsigned long long)nsub * D.E * D.M);
+  }
+  // ---- adiabatic change
+  {
+    const double DlnBDt = D.dlnb[i] / dt_full;
+    const double DlnNDt = D.dlnn[i] / dt_full;
+    const double damp = (DlnNDt < 0) ? 1.0 - D.dampCooling : 1.0;
+    const double DuParDt = D.dupar[i] / dt_full;
+    double vmax = 0.0;
+    for (int m = 0; m < D.M; m++) {
+      const double mu = D.mugrid[m];
+      const double a = -mu * DuParDt;
+      const double b = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
+      for (int e = 0; e < D.E; e++) {
+        const double vel = damp * (a * D.vgrid_i[e] + b);
+        if (fabs(vel) > vmax) vmax = fabs(vel);
+      }
+    }
+    const double dt_stable = 0.9 * D.dlnp / vmax;
+    const double xs = ceil(dt / dt_stable);
+    int nsub = 0;
+    if (!(xs < (double)kMaxSubcycles)) {
+      note_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
+    } else {
+      nsub = (int)xs;
+    }
+    D.nsub_c[i] = nsub;
+    D.dtsub_c[i] = nsub > 0 ? dt / nsub : 0.0;
+    D.c_dupar[i] = DuParDt;
+    D.c_dlnn[i] = DlnNDt;
+    D.c_dlnb[i] = DlnBDt;
+    D.c_damp[i] = damp;
+    const double rad = D.radial[i];
+    D.c_bc0[i] = log(seed_value(D, rad, 0));
+    D.c_bc1[i] = log(seed_value(D, rad, D.E - 1));
+    atomicMax(&D.diag->max_sub_change, nsub);
+    atomicAdd(&D.diag->change_sweeps, (unsigned long long)nsub * D.E * D.M);
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_seed_check: seedPopulationCheck (src/energeticParticlesBoundary.c:
+// 28-60) when streaming is off (otherwise it is fused into pass 0).
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_seed_check(Dev D) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y, s = blockIdx.z;
+  if (k >= D.n_active[s]) return;
+  const int t = s * D.E + e;
+  const double sd = seed_value(D, D.radial[(size_t)s * D.Npad + k], e);
+  double *f = fbuf(D, D.loc[t]) + (size_t)t * D.M * D.Npad + k;
+  for (int m = 0; m < D.M; m++) {
+    const double v = f[(size_t)m * D.Npad];
+    if (v < sd) f[(size_t)m * D.Npad] = sd;
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_stream_pass: DiffuseStreamData (src/energeticParticles.c:2352-2430).
+//
+// Block = one chunk of one (stream, energy) slab; a lane owns one kept
+// node and keeps its M mu values in registers.  Each launch advances the
+// slab by up to `halo` Euler subcycles: the chunk loads `halo` extra nodes
+// on each side, the upwind neighbour comes from the adjacent lane (DPP) or,
+// across waves, through a small double-buffered LDS edge array, and after
+// the subcycles only the interior lanes (whose dependency cone stayed
+// inside the chunk) are written, to the other f buffer.
+// MPOS = first mu index with mugrid >= 0 (upwind direction).
+// ------------------------------------------------------------------------
+template <int M, int MPOS>
+__global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_seed, int substep) {
+  const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
+  const int task = s * D.E + e;
+  const int nsteps = D.nsteps[task];
+  const int h = D.halo[task];
+  const int done = pass * h;
+  if (done >= nsteps) return;
+  const int nsub = min(h, nsteps - done);
+  const bool last = (done + nsub == nsteps);
+  const int nk = D.nkept[s];
+  const int W = blockDim.x;
+  const int I = W - 2 * h;
+  if (c * I >= nk) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = W >> 6;
+  const int k = c * I - h + tid;
+  const bool valid = (k >= 0) && (k < nk);
+  const size_t on = (size_t)s * D.Npad;
+  const int node = valid ? (D.thin[s] ? D.list[on + k] : k) : 0;
+  const int cur = D.loc[task] ^ (pass & 1);
+  const double *__restrict__ src = fbuf(D, cur) + (size_t)task * M * D.Npad;
+  double *__restrict__ dst = fbuf(D, cur ^ 1) + (size_t)task * M * D.Npad;
+
+  double f[M];
+  double sd = 0.0;
+  if (valid && do_seed) sd = seed_value(D, D.radial[on + node], e);
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    double v = 0.0;
+    if (valid) v = src[(size_t)m * D.Npad + node];
+    if (do_seed && v < sd) v = sd;
+    f[m] = v;
+  }
+  const size_t ok = (size_t)task * D.Npad;
+  const double dsm = valid ? D.dsm[ok + k] : 0.0;
+  const double q = valid ? D.q[ok + k] : 1.0;
+  // Stream ends are held: kept node 0 for mu >= 0, the last one for mu < 0.
+  const double dsm_pos = (k == 0) ? 0.0 : dsm;
+  const double dsm_neg = (k == nk - 1) ? 0.0 : dsm;
+  const bool bc = D.useBC && valid && (k < 3);
+  const double bcv = bc ? D.seed3[(size_t)task * 3 + k] : 0.0;
+  const double invM = 1.0 / M;
+  __shared__ double edge[2][16][M];
+  __shared__ double del[M];
+  if (tid < M) del[tid] = D.del_fac[(size_t)task * M + tid];
+
+  for (int j = 0; j < nsub; ++j) {
+    if (bc) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) f[m] = bcv;
+    }
+    const int b = j & 1;
+    if (lane == 63) {
+#pragma unroll
+      for (int m = MPOS; m < M; ++m) edge[b][w][m] = f[m];
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int m = 0; m < MPOS; ++m) edge[b][w][m] = f[m];
+    }
+    __syncthreads();
+    // Re-read the uniform del_fac row from LDS every subcycle (an opaque
+    // zero offset stops the compiler from hoisting it into M*2 VGPRs).
+    int z = 0;
+    asm volatile("" : "+v"(z));
+    const double *delz = del + z;
+    double sum = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const double fm = f[m];
+      double fn;
+      if (m >= MPOS) {
+        const double old = (w > 0) ? edge[b][w - 1][m] : fm;
+        const double nb = dpp_shr1(fm, old);
+        fn = fm + (dsm_pos * delz[m]) * (nb - fm);
+      } else {
+        const double old = (w < nw - 1) ? edge[b][w + 1][m] : fm;
+        const double nb = dpp_shl1(fm, old);
+        fn = fm + (dsm_neg * delz[m]) * (fm - nb);
+      }
+      f[m] = fn;
+      sum = sum + fn;
+    }
+    const double iso = invM * sum;
+#pragma unroll
+    for (int m = 0; m < M; ++m) f[m] = iso + (f[m] - iso) * q;
+  }
+
+  if (valid && tid >= h && tid < h + I) {
+    double *__restrict__ out = dst + node;
+    int bad = -1, bad_inf = 0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      double v = f[m];
+      if (last) {
+        if (bad < 0 && !isfinite(v)) {
+          bad = m;
+          bad_inf = isinf(v) ? 1 : 0;
+        }
+        if (v < kDblMin) v = kDblMin;
+      }
+      out[(size_t)m * D.Npad] = v;
+    }
+    if (bad >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad, node, bad_inf));
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_stream_finish: one block per (stream, energy).  Records which buffer
+// now holds the slab and copies thinned nodes from their reference node,
+// with the NaN/Inf check and DBL_MIN floor (src/energeticParticles.c:
+// 2437-2471).
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
+  const int task = blockIdx.x;
+  const int s = task / D.E, e = task % D.E;
+  const int loc_new = D.loc[task] ^ (D.passes[task] & 1);
+  __syncthreads();
+  if (threadIdx.x == 0) D.loc[task] = (uint8_t)loc_new;
+  if (!D.thin[s]) return;
+  const int n = D.n_active[s];
+  const size_t on = (size_t)s * D.Npad;
+  double *f = fbuf(D, loc_new) + (size_t)task * D.M * D.Npad;
+  for (int idx = threadIdx.x; idx < n * D.M; idx += blockDim.x) {
+    const int m = idx / n, shell = idx % n;
+    const int r = D.ref[on + shell];
+    if (r != shell) {
+      double v = f[(size_t)m * D.Npad + r];
+      if (isnan(v))
+        note_error(D, err_key(substep, 0, s, 1, e, m, shell, 0));
+      else if (isinf(v))
+        note_error(D, err_key(substep, 0, s, 1, e, m, shell, 1));
+      if (v < kDblMin) v = kDblMin;
+      f[(size_t)m * D.Npad + shell] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_focus: AdiabaticFocusing with the upwind operator
+// (src/energeticParticles.c:1676-1803, :1806-1896).  A lane per node, the
+// four waves of a block split the energies; the mu row of one (node,
+// energy) and its half-mesh velocities stay in registers for all
+// subcycles.  The upwind flux v*0.5*((1-cce)f[m]+(1+cce)f[m-1]) equals
+// v*f_upwind exactly for finite f, and the zero-flux walls reproduce the
+// operator's boundary rows.
+// ------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
+  const int s = blockIdx.y;
+  const int node = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int w = threadIdx.x >> 6;
+  const int n = D.n_active[s];
+  if (node < 1 || node >= n) return;
+  const size_t i = (size_t)s * D.Npad + node;
+  const int nsub = D.nsub_f[i];
+  const double dtsub = D.dtsub_f[i];
+  const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], b = D.f_b[i];
+  const double dmu_i = 1.0 / D.dmu;
+  const double half = 0.5, one = 1.0, two = 2.0;
+  for (int e = w; e < D.E; e += 4) {
+    const int task = s * D.E + e;
+    double *fp = fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad + node;
+    const double v = D.vgrid[e];
+    const double a = -v * dlnBds - (two / v) * DuParDt;
+    double va[M + 1];
+    double vprev = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const double mu = D.mugrid[m];
+      const double vel = half * (one - mu * mu) * (a + mu * b);
+      if (m > 0) va[m] = half * (vel + vprev);
+      vprev = vel;
+    }
+    double f[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
+    for (int it = 0; it < nsub; ++it) {
+      double flux_lo = 0.0;  // zero-flux wall at mu = -1
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        double flux_hi = 0.0;  // zero-flux wall at mu = +1
+        if (m + 1 < M) flux_hi = va[m + 1] * (signbit(va[m + 1]) ? f[m + 1] : f[m]);
+        const double f1 = (flux_hi - flux_lo) * dmu_i;
+        f[m] = f[m] - dtsub * f1;
+        flux_lo = flux_hi;
+      }
+    }
+    int bad = -1, bad_inf = 0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      double val = f[m];
+      if (bad < 0 && !isfinite(val)) {
+        bad = m;
+        bad_inf = isinf(val) ? 1 : 0;
+      }
+      if (val < kDblMin) val = kDblMin;
+      if (nsub > 0 || val != f[m]) fp[(size_t)m * D.Npad] = val;
+    }
+    if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_change: AdiabaticChange with the upwind operator, advecting ln f in
+// ln p (src/energeticParticles.c:859-1067, :1081-1176).  A lane per node,
+// the four waves of a block split the mu cells; the energy column of one
+// (node, mu) stays in registers.  EMAX >= E is the register capacity.
+// ------------------------------------------------------------------------
+template <int EMAX>
+__global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
+  const int s = blockIdx.y;
+  const int node = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int w = threadIdx.x >> 6;
+  const int n = D.n_active[s];
+  if (node < 1 || node >= n) return;
+  const int E = D.E;
+  const size_t i = (size_t)s * D.Npad + node;
+  const int nsub = D.nsub_c[i];
+  const double dtsub = D.dtsub_c[i];
+  const double DuParDt = D.c_dupar[i], DlnNDt = D.c_dlnn[i], DlnBDt = D.c_dlnb[i];
+  const double damp = D.c_damp[i], bc0 = D.c_bc0[i], bc1 = D.c_bc1[i];
+  const double dlnp_i = 1.0 / D.dlnp;
+  const double half = 0.5, one = 1.0;
+  const size_t slab = (size_t)D.M * D.Npad;
+  for (int m = w; m < D.M; m += 4) {
+    const double mu = D.mugrid[m];
+    const double a = -mu * DuParDt;
+    const double bb = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
+    double lf[EMAX];
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+      if (e < E) {
+        const int task = s * E + e;
+        const double *fp = fbuf(D, D.loc[task]) + (size_t)task * slab + (size_t)m * D.Npad + node;
+        lf[e] = log(*fp);
+      }
+    }
+    for (int it = 0; it < nsub; ++it) {
+      // keep the per-energy velocities recomputed (not hoisted into VGPRs)
+      double ah = a, bh = bb;
+      asm volatile("" : "+v"(ah), "+v"(bh));
+      double old_prev = 0.0;  // ln f[e-1] before this subcycle
+      double vel_prev = 0.0;
+      double flux_e = 0.0;    // flux at interface e (between e-1 and e)
+#pragma unroll
+      for (int e = 0; e < EMAX; ++e) {
+        if (e < E) {
+          const double cur = lf[e];
+          const double nxt = (e + 1 < E) ? lf[e + 1] : 0.0;
+          const double vel = damp * (ah * D.vgrid_i[e] + bh);
+          double f1;
+          if (e == 0) {
+            const double vel1 = damp * (ah * D.vgrid_i[1] + bh);
+            const double va1 = half * (vel1 + vel);
+            if (va1 <= 0)
+              f1 = va1 * (nxt - cur) * dlnp_i;
+            else
+              f1 = va1 * (cur - bc0) * dlnp_i;
+            flux_e = va1 * (signbit(va1) ? nxt : cur);  // interface 1
+          } else if (e == E - 1) {
+            const double vr = half * (vel + vel_prev);
+            if (vr >= 0)
+              f1 = vr * (cur - old_prev) * dlnp_i;
+            else
+              f1 = vr * (bc1 - cur) * dlnp_i;
+          } else {
+            const double vel1 = damp * (ah * D.vgrid_i[e + 1] + bh);
+            const double va1 = half * (vel1 + vel);
+            const double flux_hi = va1 * (signbit(va1) ? nxt : cur);
+            f1 = (flux_hi - flux_e) * dlnp_i;
+            flux_e = flux_hi;
+          }
+          lf[e] = cur - dtsub * f1;
+          old_prev = cur;
+          vel_prev = vel;
+        }
+      }
+    }
+    int bad = -1, bad_inf = 0;
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+      if (e < E) {
+        double val = exp(lf[e]);
+        if (bad < 0 && !isfinite(val)) {
+          bad = e;
+          bad_inf = isinf(val) ? 1 : 0;
+        }
+        if (val < kDblMin) val = kDblMin;
+        const int task = s * E + e;
+        fbuf(D, D.loc[task])[(size_t)task * slab + (size_t)m * D.Npad + node] = val;
+      }
+    }
+    if (bad >= 0) note_error(D, err_key(substep, 1, s, 1, node, bad, m, bad_inf));
+  }
+}
+
+// ------------------------------------------------------------------------
+// Layout conversion and initialisation.
+// ------------------------------------------------------------------------
+// host order [s][node][e][m] (staged in `stage`) -> device [s][e][m][node]
+__global__ __launch_bounds__(256) void k_from_reference(Dev D, const double *stage, double *dstbuf) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y, s = blockIdx.z;
+  if (node >= D.N) return;
+  const size_t src = (((size_t)s * D.N + node) * D.E + e) * D.M;
+  for (int m = 0; m < D.M; m++) dstbuf[fidx(D, s, e, m, node)] = stage[src + m];
+}
+
+__global__ __launch_bounds__(256) void k_to_reference(Dev D, const double *srcbuf, double *stage) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y, s = blockIdx.z;
+  if (node >= D.N) return;
+  const size_t dst = (((size_t)s * D.N + node) * D.E + e) * D.M;
+  for (int m = 0; m < D.M; m++) stage[dst + m] = srcbuf[fidx(D, s, e, m, node)];
+}
+
+// Move every slab whose current copy sits in buffer B back to A.
+__global__ __launch_bounds__(256) void k_normalize(Dev D) {
+  const int task = blockIdx.x;
+  if (D.loc[task] == 0) return;
+  const size_t off = (size_t)task * D.M * D.Npad;
+  for (size_t i = threadIdx.x; i < (size_t)D.M * D.Npad; i += blockDim.x) D.fA[off + i] = D.fB[off + i];
+  __syncthreads();
+  if (threadIdx.x == 0) D.loc[task] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_fill(double *p, size_t n, double v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// initEnergeticParticles (src/energeticParticlesInit.c:106-147).
+__global__ __launch_bounds__(256) void k_init_seed(Dev D) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y, s = blockIdx.z;
+  if (k >= D.n_active[s]) return;
+  const size_t i = (size_t)s * D.Npad + k;
+  const double rad = pow(D.rmag[i] / D.normRadius, -D.beta);
+  const double sd = seed_value(D, rad, e);
+  const int t = s * D.E + e;
+  double *f = fbuf(D, D.loc[t]) + (size_t)t * D.M * D.Npad + k;
+  for (int m = 0; m < D.M; m++) f[(size_t)m * D.Npad] = sd;
+}
+
+// ------------------------------------------------------------------------
+// Host-side launchers.
+// ------------------------------------------------------------------------
+#define EP_CHECK_LAUNCH() \
+  do {                     \
+    hipError_t e_ = hipGetLastError(); \
+    if (e_ != hipSuccess) return e_;   \
+  } while (0)
+
+hipError_t launch_stream_prep(const Dev &D, hipStream_t st) {
+  hipLaunchKernelGGL(k_stream_prep, dim3(D.S), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_task_prep, dim3((D.S * D.E + 255) / 256), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_stream_coef, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_node_prep(const Dev &D, hipStream_t st) {
+  hipLaunchKernelGGL(k_node_prep, dim3((D.N + 255) / 256, D.S), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_seed_check(const Dev &D, hipStream_t st) {
+  hipLaunchKernelGGL(k_seed_check, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+template <int M, int MPOS>
+static hipError_t stream_pass_t(const Dev &D, int pass, int do_seed, int substep, int chunks,
+                                hipStream_t st) {
+  hipLaunchKernelGGL((k_stream_pass<M, MPOS>), dim3(chunks, D.E, D.S), dim3(D.stream_block), 0, st, D,
+                     pass, do_seed, substep);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+#define EP_PASS_CASE(MM)                                                                 \
+  case MM:                                                                               \
+    if (D.mpos == (MM) / 2) return stream_pass_t<MM, (MM) / 2>(D, pass, do_seed, sub, chunks, st); \
+    if (D.mpos == ((MM) + 1) / 2)                                                        \
+      return stream_pass_t<MM, ((MM) + 1) / 2>(D, pass, do_seed, sub, chunks, st);      \
+    return hipErrorInvalidValue;
+
+hipError_t launch_stream_pass(const Dev &D, int pass, int do_seed, int sub, int chunks, hipStream_t st) {
+  switch (D.M) {
+    EP_PASS_CASE(2) EP_PASS_CASE(3) EP_PASS_CASE(4) EP_PASS_CASE(5) EP_PASS_CASE(6)
+    EP_PASS_CASE(7) EP_PASS_CASE(8) EP_PASS_CASE(9) EP_PASS_CASE(10) EP_PASS_CASE(11)
+    EP_PASS_CASE(12) EP_PASS_CASE(13) EP_PASS_CASE(14) EP_PASS_CASE(15) EP_PASS_CASE(16)
+    EP_PASS_CASE(17) EP_PASS_CASE(18) EP_PASS_CASE(19) EP_PASS_CASE(20) EP_PASS_CASE(21)
+    EP_PASS_CASE(22) EP_PASS_CASE(23) EP_PASS_CASE(24) EP_PASS_CASE(25) EP_PASS_CASE(26)
+    EP_PASS_CASE(27) EP_PASS_CASE(28) EP_PASS_CASE(29) EP_PASS_CASE(30) EP_PASS_CASE(31)
+    EP_PASS_CASE(32)
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
+  hipLaunchKernelGGL(k_stream_finish, dim3(D.S * D.E), dim3(256), 0, st, D, sub);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+#define EP_FOCUS_CASE(MM)                                                                     \
+  case MM:                                                                                    \
+    hipLaunchKernelGGL((k_focus<MM>), dim3((D.N + 63) / 64, D.S), dim3(256), 0, st, D, sub); \
+    break;
+
+hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
+  switch (D.M) {
+    EP_FOCUS_CASE(2) EP_FOCUS_CASE(3) EP_FOCUS_CASE(4) EP_FOCUS_CASE(5) EP_FOCUS_CASE(6)
+    EP_FOCUS_CASE(7) EP_FOCUS_CASE(8) EP_FOCUS_CASE(9) EP_FOCUS_CASE(10) EP_FOCUS_CASE(11)
+    EP_FOCUS_CASE(12) EP_FOCUS_CASE(13) EP_FOCUS_CASE(14) EP_FOCUS_CASE(15) EP_FOCUS_CASE(16)
+    EP_FOCUS_CASE(17) EP_FOCUS_CASE(18) EP_FOCUS_CASE(19) EP_FOCUS_CASE(20) EP_FOCUS_CASE(21)
+    EP_FOCUS_CASE(22) EP_FOCUS_CASE(23) EP_FOCUS_CASE(24) EP_FOCUS_CASE(25) EP_FOCUS_CASE(26)
+    EP_FOCUS_CASE(27) EP_FOCUS_CASE(28) EP_FOCUS_CASE(29) EP_FOCUS_CASE(30) EP_FOCUS_CASE(31)
+    EP_FOCUS_CASE(32)
+    default:
+      return hipErrorInvalidValue;
+  }
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
+  const dim3 grid((D.N + 63) / 64, D.S), block(256);
+  if (D.E <= 8)
+    hipLaunchKernelGGL((k_change<8>), grid, block, 0, st, D, sub);
+  else if (D.E <= 16)
+    hipLaunchKernelGGL((k_change<16>), grid, block, 0, st, D, sub);
+  else if (D.E <= 32)
+    hipLaunchKernelGGL((k_change<32>), grid, block, 0, st, D, sub);
+  else if (D.E <= 48)
+    hipLaunchKernelGGL((k_change<48>), grid, block, 0, st, D, sub);
+  else if (D.E <= 64)
+    hipLaunchKernelGGL((k_change<64>), grid, block, 0, st, D, sub);
+  else
+    return hipErrorInvalidValue;
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st) {
+  hipLaunchKernelGGL(k_from_reference, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D, stage,
+                     dstbuf);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_reference, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D, srcbuf,
+                     stage);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_normalize(const Dev &D, hipStream_t st) {
+  hipLaunchKernelGGL(k_normalize, dim3(D.S * D.E), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, st, p, n, v);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_init_seed(const Dev &D, hipStream_t st) {
+  hipLaunchKernelGGL(k_init_seed, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_dpp_probe(const double *in, double *out, double *out2, hipStream_t st) {
+  hipLaunchKernelGGL(k_dpp_probe, dim3(1), dim3(64), 0, st, in, out, out2);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+}  // namespace ep
+
+namespace ep {
+__global__ void k_reset_diag(DevDiag *d) {
+  d->max_sub_focus = 0;
+  d->max_sub_change = 0;
+  d->max_nsteps = 0;
+  d->max_passes = 0;
+  d->max_chunks = 0;
+  d->any_thin = 0;
+  d->min_tau_bits = (unsigned long long)__double_as_longlong(DBL_MAX);
+  d->err_key = ~0ull;
+  d->stream_sweeps = 0;
+  d->focus_sweeps = 0;
+  d->change_sweeps = 0;
+}
+
+hipError_t launch_reset_diag(DevDiag *d, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_diag, dim3(1), dim3(1), 0, st, d);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+}  // namespace ep

@@ -1,0 +1,23 @@
+// ep_launch.h -- host-side launchers of the kernels in ep_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ep_internal.h"
+
+namespace ep {
+hipError_t launch_stream_prep(const Dev &D, hipStream_t st);
+hipError_t launch_node_prep(const Dev &D, hipStream_t st);
+hipError_t launch_seed_check(const Dev &D, hipStream_t st);
+hipError_t launch_stream_pass(const Dev &D, int pass, int do_seed, int substep, int chunks,
+                              hipStream_t st);
+hipError_t launch_stream_finish(const Dev &D, int substep, hipStream_t st);
+hipError_t launch_focus(const Dev &D, int substep, hipStream_t st);
+hipError_t launch_change(const Dev &D, int substep, hipStream_t st);
+hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st);
+hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st);
+hipError_t launch_normalize(const Dev &D, hipStream_t st);
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
+hipError_t launch_init_seed(const Dev &D, hipStream_t st);
+hipError_t launch_dpp_probe(const double *in, double *out, double *out2, hipStream_t st);
+hipError_t launch_reset_diag(DevDiag *d, hipStream_t st);
+}  // namespace ep

@@ -1,0 +1,253 @@
+// ep_mas.hip -- MAS trilinear interpolation of B, V, rho onto the nodes.
+//
+// One thread per node.  Inside masRadialMax it runs masTriLinear
+// (src/masInterp.c:539-741): position to (r [Rs], theta, phi - phiOffset),
+// for each of the 7 fields and both time levels a binary search on each of
+// the field's own FP32 meshes (masTriLinearBinarySearch, :117-171), an
+// 8-corner trilinear stencil in FP64 on FP32 data (masInterpolate,
+// :177-200), the (1-s) / s time blend and the unit conversion; outside it
+// applies masWind (:1007-1065).  Fields stay resident in HBM as FP32
+// (phi, theta, r) arrays with r fastest.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ep_internal.h"
+#include "ep_state.h"
+
+namespace ep {
+
+struct MasDev {
+  const float *field[2][7];
+  const float *mesh[2][21];
+  int dims[2][21];
+};
+
+__device__ __forceinline__ double mas_bsearch(const float *__restrict__ A, double key, int *imin,
+                                              int *imax, int lower, int upper) {
+  while (upper != (lower + 1)) {
+    const int imid = (lower + upper) / 2;
+    if (A[imid] < key)
+      lower = imid;
+    else
+      upper = imid;
+  }
+  *imin = lower;
+  *imax = upper;
+  if (A[lower] > key) return 0.0;
+  if (A[upper] < key) return 1.0;
+  return ((key - A[lower]) / (A[upper] - A[lower]));
+}
+
+__device__ __forceinline__ double mas_field(const MasDev &M, int lv, int k, double rr, double th,
+                                            double ph) {
+  const int nr = M.dims[lv][3 * k], nt = M.dims[lv][3 * k + 1], np = M.dims[lv][3 * k + 2];
+  int r0, r1, t0, t1, p0, p1;
+  const double rd = mas_bsearch(M.mesh[lv][3 * k + 0], rr, &r0, &r1, 0, nr - 1);
+  const double td = mas_bsearch(M.mesh[lv][3 * k + 1], th, &t0, &t1, 0, nt - 1);
+  const double pd = mas_bsearch(M.mesh[lv][3 * k + 2], ph, &p0, &p1, 0, np - 1);
+  const float *__restrict__ V = M.field[lv][k];
+  const size_t snr = nr, snrt = (size_t)nr * nt;
+#define VI(r, t, p) V[(size_t)(r) + snr * (t) + snrt * (p)]
+  const double c00 = VI(r0, t0, p0) * (1.0 - rd) + VI(r1, t0, p0) * rd;
+  const double c10 = VI(r0, t0, p1) * (1.0 - rd) + VI(r1, t0, p1) * rd;
+  const double c01 = VI(r0, t1, p0) * (1.0 - rd) + VI(r1, t1, p0) * rd;
+  const double c11 = VI(r0, t1, p1) * (1.0 - rd) + VI(r1, t1, p1) * rd;
+#undef VI
+  const double c0 = c00 * (1.0 - pd) + c10 * pd;
+  const double c1 = c01 * (1.0 - pd) + c11 * pd;
+  return (c0 * (1.0 - td) + c1 * td);
+}
+
+// MAS unit conversions (src/baseTypes.h:107-115).  The reference's macros
+// are not parenthesised, so `x * MAS_B_CONVERT` multiplies and divides x
+// by each factor in turn; these helpers keep that order.
+__device__ __forceinline__ double mas_b_apply(double x) {
+  return x * sqrt(4.0 * kPI * 1.6726e-16) * 6.96e10 / 1445.87003080685 /
+         (sqrt(1.6726e-24 * 1.0) * 2.99792458e10);
+}
+__device__ __forceinline__ double mas_v_apply(double x) {
+  return x * (6.96e10 / 1445.87003080685) / 2.99792458e10;
+}
+__device__ __forceinline__ double mas_rho_apply(double x) { return x * (1.6726e-16 / 1.6726e-24) / 1.0; }
+
+__global__ __launch_bounds__(256) void k_mas_interp(Dev D, MasDev M, double s, double phi_offset,
+                                                    double radial_max, double vmin, const double *prev,
+                                                    double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int st = blockIdx.y;
+  if (k >= D.n_active[st]) return;
+  const size_t i = (size_t)st * D.Npad + k;
+  const size_t SN = (size_t)D.S * D.Npad;
+  const double rs = D.rScale;
+  // cartToSphPosAu (src/geometry.c:131-160, :35-45)
+  const double ax = D.x[i] * rs, ay = D.y[i] * rs, az = D.z[i] * rs;
+  const double pr = sqrt(ax * ax + ay * ay + az * az);
+  const double pth = acos(az / pr);
+  double pph = atan2(ay, ax);
+  while (pph < 0.0) pph += (2.0 * kPI);
+  if (pph > 2 * kPI)
+    while (pph > 2 * kPI) pph -= 2 * kPI;
+  double br, bt, bp, vr, vt, vp, dens;
+  if (pr <= (radial_max * kRSAU)) {
+    const double rr = pr / kRSAU, th = pth;
+    double ph = pph - phi_offset;
+    if (ph < 0.0)
+      while (ph < 0.0) ph += (2.0 * kPI);
+    if (ph > (2.0 * kPI))
+      while (ph > (2.0 * kPI)) ph -= (2.0 * kPI);
+    bp = mas_b_apply((1.0 - s) * mas_field(M, 0, 0, rr, th, ph) + s * mas_field(M, 1, 0, rr, th, ph));
+    bt = mas_b_apply((1.0 - s) * mas_field(M, 0, 1, rr, th, ph) + s * mas_field(M, 1, 1, rr, th, ph));
+    br = mas_b_apply((1.0 - s) * mas_field(M, 0, 2, rr, th, ph) + s * mas_field(M, 1, 2, rr, th, ph));
+    vp = mas_v_apply((1.0 - s) * mas_field(M, 0, 3, rr, th, ph) + s * mas_field(M, 1, 3, rr, th, ph));
+    vt = mas_v_apply((1.0 - s) * mas_field(M, 0, 4, rr, th, ph) + s * mas_field(M, 1, 4, rr, th, ph));
+    vr = mas_v_apply((1.0 - s) * mas_field(M, 0, 5, rr, th, ph) + s * mas_field(M, 1, 5, rr, th, ph));
+    if (vr < (vmin / 2.99792458e10)) vr = (vmin / 2.99792458e10);
+    dens = mas_rho_apply((1.0 - s) * mas_field(M, 0, 6, rr, th, ph) + s * mas_field(M, 1, 6, rr, th, ph));
+  } else {
+    // masWind: scale the node's previous values by r_old / r
+    const double ox = prev[0 * SN + i], oy = prev[1 * SN + i], oz = prev[2 * SN + i];
+    const double rOldmag = sqrt(ox * ox + oy * oy + oz * oz);
+    const double oneOverR = rOldmag / D.rmag[i];
+    br = prev[3 * SN + i] * oneOverR * oneOverR;
+    bt = prev[4 * SN + i] * oneOverR;
+    bp = prev[5 * SN + i] * oneOverR;
+    vr = prev[6 * SN + i];
+    vt = prev[7 * SN + i] * oneOverR;
+    vp = prev[8 * SN + i] * oneOverR;
+    dens = prev[9 * SN + i] * oneOverR * oneOverR;
+  }
+  out[0 * SN + i] = br;
+  out[1 * SN + i] = bt;
+  out[2 * SN + i] = bp;
+  out[3 * SN + i] = vr;
+  out[4 * SN + i] = vt;
+  out[5 * SN + i] = vp;
+  out[6 * SN + i] = dens;
+}
+
+}  // namespace ep
+
+using namespace ep;
+
+extern "C" {
+
+int ep_mas_set_level(int32_t level, const float *const fields[7], const float *const meshes[21],
+                     const int32_t dims[21]) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (level < 0 || level > 1 || !fields || !meshes || !dims)
+    return fail(EP_ERR_INVALID_ARG, "bad MAS level arguments");
+  MasLevel &L = g.mas[level];
+  for (int k = 0; k < 7; k++) {
+    for (int a = 0; a < 3; a++) {
+      const int n = dims[3 * k + a];
+      if (n < 2 || !meshes[3 * k + a] || !fields[k])
+        return fail(EP_ERR_INVALID_ARG, "MAS field %d axis %d: need >= 2 points", k, a);
+    }
+  }
+  // (re)allocate when the shape changes
+  for (int k = 0; k < 21; k++) {
+    if (L.set && L.dims[k] == dims[k]) continue;
+    L.set = false;
+  }
+  if (!L.set) {
+    for (int k = 0; k < 7; k++) {
+      const size_t cnt = (size_t)dims[3 * k] * dims[3 * k + 1] * dims[3 * k + 2];
+      void *p = nullptr;
+      hipError_t e = hipMalloc(&p, cnt * sizeof(float));
+      if (e != hipSuccess) return hip_fail(e, "MAS field alloc");
+      g.mas_allocs.push_back(p);
+      L.field[k] = (float *)p;
+    }
+    for (int k = 0; k < 21; k++) {
+      void *p = nullptr;
+      hipError_t e = hipMalloc(&p, dims[k] * sizeof(float));
+      if (e != hipSuccess) return hip_fail(e, "MAS mesh alloc");
+      g.mas_allocs.push_back(p);
+      L.mesh[k] = (float *)p;
+      L.dims[k] = dims[k];
+    }
+  }
+  for (int k = 0; k < 7; k++) {
+    const size_t cnt = (size_t)dims[3 * k] * dims[3 * k + 1] * dims[3 * k + 2];
+    hipError_t e = hipMemcpy(L.field[k], fields[k], cnt * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "MAS field upload");
+  }
+  for (int k = 0; k < 21; k++) {
+    hipError_t e = hipMemcpy(L.mesh[k], meshes[k], dims[k] * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "MAS mesh upload");
+  }
+  L.set = true;
+  return EP_OK;
+}
+
+int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev, ep_mas_out *out,
+                  double *device_ms) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
+  if (!mp) return fail(EP_ERR_INVALID_ARG, "null MAS params");
+  // The phi wrap loops of masTriLinear add or subtract 2 pi until the angle
+  // is in [0, 2 pi]; bound the offset so that they end after a few turns.
+  if (!(fabs(mp->phi_offset) <= 1.0e3) || !std::isfinite(mp->s_cor) || !std::isfinite(mp->mas_radial_max))
+    return fail(EP_ERR_INVALID_ARG, "phi_offset must be finite with |phi_offset| <= 1000");
+  if (!g.mas[0].set || !g.mas[1].set) return fail(EP_ERR_INVALID_ARG, "set both MAS levels first");
+  const size_t S = g.d.num_streams, N = g.d.num_nodes, SN = S * g.Npad;
+  if (!g.mas_out) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, 17 * SN * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "MAS output alloc");
+    g.mas_allocs.push_back(p);
+    g.mas_out = (double *)p;
+    g.mas_prev = g.mas_out + 7 * SN;
+    hipMemset(g.mas_prev, 0, 10 * SN * sizeof(double));
+  }
+  if (prev) {
+    const double *src[10] = {prev->rold_x, prev->rold_y, prev->rold_z, prev->br, prev->btheta,
+                             prev->bphi, prev->vr, prev->vtheta, prev->vphi, prev->density};
+    for (int i = 0; i < 10; i++) {
+      if (!src[i]) return fail(EP_ERR_INVALID_ARG, "prev array %d is null", i);
+      hipError_t e = hipMemcpy2D(g.mas_prev + i * SN, g.Npad * 8, src[i], N * 8, N * 8, S,
+                                 hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_fail(e, "prev upload");
+    }
+  }
+  MasDev M;
+  for (int lv = 0; lv < 2; lv++) {
+    for (int k = 0; k < 7; k++) M.field[lv][k] = g.mas[lv].field[k];
+    for (int k = 0; k < 21; k++) {
+      M.mesh[lv][k] = g.mas[lv].mesh[k];
+      M.dims[lv][k] = g.mas[lv].dims[k];
+    }
+  }
+  Dev D = make_dev();
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, g.st);
+  hipLaunchKernelGGL(k_mas_interp, dim3((N + 255) / 256, S), dim3(256), 0, g.st, D, M, mp->s_cor,
+                     mp->phi_offset, mp->mas_radial_max, mp->mas_vmin, g.mas_prev, g.mas_out);
+  hipError_t le = hipGetLastError();
+  hipEventRecord(e1, g.st);
+  hipError_t se = hipStreamSynchronize(g.st);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (le != hipSuccess) return hip_fail(le, "k_mas_interp launch");
+  if (se != hipSuccess) return hip_fail(se, "k_mas_interp");
+  if (device_ms) *device_ms = ms;
+  if (out) {
+    double *dst[7] = {out->br, out->btheta, out->bphi, out->vr, out->vtheta, out->vphi, out->density};
+    for (int i = 0; i < 7; i++) {
+      if (!dst[i]) continue;
+      hipError_t e = hipMemcpy2D(dst[i], N * 8, g.mas_out + i * SN, g.Npad * 8, N * 8, S,
+                                 hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "MAS output download");
+    }
+  }
+  return EP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// ep_state.h -- host-side record of the library's device allocations.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "ep_internal.h"
+
+namespace ep {
+
+// One MAS time level resident on the device: 7 FP32 fields and their
+// 21 1-D meshes (src/readMAS.c:1005-1069 reads the same from HDF5).
+struct MasLevel {
+  float *field[7] = {nullptr};
+  float *mesh[21] = {nullptr};
+  int dims[21] = {0};
+  bool set = false;
+};
+
+struct State {
+  bool init = false, nodes_set = false;
+  ep_dims d{};
+  ep_params p{};
+  int Npad = 0;
+  hipStream_t st = nullptr;
+  int stream_block = 256, stream_halo = 32;
+  // host copies of the grids
+  std::vector<double> egrid, vgrid, vgrid_i, rigidity, rig_lamo, plt, ext, mugrid;
+  double dlnp = 0, dmu = 0, normJ0 = 0, normRadius = 0;
+  int mpos = 0;
+  std::vector<int32_t> h_nactive;
+  // device buffers
+  double *egrid_d = nullptr, *vgrid_d = nullptr, *vgrid_i_d = nullptr, *rig_lamo_d = nullptr,
+         *plt_d = nullptr, *ext_d = nullptr, *mugrid_d = nullptr;
+  double *fA = nullptr, *fB = nullptr;
+  uint8_t *loc = nullptr;
+  double *nx = nullptr, *ny = nullptr, *nz = nullptr, *nrmag = nullptr, *nbmag = nullptr,
+         *ndlnb = nullptr, *ndlnn = nullptr, *ndupar = nullptr;
+  int32_t *n_active = nullptr;
+  double *ds = nullptr, *bplus = nullptr, *bminus = nullptr, *mfp_r = nullptr, *radial = nullptr;
+  int32_t *list = nullptr, *ref = nullptr, *nkept = nullptr, *thin = nullptr;
+  double *dsil = nullptr, *dsmin = nullptr;
+  int32_t *nsteps = nullptr, *halo = nullptr, *passes = nullptr, *chunks = nullptr;
+  double *dtprop = nullptr, *del_fac = nullptr, *seed3 = nullptr, *dsm = nullptr, *q = nullptr;
+  int32_t *nsub_f = nullptr, *nsub_c = nullptr;
+  double *dtsub_f = nullptr, *dtsub_c = nullptr, *f_dlnbds = nullptr, *f_dupar = nullptr,
+         *f_b = nullptr, *c_dupar = nullptr, *c_dlnn = nullptr, *c_dlnb = nullptr,
+         *c_damp = nullptr, *c_bc0 = nullptr, *c_bc1 = nullptr;
+  DevDiag *diag = nullptr;
+  std::vector<void *> allocs;
+  std::vector<hipEvent_t> events;
+  // MAS interpolation
+  MasLevel mas[2];
+  std::vector<void *> mas_allocs;
+  double *mas_out = nullptr;   // 7 x S x Npad
+  double *mas_prev = nullptr;  // 10 x S x Npad
+  void mas_free() {
+    for (void *p : mas_allocs) hipFree(p);
+    mas_allocs.clear();
+    mas[0] = MasLevel();
+    mas[1] = MasLevel();
+    mas_out = nullptr;
+    mas_prev = nullptr;
+  }
+};
+
+extern State g;
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+Dev make_dev();
+
+}  // namespace ep

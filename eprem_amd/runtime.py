@@ -1,0 +1,179 @@
+"""ctypes binding of libeprem_hip.so (the C ABI in include/eprem_hip.h).
+
+This is the product path: there is no CPU fallback.  Loading fails loudly
+if the library has not been built, and `EpSolver` raises on every non-zero
+return code with the library's own message (the reference panics at the
+same points, src/error.c:39-49).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (EP_LAYOUT_REFERENCE, EP_OK, EP_OP_ALL, EpDiag, EpDims, EpMasOut, EpMasParams,
+                  EpMasPrev, EpNodeSoa, EpParams, dptr, fptr, iptr)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libeprem_hip.so")
+
+_lib = None
+
+
+class EpError(RuntimeError):
+    def __init__(self, code, msg, diag=None):
+        super().__init__("%s (%d): %s" % (abi.ERR_NAMES.get(code, "?"), code, msg))
+        self.code = code
+        self.diag = diag
+
+
+def lib():
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libeprem_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.ep_init.argtypes = [C.POINTER(EpDims), C.POINTER(EpParams)]
+        L.ep_set_params.argtypes = [C.POINTER(EpParams)]
+        L.ep_set_tuning.argtypes = [C.c_int32, C.c_int32]
+        L.ep_get_grids.argtypes = [DP, DP, DP, DP, DP, DP]
+        L.ep_set_nodes.argtypes = [C.POINTER(EpNodeSoa), IP]
+        L.ep_upload_f.argtypes = [DP, C.c_int32]
+        L.ep_download_f.argtypes = [DP, C.c_int32]
+        L.ep_init_seed_population.argtypes = []
+        L.ep_step.argtypes = [C.c_uint32, C.POINTER(EpDiag)]
+        L.ep_get_node_outputs.argtypes = [DP, DP, DP]
+        L.ep_mas_set_level.argtypes = [C.c_int32, C.POINTER(C.POINTER(C.c_float)),
+                                       C.POINTER(C.POINTER(C.c_float)), IP]
+        L.ep_interp_mhd.argtypes = [C.POINTER(EpMasParams), C.POINTER(EpMasPrev),
+                                    C.POINTER(EpMasOut), DP]
+        L.ep_last_error.argtypes = [C.c_char_p, C.c_size_t]
+        L.ep_finalize.argtypes = []
+        L.ep_selftest_dpp.argtypes = [DP, DP, DP]
+        for name in abi.EXPORTED_SYMBOLS:
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(512)
+    lib().ep_last_error(buf, 512)
+    return buf.value.decode()
+
+
+def _check(rc, diag=None):
+    if rc != EP_OK:
+        raise EpError(rc, last_error(), diag)
+
+
+NODE_FIELDS = ("x", "y", "z", "rmag", "bmag", "dlnb", "dlnn", "dupar")
+
+
+class EpSolver:
+    """Owns the library's device state for one problem size.
+
+    Mirrors the reference's driver interface: `step()` is one call of
+    updateEnergeticParticles() (src/energeticParticles.c:69-203).
+    """
+
+    def __init__(self, dims, params, stream_block=None, stream_halo=None):
+        self.dims = dims if isinstance(dims, EpDims) else EpDims(*dims)
+        self.params = params
+        _check(lib().ep_init(C.byref(self.dims), C.byref(params)))
+        if stream_block or stream_halo:
+            self.set_tuning(stream_block or 256, stream_halo or 32)
+        self._keep = None
+
+    @property
+    def shape_f(self):
+        d = self.dims
+        return (d.num_streams, d.num_nodes, d.num_energies, d.num_mu)
+
+    def set_tuning(self, stream_block, stream_halo):
+        _check(lib().ep_set_tuning(int(stream_block), int(stream_halo)))
+
+    def set_params(self, params):
+        self.params = params
+        _check(lib().ep_set_params(C.byref(params)))
+
+    def grids(self):
+        E, M = self.dims.num_energies, self.dims.num_mu
+        out = {"egrid": np.zeros(E), "vgrid": np.zeros(E), "mugrid": np.zeros(M), "rigidity": np.zeros(E)}
+        dl, dm = C.c_double(), C.c_double()
+        _check(lib().ep_get_grids(dptr(out["egrid"]), dptr(out["vgrid"]), dptr(out["mugrid"]),
+                                  dptr(out["rigidity"]), C.byref(dl), C.byref(dm)))
+        out["dlnp"], out["dmu"] = dl.value, dm.value
+        return out
+
+    def set_nodes(self, nodes, n_active=None):
+        S, N = self.dims.num_streams, self.dims.num_nodes
+        arrs = {k: np.ascontiguousarray(nodes[k], dtype=np.float64).reshape(S, N) for k in NODE_FIELDS}
+        soa = EpNodeSoa(*[dptr(arrs[k]) for k in NODE_FIELDS])
+        na = None if n_active is None else np.ascontiguousarray(n_active, dtype=np.int32)
+        _check(lib().ep_set_nodes(C.byref(soa), iptr(na)))
+
+    def upload_f(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == self.shape_f, (f.shape, self.shape_f)
+        _check(lib().ep_upload_f(dptr(f), EP_LAYOUT_REFERENCE))
+
+    def download_f(self):
+        f = np.empty(self.shape_f, dtype=np.float64)
+        _check(lib().ep_download_f(dptr(f), EP_LAYOUT_REFERENCE))
+        return f
+
+    def init_seed_population(self):
+        _check(lib().ep_init_seed_population())
+
+    def step(self, op_flags=EP_OP_ALL, raise_on_error=True):
+        dg = EpDiag()
+        rc = lib().ep_step(op_flags, C.byref(dg))
+        if raise_on_error:
+            _check(rc, dg.as_dict())
+        return rc, dg.as_dict()
+
+    def node_outputs(self):
+        S, N = self.dims.num_streams, self.dims.num_nodes
+        ds, bp, bm = (np.zeros((S, N)) for _ in range(3))
+        _check(lib().ep_get_node_outputs(dptr(ds), dptr(bp), dptr(bm)))
+        return ds, bp, bm
+
+    def mas_set_level(self, level, fields, meshes):
+        fields = [np.ascontiguousarray(a, dtype=np.float32) for a in fields]
+        meshes = [np.ascontiguousarray(a, dtype=np.float32) for a in meshes]
+        dims = np.array([len(m) for m in meshes], dtype=np.int32)
+        FP = C.POINTER(C.c_float)
+        fa = (FP * 7)(*[fptr(a) for a in fields])
+        ma = (FP * 21)(*[fptr(a) for a in meshes])
+        _check(lib().ep_mas_set_level(int(level), fa, ma, iptr(dims)))
+
+    def interp_mhd(self, s_cor, phi_offset, mas_radial_max, mas_vmin, prev=None):
+        S, N = self.dims.num_streams, self.dims.num_nodes
+        mp = EpMasParams(s_cor, phi_offset, mas_radial_max, mas_vmin)
+        keep = []
+        pv = None
+        if prev is not None:
+            names = ("rold_x", "rold_y", "rold_z", "br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")
+            arrs = [np.ascontiguousarray(prev[k], dtype=np.float64).reshape(S, N) for k in names]
+            keep.extend(arrs)
+            pv = EpMasPrev(*[dptr(a) for a in arrs])
+        outs = {k: np.zeros((S, N)) for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")}
+        mo = EpMasOut(*[dptr(outs[k]) for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")])
+        ms = C.c_double()
+        _check(lib().ep_interp_mhd(C.byref(mp), C.byref(pv) if pv is not None else None, C.byref(mo),
+                                   C.byref(ms)))
+        outs["device_ms"] = ms.value
+        return outs
+
+    def close(self):
+        lib().ep_finalize()
+
+
+def selftest_dpp():
+    x = np.arange(64, dtype=np.float64) + 0.5
+    a, b = np.zeros(64), np.zeros(64)
+    _check(lib().ep_selftest_dpp(dptr(x), dptr(a), dptr(b)))
+    return x, a, b

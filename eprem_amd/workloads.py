@@ -1,0 +1,149 @@
+"""Synthetic workloads on the cube-shell grid.
+
+Node positions follow what the reference's host loop produces for a steady
+radial wind: every global step a node is spawned at the inner boundary
+(rmag = 1, i.e. r = rScale) on each stream, rotated by the Sun's azimuth
+(spawnNewShell, src/simCore.c:629-687), and every older node moves out
+radially by V*tDel (moveNodes/delrFlow, src/simCore.c:736-785,
+src/flow.c:169-178).  Node k of a stream is therefore at
+rmag = 1 + k*V*tDel/rScale and azimuth phi0 - k*omega*tDel: an Archimedean
+(Parker) spiral.  The MHD values are the reference's analytic Parker model
+(mhdB, mhdV, mhdDensity, src/flow.c:241-403) and the derivative terms are
+formed as in updateMhd (src/flow.c:131-149).
+
+Seeded perturbations (radial speed, density, field strength, node jitter)
+make every operator branch reachable: compression and rarefaction, both
+signs of the parallel velocity change, thinning of close nodes.
+"""
+import math
+
+import numpy as np
+
+from . import constants as K
+from .abi import EpDims
+
+
+def cube_directions(rows, cols):
+    """Unit vectors of the 6*rows*cols stream footpoints, stream index =
+    col + cols*(row + rows*face) (initCubeCoords + initSphereCoords,
+    src/cubeShellInit.c:529-822; the coordinate walk accumulates cell
+    widths the same way)."""
+    out = []
+    cw, chh = 1.0 / cols, 1.0 / rows
+    for face in range(6):
+        for row in range(rows):
+            for col in range(cols):
+                u, u2, w = -0.5, 0.5, 0.5
+                u += 0.5 * cw
+                u2 -= 0.5 * cw
+                w -= 0.5 * chh
+                for _ in range(col):
+                    u += cw
+                    u2 -= cw
+                for _ in range(row):
+                    w -= chh
+                x, y, z = {0: (u, w, 0.5), 1: (u2, w, -0.5), 2: (0.5, u, w), 3: (-0.5, u2, w),
+                           4: (u2, 0.5, w), 5: (u, -0.5, w)}[face]
+                r = math.sqrt(x * x + y * y + z * z)
+                x, y, z = x / r, y / r, z / r
+                out.append((math.atan2(y, x), math.acos(z)))
+    return out
+
+
+def parker_nodes(rows, cols, n_nodes, tdel_days=1.0 / 64, flow_cms=400.0e5, density_au=8.3,
+                 b_au=2.5e-5, omega=None, rscale=0.005, seed=None, jitter=0.0, perturb=0.0,
+                 thin_frac=0.0):
+    """Node SoA arrays [S, N] for a Parker-spiral snapshot.
+
+    perturb > 0 multiplies V, n and B by seeded smooth random factors
+    (relative amplitude `perturb`) that differ between the current and the
+    previous time level; jitter > 0 moves nodes by a random fraction of the
+    spacing; thin_frac > 0 pulls that fraction of node pairs to within the
+    thinning distance dsh_min = 5e-5 AU.
+    """
+    rng = np.random.default_rng(seed)
+    if omega is None:
+        omega = (0.004144 / K.MAS_TIME_NORM) * K.TAU
+    S = 6 * rows * cols
+    tdel = tdel_days / K.DAY
+    us = flow_cms / K.C
+    bs = b_au / K.MHD_B_NORM
+    dirs = cube_directions(rows, cols)
+    k = np.arange(n_nodes, dtype=np.float64)
+    step = us * tdel / rscale                      # radial move per step (rScale units)
+    rm = 1.0 + k * step
+    rm_old = np.maximum(rm - step, 1.0)
+    out = {f: np.zeros((S, n_nodes)) for f in ("x", "y", "z", "rmag", "bmag", "dlnb", "dlnn", "dupar")}
+    for s, (azi0, zen) in enumerate(dirs):
+        r = rm.copy()
+        if jitter > 0:
+            r[1:] += rng.uniform(-jitter, jitter, n_nodes - 1) * step
+        azi = azi0 - k * omega * tdel
+        x = r * np.sin(zen) * np.cos(azi)
+        y = r * np.sin(zen) * np.sin(azi)
+        z = r * np.cos(zen) * np.ones_like(r)
+        if thin_frac > 0:
+            pick = rng.random(n_nodes) < thin_frac
+            pick[:2] = False
+            for i in np.nonzero(pick)[0]:
+                # put node i within 2e-5 AU of node i-1
+                d = np.array([x[i] - x[i - 1], y[i] - y[i - 1], z[i] - z[i - 1]])
+                d *= (2.0e-5 / rscale) / max(np.linalg.norm(d), 1e-300)
+                x[i], y[i], z[i] = x[i - 1] + d[0], y[i - 1] + d[1], z[i - 1] + d[2]
+        rr = np.sqrt(x * x + y * y + z * z)
+        th = np.arccos(z / rr)
+        rau, rau_old = rr * rscale, rm_old * rscale
+
+        def fac(amp):
+            if amp <= 0:
+                return np.ones(n_nodes), np.ones(n_nodes)
+            base = np.cumsum(rng.normal(0, amp, n_nodes)) / np.sqrt(np.arange(1, n_nodes + 1))
+            new = np.exp(np.clip(base + rng.normal(0, amp, n_nodes), -1, 1))
+            old = np.exp(np.clip(base, -1, 1))
+            return new, old
+
+        fv, fv_old = fac(perturb)
+        fn, fn_old = fac(perturb)
+        fb, fb_old = fac(perturb)
+        vr, vr_old = us * fv, us * fv_old
+        dens, dens_old = density_au / rau ** 2 * fn, density_au / rau_old ** 2 * fn_old
+        br = bs / rau ** 2 * fb
+        bp = -1.0 * rau * br * (omega / (vr + K.VERYSMALL)) * np.sin(th)
+        bmag = np.sqrt(br * br + bp * bp)
+        br_o = bs / rau_old ** 2 * fb_old
+        bp_o = -1.0 * rau_old * br_o * (omega / (vr_old + K.VERYSMALL)) * np.sin(th)
+        bmag_old = np.sqrt(br_o * br_o + bp_o * bp_o)
+        dupar = ((vr - vr_old) * br) / bmag
+        dlnb = np.log(bmag / bmag_old)
+        dlnn = np.log(dens / dens_old)
+        dupar[0] = dlnb[0] = dlnn[0] = 0.0
+        out["x"][s], out["y"][s], out["z"][s], out["rmag"][s] = x, y, z, rr
+        out["bmag"][s], out["dlnb"][s], out["dlnn"][s], out["dupar"][s] = bmag, dlnb, dlnn, dupar
+    return out
+
+
+def seeded_f(p, grids, nodes, noise=0.0, seed=None):
+    """f[S, N, E, M] = sepSeedFunction(E, rmag) (initEnergeticParticles)
+    times seeded log-normal noise (anisotropic when noise > 0)."""
+    rng = np.random.default_rng(seed)
+    eg = grids["egrid"]
+    rm = nodes["rmag"]
+    efac = K.MEV / (K.MP * K.C * K.C)
+    normJ0 = p.boundaryFunctAmplitude * (K.MP * K.C) / (K.MHD_DENSITY_NORM * K.MEV)
+    normR = p.boundaryFunctR0 / p.rScale
+    radial = (rm / normR) ** (-p.boundaryFunctBeta)
+    plt = (eg / (1.0 * efac)) ** (-p.boundaryFunctGamma)
+    ext = np.exp(-(eg / (p.boundaryFunctEcutoff * efac)))
+    f = normJ0 * radial[:, :, None] * plt[None, None, :] * ext[None, None, :]
+    f = f / p.boundaryFunctXi / (2.0 * eg[None, None, :])
+    f = np.maximum(f, 2.2250738585072014e-308)
+    M = len(grids["mugrid"])
+    f = np.repeat(f[..., None], M, axis=3)
+    if noise > 0:
+        f = f * np.exp(rng.normal(0, noise, f.shape))
+    return np.ascontiguousarray(f)
+
+
+def dims_of(nodes, E, M):
+    S, N = nodes["x"].shape
+    return EpDims(S, N, E, M)

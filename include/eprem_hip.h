@@ -133,6 +133,11 @@ typedef struct ep_diag {
   int64_t cell_updates;            /* active cells x EP substeps          */
   double  device_ms;               /* HIP-event time of the whole step    */
   double  stream_ms, focus_ms, change_ms, prep_ms; /* per-operator time   */
+  /* Work done, in cell-sweeps (one cell advanced one CFL subcycle of one
+   * operator), summed over the step's EP substeps. */
+  int64_t stream_sweeps, focus_sweeps, change_sweeps;
+  int32_t stream_launches;         /* streaming pass launches in the step  */
+  int32_t pad_;
 } ep_diag;
 
 /* ------------------------------------------------------------------ */
@@ -221,6 +226,17 @@ int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev,
 /* ---------------- misc --------------------------------------------- */
 int ep_last_error(char *buf, size_t len);
 int ep_finalize(void);
+
+/* Tiling of the streaming kernel: nodes per block (64..512, multiple of
+ * 64) and Euler subcycles per pass (halo, 1..block/2-1).  No counterpart
+ * in the reference (its DiffuseStreamData loops on the CPU); set by the
+ * autotuner or left at the defaults (256, 32). */
+int ep_set_tuning(int32_t stream_block, int32_t stream_halo);
+
+/* Device self-test of the cross-lane shift the streaming kernel uses: for
+ * one 64-lane wave, shr[i] = in[i-1] (shr[0] = -1) and shl[i] = in[i+1]
+ * (shl[63] = -2).  No counterpart in the reference; used by the tests. */
+int ep_selftest_dpp(const double *in64, double *shr64, double *shl64);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,201 @@
+"""GPU parity: the HIP library (through the C ABI) against the FP64 oracle.
+
+Bar: 1e-10 relative on every cell of f (the arithmetic keeps the
+reference's operation order without FMA contraction, so the observed
+error is the device libm's exp/log/pow rounding, ~1e-15), identical
+integer diagnostics (subcycle counts) and identical error codes/locations.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from eprem_amd import abi, config, runtime, workloads
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def params(**kw):
+    base = dict(numEpSteps=4, tDel=1.0 / 64, eMin=0.01, eMax=1000.0, lamo=0.1, mfpRadialPower=1.0,
+                boundaryFunctEcutoff=1.0e30)
+    base.update(kw)
+    return config.default_params(**base)
+
+
+def run_both(p, nodes, E, M, f0=None, n_active=None, op_flags=abi.EP_OP_ALL, steps=1, tiling=None,
+             noise=0.3, seed=2):
+    dims = workloads.dims_of(nodes, E, M)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        if tiling:
+            sol.set_tuning(*tiling)
+        grids = sol.grids()
+        if f0 is None:
+            f0 = workloads.seeded_f(p, grids, nodes, noise=noise, seed=seed)
+        sol.set_nodes(nodes, n_active)
+        sol.upload_f(f0)
+        gd = []
+        for _ in range(steps):
+            rc, dg = sol.step(op_flags, raise_on_error=False)
+            gd.append((rc, dg))
+            if rc:
+                break
+        fg = sol.download_f()
+        outs = sol.node_outputs()
+    finally:
+        sol.close()
+    fo = f0.copy()
+    od = []
+    for _ in range(steps):
+        rc, do, fo, ds, bp, bm = oracle.update_ep(dims, p, nodes, fo, n_active=n_active, op_flags=op_flags)
+        od.append((rc, do))
+        if rc:
+            break
+    return fg, fo, gd, od, outs, (ds, bp, bm)
+
+
+def max_rel(a, b, n_active=None):
+    if n_active is not None:
+        mask = np.zeros(a.shape[:2], bool)
+        for s, n in enumerate(n_active):
+            mask[s, :n] = True
+        a, b = a[mask], b[mask]
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+def assert_diag_equal(gd, od):
+    assert len(gd) == len(od)
+    for (rg, dg), (ro, do) in zip(gd, od):
+        assert rg == ro, (abi.ERR_NAMES.get(rg), abi.ERR_NAMES.get(ro))
+        for k in ("max_subcycles_change", "max_subcycles_focusing", "max_nsteps_stream"):
+            assert dg[k] == do[k], (k, dg[k], do[k])
+        assert dg["min_tau"] == pytest.approx(do["min_tau"], rel=1e-14)
+        assert dg["cell_updates"] == do["cell_updates"]
+
+
+def test_dpp_lane_layout():
+    x, shr, shl = runtime.selftest_dpp()
+    np.testing.assert_array_equal(shr[1:], x[:-1])
+    assert shr[0] == -1.0
+    np.testing.assert_array_equal(shl[:-1], x[1:])
+    assert shl[63] == -2.0
+
+
+@pytest.mark.parametrize("E,M", [(12, 6), (20, 20), (8, 5)])
+def test_grids_bit_exact(E, M):
+    p = params()
+    sol = runtime.EpSolver((6, 8, E, M), p)
+    try:
+        g = sol.grids()
+    finally:
+        sol.close()
+    o = oracle.build_grids(p, E, M)
+    for k in ("egrid", "vgrid", "mugrid", "rigidity", "dlnp", "dmu"):
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+
+
+@pytest.mark.parametrize("case", [
+    dict(rows=1, cols=1, N=64, E=12, M=6),
+    dict(rows=1, cols=1, N=80, E=20, M=20),
+    dict(rows=1, cols=1, N=70, E=9, M=7),
+    dict(rows=1, cols=1, N=50, E=50, M=30),
+    dict(rows=1, cols=1, N=40, E=64, M=32),
+    dict(rows=2, cols=2, N=60, E=12, M=6),
+])
+def test_step_parity(case):
+    p = params()
+    nodes = workloads.parker_nodes(case["rows"], case["cols"], case["N"], seed=1, perturb=0.2, jitter=0.2)
+    fg, fo, gd, od, outs, oouts = run_both(p, nodes, case["E"], case["M"])
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo) <= RTOL
+    for a, b in zip(outs, oouts):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("flags", [abi.EP_OP_STREAM, abi.EP_OP_FOCUS, abi.EP_OP_CHANGE,
+                                   abi.EP_OP_SEEDCHECK, abi.EP_OP_STREAM | abi.EP_OP_SEEDCHECK])
+def test_single_operator_parity(flags):
+    p = params()
+    nodes = workloads.parker_nodes(1, 1, 64, seed=3, perturb=0.3, jitter=0.1)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 12, 8, op_flags=flags)
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo) <= RTOL
+
+
+@pytest.mark.parametrize("tiling", [(64, 1), (64, 20), (128, 8), (256, 32), (512, 64), (512, 200)])
+def test_streaming_tiling_invariance(tiling):
+    """Results do not depend on the chunk size or subcycles per pass."""
+    p = params(numEpSteps=2)
+    nodes = workloads.parker_nodes(1, 1, 300, seed=4, perturb=0.2, jitter=0.3)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 10, 6, tiling=tiling)
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo) <= RTOL
+
+
+def test_thinning_parity():
+    """Nodes closer than dsh_min are thinned (GetStreamList) and copied
+    from their reference node after streaming."""
+    p = params()
+    nodes = workloads.parker_nodes(1, 1, 120, seed=5, perturb=0.1, thin_frac=0.15)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 10, 6)
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo) <= RTOL
+
+
+def test_ragged_streams_parity():
+    """Per-stream active node counts (batched form)."""
+    p = params()
+    nodes = workloads.parker_nodes(1, 1, 200, seed=6, perturb=0.2, jitter=0.1)
+    n_active = np.array([200, 37, 150, 3, 99, 128], np.int32)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 10, 6, n_active=n_active)
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo, n_active) <= RTOL
+
+
+def test_multi_step_parity():
+    p = params(numEpSteps=3)
+    nodes = workloads.parker_nodes(1, 1, 64, seed=7, perturb=0.2, jitter=0.2)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 12, 8, steps=3)
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo) <= RTOL
+
+
+@pytest.mark.parametrize("kw", [dict(useEPBoundary=0), dict(mfpType=2), dict(dampCooling=0.5),
+                                dict(checkSeedPopulation=0), dict(lamo=1.0e-4)])
+def test_option_parity(kw):
+    p = params(**kw)
+    nodes = workloads.parker_nodes(1, 1, 64, seed=8, perturb=0.3, jitter=0.2)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 12, 8)
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo) <= RTOL
+
+
+def test_nan_error_location():
+    """A NaN in f panics in the reference (checkNaN); the library returns
+    EP_ERR_NAN at the same (stream, node, energy, mu)."""
+    p = params()
+    nodes = workloads.parker_nodes(1, 1, 64, seed=9, perturb=0.2)
+    dims = workloads.dims_of(nodes, 12, 8)
+    g = oracle.build_grids(p, 12, 8)
+    f0 = workloads.seeded_f(p, g, nodes, noise=0.2, seed=1)
+    f0[3, 17, 5, 2] = np.nan
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 12, 8, f0=f0)
+    (rg, dg), (ro, do) = gd[0], od[0]
+    assert rg == ro == abi.EP_ERR_NAN
+    for k in ("err_stream", "err_node", "err_energy", "err_mu"):
+        assert dg[k] == do[k], (k, dg[k], do[k])
+
+
+def test_ds_underflow_error():
+    """Two coincident kept nodes give ds = 0: FindSegmentLengths panics.
+    With any positive dsh_min GetStreamList thins one of two coincident
+    nodes, so the test disables thinning with a negative dsh_min (outside
+    the input file's accepted range, but the same for both sides)."""
+    p = params(dsh_min=-1.0)
+    nodes = workloads.parker_nodes(1, 1, 32, seed=10)
+    for k in ("x", "y", "z", "rmag"):
+        nodes[k][2, 11] = nodes[k][2, 10]
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 8, 6)
+    assert gd[0][0] == od[0][0] == abi.EP_ERR_DS_UNDERFLOW
+    assert gd[0][1]["err_stream"] == od[0][1]["err_stream"] == 2

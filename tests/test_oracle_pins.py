@@ -1,0 +1,268 @@
+"""Closed-form pins of the FP64 oracle, one per operator.
+
+The reference's digests pin the whole chain end to end (tests in
+test_oracle_golden.py).  These tests pin each operator separately against
+statements of the physics or mathematics that do not restate the
+oracle's code:
+
+* streaming: first-order upwind advection is exact for a profile linear
+  in s (away from the upstream boundary), so f(s) -> f(s - v mu dt);
+* relaxation: with no spatial gradient the BGK step gives
+  iso + (f - iso) exp(-dt/tau), tau = lambda/v,
+  lambda = lamo (R/1 GV)^(1/3) (r/1 AU)^p;
+* focusing: zero-flux walls conserve sum_mu f; for an isotropic f one
+  step changes each interior cell by -dt f d(mudot)/dmu where mudot is the
+  focused-transport pitch-angle rate, central-differenced (exact for the
+  cubic mudot up to its third-derivative term);
+* adiabatic change: ln f linear in ln p is shifted by -beta * (dlnp/dt)
+  * dt / dlnp, with dlnp/dt from the focused-transport energy equation;
+* thinning: hand-derived shell lists for small node lines;
+* mean free path (B form) and MAS trilinear interpolation: closed forms.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from eprem_amd import abi, config, constants as K
+from eprem_amd.abi import EpDims
+
+
+def line_nodes(n, ds_au, rscale=1.0, r0_au=1.0, direction=(0.6, 0.0, 0.8), bmag=None):
+    """n nodes on a straight radial line with uniform spacing (AU)."""
+    d = np.asarray(direction, float)
+    d /= np.linalg.norm(d)
+    s = r0_au + ds_au * np.arange(n)
+    pos = np.outer(s / rscale, d)
+    nodes = {"x": pos[:, 0][None].copy(), "y": pos[:, 1][None].copy(), "z": pos[:, 2][None].copy(),
+             "rmag": np.linalg.norm(pos, axis=1)[None].copy()}
+    nodes["bmag"] = (np.ones((1, n)) if bmag is None else np.asarray(bmag, float)[None].copy())
+    for k in ("dlnb", "dlnn", "dupar"):
+        nodes[k] = np.zeros((1, n))
+    return nodes
+
+
+def base_params(**kw):
+    p = dict(rScale=1.0, numEpSteps=1, tDel=0.05, eMin=1.0, eMax=100.0, lamo=1.0e12,
+             useEPBoundary=0, checkSeedPopulation=0, mfpRadialPower=1.0)
+    p.update(kw)
+    return config.default_params(**p)
+
+
+def test_streaming_linear_profile_is_advected_exactly():
+    n, E, M = 200, 4, 6
+    ds = 0.01
+    p = base_params(tDel=5.0e-4)      # days: dt = 0.087 AU/c, a few nodes of travel
+    nodes = line_nodes(n, ds)
+    g = oracle.build_grids(p, E, M)
+    alpha, beta = 1.0, 1.0e-3
+    f = np.empty((1, n, E, M))
+    f[:] = (alpha + beta * np.arange(n))[None, :, None, None]
+    rc, d, fo, _, _, _ = oracle.update_ep(EpDims(1, n, E, M), p, nodes, f, op_flags=abi.EP_OP_STREAM)
+    assert rc == 0
+    dt = p.tDel / p.numEpSteps
+    nmax = d["max_nsteps_stream"]
+    k = np.arange(n)
+    interior = (k >= nmax + 1) & (k <= n - 2 - nmax)
+    assert interior.sum() > 100
+    for e in range(E):
+        for m in range(M):
+            shift = g["vgrid"][e] * g["mugrid"][m] * dt / ds  # nodes travelled
+            expect = alpha + beta * (k - shift)
+            np.testing.assert_allclose(fo[0, interior, e, m], expect[interior], rtol=1e-12)
+
+
+def test_relaxation_closed_form():
+    n, E, M = 40, 5, 6
+    # r-independent mean free path, so that every node relaxes alike and no
+    # gradient along s appears (then the upwind term stays exactly zero)
+    p = base_params(tDel=3.0e-4, lamo=0.02, mfpRadialPower=0.0, rigidityPower=1.0 / 3.0)
+    nodes = line_nodes(n, 0.02)
+    g = oracle.build_grids(p, E, M)
+    rng = np.random.default_rng(0)
+    prof = rng.uniform(0.5, 2.0, (E, M))
+    f = np.broadcast_to(prof, (1, n, E, M)).copy()     # no gradient along s
+    rc, d, fo, _, _, _ = oracle.update_ep(EpDims(1, n, E, M), p, nodes, f, op_flags=abi.EP_OP_STREAM)
+    assert rc == 0
+    dt = p.tDel / p.numEpSteps
+    iso = prof.mean(axis=1)
+    # rigidity in GV of a proton with momentum p (units of m_p c)
+    R = (g["pgrid"] * K.MP * K.C * K.C / K.GEV)
+    r_au = nodes["rmag"][0] * p.rScale
+    for e in range(E):
+        lam = p.lamo * R[e] ** (1.0 / 3.0) * r_au ** 0.0
+        tau = lam / g["vgrid"][e]
+        assert 0.05 < np.exp(-dt / tau).min() < 0.95       # partially relaxed
+        expect = iso[e] + (prof[e][None, :] - iso[e]) * np.exp(-dt / tau)[:, None]
+        np.testing.assert_allclose(fo[0, :, e, :], expect, rtol=1e-11)
+
+
+def focusing_case(a_sign=1.0):
+    n, E, M = 5, 3, 8
+    ds = 0.01
+    grad = 0.3 * a_sign                                  # d ln B / ds [1/AU]
+    bm = 5.0 * np.exp(grad * ds * np.arange(n))
+    nodes = line_nodes(n, ds, bmag=bm)
+    nodes["dlnn"][0, 1:] = 1.0e-4
+    nodes["dlnb"][0, 1:] = -2.0e-4
+    nodes["dupar"][0, 1:] = 3.0e-6
+    return n, E, M, ds, grad, nodes
+
+
+def test_focusing_isotropic_closed_form():
+    n, E, M, ds, grad, nodes = focusing_case()
+    p = base_params(tDel=2.0e-3, lamo=1.0)
+    g = oracle.build_grids(p, E, M)
+    F = 0.7
+    f = np.full((1, n, E, M), F)
+    rc, d, fo, _, _, _ = oracle.update_ep(EpDims(1, n, E, M), p, nodes, f, op_flags=abi.EP_OP_FOCUS)
+    assert rc == 0 and d["max_subcycles_focusing"] == 1
+    dt = p.tDel / p.numEpSteps
+    Dt = p.tDel                                        # MHD differences span one global step
+    mu, h = g["mugrid"], g["dmu"]
+    for k in range(1, n - 1):
+        divV = -nodes["dlnn"][0, k] / Dt               # div V = -dln n/dt
+        bbgradV = (nodes["dlnb"][0, k] - nodes["dlnn"][0, k]) / Dt  # b b : grad V = dln(B/n)/dt
+        for e in range(E):
+            v = g["vgrid"][e]
+            a = -v * grad - 2.0 * (nodes["dupar"][0, k] / Dt) / v
+            b = divV - 3.0 * bbgradV
+            # mudot = (1-mu^2)/2 (a + b mu) is cubic in mu: its central difference is
+            # mudot' + h^2/6 mudot''' with mudot''' = -3b
+            dmudot = 0.5 * (b - 2 * a * mu - 3 * b * mu ** 2) - b * h * h / 2.0
+            expect = F - dt * F * dmudot
+            np.testing.assert_allclose(fo[0, k, e, 1:M - 1], expect[1:M - 1], rtol=1e-12)
+            # wall cells: only the inner face carries flux
+            vel = 0.5 * (1 - mu ** 2) * (a + b * mu)
+            assert fo[0, k, e, 0] == pytest.approx(F - dt * F * 0.5 * (vel[0] + vel[1]) / h, rel=1e-12)
+
+
+def test_focusing_conserves_and_focuses_outward():
+    n, E, M, ds, grad, nodes = focusing_case(a_sign=-1.0)   # B decreasing outward
+    p = base_params(tDel=0.5, lamo=1.0)
+    rng = np.random.default_rng(1)
+    f = rng.uniform(0.5, 1.5, (1, n, E, M))
+    rc, d, fo, _, _, _ = oracle.update_ep(EpDims(1, n, E, M), p, nodes, f, op_flags=abi.EP_OP_FOCUS)
+    assert rc == 0 and d["max_subcycles_focusing"] > 1
+    np.testing.assert_allclose(fo.sum(axis=3)[0, 1:], f.sum(axis=3)[0, 1:], rtol=1e-13)
+    g = oracle.build_grids(p, E, M)
+    m1_before = (f[0, 1:-1] * g["mugrid"]).sum(axis=2)
+    m1_after = (fo[0, 1:-1] * g["mugrid"]).sum(axis=2)
+    assert np.all(m1_after > m1_before)                # focusing toward mu = +1
+
+
+@pytest.mark.parametrize("damp,dlnn", [(0.0, 2.0e-3), (0.4, -2.0e-3)])
+def test_adiabatic_change_shift(damp, dlnn):
+    n, E, M = 4, 30, 6
+    p = base_params(tDel=0.2, dampCooling=damp, eMin=0.1, eMax=1000.0)
+    nodes = line_nodes(n, 0.01)
+    dlnb = 1.0e-3
+    nodes["dlnn"][0, 1:] = dlnn
+    nodes["dlnb"][0, 1:] = dlnb
+    g = oracle.build_grids(p, E, M)
+    alpha, beta = -3.0, -0.05
+    lnf = alpha + beta * np.arange(E)
+    f = np.broadcast_to(np.exp(lnf)[None, None, :, None], (1, n, E, M)).copy()
+    rc, d, fo, _, _, _ = oracle.update_ep(EpDims(1, n, E, M), p, nodes, f, op_flags=abi.EP_OP_CHANGE)
+    assert rc == 0
+    N = d["max_subcycles_change"]
+    Dt = p.tDel
+    dt = p.tDel / p.numEpSteps
+    mu = g["mugrid"]
+    divV = -dlnn / Dt
+    bbgradV = (dlnb - dlnn) / Dt
+    # d ln p / dt = -(1-mu^2)/2 (div V - bb:gradV) - mu^2 bb:gradV   (no parallel
+    # acceleration here); cooling (div V > 0) is damped by (1 - dampCooling)
+    rate = -(1 - mu ** 2) / 2 * (divV - bbgradV) - mu ** 2 * bbgradV
+    if divV > 0:
+        rate = rate * (1.0 - damp)
+    for k in range(1, n):
+        for m in range(M):
+            c = rate[m] * dt / g["dlnp"]               # cells travelled in ln p
+            e = np.arange(E)
+            inner = (e >= N + 1) & (e <= E - 2 - N)
+            expect = alpha + beta * (e - c)
+            np.testing.assert_allclose(np.log(fo[0, k, inner, m]), expect[inner], rtol=1e-12, atol=1e-12)
+
+
+def test_thinning_hand_cases():
+    p = base_params(dsh_min=5.0e-5)
+    s = np.array([0.0, 1.0, 1.0 + 1e-6, 2.0, 3.0, 3.0 + 2e-6, 4.0])
+    z = np.zeros_like(s)
+    size, lst, ref, dsl, dsil, dmin = oracle.get_stream_list(p, s, z, z)
+    assert size == 5
+    assert list(lst) == [0, 2, 3, 5, 6]
+    assert list(ref) == [0, 2, 2, 3, 5, 5, 6]
+    assert dmin == pytest.approx(0.999998, rel=1e-12)
+    assert dsl[1] == pytest.approx(1.0, rel=1e-12)
+    # a run of close nodes collapses onto the next kept node
+    s = np.array([0.0, 1.0, 1.0 + 1e-6, 1.0 + 2e-6, 2.0])
+    z = np.zeros_like(s)
+    size, lst, ref, _, _, _ = oracle.get_stream_list(p, s, z, z)
+    assert list(lst) == [0, 3, 4]
+    assert list(ref) == [0, 3, 3, 3, 4]
+
+
+def test_mfp_b_form():
+    p = base_params(mfpType=2, lamo=0.3, mfpBPower=0.75, mhdBAu=5.0e-5)
+    rig = 2.0 ** (1.0 / 3.0)
+    for b in (1.0e-6, 3.0e-4):
+        expect = 0.3 * rig * (p.mhdBsAu / b) ** 0.75
+        assert oracle.mfp(p, rig, 1.0, b) == pytest.approx(expect, rel=1e-14)
+
+
+def multilinear(rr, th, ph):
+    return 2.0 + 0.3 * rr - 0.5 * th + 0.1 * ph + 0.02 * rr * th - 0.01 * rr * ph + 0.04 * th * ph \
+        + 0.003 * rr * th * ph
+
+
+def test_mas_trilinear_exact_for_multilinear_fields():
+    rng = np.random.default_rng(3)
+    rm = np.sort(rng.uniform(1.0, 20.0, 17)).astype(np.float32)
+    tm = np.sort(rng.uniform(0.0, math.pi, 13)).astype(np.float32)
+    pm = np.sort(rng.uniform(0.0, 2 * math.pi, 11)).astype(np.float32)
+    P, T, R = np.meshgrid(pm.astype(float), tm.astype(float), rm.astype(float), indexing="ij")
+    field = multilinear(R, T, P).astype(np.float32).ravel()
+    fields = [field] * 7
+    meshes = [rm, tm, pm] * 7
+    # query points inside the mesh, given as Cartesian node positions (AU / rScale)
+    n = 50
+    rq = rng.uniform(rm[1], rm[-2], n)
+    tq = rng.uniform(tm[1], tm[-2], n)
+    pq = rng.uniform(pm[1], pm[-2], n)
+    rscale = 0.01
+    rau = rq * K.RSAU
+    x = rau * np.sin(tq) * np.cos(pq) / rscale
+    y = rau * np.sin(tq) * np.sin(pq) / rscale
+    z = rau * np.cos(tq) / rscale
+    rmag = np.sqrt(x * x + y * y + z * z)
+    out = oracle.mas_interp(rscale, x, y, z, rmag, (fields, meshes), (fields, meshes), 0.3, 0.0, 25.0, 0.0)
+    # nodal values carry FP32 rounding (~6e-8 relative)
+    exact = multilinear(rq, tq, pq)
+    np.testing.assert_allclose(out["density"] / ((K.MAS_RHO_NORM / K.MP) / 1.0), exact, rtol=5e-7)
+    np.testing.assert_allclose(out["btheta"] / K.MAS_B_CONVERT, exact, rtol=5e-7)
+
+
+def test_mas_time_blend_and_clamp():
+    rm = np.array([1.0, 2.0, 4.0], np.float32)
+    tm = np.array([0.0, 1.0, 2.0, 3.2], np.float32)
+    pm = np.array([0.0, 3.0, 6.3], np.float32)
+    f0 = np.full(3 * 4 * 3, 1.0, np.float32)
+    f1 = np.full(3 * 4 * 3, 3.0, np.float32)
+    f0[0::3] = 5.0      # value at r = rm[0] on level 0
+    lv0 = ([f0] * 7, [rm, tm, pm] * 7)
+    lv1 = ([f1] * 7, [rm, tm, pm] * 7)
+    rscale = 0.01
+    # r below the mesh clamps to alpha = 0 (the first radius)
+    r_rs = 0.5
+    th, ph = 1.5, 1.0
+    rau = r_rs * K.RSAU
+    x = np.array([rau * math.sin(th) * math.cos(ph) / rscale])
+    y = np.array([rau * math.sin(th) * math.sin(ph) / rscale])
+    z = np.array([rau * math.cos(th) / rscale])
+    rmag = np.sqrt(x * x + y * y + z * z)
+    s = 0.25
+    out = oracle.mas_interp(rscale, x, y, z, rmag, lv0, lv1, s, 0.0, 25.0, 0.0)
+    assert out["density"][0] / ((K.MAS_RHO_NORM / K.MP) / 1.0) == pytest.approx((1 - s) * 5.0 + s * 3.0,
+                                                                               rel=1e-14)
