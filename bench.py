@@ -45,6 +45,18 @@ def make_params(c):
         boundaryFunctEcutoff=1.0e30)
 
 
+def bench_nodes(c):
+    """Parker spiral (400 km/s, B_r 2.5 nT at 1 AU, 25-day rotation) with a
+    smooth seeded 10% perturbation (correlation ~20 nodes), 2% spacing
+    jitter and a Gaussian CME shell at 0.5 AU (1500 km/s, density and field
+    x4, 60 deg wide, 0.05 AU thick)."""
+    return workloads.parker_nodes(
+        c["rows"], c["cols"], c["N"], tdel_days=1.0 / 64, flow_cms=400e5, b_au=2.5e-5,
+        omega=2 * math.pi * K.TAU / (25 * 86400.0), rscale=0.00465047, seed=20261203, perturb=0.1,
+        smooth=20, jitter=0.02,
+        cme=dict(r_au=0.5, width_au=0.05, speed_cms=1500e5, dens=4.0, ang_deg=60.0))
+
+
 def cpu_baseline(c, p, nodes, budget_s=20.0):
     """The FP64 oracle (OpenMP over streams) on a bounded stream subset of
     the same workload, scaled per cell-update."""
@@ -84,9 +96,7 @@ def main():
         sys.exit(2)
     c = CONFIGS[args.config]
     p = make_params(c)
-    nodes = workloads.parker_nodes(c["rows"], c["cols"], c["N"], tdel_days=1.0 / 64, flow_cms=400e5,
-                                   b_au=2.5e-5, omega=2 * math.pi * K.TAU / (25 * 86400.0),
-                                   rscale=0.00465047, seed=20261203, perturb=0.15, jitter=0.05)
+    nodes = bench_nodes(c)
     dims = workloads.dims_of(nodes, c["E"], c["M"])
     sol = runtime.EpSolver(dims, p)
     if args.block or args.halo:

@@ -117,6 +117,16 @@ class EpDiag(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class EpObservers(C.Structure):
+    _fields_ = [
+        ("num", C.c_int32),
+        ("r_au", _DP),
+        ("theta", _DP),
+        ("phi", _DP),
+        ("idw_p", C.c_double),
+    ]
+
+
 class EpMasParams(C.Structure):
     _fields_ = [
         ("s_cor", C.c_double),
@@ -174,4 +184,6 @@ EXPORTED_SYMBOLS = (
     "ep_finalize",
     "ep_set_tuning",
     "ep_selftest_dpp",
+    "ep_observe",
+    "ep_stream_flux",
 )

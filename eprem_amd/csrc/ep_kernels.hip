@@ -447,8 +447,14 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
   const double bcv = bc ? D.seed3[(size_t)task * 3 + k] : 0.0;
   const double invM = 1.0 / M;
   __shared__ double edge[2][16][M];
-  __shared__ double del[M];
-  if (tid < M) del[tid] = D.del_fac[(size_t)task * M + tid];
+  // coef[m] = ds_i_multiplier * del_fac[m]: the product the reference forms
+  // every subcycle is loop-invariant, so it is formed once per pass.
+  double coef[M];
+  {
+    const double *__restrict__ del = D.del_fac + (size_t)task * M;
+#pragma unroll
+    for (int m = 0; m < M; ++m) coef[m] = ((m >= MPOS) ? dsm_pos : dsm_neg) * del[m];
+  }
 
   for (int j = 0; j < nsub; ++j) {
     if (bc) {
@@ -465,11 +471,6 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
       for (int m = 0; m < MPOS; ++m) edge[b][w][m] = f[m];
     }
     __syncthreads();
-    // Re-read the uniform del_fac row from LDS every subcycle (an opaque
-    // zero offset stops the compiler from hoisting it into M*2 VGPRs).
-    int z = 0;
-    asm volatile("" : "+v"(z));
-    const double *delz = del + z;
     double sum = 0.0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
@@ -478,11 +479,11 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
       if (m >= MPOS) {
         const double old = (w > 0) ? edge[b][w - 1][m] : fm;
         const double nb = dpp_shr1(fm, old);
-        fn = fm + (dsm_pos * delz[m]) * (nb - fm);
+        fn = fm + coef[m] * (nb - fm);
       } else {
         const double old = (w < nw - 1) ? edge[b][w + 1][m] : fm;
         const double nb = dpp_shl1(fm, old);
-        fn = fm + (dsm_neg * delz[m]) * (fm - nb);
+        fn = fm + coef[m] * (fm - nb);
       }
       f[m] = fn;
       sum = sum + fn;
@@ -553,18 +554,22 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
 // ------------------------------------------------------------------------
 template <int M>
 __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
-  const int s = blockIdx.y;
-  const int node = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int w = threadIdx.x >> 6;
+  // blockIdx.x = (stream, group of 4 energies), blockIdx.y = node chunk:
+  // the inner chunks, where focusing needs the most subcycles, are
+  // dispatched first.
+  const int groups = (D.E + 3) >> 2;
+  const int s = blockIdx.x / groups;
+  const int e = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
+  const int node = blockIdx.y * 64 + (threadIdx.x & 63);
   const int n = D.n_active[s];
-  if (node < 1 || node >= n) return;
+  if (e >= D.E || node < 1 || node >= n) return;
   const size_t i = (size_t)s * D.Npad + node;
   const int nsub = D.nsub_f[i];
   const double dtsub = D.dtsub_f[i];
   const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], b = D.f_b[i];
   const double dmu_i = 1.0 / D.dmu;
   const double half = 0.5, one = 1.0, two = 2.0;
-  for (int e = w; e < D.E; e += 4) {
+  {
     const int task = s * D.E + e;
     double *fp = fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad + node;
     const double v = D.vgrid[e];
@@ -615,11 +620,13 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
 // ------------------------------------------------------------------------
 template <int EMAX>
 __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
-  const int s = blockIdx.y;
-  const int node = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int w = threadIdx.x >> 6;
+  // blockIdx.x = (stream, group of 4 mu cells), blockIdx.y = node chunk
+  const int groups = (D.M + 3) >> 2;
+  const int s = blockIdx.x / groups;
+  const int m = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
+  const int node = blockIdx.y * 64 + (threadIdx.x & 63);
   const int n = D.n_active[s];
-  if (node < 1 || node >= n) return;
+  if (m >= D.M || node < 1 || node >= n) return;
   const int E = D.E;
   const size_t i = (size_t)s * D.Npad + node;
   const int nsub = D.nsub_c[i];
@@ -629,7 +636,7 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   const double dlnp_i = 1.0 / D.dlnp;
   const double half = 0.5, one = 1.0;
   const size_t slab = (size_t)D.M * D.Npad;
-  for (int m = w; m < D.M; m += 4) {
+  {
     const double mu = D.mugrid[m];
     const double a = -mu * DuParDt;
     const double bb = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
@@ -818,7 +825,8 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
 
 #define EP_FOCUS_CASE(MM)                                                                     \
   case MM:                                                                                    \
-    hipLaunchKernelGGL((k_focus<MM>), dim3((D.N + 63) / 64, D.S), dim3(256), 0, st, D, sub); \
+    hipLaunchKernelGGL((k_focus<MM>), dim3(D.S * ((D.E + 3) / 4), (D.N + 63) / 64), dim3(256), 0, st, \
+                       D, sub);                                                               \
     break;
 
 hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
@@ -838,7 +846,7 @@ hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
 }
 
 hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
-  const dim3 grid((D.N + 63) / 64, D.S), block(256);
+  const dim3 grid(D.S * ((D.M + 3) / 4), (D.N + 63) / 64), block(256);
   if (D.E <= 8)
     hipLaunchKernelGGL((k_change<8>), grid, block, 0, st, D, sub);
   else if (D.E <= 16)

@@ -54,9 +54,16 @@ struct State {
   std::vector<void *> mas_allocs;
   double *mas_out = nullptr;   // 7 x S x Npad
   double *mas_prev = nullptr;  // 10 x S x Npad
+  // point observers and stream flux (allocated on first use)
+  double *obs_buf = nullptr, *flux_buf = nullptr;
+  size_t obs_cap = 0, flux_cap = 0;
   void mas_free() {
-    for (void *p : mas_allocs) hipFree(p);
+    for (void *p : mas_allocs) (void)hipFree(p);
     mas_allocs.clear();
+    if (obs_buf) (void)hipFree(obs_buf);
+    if (flux_buf) (void)hipFree(flux_buf);
+    obs_buf = flux_buf = nullptr;
+    obs_cap = flux_cap = 0;
     mas[0] = MasLevel();
     mas[1] = MasLevel();
     mas_out = nullptr;

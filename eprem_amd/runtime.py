@@ -52,6 +52,8 @@ def lib():
         L.ep_last_error.argtypes = [C.c_char_p, C.c_size_t]
         L.ep_finalize.argtypes = []
         L.ep_selftest_dpp.argtypes = [DP, DP, DP]
+        L.ep_observe.argtypes = [C.POINTER(abi.EpObservers), DP, DP]
+        L.ep_stream_flux.argtypes = [DP]
         for name in abi.EXPORTED_SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -167,6 +169,23 @@ class EpSolver:
                                    C.byref(ms)))
         outs["device_ms"] = ms.value
         return outs
+
+    def observe(self, r_au, theta, phi, idw_p=3.0):
+        """Distribution [O, E, M] and omnidirectional flux [O, E] at point
+        observers (code units)."""
+        r, t, ph = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (r_au, theta, phi))
+        o = len(r)
+        ob = abi.EpObservers(o, dptr(r), dptr(t), dptr(ph), float(idw_p))
+        E, M = self.dims.num_energies, self.dims.num_mu
+        dist, flux = np.zeros((o, E, M)), np.zeros((o, E))
+        _check(lib().ep_observe(C.byref(ob), dptr(dist), dptr(flux)))
+        return dist, flux
+
+    def stream_flux(self):
+        d = self.dims
+        flux = np.zeros((d.num_streams, d.num_nodes, d.num_energies))
+        _check(lib().ep_stream_flux(dptr(flux)))
+        return flux
 
     def close(self):
         lib().ep_finalize()

@@ -52,14 +52,19 @@ def cube_directions(rows, cols):
 
 def parker_nodes(rows, cols, n_nodes, tdel_days=1.0 / 64, flow_cms=400.0e5, density_au=8.3,
                  b_au=2.5e-5, omega=None, rscale=0.005, seed=None, jitter=0.0, perturb=0.0,
-                 thin_frac=0.0):
+                 thin_frac=0.0, smooth=0, cme=None):
     """Node SoA arrays [S, N] for a Parker-spiral snapshot.
 
     perturb > 0 multiplies V, n and B by seeded smooth random factors
     (relative amplitude `perturb`) that differ between the current and the
     previous time level; jitter > 0 moves nodes by a random fraction of the
     spacing; thin_frac > 0 pulls that fraction of node pairs to within the
-    thinning distance dsh_min = 5e-5 AU.
+    thinning distance dsh_min = 5e-5 AU.  smooth > 0 low-pass filters the
+    perturbations over that many nodes (a physical, smooth field along the
+    stream).  cme = dict(r_au=..., width_au=..., speed_cms=..., dens=...,
+    ang_deg=..., theta0=..., phi0=...) adds a Gaussian CME shell that moved
+    from r_au - speed*tDel to r_au during the step, compressing density and
+    field by up to `dens` and raising the radial speed inside the shell.
     """
     rng = np.random.default_rng(seed)
     if omega is None:
@@ -97,6 +102,13 @@ def parker_nodes(rows, cols, n_nodes, tdel_days=1.0 / 64, flow_cms=400.0e5, dens
         def fac(amp):
             if amp <= 0:
                 return np.ones(n_nodes), np.ones(n_nodes)
+            if smooth > 0:
+                ker = np.exp(-0.5 * (np.arange(-3 * smooth, 3 * smooth + 1) / smooth) ** 2)
+                ker /= np.sqrt((ker ** 2).sum())
+                lp = lambda v: np.convolve(v, ker, mode="same")
+                base = lp(rng.normal(0, amp, n_nodes))
+                new = np.exp(np.clip(base + 0.3 * lp(rng.normal(0, amp, n_nodes)), -1, 1))
+                return new, np.exp(np.clip(base, -1, 1))
             base = np.cumsum(rng.normal(0, amp, n_nodes)) / np.sqrt(np.arange(1, n_nodes + 1))
             new = np.exp(np.clip(base + rng.normal(0, amp, n_nodes), -1, 1))
             old = np.exp(np.clip(base, -1, 1))
@@ -105,6 +117,20 @@ def parker_nodes(rows, cols, n_nodes, tdel_days=1.0 / 64, flow_cms=400.0e5, dens
         fv, fv_old = fac(perturb)
         fn, fn_old = fac(perturb)
         fb, fb_old = fac(perturb)
+        if cme is not None:
+            # angular profile around the CME axis, radial Gaussian shell
+            th0, ph0 = cme.get("theta0", 0.5 * math.pi), cme.get("phi0", math.pi)
+            cosang = np.sin(th) * math.sin(th0) * np.cos(np.arctan2(y, x) - ph0) + np.cos(th) * math.cos(th0)
+            ang = np.arccos(np.clip(cosang, -1.0, 1.0))
+            wang = np.exp(-0.5 * (ang / math.radians(cme.get("ang_deg", 60.0) / 2)) ** 2)
+            rc, wr = cme["r_au"], cme["width_au"]
+            rc_old = rc - cme["speed_cms"] / K.C * tdel * 1.0
+            bump = wang * np.exp(-0.5 * ((rau - rc) / wr) ** 2)
+            bump_old = wang * np.exp(-0.5 * ((rau_old - rc_old) / wr) ** 2)
+            dv = (cme["speed_cms"] - flow_cms) / flow_cms
+            fv, fv_old = fv * (1 + dv * bump), fv_old * (1 + dv * bump_old)
+            fn, fn_old = fn * (1 + (cme["dens"] - 1) * bump), fn_old * (1 + (cme["dens"] - 1) * bump_old)
+            fb, fb_old = fb * (1 + (cme["dens"] - 1) * bump), fb_old * (1 + (cme["dens"] - 1) * bump_old)
         vr, vr_old = us * fv, us * fv_old
         dens, dens_old = density_au / rau ** 2 * fn, density_au / rau_old ** 2 * fn_old
         br = bs / rau ** 2 * fb

@@ -223,6 +223,29 @@ typedef struct ep_mas_out {
 int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev,
                   ep_mas_out *out, double *device_ms);
 
+/* ---------------- observers and flux -------------------------------- */
+
+/* Point observers (config.obsR/obsTheta/obsPhi, idw_p). */
+typedef struct ep_observers {
+  int32_t num;                 /* 1..256                                 */
+  const double *r_au;          /* radius [AU]                            */
+  const double *theta, *phi;   /* colatitude, longitude [rad]            */
+  double idw_p;                /* inverse-distance weight power          */
+} ep_observers;
+
+/* Distribution and omnidirectional flux at each observer for the current
+ * f and node positions: per stream the projection onto the observer
+ * sphere (pointObsProjectedStreamData + findIntersection,
+ * src/search.c:113-268), the inverse-distance blend over streams
+ * (writePointObserverDataNetCDF, src/unifiedOutput.c:846-935) and
+ * flux = 2 E <f>_mu (computeFlux, src/unifiedOutput.c:153-180).
+ * dist [num][E][M] and flux [num][E] (either may be NULL), code units. */
+int ep_observe(const ep_observers *obs, double *dist, double *flux);
+
+/* computeFlux (src/unifiedOutput.c:153-180) on every node:
+ * flux [num_streams][num_nodes][num_energies], code units. */
+int ep_stream_flux(double *flux);
+
 /* ---------------- misc --------------------------------------------- */
 int ep_last_error(char *buf, size_t len);
 int ep_finalize(void);

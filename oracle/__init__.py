@@ -43,6 +43,15 @@ def lib():
         L.or_mas_interp.argtypes = ([C.c_int, C.c_double, DP, DP, DP, DP, FPA, MPA, IP, FPA, MPA, IP,
                                      C.c_double, C.c_double, C.c_double, C.c_double] + [DP] * 10 +
                                     [DP] * 7)
+        L.or_observers.argtypes = ([C.POINTER(EpDims), IP, C.c_double, C.c_double, C.c_int] +
+                                   [DP] * 3 + [DP] * 4 + [DP, DP, DP, DP])
+        L.or_stream_flux.argtypes = [C.POINTER(EpDims), DP, DP, DP]
+        L.or_update_mhd.argtypes = ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, DP, DP, DP, DP, IP,
+                                     C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_float)), IP,
+                                     C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_float)), IP,
+                                     C.c_double, C.c_double, C.c_double, C.c_double] + [DP] * 10 +
+                                    [DP] * 11)
         L.or_run_parker.argtypes = [C.POINTER(SimConfig), C.POINTER(SimOutput), C.c_int]
         L.or_run_parker.restype = C.c_int
         _lib = L
@@ -211,3 +220,26 @@ def mas_interp(rScale, x, y, z, rmag, level0, level1, s_cor, phi_offset, mas_rad
                         phi_offset, mas_radial_max, mas_vmin, *[dptr(a) for a in pv],
                         *[dptr(a) for a in outs])
     return dict(zip(("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density"), outs))
+
+
+def observers(dims, nodes, f, egrid, rscale, r_au, theta, phi, idw_p=3.0, n_active=None):
+    S, N = dims.num_streams, dims.num_nodes
+    E, M = dims.num_energies, dims.num_mu
+    arr = {k: np.ascontiguousarray(nodes[k], dtype=np.float64).reshape(S * N) for k in ("x", "y", "z", "rmag")}
+    r, t, ph = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (r_au, theta, phi))
+    fo = np.ascontiguousarray(f, dtype=np.float64)
+    eg = np.ascontiguousarray(egrid, dtype=np.float64)
+    o = len(r)
+    dist, flux = np.zeros((o, E, M)), np.zeros((o, E))
+    na = None if n_active is None else np.ascontiguousarray(n_active, dtype=np.int32)
+    lib().or_observers(C.byref(dims), iptr(na), rscale, idw_p, o, dptr(r), dptr(t), dptr(ph),
+                       dptr(arr["x"]), dptr(arr["y"]), dptr(arr["z"]), dptr(arr["rmag"]), dptr(fo),
+                       dptr(eg), dptr(dist), dptr(flux))
+    return dist, flux
+
+
+def stream_flux(dims, f, egrid):
+    out = np.zeros((dims.num_streams, dims.num_nodes, dims.num_energies))
+    lib().or_stream_flux(C.byref(dims), dptr(np.ascontiguousarray(f, dtype=np.float64)),
+                         dptr(np.ascontiguousarray(egrid, dtype=np.float64)), dptr(out))
+    return out

@@ -806,6 +806,156 @@ void or_mas_interp(int n, double rScale, const double *x, const double *y,
 }
 
 /* ------------------------------------------------------------------ */
+/* updateMhd (src/flow.c:33-154)                                       */
+/* ------------------------------------------------------------------ */
+void or_update_mhd(int n, int mode, double rScale, double mhdUs, double mhdNsAu,
+                   double mhdBsAu, double omegaSun, const double *x, const double *y,
+                   const double *z, const double *rmag, const int *is_first,
+                   const float *const fields0[7], const float *const meshes0[21],
+                   const int *dims0, const float *const fields1[7],
+                   const float *const meshes1[21], const int *dims1, double s_cor,
+                   double phi_offset, double mas_radial_max, double mas_vmin,
+                   const double *rold_x, const double *rold_y, const double *rold_z,
+                   const double *old_br, const double *old_bt, const double *old_bp,
+                   const double *old_vr, const double *old_vt, const double *old_vp,
+                   const double *old_d, double *br, double *bt, double *bp, double *vr,
+                   double *vt, double *vp, double *dens, double *bmag, double *dupar,
+                   double *dlnb, double *dlnn) {
+  int i;
+  if (mode == 1)
+    or_mas_interp(n, rScale, x, y, z, rmag, fields0, meshes0, dims0, fields1, meshes1, dims1,
+                  s_cor, phi_offset, mas_radial_max, mas_vmin, rold_x, rold_y, rold_z, old_br,
+                  old_bt, old_bp, old_vr, old_vt, old_vp, old_d, br, bt, bp, vr, vt, vp, dens);
+  for (i = 0; i < n; i++) {
+    const double bmag_old = sqrt(old_br[i] * old_br[i] + old_bt[i] * old_bt[i] +
+                                 old_bp[i] * old_bp[i]);
+    if (mode != 1) {
+      /* mhdDensity, mhdV, mhdB with MHD_DEFAULT (src/flow.c:241-350) */
+      double rr = rmag[i] * rScale;
+      double theta = acos(z[i] / rmag[i]);
+      dens[i] = mhdNsAu / (rr * rr);
+      vr[i] = mhdUs;
+      vt[i] = 0.0;
+      vp[i] = 0.0;
+      br[i] = mhdBsAu / (rr * rr);
+      bt[i] = 0.0;
+      bp[i] = -1.0 * rr * br[i] * (omegaSun / (vr[i] + OR_VERYSMALL)) * sin(theta);
+    }
+    bmag[i] = sqrt(br[i] * br[i] + bt[i] * bt[i] + bp[i] * bp[i]);
+    if (is_first[i]) {
+      dupar[i] = 0.0;
+      dlnb[i] = 0.0;
+      dlnn[i] = 0.0;
+    } else {
+      dupar[i] = (vr[i] - old_vr[i]) * br[i] + (vt[i] - old_vt[i]) * bt[i] +
+                 (vp[i] - old_vp[i]) * bp[i];
+      dupar[i] /= bmag[i];
+      dlnb[i] = log(bmag[i] / bmag_old);
+      dlnn[i] = log(dens[i] / old_d[i]);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Point observers and flux                                            */
+/* ------------------------------------------------------------------ */
+static double dot3(double ax, double ay, double az, double bx, double by, double bz) {
+  return ax * bx + ay * by + az * bz;
+}
+
+/* findIntersection (src/search.c:234-268) */
+static double find_intersection(const double *x0, const double *x1, double r) {
+  double dx = x1[0] - x0[0], dy = x1[1] - x0[1], dz = x1[2] - x0[2];
+  double r0square = x0[0] * x0[0] + x0[1] * x0[1] + x0[2] * x0[2];
+  double dx2 = dot3(dx, dy, dz, dx, dy, dz);
+  double b = dot3(x0[0], x0[1], x0[2], dx, dy, dz) / dx2;
+  double c = (r * r - r0square) / dx2;
+  return -b + sqrt(b * b + c);
+}
+
+void or_observers(const ep_dims *d, const int *n_active, double rScale, double idw_p, int nobs,
+                  const double *obs_r, const double *obs_theta, const double *obs_phi,
+                  const double *x, const double *y, const double *z, const double *rmag,
+                  const double *f, const double *egrid, double *dist_out, double *flux_out) {
+  const int S = d->num_streams, N = d->num_nodes, E = d->num_energies, M = d->num_mu;
+  const size_t EM = (size_t)E * M;
+  double *proj = (double *)malloc(sizeof(double) * EM);
+  int o, s, e, m;
+  for (o = 0; o < nobs; o++) {
+    double *td = dist_out + (size_t)o * EM;
+    double weightSum = 0.0, rs, th, ph, cx, cy, cz;
+    for (e = 0; e < (int)EM; e++) td[e] = 0.0;
+    /* observer position: sphToCartPos of (r/rScale, theta, phi) after the
+       phi wrap (src/unifiedOutput.c:832-851, src/geometry.c:53-60) */
+    th = obs_theta[o];
+    ph = obs_phi[o];
+    if (ph < 0.0) ph += 2.0 * OR_PI;
+    if (ph > (2.0 * OR_PI)) ph -= 2.0 * OR_PI;
+    rs = obs_r[o] / rScale;
+    cx = rs * sin(th) * cos(ph);
+    cy = rs * sin(th) * sin(ph);
+    cz = rs * cos(th);
+    for (s = 0; s < S; s++) {
+      const int n = n_active ? n_active[s] : N;
+      const size_t on = (size_t)s * N;
+      const double rSph = obs_r[o] / rScale;
+      int ss0 = 0, ss1 = n - 1, in = ss0, out = ss1;
+      double sfac, px, py, pz, distance, weight;
+      double x0[3], x1[3];
+      while ((rmag[on + ((in < ss1) ? (in + 1) : ss1)] < rSph) && (in < ss1)) in++;
+      while ((rmag[on + ((out > ss0) ? (out - 1) : ss0)] >= rSph) && (out > ss0)) out--;
+      x0[0] = x[on + in];
+      x0[1] = y[on + in];
+      x0[2] = z[on + in];
+      x1[0] = x[on + out];
+      x1[1] = y[on + out];
+      x1[2] = z[on + out];
+      sfac = (out > in) ? find_intersection(x0, x1, rSph) : 0.0;
+      px = x0[0] * (1.0 - sfac) + x1[0] * sfac;
+      py = x0[1] * (1.0 - sfac) + x1[1] * sfac;
+      pz = x0[2] * (1.0 - sfac) + x1[2] * sfac;
+      for (e = 0; e < E; e++)
+        for (m = 0; m < M; m++) {
+          const double fi = f[((on + in) * E + e) * M + m];
+          const double fo = f[((on + out) * E + e) * M + m];
+          if ((sfac >= 0.0) && (sfac <= 1.0))
+            proj[e * M + m] = fi * (1.0 - sfac) + fo * sfac;
+          else if (sfac < 0.0)
+            proj[e * M + m] = fi;
+          else
+            proj[e * M + m] = fo;
+        }
+      distance = sqrt((cx - px) * (cx - px) + (cy - py) * (cy - py) + (cz - pz) * (cz - pz));
+      weight = pow(distance, -1.0 * idw_p);
+      weightSum += weight;
+      for (e = 0; e < (int)EM; e++) td[e] += (weight * proj[e]);
+    }
+    for (e = 0; e < (int)EM; e++) td[e] /= weightSum;
+    for (e = 0; e < E; e++) {
+      double iso = 0.0;
+      for (m = 0; m < M; m++) iso += td[e * M + m];
+      iso /= M;
+      flux_out[(size_t)o * E + e] = 2.0 * egrid[e] * iso;
+    }
+  }
+  free(proj);
+}
+
+void or_stream_flux(const ep_dims *d, const double *f, const double *egrid, double *flux) {
+  const size_t SN = (size_t)d->num_streams * d->num_nodes;
+  const int E = d->num_energies, M = d->num_mu;
+  size_t i;
+  int e, m;
+  for (i = 0; i < SN; i++)
+    for (e = 0; e < E; e++) {
+      double iso = 0.0;
+      for (m = 0; m < M; m++) iso += f[(i * E + e) * M + m];
+      iso /= M;
+      flux[i * E + e] = 2.0 * egrid[e] * iso;
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Parker-wind host pipeline (src/eprem.c:426-744, masCouple = 0)      */
 /* ------------------------------------------------------------------ */
 typedef struct {
@@ -1090,47 +1240,40 @@ int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads) {
         st.rmag[i] = sqrt((rx * rx) + (ry * ry) + (rz * rz));
       }
 
-    /* updateMhd (src/flow.c:33-154) with the Parker model
-       (mhdDensity :241-275, mhdV :355-403, mhdB :280-350) */
-    for (s = 0; s < S; s++)
-      for (k = 1; k < L; k++) {
-        size_t i = (size_t)s * L + k;
-        double rr, theta, phi;
-        st.bmagOld[i] = st.bmag[i];
-        st.densOld[i] = st.dens[i];
-        st.vrOld[i] = st.vr[i];
-        st.vtOld[i] = st.vt[i];
-        st.vpOld[i] = st.vp[i];
-        rr = st.rmag[i] * c->rScale;
-        st.dens[i] = mhdNsAu / (rr * rr);
-        /* mhdV */
-        theta = acos(st.z[i] / st.rmag[i]);
-        phi = atan2(st.y[i], st.x[i]);
-        st.vr[i] = mhdUs;
-        st.vt[i] = 0.0;
-        st.vp[i] = 0.0;
+    /* updateMhd (src/flow.c:33-154) with the Parker model, through
+       or_update_mhd; then the Cartesian V used by moveNodes */
+    for (s = 0; s < S; s++) {
+      const size_t i0 = (size_t)s * L + 1;
+      const int na = L - 1;
+      double *ob = (double *)malloc(sizeof(double) * 7 * na);
+      int *first = (int *)calloc(na, sizeof(int));
+      first[0] = 1;
+      memcpy(ob + 0 * na, st.br + i0, sizeof(double) * na);
+      memcpy(ob + 1 * na, st.bt + i0, sizeof(double) * na);
+      memcpy(ob + 2 * na, st.bp + i0, sizeof(double) * na);
+      memcpy(ob + 3 * na, st.vr + i0, sizeof(double) * na);
+      memcpy(ob + 4 * na, st.vt + i0, sizeof(double) * na);
+      memcpy(ob + 5 * na, st.vp + i0, sizeof(double) * na);
+      memcpy(ob + 6 * na, st.dens + i0, sizeof(double) * na);
+      or_update_mhd(na, 0, c->rScale, mhdUs, mhdNsAu, mhdBsAu, omega, st.x + i0, st.y + i0,
+                    st.z + i0, st.rmag + i0, first, NULL, NULL, NULL, NULL, NULL, NULL, 0.0, 0.0,
+                    0.0, 0.0, NULL, NULL, NULL, ob, ob + na, ob + 2 * na, ob + 3 * na,
+                    ob + 4 * na, ob + 5 * na, ob + 6 * na, st.br + i0, st.bt + i0, st.bp + i0,
+                    st.vr + i0, st.vt + i0, st.vp + i0, st.dens + i0, st.bmag + i0,
+                    st.dupar + i0, st.dlnb + i0, st.dlnn + i0);
+      for (k = 0; k < na; k++) {
+        const size_t i = i0 + k;
+        const double theta = acos(st.z[i] / st.rmag[i]);
+        const double phi = atan2(st.y[i], st.x[i]);
         st.vvx[i] = st.vr[i] * sin(theta) * cos(phi) + st.vt[i] * cos(theta) * cos(phi) -
                     st.vp[i] * sin(phi);
         st.vvy[i] = st.vr[i] * sin(theta) * sin(phi) + st.vt[i] * cos(theta) * sin(phi) +
                     st.vp[i] * cos(phi);
         st.vvz[i] = st.vr[i] * cos(theta) - st.vt[i] * sin(theta);
-        /* mhdB */
-        st.br[i] = mhdBsAu / (rr * rr);
-        st.bt[i] = 0.0;
-        st.bp[i] = -1.0 * rr * st.br[i] * (omega / (st.vr[i] + OR_VERYSMALL)) * sin(theta);
-        st.bmag[i] = sqrt(st.br[i] * st.br[i] + st.bt[i] * st.bt[i] + st.bp[i] * st.bp[i]);
-        if (k == 1) {
-          st.dupar[i] = 0.0;
-          st.dlnb[i] = 0.0;
-          st.dlnn[i] = 0.0;
-        } else {
-          st.dupar[i] = (st.vr[i] - st.vrOld[i]) * st.br[i] + (st.vt[i] - st.vtOld[i]) * st.bt[i] +
-                        (st.vp[i] - st.vpOld[i]) * st.bp[i];
-          st.dupar[i] /= st.bmag[i];
-          st.dlnb[i] = log(st.bmag[i] / st.bmagOld[i]);
-          st.dlnn[i] = log(st.dens[i] / st.densOld[i]);
-        }
       }
+      free(ob);
+      free(first);
+    }
 
     /* particle computation (src/eprem.c:653-682) */
     if (t_global * OR_DAY >= epCalcStartTime) {

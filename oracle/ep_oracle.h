@@ -82,6 +82,42 @@ void or_mas_interp(int n, double rScale, const double *x, const double *y,
                    const double *prev_d, double *br, double *bt, double *bp,
                    double *vr, double *vt, double *vp, double *dens);
 
+/* updateMhd (src/flow.c:33-154) for a batch of n nodes.  mode 0 = the
+ * analytic wind (mhdDensity/mhdV/mhdB with MHD_DEFAULT, src/flow.c:241-403),
+ * mode 1 = MAS (or_mas_interp, same level arguments).  old_* hold the
+ * node's values before the update; is_first[i] != 0 marks the first node of
+ * a stream (its derivative terms are zeroed, src/flow.c:131-136).
+ * Outputs: new B (r, theta, phi), V (r, theta, phi), density, |B| and the
+ * derivative terms dupar, dlnb, dlnn. */
+void or_update_mhd(int n, int mode, double rScale, double mhdUs, double mhdNsAu,
+                   double mhdBsAu, double omegaSun, const double *x, const double *y,
+                   const double *z, const double *rmag, const int *is_first,
+                   const float *const fields0[7], const float *const meshes0[21],
+                   const int *dims0, const float *const fields1[7],
+                   const float *const meshes1[21], const int *dims1, double s_cor,
+                   double phi_offset, double mas_radial_max, double mas_vmin,
+                   const double *rold_x, const double *rold_y, const double *rold_z,
+                   const double *old_br, const double *old_bt, const double *old_bp,
+                   const double *old_vr, const double *old_vt, const double *old_vp,
+                   const double *old_d, double *br, double *bt, double *bp, double *vr,
+                   double *vt, double *vp, double *dens, double *bmag, double *dupar,
+                   double *dlnb, double *dlnn);
+
+/* Point observers: per stream the projection onto the observer sphere
+ * (pointObsProjectedStreamData + findIntersection, src/search.c:113-268),
+ * the inverse-distance blend over streams (src/unifiedOutput.c:846-935),
+ * and the omnidirectional flux 2 E <f>_mu of the blended distribution
+ * (computeFlux, src/unifiedOutput.c:153-180).  Observer radii in AU,
+ * angles in radians; node positions in rScale units; f in the reference
+ * layout.  dist_out [nobs][E][M], flux_out [nobs][E]. */
+void or_observers(const ep_dims *d, const int *n_active, double rScale, double idw_p, int nobs,
+                  const double *obs_r, const double *obs_theta, const double *obs_phi,
+                  const double *x, const double *y, const double *z, const double *rmag,
+                  const double *f, const double *egrid, double *dist_out, double *flux_out);
+
+/* computeFlux (src/unifiedOutput.c:153-180) for every node: flux [S][N][E]. */
+void or_stream_flux(const ep_dims *d, const double *f, const double *egrid, double *flux);
+
 /* ---------------- Parker-wind host pipeline ------------------------ */
 
 /* Inputs of the analytic-wind pipeline (src/eprem.c:426-744 with

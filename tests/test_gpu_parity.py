@@ -199,3 +199,64 @@ def test_ds_underflow_error():
     fg, fo, gd, od, _, _ = run_both(p, nodes, 8, 6)
     assert gd[0][0] == od[0][0] == abi.EP_ERR_DS_UNDERFLOW
     assert gd[0][1]["err_stream"] == od[0][1]["err_stream"] == 2
+
+
+def test_mas_interp_parity():
+    """masTriLinear/masWind on the device vs the oracle: nodes inside and
+    outside masRadialMax, two time levels blended by s_cor, phi offset."""
+    from eprem_amd import mas_synth
+    rng = np.random.default_rng(11)
+    r, t, ph = mas_synth.meshes(nr=40, nt=30, nphi=50, rmin=1.0, rmax=30.0)
+    lv0 = mas_synth.level(r, t, ph, cme_r=8.0, seed=3)
+    lv1 = mas_synth.level(r, t, ph, cme_r=12.0, seed=3)
+    # staggered meshes for some fields, as in MAS
+    m0 = list(lv0[1])
+    m0[0] = (r * 1.01).astype(np.float32)
+    m0[4] = (t + 0.01).astype(np.float32)
+    lv0 = (lv0[0], m0)
+    lv1 = (lv1[0], m0)
+    nodes = workloads.parker_nodes(1, 1, 60, tdel_days=1.0 / 512, rscale=0.00465047, seed=12, jitter=0.3)
+    p = params(rScale=0.00465047)
+    dims = workloads.dims_of(nodes, 4, 4)
+    prev = mas_synth.prev_from_nodes(nodes, p.rScale)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_nodes(nodes)
+        sol.mas_set_level(0, *lv0)
+        sol.mas_set_level(1, *lv1)
+        out = sol.interp_mhd(0.37, 0.8, 4.0, 50.0e5, prev)
+    finally:
+        sol.close()
+    S, N = nodes["x"].shape
+    ref = oracle.mas_interp(p.rScale, nodes["x"].ravel(), nodes["y"].ravel(), nodes["z"].ravel(),
+                            nodes["rmag"].ravel(), lv0, lv1, 0.37, 0.8, 4.0, 50.0e5,
+                            {k: v.ravel() for k, v in prev.items()})
+    inside = (nodes["rmag"].ravel() * p.rScale) <= 4.0 * (6.96e10 / 1.495978707e13)
+    assert 0 < inside.sum() < inside.size
+    for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density"):
+        np.testing.assert_allclose(out[k].ravel(), ref[k], rtol=1e-13, atol=0, err_msg=k)
+
+
+def test_observers_and_flux_parity():
+    """Point-observer projection, inverse-distance blend and flux on the
+    device equal the oracle's (same summation order over streams)."""
+    p = params()
+    nodes = workloads.parker_nodes(1, 1, 300, seed=13, perturb=0.1, smooth=10)
+    dims = workloads.dims_of(nodes, 10, 8)
+    g = oracle.build_grids(p, 10, 8)
+    f0 = workloads.seeded_f(p, g, nodes, noise=0.3, seed=4)
+    r_au = np.array([0.3, 1.0, 0.999, 5.0])
+    th = np.array([1.571, 0.4, 2.9, 1.0])
+    ph = np.array([3.142, -1.0, 6.5, 0.2])
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_nodes(nodes)
+        sol.upload_f(f0)
+        dist, flux = sol.observe(r_au, th, ph, idw_p=3.0)
+        sflux = sol.stream_flux()
+    finally:
+        sol.close()
+    d_o, f_o = oracle.observers(dims, nodes, f0, g["egrid"], p.rScale, r_au, th, ph, 3.0)
+    np.testing.assert_array_equal(dist, d_o)
+    np.testing.assert_array_equal(flux, f_o)
+    np.testing.assert_array_equal(sflux, oracle.stream_flux(dims, f0, g["egrid"]))

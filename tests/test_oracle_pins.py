@@ -266,3 +266,50 @@ def test_mas_time_blend_and_clamp():
     out = oracle.mas_interp(rscale, x, y, z, rmag, lv0, lv1, s, 0.0, 25.0, 0.0)
     assert out["density"][0] / ((K.MAS_RHO_NORM / K.MP) / 1.0) == pytest.approx((1 - s) * 5.0 + s * 3.0,
                                                                                rel=1e-14)
+
+
+def radial_streams(n, ds_au, dirs, rscale=1.0):
+    S = len(dirs)
+    out = {k: np.zeros((S, n)) for k in ("x", "y", "z", "rmag")}
+    r = 0.1 + ds_au * np.arange(n)
+    for s, (th, ph) in enumerate(dirs):
+        out["x"][s] = r * math.sin(th) * math.cos(ph) / rscale
+        out["y"][s] = r * math.sin(th) * math.sin(ph) / rscale
+        out["z"][s] = r * math.cos(th) / rscale
+        out["rmag"][s] = r / rscale
+    return out
+
+
+def test_observer_projection_on_a_radial_stream():
+    """A straight radial stream crosses the observer sphere between two
+    nodes; the projection is f linearly interpolated to that radius and, with
+    one stream, the inverse-distance blend returns it unchanged.  (The
+    observer sits off the stream: at zero distance the weight pow(0, -p) is
+    infinite and the reference's blend gives NaN.)"""
+    n, E, M = 30, 3, 4
+    th, ph = 1.1, 2.3
+    nodes = radial_streams(n, 0.05, [(th, ph)])
+    r = nodes["rmag"][0]
+    rng = np.random.default_rng(5)
+    a, b = rng.uniform(1, 2, (E, M)), rng.uniform(0.1, 0.2, (E, M))
+    f = (a[None] + b[None] * r[:, None, None])[None]           # linear in r
+    egrid = np.array([0.01, 0.1, 1.0])
+    robs = 0.1 + 0.05 * 12.3
+    dist, flux = oracle.observers(EpDims(1, n, E, M), nodes, f, egrid, 1.0, np.array([robs]),
+                                  np.array([th + 0.2]), np.array([ph - 0.1]), 3.0)
+    np.testing.assert_allclose(dist[0], a + b * robs, rtol=1e-12)
+    np.testing.assert_allclose(flux[0], 2 * egrid * (a + b * robs).mean(axis=1), rtol=1e-12)
+
+
+def test_observer_blend_of_identical_streams():
+    """Inverse-distance weights sum to one: identical streams give the
+    common distribution whatever the observer position."""
+    n, E, M = 20, 2, 3
+    dirs = [(0.5, 0.1), (1.2, 2.0), (2.0, 4.0), (1.6, 5.5)]
+    nodes = radial_streams(n, 0.1, dirs)
+    prof = np.random.default_rng(6).uniform(0.5, 1.5, (E, M))
+    f = np.broadcast_to(prof, (len(dirs), n, E, M)).copy()
+    dist, flux = oracle.observers(EpDims(len(dirs), n, E, M), nodes, f, np.array([0.1, 1.0]), 1.0,
+                                  np.array([0.7, 1.33]), np.array([1.0, 2.2]), np.array([0.5, -2.0]), 2.0)
+    for o in range(2):
+        np.testing.assert_allclose(dist[o], prof, rtol=1e-13)
