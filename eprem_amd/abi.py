@@ -136,6 +136,17 @@ class EpMasParams(C.Structure):
     ]
 
 
+class EpMhdParams(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("mhdUs", C.c_double),
+        ("mhdNsAu", C.c_double),
+        ("mhdBsAu", C.c_double),
+        ("omegaSun", C.c_double),
+        ("mas", EpMasParams),
+    ]
+
+
 class EpMasPrev(C.Structure):
     _fields_ = [(n, _DP) for n in ("rold_x", "rold_y", "rold_z", "br", "btheta", "bphi",
                                    "vr", "vtheta", "vphi", "density")]
@@ -186,4 +197,6 @@ EXPORTED_SYMBOLS = (
     "ep_selftest_dpp",
     "ep_observe",
     "ep_stream_flux",
+    "ep_set_positions",
+    "ep_update_mhd",
 )

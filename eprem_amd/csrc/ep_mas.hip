@@ -73,14 +73,11 @@ __device__ __forceinline__ double mas_v_apply(double x) {
 }
 __device__ __forceinline__ double mas_rho_apply(double x) { return x * (1.6726e-16 / 1.6726e-24) / 1.0; }
 
-__global__ __launch_bounds__(256) void k_mas_interp(Dev D, MasDev M, double s, double phi_offset,
-                                                    double radial_max, double vmin, const double *prev,
-                                                    double *out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int st = blockIdx.y;
-  if (k >= D.n_active[st]) return;
-  const size_t i = (size_t)st * D.Npad + k;
-  const size_t SN = (size_t)D.S * D.Npad;
+// masGetNode for one node: masTriLinear inside mas_radial_max, masWind
+// outside.  out7 = {Br, Btheta, Bphi, Vr, Vtheta, Vphi, density}.
+__device__ void mas_get_node(const Dev &D, const MasDev &M, size_t i, double s, double phi_offset,
+                             double radial_max, double vmin, const double *prev, size_t SN,
+                             double out7[7]) {
   const double rs = D.rScale;
   // cartToSphPosAu (src/geometry.c:131-160, :35-45)
   const double ax = D.x[i] * rs, ay = D.y[i] * rs, az = D.z[i] * rs;
@@ -90,7 +87,6 @@ __global__ __launch_bounds__(256) void k_mas_interp(Dev D, MasDev M, double s, d
   while (pph < 0.0) pph += (2.0 * kPI);
   if (pph > 2 * kPI)
     while (pph > 2 * kPI) pph -= 2 * kPI;
-  double br, bt, bp, vr, vt, vp, dens;
   if (pr <= (radial_max * kRSAU)) {
     const double rr = pr / kRSAU, th = pth;
     double ph = pph - phi_offset;
@@ -98,39 +94,124 @@ __global__ __launch_bounds__(256) void k_mas_interp(Dev D, MasDev M, double s, d
       while (ph < 0.0) ph += (2.0 * kPI);
     if (ph > (2.0 * kPI))
       while (ph > (2.0 * kPI)) ph -= (2.0 * kPI);
-    bp = mas_b_apply((1.0 - s) * mas_field(M, 0, 0, rr, th, ph) + s * mas_field(M, 1, 0, rr, th, ph));
-    bt = mas_b_apply((1.0 - s) * mas_field(M, 0, 1, rr, th, ph) + s * mas_field(M, 1, 1, rr, th, ph));
-    br = mas_b_apply((1.0 - s) * mas_field(M, 0, 2, rr, th, ph) + s * mas_field(M, 1, 2, rr, th, ph));
-    vp = mas_v_apply((1.0 - s) * mas_field(M, 0, 3, rr, th, ph) + s * mas_field(M, 1, 3, rr, th, ph));
-    vt = mas_v_apply((1.0 - s) * mas_field(M, 0, 4, rr, th, ph) + s * mas_field(M, 1, 4, rr, th, ph));
-    vr = mas_v_apply((1.0 - s) * mas_field(M, 0, 5, rr, th, ph) + s * mas_field(M, 1, 5, rr, th, ph));
+    out7[2] = mas_b_apply((1.0 - s) * mas_field(M, 0, 0, rr, th, ph) + s * mas_field(M, 1, 0, rr, th, ph));
+    out7[1] = mas_b_apply((1.0 - s) * mas_field(M, 0, 1, rr, th, ph) + s * mas_field(M, 1, 1, rr, th, ph));
+    out7[0] = mas_b_apply((1.0 - s) * mas_field(M, 0, 2, rr, th, ph) + s * mas_field(M, 1, 2, rr, th, ph));
+    out7[5] = mas_v_apply((1.0 - s) * mas_field(M, 0, 3, rr, th, ph) + s * mas_field(M, 1, 3, rr, th, ph));
+    out7[4] = mas_v_apply((1.0 - s) * mas_field(M, 0, 4, rr, th, ph) + s * mas_field(M, 1, 4, rr, th, ph));
+    double vr = mas_v_apply((1.0 - s) * mas_field(M, 0, 5, rr, th, ph) + s * mas_field(M, 1, 5, rr, th, ph));
     if (vr < (vmin / 2.99792458e10)) vr = (vmin / 2.99792458e10);
-    dens = mas_rho_apply((1.0 - s) * mas_field(M, 0, 6, rr, th, ph) + s * mas_field(M, 1, 6, rr, th, ph));
+    out7[3] = vr;
+    out7[6] = mas_rho_apply((1.0 - s) * mas_field(M, 0, 6, rr, th, ph) + s * mas_field(M, 1, 6, rr, th, ph));
   } else {
     // masWind: scale the node's previous values by r_old / r
     const double ox = prev[0 * SN + i], oy = prev[1 * SN + i], oz = prev[2 * SN + i];
     const double rOldmag = sqrt(ox * ox + oy * oy + oz * oz);
     const double oneOverR = rOldmag / D.rmag[i];
-    br = prev[3 * SN + i] * oneOverR * oneOverR;
-    bt = prev[4 * SN + i] * oneOverR;
-    bp = prev[5 * SN + i] * oneOverR;
-    vr = prev[6 * SN + i];
-    vt = prev[7 * SN + i] * oneOverR;
-    vp = prev[8 * SN + i] * oneOverR;
-    dens = prev[9 * SN + i] * oneOverR * oneOverR;
+    out7[0] = prev[3 * SN + i] * oneOverR * oneOverR;
+    out7[1] = prev[4 * SN + i] * oneOverR;
+    out7[2] = prev[5 * SN + i] * oneOverR;
+    out7[3] = prev[6 * SN + i];
+    out7[4] = prev[7 * SN + i] * oneOverR;
+    out7[5] = prev[8 * SN + i] * oneOverR;
+    out7[6] = prev[9 * SN + i] * oneOverR * oneOverR;
   }
-  out[0 * SN + i] = br;
-  out[1 * SN + i] = bt;
-  out[2 * SN + i] = bp;
-  out[3 * SN + i] = vr;
-  out[4 * SN + i] = vt;
-  out[5 * SN + i] = vp;
-  out[6 * SN + i] = dens;
+}
+
+__global__ __launch_bounds__(256) void k_mas_interp(Dev D, MasDev M, double s, double phi_offset,
+                                                    double radial_max, double vmin, const double *prev,
+                                                    double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int st = blockIdx.y;
+  if (k >= D.n_active[st]) return;
+  const size_t i = (size_t)st * D.Npad + k;
+  const size_t SN = (size_t)D.S * D.Npad;
+  double v[7];
+  mas_get_node(D, M, i, s, phi_offset, radial_max, vmin, prev, SN, v);
+  for (int q = 0; q < 7; q++) out[q * SN + i] = v[q];
+}
+
+// updateMhd (src/flow.c:33-154) for every node: new B, V, density (MAS or
+// the analytic wind), |B| and the derivative terms against the previous
+// values, zero on the first node of each stream (src/flow.c:131-149).
+struct MhdArgs {
+  int mode;
+  double mhdUs, mhdNsAu, mhdBsAu, omegaSun;
+  double s, phi_offset, radial_max, vmin;
+};
+
+__global__ __launch_bounds__(256) void k_update_mhd(Dev D, MasDev M, MhdArgs A, const double *prev,
+                                                    double *out, double *bmag, double *dlnb,
+                                                    double *dlnn, double *dupar) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int st = blockIdx.y;
+  if (k >= D.n_active[st]) return;
+  const size_t i = (size_t)st * D.Npad + k;
+  const size_t SN = (size_t)D.S * D.Npad;
+  double v[7];
+  if (A.mode == 1) {
+    mas_get_node(D, M, i, A.s, A.phi_offset, A.radial_max, A.vmin, prev, SN, v);
+  } else {
+    // mhdDensity, mhdV, mhdB with MHD_DEFAULT (src/flow.c:241-350)
+    const double rr = D.rmag[i] * D.rScale;
+    const double theta = acos(D.z[i] / D.rmag[i]);
+    v[6] = A.mhdNsAu / (rr * rr);
+    v[3] = A.mhdUs;
+    v[4] = 0.0;
+    v[5] = 0.0;
+    v[0] = A.mhdBsAu / (rr * rr);
+    v[1] = 0.0;
+    v[2] = -1.0 * rr * v[0] * (A.omegaSun / (v[3] + 1.0e-33)) * sin(theta);
+  }
+  for (int q = 0; q < 7; q++) out[q * SN + i] = v[q];
+  const double bm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  bmag[i] = bm;
+  if (k == 0) {
+    dupar[i] = 0.0;
+    dlnb[i] = 0.0;
+    dlnn[i] = 0.0;
+  } else {
+    const double obr = prev[3 * SN + i], obt = prev[4 * SN + i], obp = prev[5 * SN + i];
+    const double bm_old = sqrt(obr * obr + obt * obt + obp * obp);
+    double du = (v[3] - prev[6 * SN + i]) * v[0] + (v[4] - prev[7 * SN + i]) * v[1] +
+                (v[5] - prev[8 * SN + i]) * v[2];
+    du /= bm;
+    dupar[i] = du;
+    dlnb[i] = log(bm / bm_old);
+    dlnn[i] = log(v[6] / prev[9 * SN + i]);
+  }
 }
 
 }  // namespace ep
 
 using namespace ep;
+
+static int ensure_mas_buffers() {
+  const size_t SN = (size_t)g.d.num_streams * g.Npad;
+  if (!g.mas_out) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, 17 * SN * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "MAS output alloc");
+    g.mas_allocs.push_back(p);
+    g.mas_out = (double *)p;
+    g.mas_prev = g.mas_out + 7 * SN;
+    e = hipMemset(g.mas_prev, 0, 10 * SN * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "MAS prev clear");
+  }
+  return EP_OK;
+}
+
+static MasDev mas_dev() {
+  MasDev M;
+  for (int lv = 0; lv < 2; lv++) {
+    for (int k = 0; k < 7; k++) M.field[lv][k] = g.mas[lv].field[k];
+    for (int k = 0; k < 21; k++) {
+      M.mesh[lv][k] = g.mas[lv].mesh[k];
+      M.dims[lv][k] = g.mas[lv].dims[k];
+    }
+  }
+  return M;
+}
 
 extern "C" {
 
@@ -194,15 +275,8 @@ int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev, ep_mas_out *
     return fail(EP_ERR_INVALID_ARG, "phi_offset must be finite with |phi_offset| <= 1000");
   if (!g.mas[0].set || !g.mas[1].set) return fail(EP_ERR_INVALID_ARG, "set both MAS levels first");
   const size_t S = g.d.num_streams, N = g.d.num_nodes, SN = S * g.Npad;
-  if (!g.mas_out) {
-    void *p = nullptr;
-    hipError_t e = hipMalloc(&p, 17 * SN * sizeof(double));
-    if (e != hipSuccess) return hip_fail(e, "MAS output alloc");
-    g.mas_allocs.push_back(p);
-    g.mas_out = (double *)p;
-    g.mas_prev = g.mas_out + 7 * SN;
-    hipMemset(g.mas_prev, 0, 10 * SN * sizeof(double));
-  }
+  int rc = ensure_mas_buffers();
+  if (rc) return rc;
   if (prev) {
     const double *src[10] = {prev->rold_x, prev->rold_y, prev->rold_z, prev->br, prev->btheta,
                              prev->bphi, prev->vr, prev->vtheta, prev->vphi, prev->density};
@@ -213,14 +287,7 @@ int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev, ep_mas_out *
       if (e != hipSuccess) return hip_fail(e, "prev upload");
     }
   }
-  MasDev M;
-  for (int lv = 0; lv < 2; lv++) {
-    for (int k = 0; k < 7; k++) M.field[lv][k] = g.mas[lv].field[k];
-    for (int k = 0; k < 21; k++) {
-      M.mesh[lv][k] = g.mas[lv].mesh[k];
-      M.dims[lv][k] = g.mas[lv].dims[k];
-    }
-  }
+  MasDev M = mas_dev();
   Dev D = make_dev();
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -245,6 +312,92 @@ int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev, ep_mas_out *
       hipError_t e = hipMemcpy2D(dst[i], N * 8, g.mas_out + i * SN, g.Npad * 8, N * 8, S,
                                  hipMemcpyDeviceToHost);
       if (e != hipSuccess) return hip_fail(e, "MAS output download");
+    }
+  }
+  return EP_OK;
+}
+
+int ep_set_positions(const double *x, const double *y, const double *z, const double *rmag,
+                     const int32_t *n_active) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  const double *src[4] = {x, y, z, rmag};
+  double *dst[4] = {g.nx, g.ny, g.nz, g.nrmag};
+  const size_t S = g.d.num_streams, N = g.d.num_nodes;
+  for (int i = 0; i < 4; i++) {
+    if (!src[i]) return fail(EP_ERR_INVALID_ARG, "position array %d is null", i);
+    hipError_t e = hipMemcpy2D(dst[i], g.Npad * 8, src[i], N * 8, N * 8, S, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "position upload");
+  }
+  std::vector<int32_t> na(S, (int32_t)N);
+  if (n_active)
+    for (size_t s = 0; s < S; s++) {
+      if (n_active[s] < 2 || n_active[s] > (int32_t)N)
+        return fail(EP_ERR_INVALID_ARG, "n_active[%zu] = %d outside 2..%zu", s, n_active[s], N);
+      na[s] = n_active[s];
+    }
+  hipError_t e = hipMemcpy(g.n_active, na.data(), S * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "n_active");
+  g.h_nactive = na;
+  g.positions_set = true;
+  return EP_OK;
+}
+
+int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *out, double *device_ms) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!g.positions_set && !g.nodes_set) return fail(EP_ERR_INVALID_ARG, "set node positions first");
+  if (!mp || !prev) return fail(EP_ERR_INVALID_ARG, "null MHD params / previous state");
+  if (mp->mode != 0 && mp->mode != 1) return fail(EP_ERR_INVALID_ARG, "mode must be 0 or 1");
+  if (mp->mode == 1) {
+    if (!g.mas[0].set || !g.mas[1].set) return fail(EP_ERR_INVALID_ARG, "set both MAS levels first");
+    if (!(fabs(mp->mas.phi_offset) <= 1.0e3)) return fail(EP_ERR_INVALID_ARG, "phi_offset out of range");
+  }
+  const size_t S = g.d.num_streams, N = g.d.num_nodes, SN = S * g.Npad;
+  int rc = ensure_mas_buffers();
+  if (rc) return rc;
+  const double *src[10] = {prev->rold_x, prev->rold_y, prev->rold_z, prev->br, prev->btheta,
+                           prev->bphi, prev->vr, prev->vtheta, prev->vphi, prev->density};
+  for (int i = 0; i < 10; i++) {
+    if (!src[i]) return fail(EP_ERR_INVALID_ARG, "prev array %d is null", i);
+    hipError_t e = hipMemcpy2D(g.mas_prev + i * SN, g.Npad * 8, src[i], N * 8, N * 8, S,
+                               hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "prev upload");
+  }
+  MasDev M = mas_dev();
+  MhdArgs A;
+  A.mode = mp->mode;
+  A.mhdUs = mp->mhdUs;
+  A.mhdNsAu = mp->mhdNsAu;
+  A.mhdBsAu = mp->mhdBsAu;
+  A.omegaSun = mp->omegaSun;
+  A.s = mp->mas.s_cor;
+  A.phi_offset = mp->mas.phi_offset;
+  A.radial_max = mp->mas.mas_radial_max;
+  A.vmin = mp->mas.mas_vmin;
+  Dev D = make_dev();
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, g.st);
+  hipLaunchKernelGGL(k_update_mhd, dim3((N + 255) / 256, S), dim3(256), 0, g.st, D, M, A, g.mas_prev,
+                     g.mas_out, g.nbmag, g.ndlnb, g.ndlnn, g.ndupar);
+  hipError_t le = hipGetLastError();
+  (void)hipEventRecord(e1, g.st);
+  hipError_t se = hipStreamSynchronize(g.st);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (le != hipSuccess) return hip_fail(le, "k_update_mhd launch");
+  if (se != hipSuccess) return hip_fail(se, "k_update_mhd");
+  if (device_ms) *device_ms = ms;
+  g.nodes_set = true;
+  if (out) {
+    double *dst[7] = {out->br, out->btheta, out->bphi, out->vr, out->vtheta, out->vphi, out->density};
+    for (int i = 0; i < 7; i++) {
+      if (!dst[i]) continue;
+      hipError_t e = hipMemcpy2D(dst[i], N * 8, g.mas_out + i * SN, g.Npad * 8, N * 8, S,
+                                 hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "MHD output download");
     }
   }
   return EP_OK;

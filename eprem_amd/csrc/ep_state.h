@@ -18,7 +18,7 @@ struct MasLevel {
 };
 
 struct State {
-  bool init = false, nodes_set = false;
+  bool init = false, nodes_set = false, positions_set = false;
   ep_dims d{};
   ep_params p{};
   int Npad = 0;

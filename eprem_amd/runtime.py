@@ -54,6 +54,8 @@ def lib():
         L.ep_selftest_dpp.argtypes = [DP, DP, DP]
         L.ep_observe.argtypes = [C.POINTER(abi.EpObservers), DP, DP]
         L.ep_stream_flux.argtypes = [DP]
+        L.ep_set_positions.argtypes = [DP, DP, DP, DP, IP]
+        L.ep_update_mhd.argtypes = [C.POINTER(abi.EpMhdParams), C.POINTER(EpMasPrev), C.POINTER(EpMasOut), DP]
         for name in abi.EXPORTED_SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -167,6 +169,28 @@ class EpSolver:
         ms = C.c_double()
         _check(lib().ep_interp_mhd(C.byref(mp), C.byref(pv) if pv is not None else None, C.byref(mo),
                                    C.byref(ms)))
+        outs["device_ms"] = ms.value
+        return outs
+
+    def set_positions(self, nodes, n_active=None):
+        S, N = self.dims.num_streams, self.dims.num_nodes
+        arrs = [np.ascontiguousarray(nodes[k], dtype=np.float64).reshape(S, N) for k in ("x", "y", "z", "rmag")]
+        na = None if n_active is None else np.ascontiguousarray(n_active, dtype=np.int32)
+        _check(lib().ep_set_positions(*[dptr(a) for a in arrs], iptr(na)))
+
+    def update_mhd(self, prev, mode=0, mhdUs=0.0, mhdNsAu=0.0, mhdBsAu=0.0, omegaSun=0.0, s_cor=0.0,
+                   phi_offset=0.0, mas_radial_max=0.0, mas_vmin=0.0):
+        """updateMhd on the device; returns host copies of the new B, V, n."""
+        S, N = self.dims.num_streams, self.dims.num_nodes
+        names = ("rold_x", "rold_y", "rold_z", "br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")
+        arrs = [np.ascontiguousarray(prev[k], dtype=np.float64).reshape(S, N) for k in names]
+        pv = EpMasPrev(*[dptr(a) for a in arrs])
+        outs = {k: np.zeros((S, N)) for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")}
+        mo = EpMasOut(*[dptr(outs[k]) for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")])
+        mp = abi.EpMhdParams(mode, mhdUs, mhdNsAu, mhdBsAu, omegaSun,
+                             EpMasParams(s_cor, phi_offset, mas_radial_max, mas_vmin))
+        ms = C.c_double()
+        _check(lib().ep_update_mhd(C.byref(mp), C.byref(pv), C.byref(mo), C.byref(ms)))
         outs["device_ms"] = ms.value
         return outs
 

@@ -223,6 +223,29 @@ typedef struct ep_mas_out {
 int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev,
                   ep_mas_out *out, double *device_ms);
 
+/* ---------------- MHD update on the device ------------------------- */
+
+/* Node positions only, [num_streams][num_nodes] in rScale units, with the
+ * per-stream active counts (NULL = all).  Used with ep_update_mhd, which
+ * then fills the MHD node fields on the device. */
+int ep_set_positions(const double *x, const double *y, const double *z, const double *rmag,
+                     const int32_t *n_active);
+
+typedef struct ep_mhd_params {
+  int32_t mode;           /* 0: analytic wind (MHD_DEFAULT), 1: MAS        */
+  double mhdUs, mhdNsAu, mhdBsAu, omegaSun;   /* mode 0, code units        */
+  ep_mas_params mas;      /* mode 1                                        */
+} ep_mhd_params;
+
+/* updateMhd (src/flow.c:33-154) for every node at the positions on the
+ * device: new B, V and density (MAS: masGetNode; mode 0: mhdDensity, mhdV,
+ * mhdB, src/flow.c:241-403), |B| and the derivative terms mhdDuPar,
+ * mhdDlnB, mhdDlnN against `prev` (zero on the first node of each stream,
+ * src/flow.c:131-149).  The results feed ep_step directly (no host copy);
+ * `out` (may be NULL) receives host copies of B, V, density. */
+int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *out,
+                  double *device_ms);
+
 /* ---------------- observers and flux -------------------------------- */
 
 /* Point observers (config.obsR/obsTheta/obsPhi, idw_p). */

@@ -243,3 +243,40 @@ def stream_flux(dims, f, egrid):
     lib().or_stream_flux(C.byref(dims), dptr(np.ascontiguousarray(f, dtype=np.float64)),
                          dptr(np.ascontiguousarray(egrid, dtype=np.float64)), dptr(out))
     return out
+
+
+def update_mhd(nodes, prev, mode, rscale, mhdUs=0.0, mhdNsAu=0.0, mhdBsAu=0.0, omegaSun=0.0,
+               level0=None, level1=None, s_cor=0.0, phi_offset=0.0, mas_radial_max=0.0, mas_vmin=0.0):
+    """or_update_mhd over [S, N] node arrays; the first node of each stream
+    is the stream's inner node."""
+    S, N = nodes["x"].shape
+    n = S * N
+    xs = [np.ascontiguousarray(nodes[k], dtype=np.float64).ravel() for k in ("x", "y", "z", "rmag")]
+    first = np.zeros((S, N), np.int32)
+    first[:, 0] = 1
+    first = first.ravel()
+    names = ("rold_x", "rold_y", "rold_z", "br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")
+    pv = [np.ascontiguousarray(prev[k], dtype=np.float64).ravel() for k in names]
+    FPP = C.POINTER(C.POINTER(C.c_float))
+    keep = []
+
+    def lv(level):
+        if level is None:
+            return None, None, None
+        fields = [np.ascontiguousarray(a, dtype=np.float32) for a in level[0]]
+        meshes = [np.ascontiguousarray(a, dtype=np.float32) for a in level[1]]
+        dims = np.array([len(m) for m in meshes], np.int32)
+        keep.extend(fields + meshes + [dims])
+        fa = (C.POINTER(C.c_float) * 7)(*[fptr(a) for a in fields])
+        ma = (C.POINTER(C.c_float) * 21)(*[fptr(a) for a in meshes])
+        keep.extend([fa, ma])
+        return C.cast(fa, FPP), C.cast(ma, FPP), iptr(dims)
+
+    f0, m0, d0 = lv(level0)
+    f1, m1, d1 = lv(level1)
+    outs = [np.zeros(n) for _ in range(11)]
+    lib().or_update_mhd(n, mode, rscale, mhdUs, mhdNsAu, mhdBsAu, omegaSun, *[dptr(a) for a in xs],
+                        iptr(first), f0, m0, d0, f1, m1, d1, s_cor, phi_offset, mas_radial_max, mas_vmin,
+                        *[dptr(a) for a in pv], *[dptr(a) for a in outs])
+    keys = ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density", "bmag", "dupar", "dlnb", "dlnn")
+    return {k: v.reshape(S, N) for k, v in zip(keys, outs)}
