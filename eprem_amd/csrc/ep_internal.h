@@ -121,4 +121,23 @@ __device__ inline size_t fidx(const Dev &D, int s, int e, int m, int node) {
 
 __device__ inline double *fbuf(const Dev &D, int which) { return which ? D.fB : D.fA; }
 
+// Raw buffer access to one wave-uniform slab: the base address lives in
+// scalar registers (the buffer descriptor) and the per-mu offset in a
+// scalar offset, so a lane needs one 32-bit VGPR for its address instead
+// of a 64-bit pointer per mu row.  Word 3 = 32-bit data format (CDNA).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+__device__ __forceinline__ rsrc_t slab_rsrc(const double *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), 0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ double buf_ld(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+__device__ __forceinline__ void buf_st(double v, rsrc_t r, uint32_t voff, uint32_t soff) {
+  typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) b64_t;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64_t, v), r, (int)voff, (int)soff, 0);
+}
+
 }  // namespace ep

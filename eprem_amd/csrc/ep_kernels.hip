@@ -424,8 +424,10 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
   const size_t on = (size_t)s * D.Npad;
   const int node = valid ? (D.thin[s] ? D.list[on + k] : k) : 0;
   const int cur = D.loc[task] ^ (pass & 1);
-  const double *__restrict__ src = fbuf(D, cur) + (size_t)task * M * D.Npad;
-  double *__restrict__ dst = fbuf(D, cur ^ 1) + (size_t)task * M * D.Npad;
+  const uint32_t slab_bytes = (uint32_t)(M * D.Npad * 8), row = (uint32_t)D.Npad * 8;
+  const rsrc_t src = slab_rsrc(fbuf(D, cur) + (size_t)task * M * D.Npad, slab_bytes);
+  const rsrc_t dst = slab_rsrc(fbuf(D, cur ^ 1) + (size_t)task * M * D.Npad, slab_bytes);
+  const uint32_t voff = (uint32_t)node * 8;
 
   double f[M];
   double sd = 0.0;
@@ -433,7 +435,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     double v = 0.0;
-    if (valid) v = src[(size_t)m * D.Npad + node];
+    if (valid) v = buf_ld(src, voff, m * row);
     if (do_seed && v < sd) v = sd;
     f[m] = v;
   }
@@ -446,7 +448,17 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
   const bool bc = D.useBC && valid && (k < 3);
   const double bcv = bc ? D.seed3[(size_t)task * 3 + k] : 0.0;
   const double invM = 1.0 / M;
-  __shared__ double edge[2][16][M];
+  // Edge exchange between the waves of the block, double-buffered by
+  // subcycle parity.  up[r] holds the mu >= 0 values lane 0 of wave r
+  // takes as its upstream neighbour: lane 63 of wave r-1 writes row r, and
+  // lane 0 of wave 0 writes its own values into row 0 (the chunk's first
+  // lane keeps its value, as DPP leaves a lane without a source alone).
+  // dn[r] holds the mu < 0 values lane 63 of wave r-1 takes as its
+  // downstream neighbour: lane 0 of wave r writes row r, and lane 63 of the
+  // last wave writes its own values into row nw.
+  constexpr int NP = M - MPOS;
+  __shared__ double up[2][17][NP > 0 ? NP : 1];
+  __shared__ double dn[2][17][MPOS > 0 ? MPOS : 1];
   // coef[m] = ds_i_multiplier * del_fac[m]: the product the reference forms
   // every subcycle is loop-invariant, so it is formed once per pass.
   double coef[M];
@@ -455,6 +467,8 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
 #pragma unroll
     for (int m = 0; m < M; ++m) coef[m] = ((m >= MPOS) ? dsm_pos : dsm_neg) * del[m];
   }
+  const int up_w = (lane == 63) ? w + 1 : ((lane == 0 && w == 0) ? 0 : -1);
+  const int dn_w = (lane == 0) ? w : ((lane == 63 && w == nw - 1) ? nw : -1);
 
   for (int j = 0; j < nsub; ++j) {
     if (bc) {
@@ -462,27 +476,32 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
       for (int m = 0; m < M; ++m) f[m] = bcv;
     }
     const int b = j & 1;
-    if (lane == 63) {
+    if (up_w >= 0) {
 #pragma unroll
-      for (int m = MPOS; m < M; ++m) edge[b][w][m] = f[m];
+      for (int m = MPOS; m < M; ++m) up[b][up_w][m - MPOS] = f[m];
     }
-    if (lane == 0) {
+    if (dn_w >= 0) {
 #pragma unroll
-      for (int m = 0; m < MPOS; ++m) edge[b][w][m] = f[m];
+      for (int m = 0; m < MPOS; ++m) dn[b][dn_w][m] = f[m];
     }
     __syncthreads();
+    // Only lane 0 (mu >= 0) and lane 63 (mu < 0) use these values; the
+    // other lanes read the same rows harmlessly.
+    double eu[NP > 0 ? NP : 1], ed[MPOS > 0 ? MPOS : 1];
+#pragma unroll
+    for (int m = MPOS; m < M; ++m) eu[m - MPOS] = up[b][w][m - MPOS];
+#pragma unroll
+    for (int m = 0; m < MPOS; ++m) ed[m] = dn[b][w + 1][m];
     double sum = 0.0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const double fm = f[m];
       double fn;
       if (m >= MPOS) {
-        const double old = (w > 0) ? edge[b][w - 1][m] : fm;
-        const double nb = dpp_shr1(fm, old);
+        const double nb = dpp_shr1(fm, eu[m - MPOS]);
         fn = fm + coef[m] * (nb - fm);
       } else {
-        const double old = (w < nw - 1) ? edge[b][w + 1][m] : fm;
-        const double nb = dpp_shl1(fm, old);
+        const double nb = dpp_shl1(fm, ed[m]);
         fn = fm + coef[m] * (fm - nb);
       }
       f[m] = fn;
@@ -494,7 +513,6 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
   }
 
   if (valid && tid >= h && tid < h + I) {
-    double *__restrict__ out = dst + node;
     int bad = -1, bad_inf = 0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
@@ -506,7 +524,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
         }
         if (v < kDblMin) v = kDblMin;
       }
-      out[(size_t)m * D.Npad] = v;
+      buf_st(v, dst, voff, m * row);
     }
     if (bad >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad, node, bad_inf));
   }
@@ -559,7 +577,9 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
   // dispatched first.
   const int groups = (D.E + 3) >> 2;
   const int s = blockIdx.x / groups;
-  const int e = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
+  // the wave index is uniform across the wave; readfirstlane tells the
+  // compiler so, keeping e (and the slab descriptor) in scalar registers
+  const int e = (blockIdx.x % groups) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int node = blockIdx.y * 64 + (threadIdx.x & 63);
   const int n = D.n_active[s];
   if (e >= D.E || node < 1 || node >= n) return;
@@ -571,7 +591,9 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
   const double half = 0.5, one = 1.0, two = 2.0;
   {
     const int task = s * D.E + e;
-    double *fp = fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad + node;
+    const rsrc_t slab = slab_rsrc(fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad,
+                                  (uint32_t)(M * D.Npad * 8));
+    const uint32_t voff = (uint32_t)node * 8, row = (uint32_t)D.Npad * 8;
     const double v = D.vgrid[e];
     const double a = -v * dlnBds - (two / v) * DuParDt;
     double va[M + 1];
@@ -585,7 +607,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
     }
     double f[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
+    for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row);
     for (int it = 0; it < nsub; ++it) {
       double flux_lo = 0.0;  // zero-flux wall at mu = -1
 #pragma unroll
@@ -606,7 +628,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
         bad_inf = isinf(val) ? 1 : 0;
       }
       if (val < kDblMin) val = kDblMin;
-      if (nsub > 0 || val != f[m]) fp[(size_t)m * D.Npad] = val;
+      if (nsub > 0 || val != f[m]) buf_st(val, slab, voff, m * row);
     }
     if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
   }
@@ -615,15 +637,18 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
 // ------------------------------------------------------------------------
 // k_change: AdiabaticChange with the upwind operator, advecting ln f in
 // ln p (src/energeticParticles.c:859-1067, :1081-1176).  A lane per node,
-// the four waves of a block split the mu cells; the energy column of one
-// (node, mu) stays in registers.  EMAX >= E is the register capacity.
+// the four waves of a block split the mu cells.  Each subcycle is one
+// sweep up the energy column with a sliding window (the new ln f at e
+// needs the old values at e-1, e, e+1), reading ahead PF energies to keep
+// loads in flight; intermediate ln f is kept in place in the f slab, the
+// first sweep takes the log and the last one writes exp back.  A lane
+// needs a handful of registers, so the kernel runs at full occupancy.
 // ------------------------------------------------------------------------
-template <int EMAX>
 __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   // blockIdx.x = (stream, group of 4 mu cells), blockIdx.y = node chunk
   const int groups = (D.M + 3) >> 2;
   const int s = blockIdx.x / groups;
-  const int m = (blockIdx.x % groups) * 4 + (threadIdx.x >> 6);
+  const int m = (blockIdx.x % groups) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int node = blockIdx.y * 64 + (threadIdx.x & 63);
   const int n = D.n_active[s];
   if (m >= D.M || node < 1 || node >= n) return;
@@ -636,76 +661,79 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   const double dlnp_i = 1.0 / D.dlnp;
   const double half = 0.5, one = 1.0;
   const size_t slab = (size_t)D.M * D.Npad;
-  {
-    const double mu = D.mugrid[m];
-    const double a = -mu * DuParDt;
-    const double bb = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
-    double lf[EMAX];
-#pragma unroll
-    for (int e = 0; e < EMAX; ++e) {
-      if (e < E) {
-        const int task = s * E + e;
-        const double *fp = fbuf(D, D.loc[task]) + (size_t)task * slab + (size_t)m * D.Npad + node;
-        lf[e] = log(*fp);
+  const uint32_t slab_bytes = (uint32_t)(slab * 8);
+  const uint32_t voff = (uint32_t)(((size_t)m * D.Npad + node) * 8);
+  const double mu = D.mugrid[m];
+  const double a = -mu * DuParDt;
+  const double bb = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
+  const double *__restrict__ vgi = D.vgrid_i;
+  const bool upd = nsub > 0;
+  const int niter = upd ? nsub : 1;  // nsub = 0 still takes exp(log f)
+  auto slab_of = [&](int e) {
+    const int task = s * E + e;
+    return slab_rsrc(fbuf(D, D.loc[task]) + (size_t)task * slab, slab_bytes);
+  };
+  int bad = -1, bad_inf = 0;
+  for (int it = 0; it < niter; ++it) {
+    const bool first = (it == 0), fin = (it == niter - 1);
+    auto ld = [&](int e) { return buf_ld(slab_of(e), voff, 0); };
+    auto cvt = [&](double x) { return first ? log(x) : x; };
+    double r0 = cvt(ld(0));
+    double r1 = cvt(ld(1));
+    double p2 = (2 < E) ? ld(2) : 0.0;
+    double p3 = (3 < E) ? ld(3) : 0.0;
+    double p4 = (4 < E) ? ld(4) : 0.0;
+    double old_prev = 0.0;  // ln f[e-1] before this subcycle
+    double vel = damp * (a * vgi[0] + bb);
+    double vel_prev = 0.0;
+    double flux_e = 0.0;  // flux at interface e (between e-1 and e)
+    for (int e = 0; e < E; ++e) {
+      const double cur = r0;
+      const double nxt = (e + 1 < E) ? r1 : 0.0;
+      const double vel1 = (e + 1 < E) ? damp * (a * vgi[e + 1] + bb) : 0.0;
+      double f1;
+      if (e == 0) {
+        const double va1 = half * (vel1 + vel);
+        if (va1 <= 0)
+          f1 = va1 * (nxt - cur) * dlnp_i;
+        else
+          f1 = va1 * (cur - bc0) * dlnp_i;
+        flux_e = va1 * (signbit(va1) ? nxt : cur);  // interface 1
+      } else if (e == E - 1) {
+        const double vr = half * (vel + vel_prev);
+        if (vr >= 0)
+          f1 = vr * (cur - old_prev) * dlnp_i;
+        else
+          f1 = vr * (bc1 - cur) * dlnp_i;
+      } else {
+        const double va1 = half * (vel1 + vel);
+        const double flux_hi = va1 * (signbit(va1) ? nxt : cur);
+        f1 = (flux_hi - flux_e) * dlnp_i;
+        flux_e = flux_hi;
       }
-    }
-    for (int it = 0; it < nsub; ++it) {
-      // keep the per-energy velocities recomputed (not hoisted into VGPRs)
-      double ah = a, bh = bb;
-      asm volatile("" : "+v"(ah), "+v"(bh));
-      double old_prev = 0.0;  // ln f[e-1] before this subcycle
-      double vel_prev = 0.0;
-      double flux_e = 0.0;    // flux at interface e (between e-1 and e)
-#pragma unroll
-      for (int e = 0; e < EMAX; ++e) {
-        if (e < E) {
-          const double cur = lf[e];
-          const double nxt = (e + 1 < E) ? lf[e + 1] : 0.0;
-          const double vel = damp * (ah * D.vgrid_i[e] + bh);
-          double f1;
-          if (e == 0) {
-            const double vel1 = damp * (ah * D.vgrid_i[1] + bh);
-            const double va1 = half * (vel1 + vel);
-            if (va1 <= 0)
-              f1 = va1 * (nxt - cur) * dlnp_i;
-            else
-              f1 = va1 * (cur - bc0) * dlnp_i;
-            flux_e = va1 * (signbit(va1) ? nxt : cur);  // interface 1
-          } else if (e == E - 1) {
-            const double vr = half * (vel + vel_prev);
-            if (vr >= 0)
-              f1 = vr * (cur - old_prev) * dlnp_i;
-            else
-              f1 = vr * (bc1 - cur) * dlnp_i;
-          } else {
-            const double vel1 = damp * (ah * D.vgrid_i[e + 1] + bh);
-            const double va1 = half * (vel1 + vel);
-            const double flux_hi = va1 * (signbit(va1) ? nxt : cur);
-            f1 = (flux_hi - flux_e) * dlnp_i;
-            flux_e = flux_hi;
-          }
-          lf[e] = cur - dtsub * f1;
-          old_prev = cur;
-          vel_prev = vel;
-        }
-      }
-    }
-    int bad = -1, bad_inf = 0;
-#pragma unroll
-    for (int e = 0; e < EMAX; ++e) {
-      if (e < E) {
-        double val = exp(lf[e]);
+      const double nv = upd ? cur - dtsub * f1 : cur;
+      if (fin) {
+        double val = exp(nv);
         if (bad < 0 && !isfinite(val)) {
           bad = e;
           bad_inf = isinf(val) ? 1 : 0;
         }
         if (val < kDblMin) val = kDblMin;
-        const int task = s * E + e;
-        fbuf(D, D.loc[task])[(size_t)task * slab + (size_t)m * D.Npad + node] = val;
+        buf_st(val, slab_of(e), voff, 0);
+      } else {
+        buf_st(nv, slab_of(e), voff, 0);
       }
+      old_prev = cur;
+      vel_prev = vel;
+      vel = vel1;
+      r0 = r1;
+      r1 = (e + 2 < E) ? cvt(p2) : 0.0;
+      p2 = p3;
+      p3 = p4;
+      p4 = (e + 5 < E) ? ld(e + 5) : 0.0;
     }
-    if (bad >= 0) note_error(D, err_key(substep, 1, s, 1, node, bad, m, bad_inf));
   }
+  if (bad >= 0) note_error(D, err_key(substep, 1, s, 1, node, bad, m, bad_inf));
 }
 
 // ------------------------------------------------------------------------
@@ -847,18 +875,7 @@ hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
 
 hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
   const dim3 grid(D.S * ((D.M + 3) / 4), (D.N + 63) / 64), block(256);
-  if (D.E <= 8)
-    hipLaunchKernelGGL((k_change<8>), grid, block, 0, st, D, sub);
-  else if (D.E <= 16)
-    hipLaunchKernelGGL((k_change<16>), grid, block, 0, st, D, sub);
-  else if (D.E <= 32)
-    hipLaunchKernelGGL((k_change<32>), grid, block, 0, st, D, sub);
-  else if (D.E <= 48)
-    hipLaunchKernelGGL((k_change<48>), grid, block, 0, st, D, sub);
-  else if (D.E <= 64)
-    hipLaunchKernelGGL((k_change<64>), grid, block, 0, st, D, sub);
-  else
-    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_change, grid, block, 0, st, D, sub);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
