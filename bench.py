@@ -35,7 +35,18 @@ CONFIGS = {
     # rows, cols, N, E, M, numEpSteps
     "c1": dict(rows=2, cols=2, N=400, E=20, M=20, eps=10),
     "c2": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10),
+    # batched form: 1536 streams whose active-node counts are drawn from a
+    # seeded uniform 200..4000, as shells are injected
+    "c3": dict(rows=16, cols=16, N=4000, E=64, M=32, eps=10, ragged=(200, 4000)),
 }
+
+
+def active_counts(c, S):
+    """Per-stream active-node counts (all N unless the config is ragged)."""
+    if "ragged" not in c:
+        return None
+    lo, hi = c["ragged"]
+    return np.random.default_rng(20261203).integers(lo, hi + 1, S).astype(np.int32)
 
 
 def make_params(c):
@@ -57,7 +68,7 @@ def bench_nodes(c):
         cme=dict(r_au=0.5, width_au=0.05, speed_cms=1500e5, dens=4.0, ang_deg=60.0))
 
 
-def cpu_baseline(c, p, nodes, budget_s=20.0):
+def cpu_baseline(c, p, nodes, n_active=None, budget_s=20.0):
     """The FP64 oracle (OpenMP over streams) on a bounded stream subset of
     the same workload, scaled per cell-update."""
     import oracle
@@ -66,11 +77,12 @@ def cpu_baseline(c, p, nodes, budget_s=20.0):
     E, M = c["E"], c["M"]
     nsub = max(1, min(cores, 16))
     sub = {k: np.ascontiguousarray(v[:nsub]) for k, v in nodes.items()}
+    na = None if n_active is None else np.ascontiguousarray(n_active[:nsub])
     dims = workloads.dims_of(sub, E, M)
     g = oracle.build_grids(p, E, M)
     f0 = workloads.seeded_f(p, g, sub, noise=0.0)
     t0 = time.perf_counter()
-    rc, d, fo, _, _, _ = oracle.update_ep(dims, p, sub, f0, nthreads=cores)
+    rc, d, fo, _, _, _ = oracle.update_ep(dims, p, sub, f0, n_active=na, nthreads=cores)
     dt = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError("oracle failed: %d" % rc)
@@ -98,11 +110,19 @@ def main():
     p = make_params(c)
     nodes = bench_nodes(c)
     dims = workloads.dims_of(nodes, c["E"], c["M"])
+    n_active = active_counts(c, dims.num_streams)
     sol = runtime.EpSolver(dims, p)
+    sol.set_nodes(nodes, n_active)
+    sol.init_seed_population()
     if args.block or args.halo:
         sol.set_tuning(args.block or 256, args.halo or 32)
-    sol.set_nodes(nodes)
-    sol.init_seed_population()
+        tiling = {"block": args.block or 256, "halo": args.halo or 32, "source": "flags"}
+    else:
+        blk, halo, tune_ms = sol.autotune()
+        tiling = {"block": blk, "halo": halo, "source": "autotune", "stream_ms_per_substep": tune_ms}
+    # the timed steps launch kernels one by one so that HIP events bracket
+    # each operator; the captured-graph form is timed afterwards
+    sol.set_graph(False)
 
     mas_ms = None
     if not args.no_mas:
@@ -125,6 +145,12 @@ def main():
         for k in acc:
             acc[k] += dg[k]
     t1 = time.perf_counter()
+    sol.set_graph(True)
+    sol.step()  # capture
+    tg0 = time.perf_counter()
+    for _ in range(args.steps):
+        sol.step()
+    tg1 = time.perf_counter()
     if not args.no_mas:
         for _ in range(args.steps):
             mas_total += sol.interp_mhd(0.5, 0.0, 30.0, 50.0e5, prev)["device_ms"]
@@ -168,9 +194,16 @@ def main():
         "sweeps_per_step": {k: acc[k] // args.steps for k in ("stream_sweeps", "focus_sweeps",
                                                               "change_sweeps")},
         "interp_ms": mas_ms,
+        "tiling": tiling,
+        "max_subcycles": {k: dg[k] for k in ("max_nsteps_stream", "max_subcycles_focusing",
+                                             "max_subcycles_change")},
+        "graph_ms_per_step": (tg1 - tg0) * 1e3 / args.steps,
     }
+    if n_active is not None:
+        res["config"]["active_nodes"] = {"min": int(n_active.min()), "max": int(n_active.max()),
+                                         "mean": float(n_active.mean())}
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(c, p, nodes)
+        res["cpu_baseline"] = cpu_baseline(c, p, nodes, n_active)
     sol.close()
     print(json.dumps(res))
 

@@ -117,6 +117,10 @@ class EpDiag(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class EpSpawn(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("x", "y", "z", "rmag", "f")]
+
+
 class EpObservers(C.Structure):
     _fields_ = [
         ("num", C.c_int32),
@@ -199,4 +203,9 @@ EXPORTED_SYMBOLS = (
     "ep_stream_flux",
     "ep_set_positions",
     "ep_update_mhd",
+    "ep_autotune",
+    "ep_set_graph",
+    "ep_spawn_shell",
+    "ep_move_nodes",
+    "ep_get_positions",
 )

@@ -54,11 +54,12 @@ static int dalloc(T **p, size_t count, const char *what) {
 }
 
 static void free_all() {
-  for (void *p : g.allocs) hipFree(p);
+  g.drop_graph();
+  for (void *p : g.allocs) (void)hipFree(p);
   g.allocs.clear();
-  for (auto &ev : g.events) hipEventDestroy(ev);
+  for (auto &ev : g.events) (void)hipEventDestroy(ev);
   g.events.clear();
-  if (g.st) hipStreamDestroy(g.st);
+  if (g.st) (void)hipStreamDestroy(g.st);
   g.st = nullptr;
   g.mas_free();
   g.init = false;
@@ -211,6 +212,7 @@ Dev make_dev() {
   D.q = g.q;
   D.nsub_f = g.nsub_f;
   D.nsub_c = g.nsub_c;
+  D.heavy = g.heavy;
   D.dtsub_f = g.dtsub_f;
   D.dtsub_c = g.dtsub_c;
   D.f_dlnbds = g.f_dlnbds;
@@ -276,6 +278,35 @@ static int ensure_events(int count) {
   return EP_OK;
 }
 
+// The EP substep loop of one global step (src/energeticParticles.c:
+// 102-199): streaming passes and bookkeeping, then focusing and adiabatic
+// change.  With `ev` (per-operator events, not used under graph capture)
+// four events per substep bracket the operators.
+static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_t st, hipEvent_t *ev) {
+  hipError_t e = hipSuccess;
+#define EPQ(x)            \
+  do {                    \
+    e = (x);              \
+    if (e != hipSuccess) return e; \
+  } while (0)
+  for (int sub = 0; sub < sh.nsub; sub++) {
+    hipEvent_t *e4 = ev ? &ev[2 + 4 * sub] : nullptr;
+    if (sh.do_seed && !sh.do_stream) EPQ(launch_seed_check(D, st));
+    if (sh.do_stream) {
+      for (int pass = 0; pass < sh.passes; pass++)
+        EPQ(launch_stream_pass(D, pass, pass & 1, sh.do_seed && pass == 0, sub, sh.chunks, st));
+      EPQ(launch_stream_finish(D, sub, st));
+    }
+    if (e4) EPQ(hipEventRecord(e4[0], st));
+    if (sh.do_focus) EPQ(launch_focus(D, sub, st));
+    if (e4) EPQ(hipEventRecord(e4[1], st));
+    if (sh.do_change) EPQ(launch_change(D, sub, st));
+    if (e4) EPQ(hipEventRecord(e4[2], st));
+  }
+#undef EPQ
+  return e;
+}
+
 }  // namespace ep
 
 // ==========================================================================
@@ -314,7 +345,7 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
   AL(g.list, SN) AL(g.ref, SN) AL(g.nkept, S) AL(g.thin, S) AL(g.dsil, SN) AL(g.dsmin, S)
   AL(g.nsteps, SE) AL(g.halo, SE) AL(g.passes, SE) AL(g.chunks, SE) AL(g.dtprop, SE)
   AL(g.del_fac, SE * M) AL(g.seed3, SE * 3) AL(g.dsm, SE * NP) AL(g.q, SE * NP)
-  AL(g.nsub_f, SN) AL(g.nsub_c, SN) AL(g.dtsub_f, SN) AL(g.dtsub_c, SN) AL(g.f_dlnbds, SN)
+  AL(g.nsub_f, SN) AL(g.nsub_c, SN) AL(g.heavy, SN) AL(g.dtsub_f, SN) AL(g.dtsub_c, SN) AL(g.f_dlnbds, SN)
   AL(g.f_dupar, SN) AL(g.f_b, SN) AL(g.c_dupar, SN) AL(g.c_dlnn, SN) AL(g.c_dlnb, SN)
   AL(g.c_damp, SN) AL(g.c_bc0, SN) AL(g.c_bc1, SN) AL(g.diag, 1)
 #undef AL
@@ -337,6 +368,7 @@ int ep_set_params(const ep_params *params) {
   int rc = check_params(*params);
   if (rc) return rc;
   g.p = *params;
+  g.drop_graph();
   build_grids(g.p, g.d.num_energies, g.d.num_mu);
   return upload_grids();
 }
@@ -348,6 +380,83 @@ int ep_set_tuning(int32_t stream_block, int32_t stream_halo) {
     return fail(EP_ERR_INVALID_ARG, "stream_halo must be >= 1 and < stream_block/2");
   g.stream_block = stream_block;
   g.stream_halo = stream_halo;
+  g.drop_graph();
+  return EP_OK;
+}
+
+// Tile-size search for the streaming kernel.  For the current nodes it
+// times, per candidate (block, halo), every streaming pass of one substep
+// with flip = 0 (each pass reads the current slabs and writes the spare
+// buffer, so f and the step state are left as they were), keeps the
+// fastest and leaves it set.  Candidate blocks are 128..512 lanes; halos
+// keep at least half of a block's lanes in its interior.
+int ep_autotune(int32_t *best_block, int32_t *best_halo, double *best_ms) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
+  int rc = ensure_events(2);
+  if (rc) return rc;
+  hipStream_t st = g.st;
+  Dev D = make_dev();
+  DevDiag hd;
+  HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
+  HIPCHK(launch_stream_prep(D, st), "stream_prep");
+  HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
+  HIPCHK(hipStreamSynchronize(st), "prep sync");
+  if (hd.err_key != ~0ull) {
+    ep_diag dg;
+    memset(&dg, 0, sizeof(dg));
+    int code = decode_err(hd.err_key, &dg);
+    return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
+  }
+  static const int blocks[] = {128, 192, 256, 320, 384, 512};
+  static const int halos[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+  double best = 1e300;
+  int bb = g.stream_block, bh = g.stream_halo;
+  for (int W : blocks) {
+    for (int H : halos) {
+      if (4 * H > W) continue;
+      g.stream_block = W;
+      g.stream_halo = H;
+      D = make_dev();
+      HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
+      HIPCHK(launch_task_prep(D, st), "task_prep");
+      HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
+      HIPCHK(hipStreamSynchronize(st), "tune sync");
+      double t = 1e300;
+      for (int rep = 0; rep < 2; rep++) {
+        HIPCHK(hipEventRecord(g.events[0], st), "event");
+        for (int pass = 0; pass < hd.max_passes; pass++)
+          HIPCHK(launch_stream_pass(D, pass, 0, 0, 0, hd.max_chunks, st), "stream_pass");
+        HIPCHK(hipEventRecord(g.events[1], st), "event");
+        HIPCHK(hipEventSynchronize(g.events[1]), "tune sync");
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, g.events[0], g.events[1]), "elapsed");
+        if (ms < t) t = ms;
+      }
+      if (t < best) {
+        best = t;
+        bb = W;
+        bh = H;
+      }
+    }
+  }
+  g.stream_block = bb;
+  g.stream_halo = bh;
+  g.drop_graph();
+  // leave the task tables consistent with the chosen tiling
+  D = make_dev();
+  HIPCHK(launch_task_prep(D, st), "task_prep");
+  HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
+  HIPCHK(hipStreamSynchronize(st), "tune sync");
+  if (best_block) *best_block = bb;
+  if (best_halo) *best_halo = bh;
+  if (best_ms) *best_ms = best;
+  return EP_OK;
+}
+
+int ep_set_graph(int32_t enable) {
+  g.use_graph = enable != 0;
+  g.drop_graph();
   return EP_OK;
 }
 
@@ -455,19 +564,32 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
     if (out) *out = dg;
     return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
   }
-  for (int sub = 0; sub < nsub; sub++) {
-    hipEvent_t *e4 = &ev[2 + 4 * sub];
-    if (do_seed && !do_stream) HIPCHK(launch_seed_check(D, st), "seed_check");
-    if (do_stream) {
-      for (int pass = 0; pass < hd.max_passes; pass++)
-        HIPCHK(launch_stream_pass(D, pass, do_seed && pass == 0, sub, hd.max_chunks, st), "stream_pass");
-      HIPCHK(launch_stream_finish(D, sub, st), "stream_finish");
+  const StepShape shape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
+                        g.generation};
+  if (g.use_graph) {
+    // The substep loop is one HIP graph, rebuilt only when its shape (pass
+    // count, grid sizes, operators) or the parameters change.
+    if (!g.graph_exec || !(g.graph_shape == shape)) {
+      if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
+      g.graph_exec = nullptr;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+      hipError_t ce = enqueue_substeps(D, shape, st, nullptr);
+      hipGraph_t graph = nullptr;
+      hipError_t ee = hipStreamEndCapture(st, &graph);
+      if (ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(ce, "substep capture");
+      }
+      HIPCHK(ee, "end capture");
+      hipError_t ie = hipGraphInstantiate(&g.graph_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(ie, "graph instantiate");
+      g.graph_shape = shape;
     }
-    HIPCHK(hipEventRecord(e4[0], st), "event");
-    if (do_focus) HIPCHK(launch_focus(D, sub, st), "focus");
-    HIPCHK(hipEventRecord(e4[1], st), "event");
-    if (do_change) HIPCHK(launch_change(D, sub, st), "change");
-    HIPCHK(hipEventRecord(e4[2], st), "event");
+    HIPCHK(hipGraphLaunch(g.graph_exec, st), "graph launch");
+    HIPCHK(hipEventRecord(ev[2 + 4 * (nsub - 1) + 2], st), "event");
+  } else {
+    HIPCHK(enqueue_substeps(D, shape, st, ev.data()), "substeps");
   }
   HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
   HIPCHK(hipStreamSynchronize(st), "step sync");
@@ -476,7 +598,7 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   dg.device_ms = ms;
   HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]), "elapsed");
   dg.prep_ms = ms;
-  for (int sub = 0; sub < nsub; sub++) {
+  for (int sub = 0; sub < nsub && !g.use_graph; sub++) {
     hipEvent_t start = (sub == 0) ? ev[1] : ev[2 + 4 * (sub - 1) + 2];
     hipEvent_t *e4 = &ev[2 + 4 * sub];
     HIPCHK(hipEventElapsedTime(&ms, start, e4[0]), "elapsed");
@@ -542,7 +664,7 @@ int ep_selftest_dpp(const double *in64, double *shr64, double *shl64) {
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(shr64, d + 64, 64 * 8, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(shl64, d + 128, 64 * 8, hipMemcpyDeviceToHost);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "dpp selftest");
   return EP_OK;
 }

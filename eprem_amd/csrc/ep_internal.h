@@ -33,6 +33,12 @@ constexpr double kMhdDensityNorm = 1.0;
 // this bound only arise from degenerate inputs and are reported as errors.
 constexpr int kMaxSubcycles = 1 << 22;
 
+// Nodes whose focusing needs at least this many subcycles (the innermost
+// ones, where dlnB/ds is steep) are integrated with one lane per energy
+// instead of one lane per node: a node's subcycle count is the same for
+// all its energies, so those waves do not idle behind one slow lane.
+constexpr int kHeavyFocus = 32;
+
 // Error key.  The reference panics at the first bad cell in its own loop
 // order; every failing cell here computes a key that sorts in that order,
 // and the smallest key wins (one 64-bit atomicMin).  Layout, high to low:
@@ -62,6 +68,8 @@ struct DevDiag {
   unsigned long long min_tau_bits;  // positive doubles order like their bits
   unsigned long long err_key;       // ~0ull = no error
   unsigned long long stream_sweeps, focus_sweeps, change_sweeps;  // per substep
+  int focus_heavy;  // nodes in the heavy-focusing list
+  int pad_;
 };
 
 // Everything the kernels need, passed by value.
@@ -94,6 +102,7 @@ struct Dev {
   double *dsm, *q;  // [S*E*Npad] by kept index
   // node-stage coefficients [S*Npad]
   int32_t *nsub_f, *nsub_c;
+  int32_t *heavy;  // [S*Npad] node indices (s*Npad + node) with nsub_f >= kHeavyFocus
   double *dtsub_f, *dtsub_c, *f_dlnbds, *f_dupar, *f_b;
   double *c_dupar, *c_dlnn, *c_dlnb, *c_damp, *c_bc0, *c_bc1;
   DevDiag *diag;

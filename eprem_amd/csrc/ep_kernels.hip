@@ -336,6 +336,7 @@ __global__ __launch_bounds__(256) void k_node_prep(Dev D) {
     D.f_b[i] = b;
     atomicMax(&D.diag->max_sub_focus, nsub);
     atomicAdd(&D.diag->focus_sweeps, (unsigned long long)nsub * D.E * D.M);
+    if (nsub >= kHeavyFocus) D.heavy[atomicAdd(&D.diag->focus_heavy, 1)] = (int32_t)i;
   }
   // ---- adiabatic change
   {
@@ -405,7 +406,7 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
 // MPOS = first mu index with mugrid >= 0 (upwind direction).
 // ------------------------------------------------------------------------
 template <int M, int MPOS>
-__global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_seed, int substep) {
+__global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, int do_seed, int substep) {
   const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
   const int task = s * D.E + e;
   const int nsteps = D.nsteps[task];
@@ -423,7 +424,10 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int do_see
   const bool valid = (k >= 0) && (k < nk);
   const size_t on = (size_t)s * D.Npad;
   const int node = valid ? (D.thin[s] ? D.list[on + k] : k) : 0;
-  const int cur = D.loc[task] ^ (pass & 1);
+  // flip = pass & 1 in a step; the tile-size search runs every pass with
+  // flip = 0 (read the current slab, write the spare one), which leaves the
+  // state untouched
+  const int cur = D.loc[task] ^ flip;
   const uint32_t slab_bytes = (uint32_t)(M * D.Npad * 8), row = (uint32_t)D.Npad * 8;
   const rsrc_t src = slab_rsrc(fbuf(D, cur) + (size_t)task * M * D.Npad, slab_bytes);
   const rsrc_t dst = slab_rsrc(fbuf(D, cur ^ 1) + (size_t)task * M * D.Npad, slab_bytes);
@@ -563,73 +567,112 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
 
 // ------------------------------------------------------------------------
 // k_focus: AdiabaticFocusing with the upwind operator
-// (src/energeticParticles.c:1676-1803, :1806-1896).  A lane per node, the
-// four waves of a block split the energies; the mu row of one (node,
+// (src/energeticParticles.c:1676-1803, :1806-1896).  A lane per node, a
+// block per 256 nodes of one (stream, energy); the mu row of one (node,
 // energy) and its half-mesh velocities stay in registers for all
 // subcycles.  The upwind flux v*0.5*((1-cce)f[m]+(1+cce)f[m-1]) equals
 // v*f_upwind exactly for finite f, and the zero-flux walls reproduce the
 // operator's boundary rows.
 // ------------------------------------------------------------------------
+// The focusing update of one (node, energy) mu row held in registers:
+// half-mesh velocities (src/energeticParticles.c:1613-1653), then nsub
+// upwind flux subcycles with zero-flux walls, then the NaN/Inf check
+// (first bad mu) -- written out in each kernel so that the row and the
+// velocities stay in registers.
+#define EP_FOCUS_ROW(M, D, a, b, f, nsub, dtsub, bad, bad_inf)                     \
+  do {                                                                             \
+    double va_[(M) + 1];                                                           \
+    double vprev_ = 0.0;                                                           \
+    _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
+      const double mu = (D).mugrid[m];                                             \
+      const double vel = 0.5 * (1.0 - mu * mu) * ((a) + mu * (b));                 \
+      if (m > 0) va_[m] = 0.5 * (vel + vprev_);                                    \
+      vprev_ = vel;                                                                \
+    }                                                                              \
+    const double dmu_i_ = 1.0 / (D).dmu;                                           \
+    for (int it_ = 0; it_ < (nsub); ++it_) {                                       \
+      double flux_lo = 0.0; /* zero-flux wall at mu = -1 */                        \
+      _Pragma("unroll") for (int m = 0; m < (M); ++m) {                            \
+        double flux_hi = 0.0; /* zero-flux wall at mu = +1 */                      \
+        if (m + 1 < (M)) flux_hi = va_[m + 1] * (signbit(va_[m + 1]) ? f[m + 1] : f[m]); \
+        const double f1 = (flux_hi - flux_lo) * dmu_i_;                            \
+        f[m] = f[m] - (dtsub) * f1;                                                \
+        flux_lo = flux_hi;                                                         \
+      }                                                                            \
+    }                                                                              \
+    _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
+      if ((bad) < 0 && !isfinite(f[m])) {                                          \
+        (bad) = m;                                                                 \
+        (bad_inf) = isinf(f[m]) ? 1 : 0;                                           \
+      }                                                                            \
+    }                                                                              \
+  } while (0)
+
 template <int M>
 __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
-  // blockIdx.x = (stream, group of 4 energies), blockIdx.y = node chunk:
-  // the inner chunks, where focusing needs the most subcycles, are
-  // dispatched first.
-  const int groups = (D.E + 3) >> 2;
-  const int s = blockIdx.x / groups;
-  // the wave index is uniform across the wave; readfirstlane tells the
-  // compiler so, keeping e (and the slab descriptor) in scalar registers
-  const int e = (blockIdx.x % groups) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int node = blockIdx.y * 64 + (threadIdx.x & 63);
+  // blockIdx.x = ((stream, energy), chunk of 256 nodes), chunk fastest:
+  // the block reads 2 KB runs of each mu row, consecutive blocks the next.
+  // Blocks go round-robin to the 8 XCDs, so the chunk order is rotated by
+  // (stream, energy): the inner chunk (most subcycles) then lands on every
+  // XCD in turn instead of always on the same one.
+  const int nch = (D.N + 255) >> 8;
+  const int se = blockIdx.x / nch;
+  const int s = se / D.E, e = se % D.E;
+  const int chunk = (int)((blockIdx.x % nch + (unsigned)se) % nch);
+  const int node = chunk * 256 + threadIdx.x;
   const int n = D.n_active[s];
-  if (e >= D.E || node < 1 || node >= n) return;
+  if (node < 1 || node >= n) return;
   const size_t i = (size_t)s * D.Npad + node;
   const int nsub = D.nsub_f[i];
+  if (nsub >= kHeavyFocus) return;  // k_focus_heavy
   const double dtsub = D.dtsub_f[i];
   const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], b = D.f_b[i];
-  const double dmu_i = 1.0 / D.dmu;
-  const double half = 0.5, one = 1.0, two = 2.0;
-  {
+  const double two = 2.0;
+  const int task = s * D.E + e;
+  const rsrc_t slab = slab_rsrc(fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad,
+                                (uint32_t)(M * D.Npad * 8));
+  const uint32_t voff = (uint32_t)node * 8, row = (uint32_t)D.Npad * 8;
+  const double v = D.vgrid[e];
+  const double a = -v * dlnBds - (two / v) * DuParDt;
+  double f[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row);
+  int bad = -1, bad_inf = 0;
+  EP_FOCUS_ROW(M, D, a, b, f, nsub, dtsub, bad, bad_inf);
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const double val = (f[m] < kDblMin) ? kDblMin : f[m];
+    if (nsub > 0 || val != f[m]) buf_st(val, slab, voff, m * row);
+  }
+  if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
+}
+
+// k_focus_heavy: the nodes k_focus leaves out (nsub_f >= kHeavyFocus), one
+// wave per node and one lane per energy, so every lane runs the node's
+// subcycle count.  A fixed grid walks the list built by k_node_prep.
+template <int M>
+__global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
+  const int count = D.diag->focus_heavy;
+  const int e = threadIdx.x;
+  const double two = 2.0;
+  for (int h = blockIdx.x; h < count; h += gridDim.x) {
+    const size_t i = (size_t)D.heavy[h];
+    const int s = (int)(i / D.Npad), node = (int)(i % D.Npad);
+    if (e >= D.E) continue;
+    const int nsub = D.nsub_f[i];
+    const double dtsub = D.dtsub_f[i];
+    const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], b = D.f_b[i];
     const int task = s * D.E + e;
-    const rsrc_t slab = slab_rsrc(fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad,
-                                  (uint32_t)(M * D.Npad * 8));
-    const uint32_t voff = (uint32_t)node * 8, row = (uint32_t)D.Npad * 8;
+    double *fp = fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad + node;
     const double v = D.vgrid[e];
     const double a = -v * dlnBds - (two / v) * DuParDt;
-    double va[M + 1];
-    double vprev = 0.0;
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      const double mu = D.mugrid[m];
-      const double vel = half * (one - mu * mu) * (a + mu * b);
-      if (m > 0) va[m] = half * (vel + vprev);
-      vprev = vel;
-    }
     double f[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row);
-    for (int it = 0; it < nsub; ++it) {
-      double flux_lo = 0.0;  // zero-flux wall at mu = -1
-#pragma unroll
-      for (int m = 0; m < M; ++m) {
-        double flux_hi = 0.0;  // zero-flux wall at mu = +1
-        if (m + 1 < M) flux_hi = va[m + 1] * (signbit(va[m + 1]) ? f[m + 1] : f[m]);
-        const double f1 = (flux_hi - flux_lo) * dmu_i;
-        f[m] = f[m] - dtsub * f1;
-        flux_lo = flux_hi;
-      }
-    }
+    for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
     int bad = -1, bad_inf = 0;
+    EP_FOCUS_ROW(M, D, a, b, f, nsub, dtsub, bad, bad_inf);
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-      double val = f[m];
-      if (bad < 0 && !isfinite(val)) {
-        bad = m;
-        bad_inf = isinf(val) ? 1 : 0;
-      }
-      if (val < kDblMin) val = kDblMin;
-      if (nsub > 0 || val != f[m]) buf_st(val, slab, voff, m * row);
-    }
+    for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = (f[m] < kDblMin) ? kDblMin : f[m];
     if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
   }
 }
@@ -637,7 +680,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
 // ------------------------------------------------------------------------
 // k_change: AdiabaticChange with the upwind operator, advecting ln f in
 // ln p (src/energeticParticles.c:859-1067, :1081-1176).  A lane per node,
-// the four waves of a block split the mu cells.  Each subcycle is one
+// a block per 256 nodes of one (stream, mu).  Each subcycle is one
 // sweep up the energy column with a sliding window (the new ln f at e
 // needs the old values at e-1, e, e+1), reading ahead PF energies to keep
 // loads in flight; intermediate ln f is kept in place in the f slab, the
@@ -645,13 +688,13 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
 // needs a handful of registers, so the kernel runs at full occupancy.
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
-  // blockIdx.x = (stream, group of 4 mu cells), blockIdx.y = node chunk
-  const int groups = (D.M + 3) >> 2;
-  const int s = blockIdx.x / groups;
-  const int m = (blockIdx.x % groups) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int node = blockIdx.y * 64 + (threadIdx.x & 63);
+  // blockIdx.x = ((stream, mu), chunk of 256 nodes), chunk fastest
+  const int nch = (D.N + 255) >> 8;
+  const int sm = blockIdx.x / nch;
+  const int s = sm / D.M, m = sm % D.M;
+  const int node = (blockIdx.x % nch) * 256 + threadIdx.x;
   const int n = D.n_active[s];
-  if (m >= D.M || node < 1 || node >= n) return;
+  if (node < 1 || node >= n) return;
   const int E = D.E;
   const size_t i = (size_t)s * D.Npad + node;
   const int nsub = D.nsub_c[i];
@@ -793,6 +836,12 @@ __global__ __launch_bounds__(256) void k_init_seed(Dev D) {
     if (e_ != hipSuccess) return e_;   \
   } while (0)
 
+hipError_t launch_task_prep(const Dev &D, hipStream_t st) {
+  hipLaunchKernelGGL(k_task_prep, dim3((D.S * D.E + 255) / 256), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
 hipError_t launch_stream_prep(const Dev &D, hipStream_t st) {
   hipLaunchKernelGGL(k_stream_prep, dim3(D.S), dim3(256), 0, st, D);
   EP_CHECK_LAUNCH();
@@ -816,22 +865,23 @@ hipError_t launch_seed_check(const Dev &D, hipStream_t st) {
 }
 
 template <int M, int MPOS>
-static hipError_t stream_pass_t(const Dev &D, int pass, int do_seed, int substep, int chunks,
+static hipError_t stream_pass_t(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                                 hipStream_t st) {
   hipLaunchKernelGGL((k_stream_pass<M, MPOS>), dim3(chunks, D.E, D.S), dim3(D.stream_block), 0, st, D,
-                     pass, do_seed, substep);
+                     pass, flip, do_seed, substep);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
 
 #define EP_PASS_CASE(MM)                                                                 \
   case MM:                                                                               \
-    if (D.mpos == (MM) / 2) return stream_pass_t<MM, (MM) / 2>(D, pass, do_seed, sub, chunks, st); \
+    if (D.mpos == (MM) / 2) return stream_pass_t<MM, (MM) / 2>(D, pass, flip, do_seed, sub, chunks, st); \
     if (D.mpos == ((MM) + 1) / 2)                                                        \
-      return stream_pass_t<MM, ((MM) + 1) / 2>(D, pass, do_seed, sub, chunks, st);      \
+      return stream_pass_t<MM, ((MM) + 1) / 2>(D, pass, flip, do_seed, sub, chunks, st); \
     return hipErrorInvalidValue;
 
-hipError_t launch_stream_pass(const Dev &D, int pass, int do_seed, int sub, int chunks, hipStream_t st) {
+hipError_t launch_stream_pass(const Dev &D, int pass, int flip, int do_seed, int sub, int chunks,
+                              hipStream_t st) {
   switch (D.M) {
     EP_PASS_CASE(2) EP_PASS_CASE(3) EP_PASS_CASE(4) EP_PASS_CASE(5) EP_PASS_CASE(6)
     EP_PASS_CASE(7) EP_PASS_CASE(8) EP_PASS_CASE(9) EP_PASS_CASE(10) EP_PASS_CASE(11)
@@ -853,9 +903,13 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
 
 #define EP_FOCUS_CASE(MM)                                                                     \
   case MM:                                                                                    \
-    hipLaunchKernelGGL((k_focus<MM>), dim3(D.S * ((D.E + 3) / 4), (D.N + 63) / 64), dim3(256), 0, st, \
+    hipLaunchKernelGGL((k_focus<MM>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, st, \
                        D, sub);                                                               \
+    hipLaunchKernelGGL((k_focus_heavy<MM>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub);        \
     break;
+
+// Blocks of the heavy-focusing walk: enough waves to fill the chip.
+constexpr int kHeavyGrid = 4096;
 
 hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
   switch (D.M) {
@@ -874,7 +928,7 @@ hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
 }
 
 hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
-  const dim3 grid(D.S * ((D.M + 3) / 4), (D.N + 63) / 64), block(256);
+  const dim3 grid((D.N + 255) / 256 * D.S * D.M), block(256);
   hipLaunchKernelGGL(k_change, grid, block, 0, st, D, sub);
   EP_CHECK_LAUNCH();
   return hipSuccess;
@@ -933,6 +987,7 @@ __global__ void k_reset_diag(DevDiag *d) {
   d->stream_sweeps = 0;
   d->focus_sweeps = 0;
   d->change_sweeps = 0;
+  d->focus_heavy = 0;
 }
 
 hipError_t launch_reset_diag(DevDiag *d, hipStream_t st) {

@@ -8,8 +8,9 @@ namespace ep {
 hipError_t launch_stream_prep(const Dev &D, hipStream_t st);
 hipError_t launch_node_prep(const Dev &D, hipStream_t st);
 hipError_t launch_seed_check(const Dev &D, hipStream_t st);
-hipError_t launch_stream_pass(const Dev &D, int pass, int do_seed, int substep, int chunks,
+hipError_t launch_stream_pass(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                               hipStream_t st);
+hipError_t launch_task_prep(const Dev &D, hipStream_t st);
 hipError_t launch_stream_finish(const Dev &D, int substep, hipStream_t st);
 hipError_t launch_focus(const Dev &D, int substep, hipStream_t st);
 hipError_t launch_change(const Dev &D, int substep, hipStream_t st);

@@ -140,7 +140,7 @@ struct MhdArgs {
   double s, phi_offset, radial_max, vmin;
 };
 
-__global__ __launch_bounds__(256) void k_update_mhd(Dev D, MasDev M, MhdArgs A, const double *prev,
+__global__ __launch_bounds__(256) void k_update_mhd(Dev D, MasDev M, MhdArgs A, double *prev,
                                                     double *out, double *bmag, double *dlnb,
                                                     double *dlnn, double *dupar) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,11 +180,19 @@ __global__ __launch_bounds__(256) void k_update_mhd(Dev D, MasDev M, MhdArgs A, 
     dlnb[i] = log(bm / bm_old);
     dlnn[i] = log(v[6] / prev[9 * SN + i]);
   }
+  // the new values become the node's MHD state (the reference keeps them
+  // in Node_t for the next updateMhd and for moveNodes); each thread only
+  // touches its own node, after its reads
+  for (int q = 0; q < 7; q++) prev[(3 + q) * SN + i] = v[q];
 }
 
 }  // namespace ep
 
 using namespace ep;
+
+static int ensure_mas_buffers();
+
+int ensure_mhd_state() { return ensure_mas_buffers(); }
 
 static int ensure_mas_buffers() {
   const size_t SN = (size_t)g.d.num_streams * g.Npad;
@@ -289,19 +297,19 @@ int ep_interp_mhd(const ep_mas_params *mp, const ep_mas_prev *prev, ep_mas_out *
   }
   MasDev M = mas_dev();
   Dev D = make_dev();
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventRecord(e0, g.st);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, g.st);
   hipLaunchKernelGGL(k_mas_interp, dim3((N + 255) / 256, S), dim3(256), 0, g.st, D, M, mp->s_cor,
                      mp->phi_offset, mp->mas_radial_max, mp->mas_vmin, g.mas_prev, g.mas_out);
   hipError_t le = hipGetLastError();
-  hipEventRecord(e1, g.st);
+  (void)hipEventRecord(e1, g.st);
   hipError_t se = hipStreamSynchronize(g.st);
   float ms = 0;
-  hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (le != hipSuccess) return hip_fail(le, "k_mas_interp launch");
   if (se != hipSuccess) return hip_fail(se, "k_mas_interp");
   if (device_ms) *device_ms = ms;
@@ -345,7 +353,9 @@ int ep_set_positions(const double *x, const double *y, const double *z, const do
 int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *out, double *device_ms) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
   if (!g.positions_set && !g.nodes_set) return fail(EP_ERR_INVALID_ARG, "set node positions first");
-  if (!mp || !prev) return fail(EP_ERR_INVALID_ARG, "null MHD params / previous state");
+  if (!mp) return fail(EP_ERR_INVALID_ARG, "null MHD params");
+  if (!prev && !g.mhd_state)
+    return fail(EP_ERR_INVALID_ARG, "no previous MHD state on the device: pass prev");
   if (mp->mode != 0 && mp->mode != 1) return fail(EP_ERR_INVALID_ARG, "mode must be 0 or 1");
   if (mp->mode == 1) {
     if (!g.mas[0].set || !g.mas[1].set) return fail(EP_ERR_INVALID_ARG, "set both MAS levels first");
@@ -354,13 +364,15 @@ int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *
   const size_t S = g.d.num_streams, N = g.d.num_nodes, SN = S * g.Npad;
   int rc = ensure_mas_buffers();
   if (rc) return rc;
-  const double *src[10] = {prev->rold_x, prev->rold_y, prev->rold_z, prev->br, prev->btheta,
-                           prev->bphi, prev->vr, prev->vtheta, prev->vphi, prev->density};
-  for (int i = 0; i < 10; i++) {
-    if (!src[i]) return fail(EP_ERR_INVALID_ARG, "prev array %d is null", i);
-    hipError_t e = hipMemcpy2D(g.mas_prev + i * SN, g.Npad * 8, src[i], N * 8, N * 8, S,
-                               hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hip_fail(e, "prev upload");
+  if (prev) {
+    const double *src[10] = {prev->rold_x, prev->rold_y, prev->rold_z, prev->br, prev->btheta,
+                             prev->bphi, prev->vr, prev->vtheta, prev->vphi, prev->density};
+    for (int i = 0; i < 10; i++) {
+      if (!src[i]) return fail(EP_ERR_INVALID_ARG, "prev array %d is null", i);
+      hipError_t e = hipMemcpy2D(g.mas_prev + i * SN, g.Npad * 8, src[i], N * 8, N * 8, S,
+                                 hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_fail(e, "prev upload");
+    }
   }
   MasDev M = mas_dev();
   MhdArgs A;
@@ -391,6 +403,7 @@ int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *
   if (se != hipSuccess) return hip_fail(se, "k_update_mhd");
   if (device_ms) *device_ms = ms;
   g.nodes_set = true;
+  g.mhd_state = true;
   if (out) {
     double *dst[7] = {out->br, out->btheta, out->bphi, out->vr, out->vtheta, out->vphi, out->density};
     for (int i = 0; i < 7; i++) {

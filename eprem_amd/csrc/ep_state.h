@@ -17,6 +17,17 @@ struct MasLevel {
   bool set = false;
 };
 
+// What a captured substep graph depends on besides device memory.
+struct StepShape {
+  int passes, chunks, nsub, do_stream, do_focus, do_change, do_seed;
+  unsigned generation;
+  bool operator==(const StepShape &o) const {
+    return passes == o.passes && chunks == o.chunks && nsub == o.nsub && do_stream == o.do_stream &&
+           do_focus == o.do_focus && do_change == o.do_change && do_seed == o.do_seed &&
+           generation == o.generation;
+  }
+};
+
 struct State {
   bool init = false, nodes_set = false, positions_set = false;
   ep_dims d{};
@@ -24,6 +35,17 @@ struct State {
   int Npad = 0;
   hipStream_t st = nullptr;
   int stream_block = 256, stream_halo = 32;
+  // graph of the substep loop; `generation` changes whenever kernel
+  // arguments captured by value (parameters, tiling) change
+  bool use_graph = true;
+  unsigned generation = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  StepShape graph_shape{};
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    generation++;
+  }
   // host copies of the grids
   std::vector<double> egrid, vgrid, vgrid_i, rigidity, rig_lamo, plt, ext, mugrid;
   double dlnp = 0, dmu = 0, normJ0 = 0, normRadius = 0;
@@ -42,7 +64,7 @@ struct State {
   double *dsil = nullptr, *dsmin = nullptr;
   int32_t *nsteps = nullptr, *halo = nullptr, *passes = nullptr, *chunks = nullptr;
   double *dtprop = nullptr, *del_fac = nullptr, *seed3 = nullptr, *dsm = nullptr, *q = nullptr;
-  int32_t *nsub_f = nullptr, *nsub_c = nullptr;
+  int32_t *nsub_f = nullptr, *nsub_c = nullptr, *heavy = nullptr;
   double *dtsub_f = nullptr, *dtsub_c = nullptr, *f_dlnbds = nullptr, *f_dupar = nullptr,
          *f_b = nullptr, *c_dupar = nullptr, *c_dlnn = nullptr, *c_dlnb = nullptr,
          *c_damp = nullptr, *c_bc0 = nullptr, *c_bc1 = nullptr;
@@ -54,16 +76,22 @@ struct State {
   std::vector<void *> mas_allocs;
   double *mas_out = nullptr;   // 7 x S x Npad
   double *mas_prev = nullptr;  // 10 x S x Npad
-  // point observers and stream flux (allocated on first use)
-  double *obs_buf = nullptr, *flux_buf = nullptr;
-  size_t obs_cap = 0, flux_cap = 0;
+  // the MHD buffer's state half holds the latest B, V, density per node
+  // once ep_update_mhd has run (ep_move_nodes reads V from it)
+  bool mhd_state = false;
+  // point observers, stream flux and shell-spawn staging (allocated on
+  // first use)
+  double *obs_buf = nullptr, *flux_buf = nullptr, *spawn_buf = nullptr;
+  size_t obs_cap = 0, flux_cap = 0, spawn_cap = 0;
   void mas_free() {
     for (void *p : mas_allocs) (void)hipFree(p);
     mas_allocs.clear();
     if (obs_buf) (void)hipFree(obs_buf);
     if (flux_buf) (void)hipFree(flux_buf);
-    obs_buf = flux_buf = nullptr;
-    obs_cap = flux_cap = 0;
+    if (spawn_buf) (void)hipFree(spawn_buf);
+    obs_buf = flux_buf = spawn_buf = nullptr;
+    obs_cap = flux_cap = spawn_cap = 0;
+    mhd_state = false;
     mas[0] = MasLevel();
     mas[1] = MasLevel();
     mas_out = nullptr;

@@ -64,3 +64,34 @@ def parse_digest(text):
         elif line.startswith("  sha256="):
             out[cur_file][cur_var]["sha256"] = line.strip()[7:]
     return out
+
+
+def digest_file(path):
+    """Digest lines of one netCDF file, variables in sorted order, scalar
+    variables skipped, values scaled by their scale_factor as a netCDF
+    reader returns them (bin/epremdigest.py:100-113)."""
+    import os
+    from scipy.io import netcdf_file
+
+    lines = ["file %s" % os.path.basename(path)]
+    with netcdf_file(path, "r", mmap=False, maskandscale=True) as ds:
+        for k in sorted(ds.variables.keys()):
+            v = ds.variables[k]
+            if len(v.shape) == 0:
+                continue
+            arr = np.asarray(v[:])
+            arr = arr.astype(arr.dtype.newbyteorder("="))
+            lines.extend(digest_variable(k, arr))
+    return lines
+
+
+def digest_dir(rundir):
+    """Digest text of every obs*.nc file of a run directory."""
+    import glob
+    import os
+
+    lines = ["# eprem_digest granularity=global"]
+    for f in sorted(glob.glob(os.path.join(rundir, "obs*.nc"))):
+        lines.extend(digest_file(f))
+    lines.append("")
+    return "\n".join(lines)

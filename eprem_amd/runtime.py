@@ -56,6 +56,11 @@ def lib():
         L.ep_stream_flux.argtypes = [DP]
         L.ep_set_positions.argtypes = [DP, DP, DP, DP, IP]
         L.ep_update_mhd.argtypes = [C.POINTER(abi.EpMhdParams), C.POINTER(EpMasPrev), C.POINTER(EpMasOut), DP]
+        L.ep_autotune.argtypes = [IP, IP, DP]
+        L.ep_spawn_shell.argtypes = [C.POINTER(abi.EpSpawn)]
+        L.ep_move_nodes.argtypes = [C.c_double]
+        L.ep_get_positions.argtypes = [DP, DP, DP, DP, IP]
+        L.ep_set_graph.argtypes = [C.c_int32]
         for name in abi.EXPORTED_SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -98,6 +103,15 @@ class EpSolver:
 
     def set_tuning(self, stream_block, stream_halo):
         _check(lib().ep_set_tuning(int(stream_block), int(stream_halo)))
+
+    def autotune(self):
+        """Tile-size search for the streaming kernel; returns (block, halo, ms)."""
+        b, h, ms = C.c_int32(), C.c_int32(), C.c_double()
+        _check(lib().ep_autotune(C.byref(b), C.byref(h), C.byref(ms)))
+        return b.value, h.value, ms.value
+
+    def set_graph(self, enable):
+        _check(lib().ep_set_graph(1 if enable else 0))
 
     def set_params(self, params):
         self.params = params
@@ -178,19 +192,42 @@ class EpSolver:
         na = None if n_active is None else np.ascontiguousarray(n_active, dtype=np.int32)
         _check(lib().ep_set_positions(*[dptr(a) for a in arrs], iptr(na)))
 
+    def spawn_shell(self, x, y, z, rmag, f=None):
+        """Ripple every node one shell outward and make node 0 the new shell
+        (positions [S]; f [S, E, M] or None for the seed spectrum)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (x, y, z, rmag)]
+        fa = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+        sp = abi.EpSpawn(*[dptr(a) for a in arrs], dptr(fa))
+        _check(lib().ep_spawn_shell(C.byref(sp)))
+
+    def move_nodes(self, tdel):
+        _check(lib().ep_move_nodes(float(tdel)))
+
+    def positions(self):
+        S, N = self.dims.num_streams, self.dims.num_nodes
+        out = {k: np.zeros((S, N)) for k in ("x", "y", "z", "rmag")}
+        na = np.zeros(S, dtype=np.int32)
+        _check(lib().ep_get_positions(*[dptr(out[k]) for k in ("x", "y", "z", "rmag")], iptr(na)))
+        out["n_active"] = na
+        return out
+
     def update_mhd(self, prev, mode=0, mhdUs=0.0, mhdNsAu=0.0, mhdBsAu=0.0, omegaSun=0.0, s_cor=0.0,
                    phi_offset=0.0, mas_radial_max=0.0, mas_vmin=0.0):
-        """updateMhd on the device; returns host copies of the new B, V, n."""
+        """updateMhd on the device; returns host copies of the new B, V, n.
+        prev=None uses the MHD state the device holds from the last call."""
         S, N = self.dims.num_streams, self.dims.num_nodes
         names = ("rold_x", "rold_y", "rold_z", "br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")
-        arrs = [np.ascontiguousarray(prev[k], dtype=np.float64).reshape(S, N) for k in names]
-        pv = EpMasPrev(*[dptr(a) for a in arrs])
+        pv = None
+        if prev is not None:
+            arrs = [np.ascontiguousarray(prev[k], dtype=np.float64).reshape(S, N) for k in names]
+            pv = EpMasPrev(*[dptr(a) for a in arrs])
         outs = {k: np.zeros((S, N)) for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")}
         mo = EpMasOut(*[dptr(outs[k]) for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")])
         mp = abi.EpMhdParams(mode, mhdUs, mhdNsAu, mhdBsAu, omegaSun,
                              EpMasParams(s_cor, phi_offset, mas_radial_max, mas_vmin))
         ms = C.c_double()
-        _check(lib().ep_update_mhd(C.byref(mp), C.byref(pv), C.byref(mo), C.byref(ms)))
+        _check(lib().ep_update_mhd(C.byref(mp), C.byref(pv) if pv is not None else None, C.byref(mo),
+                                   C.byref(ms)))
         outs["device_ms"] = ms.value
         return outs
 

@@ -245,6 +245,37 @@ typedef struct ep_mhd_params {
  * `out` (may be NULL) receives host copies of B, V, density. */
 int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *out,
                   double *device_ms);
+/* With prev = NULL, ep_update_mhd uses the state the device already holds:
+ * the B, V and density its previous call computed (carried along by
+ * ep_spawn_shell) and the old positions ep_move_nodes recorded. */
+
+/* ---------------- node motion on the device ------------------------ */
+
+/* The new innermost shell of each stream: position [num_streams] in rScale
+ * units and, optionally, its distribution [num_streams][E][M] (NULL: the
+ * seed spectrum sepSeedFunction gives at its radius). */
+typedef struct ep_spawn {
+  const double *x, *y, *z, *rmag;
+  const double *f;
+} ep_spawn;
+
+/* rotSunAndSpawnShell's ripple and spawnNewShell (src/simCore.c:442-728):
+ * every node record (position, MHD state, f) moves one node outward, the
+ * outermost one is dropped once a stream is full, node 0 becomes the new
+ * shell, and n_active grows by one up to num_nodes.  The host computes the
+ * spawn positions (the rotating template shell). */
+int ep_spawn_shell(const ep_spawn *sp);
+
+/* moveNodes with the analytic-wind flow (src/simCore.c:736-785,
+ * delrFlow src/flow.c:169-178): every node but node 0 moves by
+ * tdel * V / rScale, V the node's flow velocity from the last
+ * ep_update_mhd made Cartesian at the node; old positions are kept for the
+ * next ep_update_mhd.  tdel in code units. */
+int ep_move_nodes(double tdel);
+
+/* Host copies of the device node positions and active counts (any pointer
+ * may be NULL). */
+int ep_get_positions(double *x, double *y, double *z, double *rmag, int32_t *n_active);
 
 /* ---------------- observers and flux -------------------------------- */
 
@@ -278,6 +309,18 @@ int ep_finalize(void);
  * in the reference (its DiffuseStreamData loops on the CPU); set by the
  * autotuner or left at the defaults (256, 32). */
 int ep_set_tuning(int32_t stream_block, int32_t stream_halo);
+
+/* Tile-size search for the streaming kernel on the current nodes: times
+ * every streaming pass of one substep for each candidate (block, halo)
+ * without changing f or the step state, and sets the fastest.  Any output
+ * pointer may be NULL.  No counterpart in the reference. */
+int ep_autotune(int32_t *stream_block, int32_t *stream_halo, double *stream_ms);
+
+/* ep_step runs its substep loop as one captured HIP graph (enable = 1, the
+ * default; rebuilt when the pass count, grid or parameters change) or as
+ * individual launches with per-operator timing in ep_diag (enable = 0).
+ * No counterpart in the reference. */
+int ep_set_graph(int32_t enable);
 
 /* Device self-test of the cross-lane shift the streaming kernel uses: for
  * one 64-lane wave, shr[i] = in[i-1] (shr[0] = -1) and shl[i] = in[i+1]

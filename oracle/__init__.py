@@ -46,6 +46,8 @@ def lib():
         L.or_observers.argtypes = ([C.POINTER(EpDims), IP, C.c_double, C.c_double, C.c_int] +
                                    [DP] * 3 + [DP] * 4 + [DP, DP, DP, DP])
         L.or_stream_flux.argtypes = [C.POINTER(EpDims), DP, DP, DP]
+        L.or_move_nodes.argtypes = [C.c_int, C.c_double, C.c_double] + [DP] * 10
+        L.or_move_nodes.restype = None
         L.or_update_mhd.argtypes = ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, DP, DP, DP, DP, IP,
                                      C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_float)), IP,
@@ -90,7 +92,7 @@ class SimOutput(C.Structure):
                 [(n, C.POINTER(C.c_double)) for n in
                  ("R", "T", "P", "Br", "Bt", "Bp", "Vr", "Vt", "Vp", "Rho", "dist",
                   "egrid", "vgrid", "mu")] +
-                [("step_diag", C.POINTER(EpDiag))])
+                [("step_diag", C.POINTER(EpDiag)), ("raw", C.c_int)])
 
 
 def build_grids(p, E, M):
@@ -167,8 +169,10 @@ def sim_config_from(cfg):
     return c
 
 
-def run_parker(cfg, max_dumps=64, max_steps=4096, nthreads=0):
-    """Run the analytic-wind pipeline; returns a dict of numpy outputs."""
+def run_parker(cfg, max_dumps=64, max_steps=4096, nthreads=0, raw=False):
+    """Run the analytic-wind pipeline; returns a dict of numpy outputs
+    (physical units, or the code-unit values the output file holds when
+    raw is set)."""
     c = sim_config_from(cfg)
     S, N, E, M = 6 * c.rows * c.cols, c.num_nodes, c.num_energies, c.num_mu
     bufs = {"time": np.zeros(max_dumps)}
@@ -179,6 +183,7 @@ def run_parker(cfg, max_dumps=64, max_steps=4096, nthreads=0):
     diags = (EpDiag * max_steps)()
     o = SimOutput()
     o.max_dumps, o.max_steps = max_dumps, max_steps
+    o.raw = 1 if raw else 0
     for k, v in bufs.items():
         setattr(o, k, dptr(v))
     o.step_diag = C.cast(diags, C.POINTER(EpDiag))
@@ -280,3 +285,26 @@ def update_mhd(nodes, prev, mode, rscale, mhdUs=0.0, mhdNsAu=0.0, mhdBsAu=0.0, o
                         *[dptr(a) for a in pv], *[dptr(a) for a in outs])
     keys = ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density", "bmag", "dupar", "dlnb", "dlnn")
     return {k: v.reshape(S, N) for k, v in zip(keys, outs)}
+
+
+def move_nodes(nodes, state, tdel, rscale, first=1):
+    """or_move_nodes on [S, N] arrays in place: nodes `first`.. of every
+    stream move; returns the old positions (rold_x, rold_y, rold_z), which
+    are the unmoved positions for nodes before `first`."""
+    S, N = nodes["x"].shape
+    pos = {k: np.ascontiguousarray(nodes[k], dtype=np.float64) for k in ("x", "y", "z", "rmag")}
+    old = {k: pos[c].copy() for k, c in (("rold_x", "x"), ("rold_y", "y"), ("rold_z", "z"))}
+    v = {k: np.ascontiguousarray(state[k], dtype=np.float64) for k in ("vr", "vtheta", "vphi")}
+    n = N - first
+    for s in range(S):
+        args = [pos["x"][s, first:], pos["y"][s, first:], pos["z"][s, first:], pos["rmag"][s, first:],
+                old["rold_x"][s, first:], old["rold_y"][s, first:], old["rold_z"][s, first:],
+                v["vr"][s, first:], v["vtheta"][s, first:], v["vphi"][s, first:]]
+        args = [np.ascontiguousarray(a) for a in args]
+        lib().or_move_nodes(n, tdel, rscale, *[dptr(a) for a in args])
+        for k, a in zip(("x", "y", "z", "rmag"), args[:4]):
+            pos[k][s, first:] = a
+        for k, a in zip(("rold_x", "rold_y", "rold_z"), args[4:7]):
+            old[k][s, first:] = a
+    nodes.update(pos)
+    return old

@@ -1009,6 +1009,37 @@ static void node_copy(or_state *st, size_t dst, size_t src, size_t cells_node) {
   memcpy(st->f + dst * cells_node, st->f + src * cells_node, sizeof(double) * cells_node);
 }
 
+/* moveNodes with delrFlow MHD_DEFAULT (src/simCore.c:736-785,
+   src/flow.c:169-178) for n nodes: the flow velocity (vr, vt, vp) made
+   Cartesian at the node (src/flow.c:355-403), old position saved, node
+   moved by tDel * V / rScale. */
+void or_move_nodes(int n, double tDel, double rScale, double *x, double *y, double *z, double *rmag,
+                   double *rox, double *roy, double *roz, const double *vr, const double *vt,
+                   const double *vp) {
+  int k;
+  for (k = 0; k < n; k++) {
+    const double theta = acos(z[k] / rmag[k]);
+    const double phi = atan2(y[k], x[k]);
+    const double vvx = vr[k] * sin(theta) * cos(phi) + vt[k] * cos(theta) * cos(phi) - vp[k] * sin(phi);
+    const double vvy = vr[k] * sin(theta) * sin(phi) + vt[k] * cos(theta) * sin(phi) + vp[k] * cos(phi);
+    const double vvz = vr[k] * cos(theta) - vt[k] * sin(theta);
+    double dx = tDel * vvx / rScale;
+    double dy = tDel * vvy / rScale;
+    double dz = tDel * vvz / rScale;
+    double rx = x[k], ry = y[k], rz = z[k];
+    rox[k] = rx;
+    roy[k] = ry;
+    roz[k] = rz;
+    rx += dx;
+    ry += dy;
+    rz += dz;
+    x[k] = rx;
+    y[k] = ry;
+    z[k] = rz;
+    rmag[k] = sqrt((rx * rx) + (ry * ry) + (rz * rz));
+  }
+}
+
 int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads) {
   const int R = c->rows, C = c->cols, N = c->num_nodes, E = c->num_energies, M = c->num_mu;
   const int S = 6 * R * C, L = N + 1; /* shells per stream incl. template */
@@ -1174,7 +1205,7 @@ int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads) {
     if ((c->dumpFreq > 0) && (num_loops % c->dumpFreq) == 0 &&
         (t_global * OR_DAY >= unifiedOutputTime) && out->num_dumps < out->max_dumps) {
       int dd = out->num_dumps;
-      out->time[dd] = t_global * OR_DAY;
+      out->time[dd] = out->raw ? t_global : t_global * OR_DAY;
       for (s = 0; s < S; s++)
         for (k = 0; k < N; k++) {
           size_t i = (size_t)s * L + k + 1;
@@ -1183,18 +1214,19 @@ int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads) {
           double azi = atan2(st.y[i], st.x[i]);
           if (azi < 0.0) azi += 2.0 * OR_PI;
           if (azi > (2.0 * OR_PI)) azi -= 2.0 * OR_PI;
-          out->R[o] = st.rmag[i] * c->rScale;
+          out->R[o] = out->raw ? st.rmag[i] : st.rmag[i] * c->rScale;
           out->T[o] = zen;
           out->P[o] = azi;
-          out->Br[o] = st.br[i] * (OR_MHD_B_NORM * 1.0e5);
-          out->Bt[o] = st.bt[i] * (OR_MHD_B_NORM * 1.0e5);
-          out->Bp[o] = st.bp[i] * (OR_MHD_B_NORM * 1.0e5);
-          out->Vr[o] = st.vr[i] * (OR_C / 1.0e5);
-          out->Vt[o] = st.vt[i] * (OR_C / 1.0e5);
-          out->Vp[o] = st.vp[i] * (OR_C / 1.0e5);
-          out->Rho[o] = st.dens[i] * OR_MHD_DENSITY_NORM;
+          out->Br[o] = out->raw ? st.br[i] : st.br[i] * (OR_MHD_B_NORM * 1.0e5);
+          out->Bt[o] = out->raw ? st.bt[i] : st.bt[i] * (OR_MHD_B_NORM * 1.0e5);
+          out->Bp[o] = out->raw ? st.bp[i] : st.bp[i] * (OR_MHD_B_NORM * 1.0e5);
+          out->Vr[o] = out->raw ? st.vr[i] : st.vr[i] * (OR_C / 1.0e5);
+          out->Vt[o] = out->raw ? st.vt[i] : st.vt[i] * (OR_C / 1.0e5);
+          out->Vp[o] = out->raw ? st.vp[i] : st.vp[i] * (OR_C / 1.0e5);
+          out->Rho[o] = out->raw ? st.dens[i] : st.dens[i] * OR_MHD_DENSITY_NORM;
           for (e = 0; e < (int)cells_node; e++)
-            out->dist[o * cells_node + e] = st.f[i * cells_node + e] * (1.0 / 27.0);
+            out->dist[o * cells_node + e] =
+                out->raw ? st.f[i * cells_node + e] : st.f[i * cells_node + e] * (1.0 / 27.0);
         }
       out->num_dumps++;
     }
@@ -1221,27 +1253,14 @@ int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads) {
 
     /* moveNodes (src/simCore.c:736-785), delrFlow MHD_DEFAULT
        (src/flow.c:169-178): active shells 2..N move */
-    for (s = 0; s < S; s++)
-      for (k = 2; k < L; k++) {
-        size_t i = (size_t)s * L + k;
-        double dx = tDel * st.vvx[i] / c->rScale;
-        double dy = tDel * st.vvy[i] / c->rScale;
-        double dz = tDel * st.vvz[i] / c->rScale;
-        double rx = st.x[i], ry = st.y[i], rz = st.z[i];
-        st.rox[i] = rx;
-        st.roy[i] = ry;
-        st.roz[i] = rz;
-        rx += dx;
-        ry += dy;
-        rz += dz;
-        st.x[i] = rx;
-        st.y[i] = ry;
-        st.z[i] = rz;
-        st.rmag[i] = sqrt((rx * rx) + (ry * ry) + (rz * rz));
-      }
+    for (s = 0; s < S; s++) {
+      size_t i = (size_t)s * L + 2;
+      or_move_nodes(L - 2, tDel, c->rScale, st.x + i, st.y + i, st.z + i, st.rmag + i, st.rox + i,
+                    st.roy + i, st.roz + i, st.vr + i, st.vt + i, st.vp + i);
+    }
 
     /* updateMhd (src/flow.c:33-154) with the Parker model, through
-       or_update_mhd; then the Cartesian V used by moveNodes */
+       or_update_mhd (moveNodes makes V Cartesian at the same position) */
     for (s = 0; s < S; s++) {
       const size_t i0 = (size_t)s * L + 1;
       const int na = L - 1;
@@ -1261,16 +1280,6 @@ int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads) {
                     ob + 4 * na, ob + 5 * na, ob + 6 * na, st.br + i0, st.bt + i0, st.bp + i0,
                     st.vr + i0, st.vt + i0, st.vp + i0, st.dens + i0, st.bmag + i0,
                     st.dupar + i0, st.dlnb + i0, st.dlnn + i0);
-      for (k = 0; k < na; k++) {
-        const size_t i = i0 + k;
-        const double theta = acos(st.z[i] / st.rmag[i]);
-        const double phi = atan2(st.y[i], st.x[i]);
-        st.vvx[i] = st.vr[i] * sin(theta) * cos(phi) + st.vt[i] * cos(theta) * cos(phi) -
-                    st.vp[i] * sin(phi);
-        st.vvy[i] = st.vr[i] * sin(theta) * sin(phi) + st.vt[i] * cos(theta) * sin(phi) +
-                    st.vp[i] * cos(phi);
-        st.vvz[i] = st.vr[i] * cos(theta) - st.vt[i] * sin(theta);
-      }
       free(ob);
       free(first);
     }

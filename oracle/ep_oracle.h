@@ -143,7 +143,7 @@ typedef struct or_sim_config {
 
 /* Per-dump stream output (writeObserverDataNetCDF,
  * src/unifiedOutput.c:442-549), values already multiplied by the
- * netCDF scale_factor (physical units).  Buffers are sized by the caller
+ * netCDF scale_factor (physical units) unless raw is set.  Buffers are sized by the caller
  * for max_dumps dumps: time[max_dumps], node fields
  * [max_dumps][S][N] (R, T, P, Br, Bt, Bp, Vr, Vt, Vp, Rho),
  * dist [max_dumps][S][N][E][M].  Per-step diagnostics go to
@@ -155,7 +155,16 @@ typedef struct or_sim_output {
   double *dist;
   double *egrid, *vgrid, *mu;
   ep_diag *step_diag;
+  int raw;  /* nonzero: store the code-unit values the file holds, before
+               the scale_factor a netCDF reader applies */
 } or_sim_output;
+
+/* moveNodes for the analytic wind (src/simCore.c:736-785): n nodes move
+ * by tDel * V / rScale, V = (vr, vt, vp) made Cartesian at the node; the
+ * old positions go to rox/roy/roz. */
+void or_move_nodes(int n, double tDel, double rScale, double *x, double *y, double *z, double *rmag,
+                   double *rox, double *roy, double *roz, const double *vr, const double *vt,
+                   const double *vp);
 
 /* Run the whole analytic-wind simulation.  Returns EP_OK or an error. */
 int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads);

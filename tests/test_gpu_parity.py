@@ -306,3 +306,158 @@ def test_update_mhd_parity(mode):
     assert rc == 0
     assert dg["max_subcycles_focusing"] == do["max_subcycles_focusing"]
     assert max_rel(fg, fo) <= RTOL
+
+
+def test_graph_replays_match_individual_launches():
+    """The captured substep graph is replayed on every later step; each
+    replay, with eager library work and host synchronisation between the
+    replays, must give exactly what individual launches give."""
+    p = params(numEpSteps=3)
+    nodes = workloads.parker_nodes(1, 1, 90, seed=21, perturb=0.2, jitter=0.2)
+    dims = workloads.dims_of(nodes, 12, 8)
+    outs = {}
+    for graph in (False, True):
+        sol = runtime.EpSolver(dims, p)
+        try:
+            sol.set_graph(graph)
+            f0 = workloads.seeded_f(p, sol.grids(), nodes, noise=0.3, seed=5)
+            sol.set_nodes(nodes)
+            sol.upload_f(f0)
+            seq = []
+            for _ in range(4):
+                _, dg = sol.step()
+                seq.append(sol.download_f())
+                sol.stream_flux()  # eager kernels between replays
+                if graph:
+                    assert dg["stream_ms"] == 0.0 and dg["device_ms"] > 0.0
+                else:
+                    assert dg["stream_ms"] > 0.0
+            outs[graph] = seq
+        finally:
+            sol.close()
+    for a, b in zip(outs[False], outs[True]):
+        np.testing.assert_array_equal(a, b)
+    # and the first step agrees with the oracle
+    fo = workloads.seeded_f(p, oracle.build_grids(p, 12, 8), nodes, noise=0.3, seed=5)
+    _, _, fo, _, _, _ = oracle.update_ep(dims, p, nodes, fo)
+    assert max_rel(outs[True][0], fo) <= RTOL
+
+
+def test_autotune_leaves_state_and_parity():
+    """The tile-size search times passes without changing f; the tiling it
+    picks gives results within the parity bar."""
+    p = params()
+    nodes = workloads.parker_nodes(1, 1, 300, seed=22, perturb=0.2, jitter=0.2)
+    dims = workloads.dims_of(nodes, 16, 12)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        f0 = workloads.seeded_f(p, sol.grids(), nodes, noise=0.3, seed=6)
+        sol.set_nodes(nodes)
+        sol.upload_f(f0)
+        block, halo, ms = sol.autotune()
+        assert block in (128, 192, 256, 320, 384, 512) and 1 <= halo and 4 * halo <= block and ms > 0
+        np.testing.assert_array_equal(sol.download_f(), f0)
+        _, dg = sol.step()
+        fg = sol.download_f()
+    finally:
+        sol.close()
+    _, do, fo, _, _, _ = oracle.update_ep(dims, p, nodes, f0)
+    assert_diag_equal([(0, dg)], [(0, do)])
+    assert max_rel(fg, fo) <= RTOL
+
+
+def test_heavy_focusing_nodes_parity():
+    """Inner nodes with steep dlnB/ds need hundreds of focusing subcycles
+    and take the one-lane-per-energy kernel; the rest take the one-lane-per-
+    node kernel.  Both must match the oracle."""
+    p = params(rScale=0.00465047, numEpSteps=2)
+    nodes = workloads.parker_nodes(1, 1, 300, rscale=0.00465047, seed=23, perturb=0.1, smooth=20,
+                                   jitter=0.02)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 20, 30)
+    assert od[0][1]["max_subcycles_focusing"] >= 64  # heavy path taken
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo) <= RTOL
+
+
+def _ripple(a, n_active, head):
+    """Shift every stream's node axis one node outward (numpy restatement of
+    the reference's memcpy ripple); node 0 takes `head`."""
+    out = a.copy()
+    for s in range(a.shape[0]):
+        n_new = min(n_active[s] + 1, a.shape[1])
+        out[s, 1:n_new] = a[s, 0:n_new - 1]
+        out[s, 0] = head[s]
+    return out
+
+
+def test_device_node_pipeline_parity():
+    """Global steps with node motion on the device -- spawn/ripple, move,
+    updateMhd from the device-held state, EP update -- against the oracle
+    doing the same steps (or_move_nodes is the function the oracle's
+    analytic-wind pipeline uses, pinned by the wind digests)."""
+    from eprem_amd import constants as K, mas_synth
+    rs = 0.00465047
+    p = params(rScale=rs, numEpSteps=2)
+    E, M = 10, 8
+    nodes = workloads.parker_nodes(1, 1, 70, tdel_days=1.0 / 64, rscale=rs, seed=31, jitter=0.1)
+    S, N = nodes["x"].shape
+    n_active = np.full(S, N, np.int32)
+    n_active[::2] = N - 3  # some streams still filling up
+    prev = mas_synth.prev_from_nodes(nodes, rs)
+    kw = dict(mhdUs=400e5 / K.C, mhdNsAu=8.3, mhdBsAu=2.5e-5 / K.MHD_B_NORM, omegaSun=0.0014)
+    tdel = p.tDel
+    g = oracle.build_grids(p, E, M)
+    f = workloads.seeded_f(p, g, nodes, noise=0.2, seed=8)
+    spawn_r = nodes["rmag"][:, 0].copy()
+    zen0 = np.arccos(nodes["z"][:, 0] / spawn_r)
+    azi0 = np.arctan2(nodes["y"][:, 0], nodes["x"][:, 0])
+    steps = 3
+
+    def spawn_pos(j):
+        azi = azi0 + 0.01 * (j + 1)
+        return (spawn_r * np.sin(zen0) * np.cos(azi), spawn_r * np.sin(zen0) * np.sin(azi),
+                spawn_r * np.cos(zen0), spawn_r.copy())
+
+    dims = workloads.dims_of(nodes, E, M)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_positions(nodes, n_active)
+        sol.update_mhd(prev, mode=0, **kw)
+        sol.upload_f(f)
+        for j in range(steps):
+            sol.spawn_shell(*spawn_pos(j))
+            sol.move_nodes(tdel)
+            sol.update_mhd(None, mode=0, **kw)
+            sol.step()
+        pos_g = sol.positions()
+        fg = sol.download_f()
+    finally:
+        sol.close()
+
+    # oracle side
+    o_nodes = {k: nodes[k].copy() for k in ("x", "y", "z", "rmag")}
+    state = oracle.update_mhd(dict(o_nodes), prev, 0, rs, **kw)
+    na = n_active.copy()
+    fo = f.copy()
+    for j in range(steps):
+        sx, sy, sz, sr = spawn_pos(j)
+        for k, h in zip(("x", "y", "z", "rmag"), (sx, sy, sz, sr)):
+            o_nodes[k] = _ripple(o_nodes[k], na, h)
+        for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density"):
+            state[k] = _ripple(state[k], na, state[k][:, 0])
+        seed0 = np.array([[oracle.seed(p, g["egrid"][e], sr[s]) for e in range(E)] for s in range(S)])
+        fo = _ripple(fo, na, np.repeat(seed0[:, :, None], M, axis=2))
+        na = np.minimum(na + 1, N)
+        old = oracle.move_nodes(o_nodes, state, tdel, rs, first=1)
+        pv = dict(state)
+        pv.update(old)
+        state = oracle.update_mhd(dict(o_nodes), pv, 0, rs, **kw)
+        full = dict(o_nodes)
+        full.update({k: state[k] for k in ("bmag", "dlnb", "dlnn", "dupar")})
+        rc, _, fo, _, _, _ = oracle.update_ep(dims, p, full, fo, n_active=na)
+        assert rc == 0
+    np.testing.assert_array_equal(pos_g["n_active"], na)
+    act = np.arange(N)[None, :] < na[:, None]
+    for k in ("x", "y", "z", "rmag"):
+        np.testing.assert_allclose(pos_g[k][act], o_nodes[k][act], rtol=1e-13, atol=1e-11, err_msg=k)
+    assert max_rel(fg, fo, na) <= RTOL
