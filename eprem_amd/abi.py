@@ -208,4 +208,5 @@ EXPORTED_SYMBOLS = (
     "ep_spawn_shell",
     "ep_move_nodes",
     "ep_get_positions",
+    "ep_set_precision",
 )

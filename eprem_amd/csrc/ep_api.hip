@@ -157,6 +157,7 @@ Dev make_dev() {
   D.numEpSteps = p.numEpSteps;
   D.mfpType = p.mfpType;
   D.useBC = (p.useEPBoundary > 0) && (p.useBoundaryFunction > 0);
+  D.fp32 = g.fp32;
   D.stream_block = g.stream_block;
   D.stream_halo = g.stream_halo;
   D.mfpRadialPower = p.mfpRadialPower;
@@ -334,11 +335,13 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
   const size_t S = d.num_streams, E = d.num_energies, M = d.num_mu, NP = g.Npad;
   const size_t SN = S * NP, SE = S * E;
   g.init = true;  // so that free_all() releases partial allocations
+  g.fp32 = 0;     // f starts as FP64 (filled below)
 #define AL(ptr, cnt) \
   if ((rc = dalloc(&(ptr), (cnt), #ptr))) { free_all(); return rc; }
   AL(g.egrid_d, E) AL(g.vgrid_d, E) AL(g.vgrid_i_d, E) AL(g.rig_lamo_d, E) AL(g.plt_d, E)
   AL(g.ext_d, E) AL(g.mugrid_d, M)
-  AL(g.fA, SE * M * NP) AL(g.fB, SE * M * NP) AL(g.loc, SE)
+  // one extra slab each: room for the FP64 scratch of the mixed-precision form
+  AL(g.fA, SE * M * NP + M * NP) AL(g.fB, SE * M * NP + M * NP) AL(g.loc, SE)
   AL(g.nx, SN) AL(g.ny, SN) AL(g.nz, SN) AL(g.nrmag, SN) AL(g.nbmag, SN) AL(g.ndlnb, SN)
   AL(g.ndlnn, SN) AL(g.ndupar, SN) AL(g.n_active, S)
   AL(g.ds, SN) AL(g.bplus, SN) AL(g.bminus, SN) AL(g.mfp_r, SN) AL(g.radial, SN)
@@ -451,6 +454,23 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, double *best_ms) {
   if (best_block) *best_block = bb;
   if (best_halo) *best_halo = bh;
   if (best_ms) *best_ms = best;
+  return EP_OK;
+}
+
+// Storage and arithmetic of the hot path: 0 = FP64 throughout (the parity
+// path), 1 = f stored as float, streaming and focusing in FP32, adiabatic
+// change (the ln f energy flux) in FP64.  The distribution on the device is
+// converted.
+int ep_set_precision(int32_t mode) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (mode != 0 && mode != 1) return fail(EP_ERR_INVALID_ARG, "precision mode must be 0 (FP64) or 1 (mixed)");
+  if (mode == g.fp32) return EP_OK;
+  Dev D = make_dev();
+  HIPCHK(launch_normalize(D, g.st), "normalize");
+  HIPCHK(launch_convert(D, mode, g.st), "convert");
+  HIPCHK(hipStreamSynchronize(g.st), "convert sync");
+  g.fp32 = mode;
+  g.drop_graph();
   return EP_OK;
 }
 

@@ -106,6 +106,7 @@ struct Dev {
   double *dtsub_f, *dtsub_c, *f_dlnbds, *f_dupar, *f_b;
   double *c_dupar, *c_dlnn, *c_dlnb, *c_damp, *c_bc0, *c_bc1;
   DevDiag *diag;
+  int fp32;  // f stored as float (mixed-precision form)
 };
 
 __device__ inline void note_error(const Dev &D, unsigned long long key) {
@@ -130,6 +131,34 @@ __device__ inline size_t fidx(const Dev &D, int s, int e, int m, int node) {
 
 __device__ inline double *fbuf(const Dev &D, int which) { return which ? D.fB : D.fA; }
 
+// f is stored as T: double on the parity path, float in the mixed-precision
+// form (the same two allocations, used at 4 bytes per cell).
+template <typename T>
+__device__ inline T *fbufT(const Dev &D, int which) {
+  return reinterpret_cast<T *>(which ? D.fB : D.fA);
+}
+
+// FP64 scratch slab of one (stream, energy) when f is stored as float: the
+// upper halves of the two f allocations, which float storage leaves unused
+// (task parity picks the allocation; each has one spare slab of room).
+__device__ inline double *spare_slab(const Dev &D, int task) {
+  const size_t slab = (size_t)D.M * D.Npad;
+  const size_t off = ((size_t)D.S * D.E * slab * 4 + 7) / 8;  // doubles
+  return ((task & 1) ? D.fB : D.fA) + off + (size_t)(task >> 1) * slab;
+}
+
+// Smallest normal value of T: the floor the reference applies as DBL_MIN.
+template <typename T>
+struct FloorT;
+template <>
+struct FloorT<double> {
+  static constexpr double v = 2.2250738585072014e-308;
+};
+template <>
+struct FloorT<float> {
+  static constexpr float v = 1.17549435e-38f;
+};
+
 // Raw buffer access to one wave-uniform slab: the base address lives in
 // scalar registers (the buffer descriptor) and the per-mu offset in a
 // scalar offset, so a lane needs one 32-bit VGPR for its address instead
@@ -147,6 +176,18 @@ __device__ __forceinline__ double buf_ld(rsrc_t r, uint32_t voff, uint32_t soff)
 __device__ __forceinline__ void buf_st(double v, rsrc_t r, uint32_t voff, uint32_t soff) {
   typedef decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) b64_t;
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64_t, v), r, (int)voff, (int)soff, 0);
+}
+
+__device__ __forceinline__ float buf_ld(rsrc_t r, uint32_t voff, uint32_t soff, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+
+__device__ __forceinline__ double buf_ld(rsrc_t r, uint32_t voff, uint32_t soff, double) {
+  return buf_ld(r, voff, soff);
+}
+
+__device__ __forceinline__ void buf_st(float v, rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, 0);
 }
 
 }  // namespace ep

@@ -48,6 +48,18 @@ __device__ __forceinline__ double dpp_shl1(double v, double old) {
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+__device__ __forceinline__ float dpp_shr1(float v, float old) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                               __builtin_bit_cast(int, v), 0x138, 0xF,
+                                                               0xF, false));
+}
+
+__device__ __forceinline__ float dpp_shl1(float v, float old) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                               __builtin_bit_cast(int, v), 0x130, 0xF,
+                                                               0xF, false));
+}
+
 // Lane-layout probe used by the tests: out[i] = in[i-1] (lane 0: -1),
 // out2[i] = in[i+1] (lane 63: -2) for one wave.
 __global__ void k_dpp_probe(const double *in, double *out, double *out2) {
@@ -380,15 +392,16 @@ __global__ __launch_bounds__(256) void k_node_prep(Dev D) {
 // k_seed_check: seedPopulationCheck (src/energeticParticlesBoundary.c:
 // 28-60) when streaming is off (otherwise it is fused into pass 0).
 // ------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.n_active[s]) return;
   const int t = s * D.E + e;
-  const double sd = seed_value(D, D.radial[(size_t)s * D.Npad + k], e);
-  double *f = fbuf(D, D.loc[t]) + (size_t)t * D.M * D.Npad + k;
+  const T sd = (T)seed_value(D, D.radial[(size_t)s * D.Npad + k], e);
+  T *f = fbufT<T>(D, D.loc[t]) + (size_t)t * D.M * D.Npad + k;
   for (int m = 0; m < D.M; m++) {
-    const double v = f[(size_t)m * D.Npad];
+    const T v = f[(size_t)m * D.Npad];
     if (v < sd) f[(size_t)m * D.Npad] = sd;
   }
 }
@@ -405,7 +418,7 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
 // inside the chunk) are written, to the other f buffer.
 // MPOS = first mu index with mugrid >= 0 (upwind direction).
 // ------------------------------------------------------------------------
-template <int M, int MPOS>
+template <int M, int MPOS, typename T>
 __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, int do_seed, int substep) {
   const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
   const int task = s * D.E + e;
@@ -428,30 +441,31 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
   // flip = 0 (read the current slab, write the spare one), which leaves the
   // state untouched
   const int cur = D.loc[task] ^ flip;
-  const uint32_t slab_bytes = (uint32_t)(M * D.Npad * 8), row = (uint32_t)D.Npad * 8;
-  const rsrc_t src = slab_rsrc(fbuf(D, cur) + (size_t)task * M * D.Npad, slab_bytes);
-  const rsrc_t dst = slab_rsrc(fbuf(D, cur ^ 1) + (size_t)task * M * D.Npad, slab_bytes);
-  const uint32_t voff = (uint32_t)node * 8;
+  constexpr uint32_t B = sizeof(T);
+  const uint32_t slab_bytes = (uint32_t)(M * D.Npad * B), row = (uint32_t)D.Npad * B;
+  const rsrc_t src = slab_rsrc((const double *)(fbufT<T>(D, cur) + (size_t)task * M * D.Npad), slab_bytes);
+  const rsrc_t dst = slab_rsrc((const double *)(fbufT<T>(D, cur ^ 1) + (size_t)task * M * D.Npad), slab_bytes);
+  const uint32_t voff = (uint32_t)node * B;
 
-  double f[M];
-  double sd = 0.0;
-  if (valid && do_seed) sd = seed_value(D, D.radial[on + node], e);
+  T f[M];
+  T sd = 0;
+  if (valid && do_seed) sd = (T)seed_value(D, D.radial[on + node], e);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    double v = 0.0;
-    if (valid) v = buf_ld(src, voff, m * row);
+    T v = 0;
+    if (valid) v = buf_ld(src, voff, m * row, T());
     if (do_seed && v < sd) v = sd;
     f[m] = v;
   }
   const size_t ok = (size_t)task * D.Npad;
   const double dsm = valid ? D.dsm[ok + k] : 0.0;
-  const double q = valid ? D.q[ok + k] : 1.0;
+  const T q = valid ? (T)D.q[ok + k] : (T)1;
   // Stream ends are held: kept node 0 for mu >= 0, the last one for mu < 0.
   const double dsm_pos = (k == 0) ? 0.0 : dsm;
   const double dsm_neg = (k == nk - 1) ? 0.0 : dsm;
   const bool bc = D.useBC && valid && (k < 3);
-  const double bcv = bc ? D.seed3[(size_t)task * 3 + k] : 0.0;
-  const double invM = 1.0 / M;
+  const T bcv = bc ? (T)D.seed3[(size_t)task * 3 + k] : (T)0;
+  const T invM = (T)(1.0 / M);
   // Edge exchange between the waves of the block, double-buffered by
   // subcycle parity.  up[r] holds the mu >= 0 values lane 0 of wave r
   // takes as its upstream neighbour: lane 63 of wave r-1 writes row r, and
@@ -461,15 +475,15 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
   // downstream neighbour: lane 0 of wave r writes row r, and lane 63 of the
   // last wave writes its own values into row nw.
   constexpr int NP = M - MPOS;
-  __shared__ double up[2][17][NP > 0 ? NP : 1];
-  __shared__ double dn[2][17][MPOS > 0 ? MPOS : 1];
+  __shared__ T up[2][17][NP > 0 ? NP : 1];
+  __shared__ T dn[2][17][MPOS > 0 ? MPOS : 1];
   // coef[m] = ds_i_multiplier * del_fac[m]: the product the reference forms
   // every subcycle is loop-invariant, so it is formed once per pass.
-  double coef[M];
+  T coef[M];
   {
     const double *__restrict__ del = D.del_fac + (size_t)task * M;
 #pragma unroll
-    for (int m = 0; m < M; ++m) coef[m] = ((m >= MPOS) ? dsm_pos : dsm_neg) * del[m];
+    for (int m = 0; m < M; ++m) coef[m] = (T)(((m >= MPOS) ? dsm_pos : dsm_neg) * del[m]);
   }
   const int up_w = (lane == 63) ? w + 1 : ((lane == 0 && w == 0) ? 0 : -1);
   const int dn_w = (lane == 0) ? w : ((lane == 63 && w == nw - 1) ? nw : -1);
@@ -491,27 +505,27 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
     __syncthreads();
     // Only lane 0 (mu >= 0) and lane 63 (mu < 0) use these values; the
     // other lanes read the same rows harmlessly.
-    double eu[NP > 0 ? NP : 1], ed[MPOS > 0 ? MPOS : 1];
+    T eu[NP > 0 ? NP : 1], ed[MPOS > 0 ? MPOS : 1];
 #pragma unroll
     for (int m = MPOS; m < M; ++m) eu[m - MPOS] = up[b][w][m - MPOS];
 #pragma unroll
     for (int m = 0; m < MPOS; ++m) ed[m] = dn[b][w + 1][m];
-    double sum = 0.0;
+    T sum = 0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      const double fm = f[m];
-      double fn;
+      const T fm = f[m];
+      T fn;
       if (m >= MPOS) {
-        const double nb = dpp_shr1(fm, eu[m - MPOS]);
+        const T nb = dpp_shr1(fm, eu[m - MPOS]);
         fn = fm + coef[m] * (nb - fm);
       } else {
-        const double nb = dpp_shl1(fm, ed[m]);
+        const T nb = dpp_shl1(fm, ed[m]);
         fn = fm + coef[m] * (fm - nb);
       }
       f[m] = fn;
       sum = sum + fn;
     }
-    const double iso = invM * sum;
+    const T iso = invM * sum;
 #pragma unroll
     for (int m = 0; m < M; ++m) f[m] = iso + (f[m] - iso) * q;
   }
@@ -520,13 +534,13 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
     int bad = -1, bad_inf = 0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      double v = f[m];
+      T v = f[m];
       if (last) {
         if (bad < 0 && !isfinite(v)) {
           bad = m;
           bad_inf = isinf(v) ? 1 : 0;
         }
-        if (v < kDblMin) v = kDblMin;
+        if (v < FloorT<T>::v) v = FloorT<T>::v;
       }
       buf_st(v, dst, voff, m * row);
     }
@@ -540,6 +554,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
 // with the NaN/Inf check and DBL_MIN floor (src/energeticParticles.c:
 // 2437-2471).
 // ------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
   const int task = blockIdx.x;
   const int s = task / D.E, e = task % D.E;
@@ -549,17 +564,17 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
   if (!D.thin[s]) return;
   const int n = D.n_active[s];
   const size_t on = (size_t)s * D.Npad;
-  double *f = fbuf(D, loc_new) + (size_t)task * D.M * D.Npad;
+  T *f = fbufT<T>(D, loc_new) + (size_t)task * D.M * D.Npad;
   for (int idx = threadIdx.x; idx < n * D.M; idx += blockDim.x) {
     const int m = idx / n, shell = idx % n;
     const int r = D.ref[on + shell];
     if (r != shell) {
-      double v = f[(size_t)m * D.Npad + r];
+      T v = f[(size_t)m * D.Npad + r];
       if (isnan(v))
         note_error(D, err_key(substep, 0, s, 1, e, m, shell, 0));
       else if (isinf(v))
         note_error(D, err_key(substep, 0, s, 1, e, m, shell, 1));
-      if (v < kDblMin) v = kDblMin;
+      if (v < FloorT<T>::v) v = FloorT<T>::v;
       f[(size_t)m * D.Npad + shell] = v;
     }
   }
@@ -579,24 +594,25 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
 // upwind flux subcycles with zero-flux walls, then the NaN/Inf check
 // (first bad mu) -- written out in each kernel so that the row and the
 // velocities stay in registers.
-#define EP_FOCUS_ROW(M, D, a, b, f, nsub, dtsub, bad, bad_inf)                     \
+#define EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf)                  \
   do {                                                                             \
-    double va_[(M) + 1];                                                           \
+    T va_[(M) + 1];                                                                \
     double vprev_ = 0.0;                                                           \
     _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
       const double mu = (D).mugrid[m];                                             \
       const double vel = 0.5 * (1.0 - mu * mu) * ((a) + mu * (b));                 \
-      if (m > 0) va_[m] = 0.5 * (vel + vprev_);                                    \
+      if (m > 0) va_[m] = (T)(0.5 * (vel + vprev_));                               \
       vprev_ = vel;                                                                \
     }                                                                              \
-    const double dmu_i_ = 1.0 / (D).dmu;                                           \
+    const T dmu_i_ = (T)(1.0 / (D).dmu);                                           \
+    const T dts_ = (T)(dtsub);                                                     \
     for (int it_ = 0; it_ < (nsub); ++it_) {                                       \
-      double flux_lo = 0.0; /* zero-flux wall at mu = -1 */                        \
+      T flux_lo = 0; /* zero-flux wall at mu = -1 */                               \
       _Pragma("unroll") for (int m = 0; m < (M); ++m) {                            \
-        double flux_hi = 0.0; /* zero-flux wall at mu = +1 */                      \
+        T flux_hi = 0; /* zero-flux wall at mu = +1 */                             \
         if (m + 1 < (M)) flux_hi = va_[m + 1] * (signbit(va_[m + 1]) ? f[m + 1] : f[m]); \
-        const double f1 = (flux_hi - flux_lo) * dmu_i_;                            \
-        f[m] = f[m] - (dtsub) * f1;                                                \
+        const T f1 = (flux_hi - flux_lo) * dmu_i_;                                 \
+        f[m] = f[m] - dts_ * f1;                                                   \
         flux_lo = flux_hi;                                                         \
       }                                                                            \
     }                                                                              \
@@ -608,7 +624,7 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
     }                                                                              \
   } while (0)
 
-template <int M>
+template <int M, typename T>
 __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
   // blockIdx.x = ((stream, energy), chunk of 256 nodes), chunk fastest:
   // the block reads 2 KB runs of each mu row, consecutive blocks the next.
@@ -629,19 +645,20 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
   const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], b = D.f_b[i];
   const double two = 2.0;
   const int task = s * D.E + e;
-  const rsrc_t slab = slab_rsrc(fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad,
-                                (uint32_t)(M * D.Npad * 8));
-  const uint32_t voff = (uint32_t)node * 8, row = (uint32_t)D.Npad * 8;
+  constexpr uint32_t B = sizeof(T);
+  const rsrc_t slab = slab_rsrc((const double *)(fbufT<T>(D, D.loc[task]) + (size_t)task * M * D.Npad),
+                                (uint32_t)(M * D.Npad * B));
+  const uint32_t voff = (uint32_t)node * B, row = (uint32_t)D.Npad * B;
   const double v = D.vgrid[e];
   const double a = -v * dlnBds - (two / v) * DuParDt;
-  double f[M];
+  T f[M];
 #pragma unroll
-  for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row);
+  for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row, T());
   int bad = -1, bad_inf = 0;
-  EP_FOCUS_ROW(M, D, a, b, f, nsub, dtsub, bad, bad_inf);
+  EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    const double val = (f[m] < kDblMin) ? kDblMin : f[m];
+    const T val = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
     if (nsub > 0 || val != f[m]) buf_st(val, slab, voff, m * row);
   }
   if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
@@ -650,7 +667,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
 // k_focus_heavy: the nodes k_focus leaves out (nsub_f >= kHeavyFocus), one
 // wave per node and one lane per energy, so every lane runs the node's
 // subcycle count.  A fixed grid walks the list built by k_node_prep.
-template <int M>
+template <int M, typename T>
 __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
   const int count = D.diag->focus_heavy;
   const int e = threadIdx.x;
@@ -663,16 +680,16 @@ __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
     const double dtsub = D.dtsub_f[i];
     const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], b = D.f_b[i];
     const int task = s * D.E + e;
-    double *fp = fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad + node;
+    T *fp = fbufT<T>(D, D.loc[task]) + (size_t)task * M * D.Npad + node;
     const double v = D.vgrid[e];
     const double a = -v * dlnBds - (two / v) * DuParDt;
-    double f[M];
+    T f[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
     int bad = -1, bad_inf = 0;
-    EP_FOCUS_ROW(M, D, a, b, f, nsub, dtsub, bad, bad_inf);
+    EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf);
 #pragma unroll
-    for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = (f[m] < kDblMin) ? kDblMin : f[m];
+    for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
     if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
   }
 }
@@ -687,6 +704,7 @@ __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
 // first sweep takes the log and the last one writes exp back.  A lane
 // needs a handful of registers, so the kernel runs at full occupancy.
 // ------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   // blockIdx.x = ((stream, mu), chunk of 256 nodes), chunk fastest
   const int nch = (D.N + 255) >> 8;
@@ -704,8 +722,9 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   const double dlnp_i = 1.0 / D.dlnp;
   const double half = 0.5, one = 1.0;
   const size_t slab = (size_t)D.M * D.Npad;
-  const uint32_t slab_bytes = (uint32_t)(slab * 8);
-  const uint32_t voff = (uint32_t)(((size_t)m * D.Npad + node) * 8);
+  constexpr uint32_t B = sizeof(T);
+  const uint32_t slab_bytes = (uint32_t)(slab * B);
+  const uint32_t voff = (uint32_t)(((size_t)m * D.Npad + node) * B);
   const double mu = D.mugrid[m];
   const double a = -mu * DuParDt;
   const double bb = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
@@ -714,12 +733,21 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   const int niter = upd ? nsub : 1;  // nsub = 0 still takes exp(log f)
   auto slab_of = [&](int e) {
     const int task = s * E + e;
-    return slab_rsrc(fbuf(D, D.loc[task]) + (size_t)task * slab, slab_bytes);
+    return slab_rsrc((const double *)(fbufT<T>(D, D.loc[task]) + (size_t)task * slab), slab_bytes);
   };
+  // With float storage, ln f between subcycles is kept in FP64 in the part
+  // of the f allocations that float storage leaves free.
+  constexpr bool kSpare = sizeof(T) != sizeof(double);
+  const uint32_t voff8 = (uint32_t)(((size_t)m * D.Npad + node) * 8);
+  auto spare_of = [&](int e) { return slab_rsrc(spare_slab(D, s * E + e), (uint32_t)(slab * 8)); };
   int bad = -1, bad_inf = 0;
   for (int it = 0; it < niter; ++it) {
     const bool first = (it == 0), fin = (it == niter - 1);
-    auto ld = [&](int e) { return buf_ld(slab_of(e), voff, 0); };
+    // the column is advanced in FP64 whatever the storage type
+    auto ld = [&](int e) {
+      if (kSpare && !first) return buf_ld(spare_of(e), voff8, 0);
+      return (double)buf_ld(slab_of(e), voff, 0, T());
+    };
     auto cvt = [&](double x) { return first ? log(x) : x; };
     double r0 = cvt(ld(0));
     double r1 = cvt(ld(1));
@@ -756,15 +784,17 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
       }
       const double nv = upd ? cur - dtsub * f1 : cur;
       if (fin) {
-        double val = exp(nv);
+        T val = (T)exp(nv);
         if (bad < 0 && !isfinite(val)) {
           bad = e;
           bad_inf = isinf(val) ? 1 : 0;
         }
-        if (val < kDblMin) val = kDblMin;
+        if (val < FloorT<T>::v) val = FloorT<T>::v;
         buf_st(val, slab_of(e), voff, 0);
+      } else if (kSpare) {
+        buf_st(nv, spare_of(e), voff8, 0);
       } else {
-        buf_st(nv, slab_of(e), voff, 0);
+        buf_st((T)nv, slab_of(e), voff, 0);
       }
       old_prev = cur;
       vel_prev = vel;
@@ -783,28 +813,33 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
 // Layout conversion and initialisation.
 // ------------------------------------------------------------------------
 // host order [s][node][e][m] (staged in `stage`) -> device [s][e][m][node]
-__global__ __launch_bounds__(256) void k_from_reference(Dev D, const double *stage, double *dstbuf) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_from_reference(Dev D, const double *stage, T *dstbuf) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (node >= D.N) return;
   const size_t src = (((size_t)s * D.N + node) * D.E + e) * D.M;
-  for (int m = 0; m < D.M; m++) dstbuf[fidx(D, s, e, m, node)] = stage[src + m];
+  for (int m = 0; m < D.M; m++) dstbuf[fidx(D, s, e, m, node)] = (T)stage[src + m];
 }
 
-__global__ __launch_bounds__(256) void k_to_reference(Dev D, const double *srcbuf, double *stage) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_to_reference(Dev D, const T *srcbuf, double *stage) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (node >= D.N) return;
   const size_t dst = (((size_t)s * D.N + node) * D.E + e) * D.M;
-  for (int m = 0; m < D.M; m++) stage[dst + m] = srcbuf[fidx(D, s, e, m, node)];
+  for (int m = 0; m < D.M; m++) stage[dst + m] = (double)srcbuf[fidx(D, s, e, m, node)];
 }
 
 // Move every slab whose current copy sits in buffer B back to A.
+template <typename T>
 __global__ __launch_bounds__(256) void k_normalize(Dev D) {
   const int task = blockIdx.x;
   if (D.loc[task] == 0) return;
   const size_t off = (size_t)task * D.M * D.Npad;
-  for (size_t i = threadIdx.x; i < (size_t)D.M * D.Npad; i += blockDim.x) D.fA[off + i] = D.fB[off + i];
+  T *a = fbufT<T>(D, 0);
+  const T *b = fbufT<T>(D, 1);
+  for (size_t i = threadIdx.x; i < (size_t)D.M * D.Npad; i += blockDim.x) a[off + i] = b[off + i];
   __syncthreads();
   if (threadIdx.x == 0) D.loc[task] = 0;
 }
@@ -815,16 +850,35 @@ __global__ __launch_bounds__(256) void k_fill(double *p, size_t n, double v) {
 }
 
 // initEnergeticParticles (src/energeticParticlesInit.c:106-147).
+template <typename T>
 __global__ __launch_bounds__(256) void k_init_seed(Dev D) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.n_active[s]) return;
   const size_t i = (size_t)s * D.Npad + k;
   const double rad = pow(D.rmag[i] / D.normRadius, -D.beta);
-  const double sd = seed_value(D, rad, e);
+  const T sd = (T)seed_value(D, rad, e);
   const int t = s * D.E + e;
-  double *f = fbuf(D, D.loc[t]) + (size_t)t * D.M * D.Npad + k;
+  T *f = fbufT<T>(D, D.loc[t]) + (size_t)t * D.M * D.Npad + k;
   for (int m = 0; m < D.M; m++) f[(size_t)m * D.Npad] = sd;
+}
+
+// Storage-type change of every slab: the current copy in buffer loc is
+// converted into the other buffer, which becomes current.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_convert(Dev D) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y, s = blockIdx.z;
+  if (k >= D.Npad) return;
+  const int t = s * D.E + e;
+  const TI *src = fbufT<TI>(D, D.loc[t]) + (size_t)t * D.M * D.Npad;
+  TO *dst = fbufT<TO>(D, D.loc[t] ^ 1) + (size_t)t * D.M * D.Npad;
+  for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad + k] = (TO)src[(size_t)m * D.Npad + k];
+}
+
+__global__ __launch_bounds__(256) void k_flip_all(Dev D) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < D.S * D.E) D.loc[t] ^= 1;
 }
 
 // ------------------------------------------------------------------------
@@ -859,7 +913,10 @@ hipError_t launch_node_prep(const Dev &D, hipStream_t st) {
 }
 
 hipError_t launch_seed_check(const Dev &D, hipStream_t st) {
-  hipLaunchKernelGGL(k_seed_check, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_seed_check<float>, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
+  else
+    hipLaunchKernelGGL(k_seed_check<double>, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
@@ -867,8 +924,12 @@ hipError_t launch_seed_check(const Dev &D, hipStream_t st) {
 template <int M, int MPOS>
 static hipError_t stream_pass_t(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                                 hipStream_t st) {
-  hipLaunchKernelGGL((k_stream_pass<M, MPOS>), dim3(chunks, D.E, D.S), dim3(D.stream_block), 0, st, D,
-                     pass, flip, do_seed, substep);
+  if (D.fp32)
+    hipLaunchKernelGGL((k_stream_pass<M, MPOS, float>), dim3(chunks, D.E, D.S), dim3(D.stream_block), 0, st,
+                       D, pass, flip, do_seed, substep);
+  else
+    hipLaunchKernelGGL((k_stream_pass<M, MPOS, double>), dim3(chunks, D.E, D.S), dim3(D.stream_block), 0, st,
+                       D, pass, flip, do_seed, substep);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
@@ -896,16 +957,25 @@ hipError_t launch_stream_pass(const Dev &D, int pass, int flip, int do_seed, int
 }
 
 hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
-  hipLaunchKernelGGL(k_stream_finish, dim3(D.S * D.E), dim3(256), 0, st, D, sub);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_stream_finish<float>, dim3(D.S * D.E), dim3(256), 0, st, D, sub);
+  else
+    hipLaunchKernelGGL(k_stream_finish<double>, dim3(D.S * D.E), dim3(256), 0, st, D, sub);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
 
 #define EP_FOCUS_CASE(MM)                                                                     \
   case MM:                                                                                    \
-    hipLaunchKernelGGL((k_focus<MM>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, st, \
-                       D, sub);                                                               \
-    hipLaunchKernelGGL((k_focus_heavy<MM>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub);        \
+    if (D.fp32) {                                                                             \
+      hipLaunchKernelGGL((k_focus<MM, float>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
+                         st, D, sub);                                                         \
+      hipLaunchKernelGGL((k_focus_heavy<MM, float>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+    } else {                                                                                  \
+      hipLaunchKernelGGL((k_focus<MM, double>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
+                         st, D, sub);                                                         \
+      hipLaunchKernelGGL((k_focus_heavy<MM, double>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+    }                                                                                         \
     break;
 
 // Blocks of the heavy-focusing walk: enough waves to fill the chip.
@@ -929,27 +999,51 @@ hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
 
 hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
   const dim3 grid((D.N + 255) / 256 * D.S * D.M), block(256);
-  hipLaunchKernelGGL(k_change, grid, block, 0, st, D, sub);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_change<float>, grid, block, 0, st, D, sub);
+  else
+    hipLaunchKernelGGL(k_change<double>, grid, block, 0, st, D, sub);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
 
 hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st) {
-  hipLaunchKernelGGL(k_from_reference, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D, stage,
-                     dstbuf);
+  const dim3 grid((D.N + 255) / 256, D.E, D.S);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_from_reference<float>, grid, dim3(256), 0, st, D, stage, (float *)dstbuf);
+  else
+    hipLaunchKernelGGL(k_from_reference<double>, grid, dim3(256), 0, st, D, stage, dstbuf);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
 
 hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st) {
-  hipLaunchKernelGGL(k_to_reference, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D, srcbuf,
-                     stage);
+  const dim3 grid((D.N + 255) / 256, D.E, D.S);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_to_reference<float>, grid, dim3(256), 0, st, D, (const float *)srcbuf, stage);
+  else
+    hipLaunchKernelGGL(k_to_reference<double>, grid, dim3(256), 0, st, D, srcbuf, stage);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
 
 hipError_t launch_normalize(const Dev &D, hipStream_t st) {
-  hipLaunchKernelGGL(k_normalize, dim3(D.S * D.E), dim3(256), 0, st, D);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_normalize<float>, dim3(D.S * D.E), dim3(256), 0, st, D);
+  else
+    hipLaunchKernelGGL(k_normalize<double>, dim3(D.S * D.E), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st) {
+  const dim3 grid((D.Npad + 255) / 256, D.E, D.S);
+  if (to_fp32)
+    hipLaunchKernelGGL((k_convert<double, float>), grid, dim3(256), 0, st, D);
+  else
+    hipLaunchKernelGGL((k_convert<float, double>), grid, dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_flip_all, dim3((D.S * D.E + 255) / 256), dim3(256), 0, st, D);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
@@ -961,7 +1055,10 @@ hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st) {
 }
 
 hipError_t launch_init_seed(const Dev &D, hipStream_t st) {
-  hipLaunchKernelGGL(k_init_seed, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_init_seed<float>, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
+  else
+    hipLaunchKernelGGL(k_init_seed<double>, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, st, D);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }

@@ -17,6 +17,7 @@ hipError_t launch_change(const Dev &D, int substep, hipStream_t st);
 hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st);
 hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st);
 hipError_t launch_normalize(const Dev &D, hipStream_t st);
+hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_init_seed(const Dev &D, hipStream_t st);
 hipError_t launch_dpp_probe(const double *in, double *out, double *out2, hipStream_t st);

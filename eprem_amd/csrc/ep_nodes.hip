@@ -68,21 +68,24 @@ __global__ __launch_bounds__(256) void k_ripple_nodes(Dev D, RippleArgs A) {
 
 // dst[m][k] = src[m][k-1]; node 0 takes spawn_f[s][e][m], or the seed
 // spectrum at the new node's radius when spawn_f is null.
+// Source and copy use the same element type, so a slab occupies the same
+// offsets in both buffers and slabs never overlap.
+template <typename T>
 __global__ __launch_bounds__(256) void k_ripple_f(Dev D, const double *spawn_f) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.n_active[s]) return;  // n_active already advanced
   const int task = s * D.E + e;
   const int cur = D.loc[task];
-  const double *src = fbuf(D, cur) + (size_t)task * D.M * D.Npad;
-  double *dst = fbuf(D, cur ^ 1) + (size_t)task * D.M * D.Npad;
+  const T *src = fbufT<T>(D, cur) + (size_t)task * D.M * D.Npad;
+  T *dst = fbufT<T>(D, cur ^ 1) + (size_t)task * D.M * D.Npad;
   if (k > 0) {
     for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad + k] = src[(size_t)m * D.Npad + k - 1];
   } else if (spawn_f) {
-    for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad] = spawn_f[(size_t)task * D.M + m];
+    for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad] = (T)spawn_f[(size_t)task * D.M + m];
   } else {
     const double radial = pow(D.rmag[(size_t)s * D.Npad] / D.normRadius, -D.beta);
-    const double sd = seed_value(D, radial, e);
+    const T sd = (T)seed_value(D, radial, e);
     for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad] = sd;
   }
 }
@@ -174,7 +177,11 @@ int ep_spawn_shell(const ep_spawn *sp) {
   A.rold = 8;
   Dev D = make_dev();
   hipLaunchKernelGGL(k_ripple_nodes, dim3(S), dim3(256), 0, g.st, D, A);
-  hipLaunchKernelGGL(k_ripple_f, dim3((g.d.num_nodes + 255) / 256, E, S), dim3(256), 0, g.st, D, spawn_f);
+  const dim3 grid((g.d.num_nodes + 255) / 256, E, S);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_ripple_f<float>, grid, dim3(256), 0, g.st, D, spawn_f);
+  else
+    hipLaunchKernelGGL(k_ripple_f<double>, grid, dim3(256), 0, g.st, D, spawn_f);
   hipLaunchKernelGGL(k_flip_loc, dim3((S * E + 255) / 256), dim3(256), 0, g.st, D);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(g.st);

@@ -29,6 +29,7 @@ struct ObsDev {
   double *dist, *flux;            // [num][E*M], [num][E]
 };
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_obs_project(Dev D, ObsDev O) {
   const int s = blockIdx.x, o = blockIdx.y;
   const int n = D.n_active[s];
@@ -75,8 +76,8 @@ __global__ __launch_bounds__(256) void k_obs_project(Dev D, ObsDev O) {
   for (int c = threadIdx.x; c < EM; c += blockDim.x) {
     const int e = c / D.M, m = c % D.M;
     const int task = s * D.E + e;
-    const double *fb = fbuf(D, D.loc[task]) + (size_t)task * D.M * D.Npad + (size_t)m * D.Npad;
-    const double fi = fb[in], fo = fb[out];
+    const T *fb = fbufT<T>(D, D.loc[task]) + (size_t)task * D.M * D.Npad + (size_t)m * D.Npad;
+    const double fi = (double)fb[in], fo = (double)fb[out];
     double v;
     if ((sf >= 0.0) && (sf <= 1.0))
       v = fi * (1.0 - sf) + fo * sf;
@@ -108,14 +109,15 @@ __global__ __launch_bounds__(256) void k_obs_reduce(Dev D, ObsDev O) {
   }
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_stream_flux(Dev D, double *flux) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.N) return;
   const int task = s * D.E + e;
-  const double *fb = fbuf(D, D.loc[task]) + (size_t)task * D.M * D.Npad + k;
+  const T *fb = fbufT<T>(D, D.loc[task]) + (size_t)task * D.M * D.Npad + k;
   double iso = 0.0;
-  for (int m = 0; m < D.M; m++) iso += fb[(size_t)m * D.Npad];
+  for (int m = 0; m < D.M; m++) iso += (double)fb[(size_t)m * D.Npad];
   iso /= D.M;
   flux[((size_t)s * D.N + k) * D.E + e] = 2.0 * D.egrid[e] * iso;
 }
@@ -162,7 +164,10 @@ int ep_observe(const ep_observers *obs, double *dist, double *flux) {
   if (e == hipSuccess) e = hipMemcpy(pp, obs->phi, O * 8, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "observer upload");
   Dev D = make_dev();
-  hipLaunchKernelGGL(k_obs_project, dim3(S, O), dim3(256), 0, g.st, D, od);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_obs_project<float>, dim3(S, O), dim3(256), 0, g.st, D, od);
+  else
+    hipLaunchKernelGGL(k_obs_project<double>, dim3(S, O), dim3(256), 0, g.st, D, od);
   hipLaunchKernelGGL(k_obs_reduce, dim3(O), dim3(256), 0, g.st, D, od);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(g.st);
@@ -193,7 +198,10 @@ int ep_stream_flux(double *flux) {
   }
   double *out = g.flux_buf;
   Dev D = make_dev();
-  hipLaunchKernelGGL(k_stream_flux, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, g.st, D, out);
+  if (D.fp32)
+    hipLaunchKernelGGL(k_stream_flux<float>, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, g.st, D, out);
+  else
+    hipLaunchKernelGGL(k_stream_flux<double>, dim3((D.N + 255) / 256, D.E, D.S), dim3(256), 0, g.st, D, out);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(g.st);
   if (e == hipSuccess) e = hipMemcpy(flux, out, cnt * 8, hipMemcpyDeviceToHost);

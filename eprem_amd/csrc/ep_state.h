@@ -38,6 +38,7 @@ struct State {
   // graph of the substep loop; `generation` changes whenever kernel
   // arguments captured by value (parameters, tiling) change
   bool use_graph = true;
+  int fp32 = 0;  // f stored as float (mixed-precision form)
   unsigned generation = 0;
   hipGraphExec_t graph_exec = nullptr;
   StepShape graph_shape{};

@@ -61,6 +61,7 @@ def lib():
         L.ep_move_nodes.argtypes = [C.c_double]
         L.ep_get_positions.argtypes = [DP, DP, DP, DP, IP]
         L.ep_set_graph.argtypes = [C.c_int32]
+        L.ep_set_precision.argtypes = [C.c_int32]
         for name in abi.EXPORTED_SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -109,6 +110,10 @@ class EpSolver:
         b, h, ms = C.c_int32(), C.c_int32(), C.c_double()
         _check(lib().ep_autotune(C.byref(b), C.byref(h), C.byref(ms)))
         return b.value, h.value, ms.value
+
+    def set_precision(self, mixed):
+        """False: FP64 throughout; True: FP32 streaming/focusing, FP64 change."""
+        _check(lib().ep_set_precision(1 if mixed else 0))
 
     def set_graph(self, enable):
         _check(lib().ep_set_graph(1 if enable else 0))

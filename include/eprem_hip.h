@@ -322,6 +322,14 @@ int ep_autotune(int32_t *stream_block, int32_t *stream_halo, double *stream_ms);
  * No counterpart in the reference. */
 int ep_set_graph(int32_t enable);
 
+/* Precision of the hot path: 0 = FP64 throughout (default; the parity
+ * path), 1 = mixed: f stored as float with streaming, relaxation and
+ * focusing in FP32 (FP32 accumulation) and the adiabatic energy change
+ * (ln f flux) in FP64.  The distribution on the device is converted; the
+ * float floor is FLT_MIN where the reference floors at DBL_MIN.  No
+ * counterpart in the reference. */
+int ep_set_precision(int32_t mode);
+
 /* Device self-test of the cross-lane shift the streaming kernel uses: for
  * one 64-lane wave, shr[i] = in[i-1] (shr[0] = -1) and shl[i] = in[i+1]
  * (shl[63] = -2).  No counterpart in the reference; used by the tests. */
