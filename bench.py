@@ -38,7 +38,78 @@ CONFIGS = {
     # batched form: 1536 streams whose active-node counts are drawn from a
     # seeded uniform 200..4000, as shells are injected
     "c3": dict(rows=16, cols=16, N=4000, E=64, M=32, eps=10, ragged=(200, 4000)),
+    # mixed-precision form on the c2 workload with a 35-decade power-law f
+    "c4": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10, mixed=True),
 }
+
+OBSERVERS = dict(r_au=np.array([0.3, 0.5, 1.0]), theta=np.array([1.2, 1.571, 2.0]),
+                 phi=np.array([0.5, 2.5, 4.5]))
+
+
+def run_mixed(args, c, p, nodes, dims):
+    """Mixed-precision form: the timed steps run with f stored as float
+    (FP32 streaming and focusing, FP64 adiabatic change); the same steps are
+    then repeated on the FP64 path (equal to the oracle to 1e-10) to state
+    the error of f, of the omnidirectional flux at three observers and of
+    its fluence (time integral over the run)."""
+    grids = None
+    sol = runtime.EpSolver(dims, p)
+    sol.set_nodes(nodes)
+    grids = sol.grids()
+    f0 = workloads.power_law_f(grids, nodes, c["M"], seed=20261203)
+    tdel_days = 1.0 / 64
+
+    def run(mixed, timed):
+        sol.set_precision(False)
+        sol.upload_f(f0)
+        sol.set_precision(mixed)
+        fluence = np.zeros((3, c["E"]))
+        wall, cells = 0.0, 0
+        flux = None
+        for i in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            _, dg = sol.step()
+            t1 = time.perf_counter()
+            if i >= args.warmup:
+                wall += t1 - t0
+                cells += dg["cell_updates"]
+            _, flux = sol.observe(OBSERVERS["r_au"], OBSERVERS["theta"], OBSERVERS["phi"])
+            fluence += flux * tdel_days
+        return sol.download_f(), flux, fluence, wall, cells
+
+    if not (args.block or args.halo):
+        sol.autotune()
+    f32, flux32, flu32, wall, cells = run(True, True)
+    f64, flux64, flu64, wall64, cells64 = run(False, False)
+    sol.close()
+    # float storage floors at FLT_MIN: the error of f is stated over the
+    # cells inside the workload's range (f >= 1e-30)
+    live = f64 >= 1e-30
+    rel = np.abs(f32[live] - f64[live]) / f64[live]
+    live_frac = float(live.mean())
+    del f32, f64, live
+
+    def err(a, b):
+        r = np.abs(a - b) / np.abs(b)
+        return {"max": float(r.max()), "median": float(np.median(r))}
+
+    return {
+        "metric": "phase-space cell-updates/s (1 GPU), surrogate vs plain at <=1% flux error",
+        "value": cells / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (change f64)", "data": "synthetic",
+        "config": {"workload": "c4: c2 shapes, mixed precision, power-law f spanning %.1e..%.1e"
+                               % (f0.min(), f0.max()), "path": "mixed precision"},
+        "fp64_same_steps": {"value": cells64 / wall64, "ms_per_step": wall64 * 1e3 / args.steps},
+        "error_vs_fp64": {"f": {"max": float(rel.max()), "median": float(np.median(rel)),
+                                "cells_at_or_above_1e-30": live_frac},
+                          "observer_flux": err(flux32, flux64), "observer_fluence": err(flu32, flu64)},
+    }
+
+
+def log(msg):
+    """Progress on stderr (the JSON result goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def active_counts(c, S):
@@ -101,6 +172,9 @@ def main():
     ap.add_argument("--halo", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mas", action="store_true")
+    ap.add_argument("--precision", type=int, default=2, choices=(0, 2),
+                    help="0: FP64 in the reference's operation order; 2: FP64 with fused "
+                         "multiply-adds in streaming (within 1e-13 of it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs every stream on one GPU; --gpus %d is not supported "
@@ -110,15 +184,21 @@ def main():
     p = make_params(c)
     nodes = bench_nodes(c)
     dims = workloads.dims_of(nodes, c["E"], c["M"])
+    if c.get("mixed"):
+        print(json.dumps(run_mixed(args, c, p, nodes, dims)))
+        return
     n_active = active_counts(c, dims.num_streams)
     sol = runtime.EpSolver(dims, p)
     sol.set_nodes(nodes, n_active)
     sol.init_seed_population()
+    sol.set_precision(args.precision)
+    log("nodes and seed population set; precision mode %d" % args.precision)
     if args.block or args.halo:
         sol.set_tuning(args.block or 256, args.halo or 32)
         tiling = {"block": args.block or 256, "halo": args.halo or 32, "source": "flags"}
     else:
         blk, halo, tune_ms = sol.autotune()
+        log("autotune: block %d halo %d" % (blk, halo))
         tiling = {"block": blk, "halo": halo, "source": "autotune", "stream_ms_per_substep": tune_ms}
     # the timed steps launch kernels one by one so that HIP events bracket
     # each operator; the captured-graph form is timed afterwards
@@ -133,6 +213,7 @@ def main():
 
     for _ in range(args.warmup):
         sol.step()
+        log("warmup step done")
         if not args.no_mas:
             sol.interp_mhd(0.5, 0.0, 30.0, 50.0e5, prev)
 
@@ -181,7 +262,9 @@ def main():
         "config": {"workload": "%s: %d streams x %d nodes x %d energies x %d mu, FP64 plain path, "
                                "%d EP substeps per step" % (args.config, dims.num_streams, c["N"], c["E"],
                                                              c["M"], c["eps"]),
-                   "path": "plain FP64 (no surrogate)"},
+                   "path": "plain FP64 (no surrogate)",
+                   "arithmetic": ("FP64, reference operation order" if args.precision == 0 else
+                                  "FP64, fused multiply-add streaming (within 1e-13 of the reference order)")},
         "roofline": {"bound": "hbm", "kernel": "k_stream_pass", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None,
@@ -203,6 +286,7 @@ def main():
         res["config"]["active_nodes"] = {"min": int(n_active.min()), "max": int(n_active.max()),
                                          "mean": float(n_active.mean())}
     if not args.no_cpu:
+        log("timed steps done; CPU baseline")
         res["cpu_baseline"] = cpu_baseline(c, p, nodes, n_active)
     sol.close()
     print(json.dumps(res))

@@ -209,4 +209,13 @@ EXPORTED_SYMBOLS = (
     "ep_move_nodes",
     "ep_get_positions",
     "ep_set_precision",
+    "ep_selftest_copy",
+    "ep_mlp_init",
+    "ep_mlp_num_params",
+    "ep_mlp_get_params",
+    "ep_mlp_set_params",
+    "ep_mlp_forward",
+    "ep_mlp_loss_grad",
+    "ep_mlp_train",
+    "ep_mlp_finalize",
 )

@@ -158,6 +158,7 @@ Dev make_dev() {
   D.mfpType = p.mfpType;
   D.useBC = (p.useEPBoundary > 0) && (p.useBoundaryFunction > 0);
   D.fp32 = g.fp32;
+  D.fast = g.fast;
   D.stream_block = g.stream_block;
   D.stream_halo = g.stream_halo;
   D.mfpRadialPower = p.mfpRadialPower;
@@ -336,6 +337,7 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
   const size_t SN = S * NP, SE = S * E;
   g.init = true;  // so that free_all() releases partial allocations
   g.fp32 = 0;     // f starts as FP64 (filled below)
+  g.fast = 0;
 #define AL(ptr, cnt) \
   if ((rc = dalloc(&(ptr), (cnt), #ptr))) { free_all(); return rc; }
   AL(g.egrid_d, E) AL(g.vgrid_d, E) AL(g.vgrid_i_d, E) AL(g.rig_lamo_d, E) AL(g.plt_d, E)
@@ -457,19 +459,24 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, double *best_ms) {
   return EP_OK;
 }
 
-// Storage and arithmetic of the hot path: 0 = FP64 throughout (the parity
-// path), 1 = f stored as float, streaming and focusing in FP32, adiabatic
-// change (the ln f energy flux) in FP64.  The distribution on the device is
-// converted.
+// Storage and arithmetic of the hot path: 0 = FP64 in the reference's
+// operation order (the parity path), 1 = f stored as float, streaming and
+// focusing in FP32, adiabatic change (the ln f energy flux) in FP64,
+// 2 = FP64 with fused multiply-adds in the streaming kernel.  The
+// distribution on the device is converted when the storage type changes.
 int ep_set_precision(int32_t mode) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
-  if (mode != 0 && mode != 1) return fail(EP_ERR_INVALID_ARG, "precision mode must be 0 (FP64) or 1 (mixed)");
-  if (mode == g.fp32) return EP_OK;
-  Dev D = make_dev();
-  HIPCHK(launch_normalize(D, g.st), "normalize");
-  HIPCHK(launch_convert(D, mode, g.st), "convert");
-  HIPCHK(hipStreamSynchronize(g.st), "convert sync");
-  g.fp32 = mode;
+  if (mode < 0 || mode > 2)
+    return fail(EP_ERR_INVALID_ARG, "precision mode must be 0 (FP64), 1 (mixed) or 2 (FP64, fused)");
+  const int fp32 = (mode == 1) ? 1 : 0;
+  if (fp32 != g.fp32) {
+    Dev D = make_dev();
+    HIPCHK(launch_normalize(D, g.st), "normalize");
+    HIPCHK(launch_convert(D, fp32, g.st), "convert");
+    HIPCHK(hipStreamSynchronize(g.st), "convert sync");
+  }
+  g.fp32 = fp32;
+  g.fast = (mode != 0) ? 1 : 0;
   g.drop_graph();
   return EP_OK;
 }
@@ -686,6 +693,38 @@ int ep_selftest_dpp(const double *in64, double *shr64, double *shl64) {
   if (e == hipSuccess) e = hipMemcpy(shl64, d + 128, 64 * 8, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "dpp selftest");
+  return EP_OK;
+}
+
+// Test hook: a plain device-to-device copy kernel of `bytes` bytes (a
+// multiple of 16), run `reps` times; *ms = mean kernel time.  The reference
+// point for HBM byte counters and bandwidth.
+int ep_selftest_copy(size_t bytes, int32_t reps, double *ms) {
+  if (bytes == 0 || bytes % 16 || reps < 1) return fail(EP_ERR_INVALID_ARG, "bytes must be a positive multiple of 16");
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, bytes), "hipMalloc");
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return hip_fail(e, "hipMalloc");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  e = hipMemset(a, 0, bytes);
+  if (e == hipSuccess) e = launch_copy_probe(a, b, bytes, 0);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy_probe(a, b, bytes, 0);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return hip_fail(e, "copy probe");
+  if (ms) *ms = t / reps;
   return EP_OK;
 }
 

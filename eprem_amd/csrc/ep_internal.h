@@ -107,6 +107,7 @@ struct Dev {
   double *c_dupar, *c_dlnn, *c_dlnb, *c_damp, *c_bc0, *c_bc1;
   DevDiag *diag;
   int fp32;  // f stored as float (mixed-precision form)
+  int fast;  // streaming with fused multiply-adds (not bit-identical)
 };
 
 __device__ inline void note_error(const Dev &D, unsigned long long key) {

@@ -21,5 +21,6 @@ hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_init_seed(const Dev &D, hipStream_t st);
 hipError_t launch_dpp_probe(const double *in, double *out, double *out2, hipStream_t st);
+hipError_t launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_reset_diag(DevDiag *d, hipStream_t st);
 }  // namespace ep

@@ -1,0 +1,437 @@
+// ep_mlp.hip -- a small multilayer perceptron on the matrix cores: forward,
+// backward and Adam, for the learned large-step surrogate of streaming.
+//
+// Every layer is Z = X W + b (ReLU on hidden layers).  Products run on
+// MFMA v_mfma_f32_32x32x8_bf16: operands rounded to bf16 (round to nearest
+// even), fp32 accumulation; master weights, activations and gradients are
+// fp32.  One wave computes a 32 x 32 tile of C = op(A) op(B); the weight
+// gradient, whose inner dimension is the batch, is split over the batch
+// with atomic accumulation.
+//
+// Parameters are one flat fp32 vector: for each layer W [in][out]
+// (row-major) then b [out].
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ep_internal.h"
+#include "ep_state.h"
+
+namespace ep {
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ short to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (short)(u >> 16);
+}
+
+struct GemmArgs {
+  const float *A, *B;
+  float *C;
+  const float *bias;
+  int M, N, K;
+  int lda, ldb, ldc;
+  int transA, transB;  // element (i,k) of A at A[k*lda+i] when transA, ...
+  int relu, atomic;    // epilogue: bias + ReLU, or atomic accumulation
+  int kchunk;          // inner range per blockIdx.z (multiple of 8)
+};
+
+// MFMA 32x32x8 bf16 operand layout: lane l holds A[l%32][4*(l/32) + q] and
+// B[4*(l/32) + q][l%32], q = 0..3; result element r of lane l is
+// C[8*(r/4) + 4*(l/32) + r%4][l%32].
+__global__ __launch_bounds__(64) void k_gemm(GemmArgs g) {
+  const int lane = threadIdx.x;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int kbeg = blockIdx.z * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const int li = lane & 31, lk = (lane >> 5) * 4;
+  const int i = i0 + li, j = j0 + li;
+  f32x16_t acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = kbeg; k0 < kend; k0 += 8) {
+    bf16x4_t a, b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + lk + q;
+      float av = 0.0f, bv = 0.0f;
+      if (k < kend) {
+        if (i < g.M) av = g.transA ? g.A[(size_t)k * g.lda + i] : g.A[(size_t)i * g.lda + k];
+        if (j < g.N) bv = g.transB ? g.B[(size_t)j * g.ldb + k] : g.B[(size_t)k * g.ldb + j];
+      }
+      a[q] = to_bf16(av);
+      b[q] = to_bf16(bv);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int ii = i0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int jj = j0 + (lane & 31);
+    if (ii < g.M && jj < g.N) {
+      float v = acc[r];
+      if (g.atomic) {
+        atomicAdd(&g.C[(size_t)ii * g.ldc + jj], v);
+      } else {
+        if (g.bias) v += g.bias[jj];
+        if (g.relu) v = v > 0.0f ? v : 0.0f;
+        g.C[(size_t)ii * g.ldc + jj] = v;
+      }
+    }
+  }
+}
+
+// out - y and the mean-squared-error gradient 2 (out - y) / (n * m); the
+// loss sum goes to *loss (double, atomically).
+__global__ __launch_bounds__(256) void k_mse(const float *out, const float *y, float *dout, size_t count,
+                                             float scale, double *loss) {
+  __shared__ double part[256];
+  double acc = 0.0;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (size_t)gridDim.x * blockDim.x) {
+    const float d = out[t] - y[t];
+    dout[t] = scale * d;
+    acc += (double)d * (double)d;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+}
+
+// dZ = dA where the layer's ReLU output is positive, else 0 (in place).
+__global__ __launch_bounds__(256) void k_relu_back(const float *act, float *d, size_t count) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (size_t)gridDim.x * blockDim.x)
+    if (!(act[t] > 0.0f)) d[t] = 0.0f;
+}
+
+// db[j] += sum_i dZ[i][j]
+__global__ __launch_bounds__(256) void k_bias_grad(const float *dz, float *db, int n, int m) {
+  const int j = blockIdx.x;
+  float acc = 0.0f;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += dz[(size_t)i * m + j];
+  __shared__ float part[256];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[j] += part[0];
+}
+
+__global__ __launch_bounds__(256) void k_zero(float *p, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    p[t] = 0.0f;
+}
+
+// Adam (Kingma & Ba 2015) with bias correction factors c1 = 1 - beta1^t,
+// c2 = 1 - beta2^t.
+__global__ __launch_bounds__(256) void k_adam(float *w, const float *gr, float *m, float *v, size_t n, float lr,
+                                              float b1, float b2, float c1, float c2, float eps) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const float g = gr[t];
+    const float mt = b1 * m[t] + (1.0f - b1) * g;
+    const float vt = b2 * v[t] + (1.0f - b2) * g * g;
+    m[t] = mt;
+    v[t] = vt;
+    w[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+  }
+}
+
+// Gather a minibatch: rows idx[0..n) of X and Y.
+__global__ __launch_bounds__(256) void k_gather_rows(const float *src, const int32_t *idx, float *dst, int n,
+                                                     int width) {
+  const size_t total = (size_t)n * width;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = t / width, c = t % width;
+    dst[t] = src[(size_t)idx[r] * width + c];
+  }
+}
+
+struct Mlp {
+  bool init = false;
+  std::vector<int> sizes;       // layer widths, input first
+  std::vector<size_t> woff, boff;
+  size_t nparam = 0;
+  float *param = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  long step = 0;
+  // work buffers for up to `cap` rows
+  int cap = 0;
+  std::vector<float *> act;     // act[0] = input rows, act[l] = layer l output
+  std::vector<float *> dact;    // gradients w.r.t. act[l]
+  float *ybuf = nullptr;
+  double *loss = nullptr;
+  void release() {
+    for (float *p : act) (void)hipFree(p);
+    for (float *p : dact) (void)hipFree(p);
+    act.clear();
+    dact.clear();
+    if (ybuf) (void)hipFree(ybuf);
+    ybuf = nullptr;
+    cap = 0;
+  }
+  void free_all() {
+    release();
+    float *ps[4] = {param, grad, adam_m, adam_v};
+    for (float *p : ps)
+      if (p) (void)hipFree(p);
+    param = grad = adam_m = adam_v = nullptr;
+    if (loss) (void)hipFree(loss);
+    loss = nullptr;
+    init = false;
+  }
+};
+
+static Mlp mlp;
+
+static int mlp_reserve(int rows) {
+  if (rows <= mlp.cap) return EP_OK;
+  mlp.release();
+  const int L = (int)mlp.sizes.size();
+  mlp.act.assign(L, nullptr);
+  mlp.dact.assign(L, nullptr);
+  for (int l = 0; l < L; l++) {
+    hipError_t e = hipMalloc(&mlp.act[l], (size_t)rows * mlp.sizes[l] * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&mlp.dact[l], (size_t)rows * mlp.sizes[l] * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "MLP work buffers");
+  }
+  hipError_t e = hipMalloc(&mlp.ybuf, (size_t)rows * mlp.sizes.back() * sizeof(float));
+  if (e != hipSuccess) return hip_fail(e, "MLP work buffers");
+  mlp.cap = rows;
+  return EP_OK;
+}
+
+static hipError_t gemm(const float *A, const float *B, float *C, const float *bias, int M, int N, int K, int lda,
+                       int ldb, int ldc, int transA, int transB, int relu, int atomic, int splitk,
+                       hipStream_t st) {
+  GemmArgs g{A, B, C, bias, M, N, K, lda, ldb, ldc, transA, transB, relu, atomic, 0};
+  int chunk = (K + splitk - 1) / splitk;
+  chunk = (chunk + 7) / 8 * 8;
+  g.kchunk = chunk;
+  const dim3 grid((M + 31) / 32, (N + 31) / 32, (K + chunk - 1) / chunk);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(64), 0, st, g);
+  return hipGetLastError();
+}
+
+// Forward pass over n rows already in act[0].
+static hipError_t forward(int n, hipStream_t st) {
+  const int L = (int)mlp.sizes.size();
+  for (int l = 1; l < L; l++) {
+    const int in = mlp.sizes[l - 1], out = mlp.sizes[l];
+    hipError_t e = gemm(mlp.act[l - 1], mlp.param + mlp.woff[l - 1], mlp.act[l], mlp.param + mlp.boff[l - 1], n,
+                        out, in, in, out, out, 0, 0, (l < L - 1) ? 1 : 0, 0, 1, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Backward pass from the targets in ybuf; accumulates into grad (zeroed
+// here) and adds the summed squared error to *loss.
+static hipError_t backward(int n, hipStream_t st) {
+  const int L = (int)mlp.sizes.size();
+  const int m = mlp.sizes.back();
+  hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, st, mlp.grad, mlp.nparam);
+  const size_t cnt = (size_t)n * m;
+  hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, st, mlp.act[L - 1], mlp.ybuf, mlp.dact[L - 1], cnt,
+                     2.0f / (float)cnt, mlp.loss);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (int l = L - 1; l >= 1; l--) {
+    const int in = mlp.sizes[l - 1], out = mlp.sizes[l];
+    if (l < L - 1) {
+      hipLaunchKernelGGL(k_relu_back, dim3(1024), dim3(256), 0, st, mlp.act[l], mlp.dact[l], (size_t)n * out);
+    }
+    // dW = act[l-1]^T dZ   (in x out, inner dimension n)
+    e = gemm(mlp.act[l - 1], mlp.dact[l], mlp.grad + mlp.woff[l - 1], nullptr, in, out, n, in, out, out, 1, 0, 0,
+             1, (n + 511) / 512, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bias_grad, dim3(out), dim3(256), 0, st, mlp.dact[l], mlp.grad + mlp.boff[l - 1], n,
+                       out);
+    // dX = dZ W^T   (n x in)
+    if (l > 1) {
+      e = gemm(mlp.dact[l], mlp.param + mlp.woff[l - 1], mlp.dact[l - 1], nullptr, n, in, out, out, out, in, 0, 1,
+               0, 0, 1, st);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ep
+
+using namespace ep;
+
+extern "C" {
+
+int ep_mlp_init(const int32_t *sizes, int32_t nsizes, uint64_t seed) {
+  if (!sizes || nsizes < 2 || nsizes > 8) return fail(EP_ERR_INVALID_ARG, "2..8 layer sizes");
+  for (int i = 0; i < nsizes; i++)
+    if (sizes[i] < 1 || sizes[i] > 4096) return fail(EP_ERR_INVALID_ARG, "layer width outside 1..4096");
+  mlp.free_all();
+  mlp.sizes.assign(sizes, sizes + nsizes);
+  mlp.woff.clear();
+  mlp.boff.clear();
+  size_t off = 0;
+  for (int l = 1; l < nsizes; l++) {
+    mlp.woff.push_back(off);
+    off += (size_t)sizes[l - 1] * sizes[l];
+    mlp.boff.push_back(off);
+    off += sizes[l];
+  }
+  mlp.nparam = off;
+  // He-uniform weights from a splitmix64 stream, zero biases
+  std::vector<float> h(off, 0.0f);
+  uint64_t x = seed;
+  auto next = [&]() {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int l = 1; l < nsizes; l++) {
+    const double lim = std::sqrt(6.0 / sizes[l - 1]);
+    for (size_t t = 0; t < (size_t)sizes[l - 1] * sizes[l]; t++)
+      h[mlp.woff[l - 1] + t] = (float)((2.0 * next() - 1.0) * lim);
+  }
+  float **bufs[4] = {&mlp.param, &mlp.grad, &mlp.adam_m, &mlp.adam_v};
+  for (float **b : bufs) {
+    hipError_t e = hipMalloc(b, off * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "MLP parameters");
+    e = hipMemset(*b, 0, off * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "MLP parameters");
+  }
+  hipError_t e = hipMalloc(&mlp.loss, sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(mlp.param, h.data(), off * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "MLP init");
+  mlp.step = 0;
+  mlp.init = true;
+  return EP_OK;
+}
+
+int ep_mlp_num_params(int64_t *count) {
+  if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
+  if (count) *count = (int64_t)mlp.nparam;
+  return EP_OK;
+}
+
+int ep_mlp_get_params(float *p) {
+  if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
+  hipError_t e = hipMemcpy(p, mlp.param, mlp.nparam * sizeof(float), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "MLP params download");
+  return EP_OK;
+}
+
+int ep_mlp_set_params(const float *p) {
+  if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
+  hipError_t e = hipMemcpy(mlp.param, p, mlp.nparam * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "MLP params upload");
+  return EP_OK;
+}
+
+int ep_mlp_forward(const float *X, int32_t n, float *Y) {
+  if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
+  if (!X || !Y || n < 1) return fail(EP_ERR_INVALID_ARG, "bad rows");
+  int rc = mlp_reserve(n);
+  if (rc) return rc;
+  const int L = (int)mlp.sizes.size();
+  hipError_t e = hipMemcpy(mlp.act[0], X, (size_t)n * mlp.sizes[0] * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = forward(n, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(Y, mlp.act[L - 1], (size_t)n * mlp.sizes[L - 1] * sizeof(float), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "MLP forward");
+  return EP_OK;
+}
+
+// Loss (mean squared error over n rows and all outputs) and its gradient
+// with respect to the flat parameter vector.
+int ep_mlp_loss_grad(const float *X, const float *Y, int32_t n, double *loss, float *grad) {
+  if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
+  if (!X || !Y || n < 1) return fail(EP_ERR_INVALID_ARG, "bad rows");
+  int rc = mlp_reserve(n);
+  if (rc) return rc;
+  const size_t m = mlp.sizes.back();
+  hipError_t e = hipMemcpy(mlp.act[0], X, (size_t)n * mlp.sizes[0] * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(mlp.ybuf, Y, (size_t)n * m * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(mlp.loss, 0, sizeof(double));
+  if (e == hipSuccess) e = forward(n, 0);
+  if (e == hipSuccess) e = backward(n, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  double ls = 0;
+  if (e == hipSuccess) e = hipMemcpy(&ls, mlp.loss, sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && grad) e = hipMemcpy(grad, mlp.grad, mlp.nparam * sizeof(float), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "MLP loss/gradient");
+  if (loss) *loss = ls / ((double)n * m);
+  return EP_OK;
+}
+
+// Minibatch training with Adam on n rows (X [n][in], Y [n][out] on the
+// host, copied to the device once); loss_per_epoch[epochs] receives the
+// mean training loss of each epoch.
+int ep_mlp_train(const float *X, const float *Y, int32_t n, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, float *loss_per_epoch) {
+  if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
+  if (!X || !Y || n < 1 || epochs < 1 || batch < 1) return fail(EP_ERR_INVALID_ARG, "bad training arguments");
+  const int in = mlp.sizes[0], out = mlp.sizes.back();
+  int rc = mlp_reserve(batch);
+  if (rc) return rc;
+  float *dx = nullptr, *dy = nullptr;
+  int32_t *didx = nullptr;
+  hipError_t e = hipMalloc(&dx, (size_t)n * in * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dy, (size_t)n * out * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&didx, (size_t)n * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMemcpy(dx, X, (size_t)n * in * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dy, Y, (size_t)n * out * sizeof(float), hipMemcpyHostToDevice);
+  std::vector<int32_t> perm(n);
+  uint64_t x = seed ^ 0x5DEECE66Dull;
+  auto rnd = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep_i = 0; ep_i < epochs && e == hipSuccess; ep_i++) {
+    for (int i = 0; i < n; i++) perm[i] = i;
+    for (int i = n - 1; i > 0; i--) std::swap(perm[i], perm[rnd() % (uint64_t)(i + 1)]);
+    e = hipMemcpy(didx, perm.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(mlp.loss, 0, sizeof(double));
+    int batches = 0;
+    for (int b0 = 0; b0 + batch <= n && e == hipSuccess; b0 += batch, batches++) {
+      hipLaunchKernelGGL(k_gather_rows, dim3(1024), dim3(256), 0, 0, dx, didx + b0, mlp.act[0], batch, in);
+      hipLaunchKernelGGL(k_gather_rows, dim3(1024), dim3(256), 0, 0, dy, didx + b0, mlp.ybuf, batch, out);
+      e = forward(batch, 0);
+      if (e == hipSuccess) e = backward(batch, 0);
+      mlp.step++;
+      const float c1 = 1.0f - powf(b1, (float)mlp.step), c2 = 1.0f - powf(b2, (float)mlp.step);
+      hipLaunchKernelGGL(k_adam, dim3(256), dim3(256), 0, 0, mlp.param, mlp.grad, mlp.adam_m, mlp.adam_v,
+                         mlp.nparam, lr, b1, b2, c1, c2, eps);
+      if (e == hipSuccess) e = hipGetLastError();
+    }
+    double ls = 0;
+    if (e == hipSuccess) e = hipMemcpy(&ls, mlp.loss, sizeof(double), hipMemcpyDeviceToHost);
+    if (loss_per_epoch) loss_per_epoch[ep_i] = (float)(ls / ((double)batches * batch * out));
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  (void)hipFree(didx);
+  if (e != hipSuccess) return hip_fail(e, "MLP training");
+  return EP_OK;
+}
+
+int ep_mlp_finalize(void) {
+  mlp.free_all();
+  return EP_OK;
+}
+
+}  // extern "C"

@@ -39,6 +39,7 @@ struct State {
   // arguments captured by value (parameters, tiling) change
   bool use_graph = true;
   int fp32 = 0;  // f stored as float (mixed-precision form)
+  int fast = 0;  // streaming with fused multiply-adds
   unsigned generation = 0;
   hipGraphExec_t graph_exec = nullptr;
   StepShape graph_shape{};

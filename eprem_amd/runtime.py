@@ -62,6 +62,16 @@ def lib():
         L.ep_get_positions.argtypes = [DP, DP, DP, DP, IP]
         L.ep_set_graph.argtypes = [C.c_int32]
         L.ep_set_precision.argtypes = [C.c_int32]
+        L.ep_selftest_copy.argtypes = [C.c_size_t, C.c_int32, DP]
+        FP = C.POINTER(C.c_float)
+        L.ep_mlp_init.argtypes = [IP, C.c_int32, C.c_uint64]
+        L.ep_mlp_num_params.argtypes = [C.POINTER(C.c_int64)]
+        L.ep_mlp_get_params.argtypes = [FP]
+        L.ep_mlp_set_params.argtypes = [FP]
+        L.ep_mlp_forward.argtypes = [FP, C.c_int32, FP]
+        L.ep_mlp_loss_grad.argtypes = [FP, FP, C.c_int32, DP, FP]
+        L.ep_mlp_train.argtypes = [FP, FP, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64, FP]
+        L.ep_mlp_finalize.argtypes = []
         for name in abi.EXPORTED_SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -111,9 +121,11 @@ class EpSolver:
         _check(lib().ep_autotune(C.byref(b), C.byref(h), C.byref(ms)))
         return b.value, h.value, ms.value
 
-    def set_precision(self, mixed):
-        """False: FP64 throughout; True: FP32 streaming/focusing, FP64 change."""
-        _check(lib().ep_set_precision(1 if mixed else 0))
+    def set_precision(self, mode):
+        """0 / False: FP64 in the reference's operation order; 1 / True: FP32
+        streaming and focusing with FP64 adiabatic change; 2: FP64 with fused
+        multiply-adds in streaming."""
+        _check(lib().ep_set_precision(int(mode)))
 
     def set_graph(self, enable):
         _check(lib().ep_set_graph(1 if enable else 0))
@@ -262,3 +274,66 @@ def selftest_dpp():
     a, b = np.zeros(64), np.zeros(64)
     _check(lib().ep_selftest_dpp(dptr(x), dptr(a), dptr(b)))
     return x, a, b
+
+
+def copy_probe(nbytes=1 << 30, reps=20):
+    """Mean time (ms) and bandwidth (GB/s, read + write) of a plain device
+    copy of nbytes bytes."""
+    ms = C.c_double()
+    _check(lib().ep_selftest_copy(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+
+
+if __name__ == "__main__":
+    import sys
+    n = int(sys.argv[1]) if len(sys.argv) > 1 else 1 << 30
+    t, bw = copy_probe(n)
+    print("copy %d bytes: %.4f ms, %.1f GB/s (read + write)" % (n, t, bw))
+
+
+class Mlp:
+    """The device multilayer perceptron (matrix-core products, bf16 operands,
+    fp32 accumulation and master weights)."""
+
+    def __init__(self, sizes, seed=1):
+        self.sizes = list(sizes)
+        sz = np.ascontiguousarray(self.sizes, dtype=np.int32)
+        _check(lib().ep_mlp_init(iptr(sz), len(sz), int(seed)))
+        n = C.c_int64()
+        _check(lib().ep_mlp_num_params(C.byref(n)))
+        self.nparam = n.value
+
+    def params(self):
+        p = np.zeros(self.nparam, np.float32)
+        _check(lib().ep_mlp_get_params(fptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        _check(lib().ep_mlp_set_params(fptr(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.sizes[-1]), np.float32)
+        _check(lib().ep_mlp_forward(fptr(x), x.shape[0], fptr(y)))
+        return y
+
+    def loss_grad(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.nparam, np.float32)
+        loss = C.c_double()
+        _check(lib().ep_mlp_loss_grad(fptr(x), fptr(y), x.shape[0], C.byref(loss), fptr(g)))
+        return loss.value, g
+
+    def train(self, x, y, epochs, batch, lr, seed=7):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        hist = np.zeros(epochs, np.float32)
+        _check(lib().ep_mlp_train(fptr(x), fptr(y), x.shape[0], int(epochs), int(batch), float(lr), int(seed),
+                                  fptr(hist)))
+        return hist
+
+    def close(self):
+        lib().ep_mlp_finalize()

@@ -105,7 +105,8 @@ def parker_nodes(rows, cols, n_nodes, tdel_days=1.0 / 64, flow_cms=400.0e5, dens
             if smooth > 0:
                 ker = np.exp(-0.5 * (np.arange(-3 * smooth, 3 * smooth + 1) / smooth) ** 2)
                 ker /= np.sqrt((ker ** 2).sum())
-                lp = lambda v: np.convolve(v, ker, mode="same")
+                half = len(ker) // 2
+                lp = lambda v: np.convolve(v, ker, mode="full")[half:half + len(v)]
                 base = lp(rng.normal(0, amp, n_nodes))
                 new = np.exp(np.clip(base + 0.3 * lp(rng.normal(0, amp, n_nodes)), -1, 1))
                 return new, np.exp(np.clip(base, -1, 1))
@@ -173,3 +174,20 @@ def seeded_f(p, grids, nodes, noise=0.0, seed=None):
 def dims_of(nodes, E, M):
     S, N = nodes["x"].shape
     return EpDims(S, N, E, M)
+
+
+def power_law_f(grids, nodes, M, seed=None, f_max=1.0e5, index=(-5.0, -2.0), radial_power=-2.0):
+    """f[S, N, E, M] for the mixed-precision workload: per stream a seeded
+    power law E^g with g uniform in `index`, times (r/r_inner)^radial_power,
+    scaled to f_max at the lowest energy of the inner node, isotropic.  With
+    the default indices and 10 keV - 1 GeV it spans about 1e5 down to
+    1e-30."""
+    rng = np.random.default_rng(seed)
+    eg = grids["egrid"]
+    rm = nodes["rmag"]
+    S = rm.shape[0]
+    g = rng.uniform(index[0], index[1], S)
+    spec = (eg[None, :] / eg[0]) ** g[:, None]                      # [S, E]
+    rad = (rm / rm[:, :1]) ** radial_power                          # [S, N]
+    f = f_max * rad[:, :, None] * spec[:, None, :]
+    return np.repeat(f[..., None], M, axis=3)

@@ -300,6 +300,23 @@ int ep_observe(const ep_observers *obs, double *dist, double *flux);
  * flux [num_streams][num_nodes][num_energies], code units. */
 int ep_stream_flux(double *flux);
 
+/* ---------------- learned surrogate: multilayer perceptron --------- */
+/* No counterpart in the reference.  A fully connected network (ReLU on
+ * hidden layers) whose products run on the matrix cores (bf16 operands,
+ * fp32 accumulation); parameters are one flat fp32 vector, per layer
+ * W [in][out] row-major then b [out].  Rows are fp32 [n][width]. */
+int ep_mlp_init(const int32_t *sizes, int32_t nsizes, uint64_t seed);
+int ep_mlp_num_params(int64_t *count);
+int ep_mlp_get_params(float *params);
+int ep_mlp_set_params(const float *params);
+int ep_mlp_forward(const float *x, int32_t n, float *y);
+/* mean squared error over n rows and its gradient (flat, like params) */
+int ep_mlp_loss_grad(const float *x, const float *y, int32_t n, double *loss, float *grad);
+/* Adam minibatch training; loss_per_epoch[epochs] may be NULL */
+int ep_mlp_train(const float *x, const float *y, int32_t n, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, float *loss_per_epoch);
+int ep_mlp_finalize(void);
+
 /* ---------------- misc --------------------------------------------- */
 int ep_last_error(char *buf, size_t len);
 int ep_finalize(void);
@@ -322,18 +339,26 @@ int ep_autotune(int32_t *stream_block, int32_t *stream_halo, double *stream_ms);
  * No counterpart in the reference. */
 int ep_set_graph(int32_t enable);
 
-/* Precision of the hot path: 0 = FP64 throughout (default; the parity
- * path), 1 = mixed: f stored as float with streaming, relaxation and
- * focusing in FP32 (FP32 accumulation) and the adiabatic energy change
- * (ln f flux) in FP64.  The distribution on the device is converted; the
- * float floor is FLT_MIN where the reference floors at DBL_MIN.  No
- * counterpart in the reference. */
+/* Precision of the hot path: 0 = FP64 in the reference's operation order
+ * (default; the parity path), 1 = mixed: f stored as float with streaming,
+ * relaxation and focusing in FP32 (FP32 accumulation) and the adiabatic
+ * energy change (ln f flux) in FP64, 2 = FP64 with fused multiply-adds
+ * in streaming and relaxation (same results to ~1e-15 relative, 6 instead
+ * of 9 vector operations per cell and subcycle).  The distribution on the
+ * device is converted when the storage type changes; the float floor is
+ * FLT_MIN where the reference floors at DBL_MIN.  No counterpart in the
+ * reference. */
 int ep_set_precision(int32_t mode);
 
 /* Device self-test of the cross-lane shift the streaming kernel uses: for
  * one 64-lane wave, shr[i] = in[i-1] (shr[0] = -1) and shl[i] = in[i+1]
  * (shl[63] = -2).  No counterpart in the reference; used by the tests. */
 int ep_selftest_dpp(const double *in64, double *shr64, double *shl64);
+
+/* Test hook: a plain copy kernel of `bytes` bytes (multiple of 16) on a
+ * scratch pair of buffers, `reps` times; *ms = mean kernel time.  The
+ * reference point for HBM byte counters and bandwidth. */
+int ep_selftest_copy(size_t bytes, int32_t reps, double *ms);
 
 #ifdef __cplusplus
 }
