@@ -40,7 +40,71 @@ CONFIGS = {
     "c3": dict(rows=16, cols=16, N=4000, E=64, M=32, eps=10, ragged=(200, 4000)),
     # mixed-precision form on the c2 workload with a 35-decade power-law f
     "c4": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10, mixed=True),
+    # learned large-step surrogate of K = 16 streaming subcycles on the c2
+    # workload, trained on pairs from the plain solver
+    "c5": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10, surrogate=16),
 }
+
+
+def run_surrogate(args, c, p, dims):
+    """Learned surrogate: 1e6 training pairs (x: ln f at a node and its two
+    neighbours + 6 scalars, y: ln f after K plain subcycles) from three
+    seeded wind/CME variants of the c2 workload, device training (Adam),
+    held-out error of the node flux <f>_mu against the plain path, and the
+    network's throughput beside the plain streaming kernel's, both in
+    cell-subcycles/s of streaming."""
+    from eprem_amd import surrogate
+
+    Ks = c["surrogate"]
+    xs, ys = [], []
+    plain = None
+    for vi, (seed, cme_r) in enumerate(((20261203, 0.5), (7, 0.3), (11, 0.8))):
+        nodes = bench_nodes(c, seed=seed, cme_r=cme_r)
+        sol = runtime.EpSolver(dims, p)
+        sol.set_nodes(nodes)
+        sol.init_seed_population()
+        sol.set_graph(False)
+        _, dg = sol.step()  # develop anisotropy from the isotropic seed
+        if plain is None:
+            _, dg = sol.step()
+            plain = dg
+        x, y = sol.surrogate_samples(Ks, 380000, seed=seed)
+        sol.close()
+        xs.append(x)
+        ys.append(y)
+        log("surrogate samples from variant %d: %d" % (vi, len(x)))
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    rng = np.random.default_rng(1)
+    order = rng.permutation(len(x))
+    x, y = x[order], y[order]
+    cut = int(0.9 * len(x))
+    t0 = time.perf_counter()
+    net, hist = surrogate.train(x[:cut], y[:cut], epochs=8, batch=8192, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    log("trained: loss %.3e -> %.3e in %.1f s" % (hist[0], hist[-1], train_s))
+    pred = net.forward(x[cut:])
+    rows = 1 << 20
+    ms = net.bench(rows, reps=5)
+    net.close()
+    e_net = surrogate.flux_error(pred, y[cut:])
+    e_base = surrogate.flux_error(surrogate.relaxation_step(x[cut:], c["M"]), y[cut:])
+    sur_rate = rows / (ms * 1e-3) * c["M"] * Ks
+    plain_rate = plain["stream_sweeps"] / (plain["stream_ms"] * 1e-3)
+    within = float(np.mean(e_net <= 0.01))
+    return {
+        "metric": "phase-space cell-updates/s (1 GPU), surrogate vs plain at <=1% flux error",
+        "value": sur_rate, "unit": "cell-subcycles/s of streaming (network forward, no fallback)",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 MFMA, fp32 accumulate", "data": "synthetic",
+        "config": {"workload": "c5: c2 shapes, MLP %d-128-128-128-%d replacing K=%d streaming subcycles"
+                               % (3 * c["M"] + 6, c["M"], Ks), "pairs": int(len(x)), "train_s": train_s},
+        "plain_streaming": {"value": plain_rate, "unit": "cell-subcycles/s", "kernel": "k_stream_pass"},
+        "surrogate_over_plain": sur_rate / plain_rate,
+        "flux_error_heldout": {"network": surrogate.summary(e_net),
+                               "without_network_relaxation_only": surrogate.summary(e_base)},
+        "fallback_fraction_at_1pct": 1.0 - within,
+        "loss_per_epoch": [float(v) for v in hist],
+    }
 
 OBSERVERS = dict(r_au=np.array([0.3, 0.5, 1.0]), theta=np.array([1.2, 1.571, 2.0]),
                  phi=np.array([0.5, 2.5, 4.5]))
@@ -127,16 +191,16 @@ def make_params(c):
         boundaryFunctEcutoff=1.0e30)
 
 
-def bench_nodes(c):
+def bench_nodes(c, seed=20261203, cme_r=0.5):
     """Parker spiral (400 km/s, B_r 2.5 nT at 1 AU, 25-day rotation) with a
     smooth seeded 10% perturbation (correlation ~20 nodes), 2% spacing
     jitter and a Gaussian CME shell at 0.5 AU (1500 km/s, density and field
     x4, 60 deg wide, 0.05 AU thick)."""
     return workloads.parker_nodes(
         c["rows"], c["cols"], c["N"], tdel_days=1.0 / 64, flow_cms=400e5, b_au=2.5e-5,
-        omega=2 * math.pi * K.TAU / (25 * 86400.0), rscale=0.00465047, seed=20261203, perturb=0.1,
+        omega=2 * math.pi * K.TAU / (25 * 86400.0), rscale=0.00465047, seed=seed, perturb=0.1,
         smooth=20, jitter=0.02,
-        cme=dict(r_au=0.5, width_au=0.05, speed_cms=1500e5, dens=4.0, ang_deg=60.0))
+        cme=dict(r_au=cme_r, width_au=0.05, speed_cms=1500e5, dens=4.0, ang_deg=60.0))
 
 
 def cpu_baseline(c, p, nodes, n_active=None, budget_s=20.0):
@@ -186,6 +250,9 @@ def main():
     dims = workloads.dims_of(nodes, c["E"], c["M"])
     if c.get("mixed"):
         print(json.dumps(run_mixed(args, c, p, nodes, dims)))
+        return
+    if c.get("surrogate"):
+        print(json.dumps(run_surrogate(args, c, p, dims)))
         return
     n_active = active_counts(c, dims.num_streams)
     sol = runtime.EpSolver(dims, p)

@@ -35,7 +35,8 @@ EP_OP_STREAM = 0x1
 EP_OP_FOCUS = 0x2
 EP_OP_CHANGE = 0x4
 EP_OP_SEEDCHECK = 0x8
-EP_OP_ALL = 0xF
+EP_OP_SHELL = 0x10
+EP_OP_ALL = 0x1F
 
 
 class EpDims(C.Structure):
@@ -218,4 +219,7 @@ EXPORTED_SYMBOLS = (
     "ep_mlp_loss_grad",
     "ep_mlp_train",
     "ep_mlp_finalize",
+    "ep_mlp_bench",
+    "ep_surrogate_samples",
+    "ep_shell_diffusion",
 )

@@ -250,10 +250,16 @@ static int decode_err(unsigned long long key, ep_diag *dg) {
     dg->err_energy = a;
     dg->err_mu = b;
     dg->err_node = c;
-  } else {
+  } else if (group == 1) {
     code = inf ? EP_ERR_INF : EP_ERR_NAN;
     dg->err_node = a;
     dg->err_energy = b;
+    dg->err_mu = c;
+  } else {  // shell diffusion: [node][energy | stream][mu] in the reference's order
+    code = inf ? EP_ERR_INF : EP_ERR_NAN;
+    dg->err_node = s;
+    dg->err_energy = a >> 7;
+    dg->err_stream = ((a & 0x7F) << 7) | b;
     dg->err_mu = c;
   }
   dg->err_code = code;
@@ -303,6 +309,7 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
     if (sh.do_focus) EPQ(launch_focus(D, sub, st));
     if (e4) EPQ(hipEventRecord(e4[1], st));
     if (sh.do_change) EPQ(launch_change(D, sub, st));
+    if (sh.do_shell) EPQ(launch_shell_diffuse(D, sub, st));  // DiffuseShellData, after the stream loop
     if (e4) EPQ(hipEventRecord(e4[2], st));
   }
 #undef EPQ
@@ -591,7 +598,15 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
     if (out) *out = dg;
     return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
   }
-  const StepShape shape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
+  const int do_shell = (g.kperxkpar > 0.0) && (op_flags & EP_OP_SHELL) ? 1 : 0;
+  if (do_shell) {
+    // once per global step: node positions and B are fixed inside it
+    if (!g.mhd_state) return fail(EP_ERR_INVALID_ARG, "shell diffusion needs the device MHD state (ep_update_mhd)");
+    for (int32_t n : g.h_nactive)
+      if (n != g.d.num_nodes) return fail(EP_ERR_UNSUPPORTED, "shell diffusion needs every stream full");
+    HIPCHK(launch_shell_links(D, st), "shell links");
+  }
+  const StepShape shape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed, do_shell,
                         g.generation};
   if (g.use_graph) {
     // The substep loop is one HIP graph, rebuilt only when its shape (pass

@@ -1051,6 +1051,12 @@ hipError_t launch_normalize(const Dev &D, hipStream_t st) {
   return hipSuccess;
 }
 
+hipError_t launch_flip_all(const Dev &D, hipStream_t st) {
+  hipLaunchKernelGGL(k_flip_all, dim3((D.S * D.E + 255) / 256), dim3(256), 0, st, D);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
 hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st) {
   const dim3 grid((D.Npad + 255) / 256, D.E, D.S);
   if (to_fp32)

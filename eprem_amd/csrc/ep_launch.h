@@ -18,6 +18,9 @@ hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstb
 hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st);
 hipError_t launch_normalize(const Dev &D, hipStream_t st);
 hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st);
+hipError_t launch_flip_all(const Dev &D, hipStream_t st);
+hipError_t launch_shell_links(const Dev &D, hipStream_t st);
+hipError_t launch_shell_diffuse(const Dev &D, int sub, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_init_seed(const Dev &D, hipStream_t st);
 hipError_t launch_dpp_probe(const double *in, double *out, double *out2, hipStream_t st);

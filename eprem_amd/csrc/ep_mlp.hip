@@ -429,6 +429,32 @@ int ep_mlp_train(const float *X, const float *Y, int32_t n, int32_t epochs, int3
   return EP_OK;
 }
 
+// Forward pass over n rows already resident on the device (whatever the
+// work buffer holds), reps times; *ms = mean time per pass.  Measures the
+// network alone, without host transfers.
+int ep_mlp_bench(int32_t n, int32_t reps, double *ms) {
+  if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
+  if (n < 1 || reps < 1) return fail(EP_ERR_INVALID_ARG, "bad rows");
+  int rc = mlp_reserve(n);
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, 0, mlp.act[0], (size_t)n * mlp.sizes[0]);
+  hipError_t e = forward(n, 0);
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = forward(n, 0);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(e, "MLP bench");
+  if (ms) *ms = t / reps;
+  return EP_OK;
+}
+
 int ep_mlp_finalize(void) {
   mlp.free_all();
   return EP_OK;

@@ -19,12 +19,12 @@ struct MasLevel {
 
 // What a captured substep graph depends on besides device memory.
 struct StepShape {
-  int passes, chunks, nsub, do_stream, do_focus, do_change, do_seed;
+  int passes, chunks, nsub, do_stream, do_focus, do_change, do_seed, do_shell;
   unsigned generation;
   bool operator==(const StepShape &o) const {
     return passes == o.passes && chunks == o.chunks && nsub == o.nsub && do_stream == o.do_stream &&
            do_focus == o.do_focus && do_change == o.do_change && do_seed == o.do_seed &&
-           generation == o.generation;
+           do_shell == o.do_shell && generation == o.generation;
   }
 };
 
@@ -78,6 +78,10 @@ struct State {
   std::vector<void *> mas_allocs;
   double *mas_out = nullptr;   // 7 x S x Npad
   double *mas_prev = nullptr;  // 10 x S x Npad
+  // shell diffusion (off while kperxkpar == 0)
+  double kperxkpar = 0.0;
+  int32_t *shell_nbr = nullptr, *shell_src = nullptr, *shell_dir = nullptr;
+  double *shell_dlper = nullptr;
   // the MHD buffer's state half holds the latest B, V, density per node
   // once ep_update_mhd has run (ep_move_nodes reads V from it)
   bool mhd_state = false;
@@ -92,6 +96,13 @@ struct State {
     if (flux_buf) (void)hipFree(flux_buf);
     if (spawn_buf) (void)hipFree(spawn_buf);
     obs_buf = flux_buf = spawn_buf = nullptr;
+    int32_t *ip[3] = {shell_nbr, shell_src, shell_dir};
+    for (int32_t *q : ip)
+      if (q) (void)hipFree(q);
+    if (shell_dlper) (void)hipFree(shell_dlper);
+    shell_nbr = shell_src = shell_dir = nullptr;
+    shell_dlper = nullptr;
+    kperxkpar = 0.0;
     obs_cap = flux_cap = spawn_cap = 0;
     mhd_state = false;
     mas[0] = MasLevel();

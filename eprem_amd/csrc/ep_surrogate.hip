@@ -1,0 +1,123 @@
+// ep_surrogate.hip -- training pairs for the learned large-step surrogate.
+//
+// The surrogate maps the distribution around one kept node of one
+// (stream, energy) slab to the distribution at that node K streaming and
+// relaxation subcycles later (DiffuseStreamData, src/energeticParticles.c:
+// 2352-2430).  Pairs come from the plain solver: one streaming pass of
+// exactly K subcycles (halo = K, read-only with respect to the step state:
+// the result goes to the spare f buffer) on the current nodes and f.
+//
+// Features of a sample (96): ln f at the kept nodes k-1, k, k+1 over the M
+// mu cells, each minus the mean ln f at node k (3M values, M = 30 gives
+// 90), then six scalars: K v dtProp / ds_k (distance travelled in cells),
+// K dtProp / tau_k (relaxation), ln(ds_k/ds_{k-1}), ln(ds_{k+1}/ds_k),
+// v, and the fraction of the substep the K subcycles cover.
+// Target (M): ln f at node k after the K subcycles minus the same mean.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ep_internal.h"
+#include "ep_launch.h"
+#include "ep_state.h"
+
+namespace ep {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// One thread per sample; rows whose random (stream, energy, node) does not
+// qualify (too few subcycles, thinned stream, end node) get valid = 0.
+__global__ __launch_bounds__(256) void k_samples(Dev D, int K, int n, uint64_t seed, float *X, float *Y,
+                                                 int32_t *valid) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int M = D.M, nin = 3 * M + 6;
+  const uint64_t h = mix64(seed ^ (uint64_t)r * 0xD1B54A32D192ED03ull);
+  const int s = (int)(h % (uint64_t)D.S);
+  const int e = (int)((h >> 20) % (uint64_t)D.E);
+  const int task = s * D.E + e;
+  const int nk = D.nkept[s];
+  int ok = (D.thin[s] == 0) && (D.nsteps[task] >= K) && (nk >= 4);
+  const int kk = ok ? 1 + (int)((h >> 40) % (uint64_t)(nk - 2)) : 1;
+  valid[r] = ok;
+  if (!ok) return;
+  const size_t on = (size_t)s * D.Npad;
+  const double *cur = fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad;
+  const double *aft = fbuf(D, D.loc[task] ^ 1) + (size_t)task * M * D.Npad;
+  float *x = X + (size_t)r * nin;
+  double c = 0.0;
+  for (int m = 0; m < M; m++) c += log(cur[(size_t)m * D.Npad + kk]);
+  c /= M;
+  for (int q = 0; q < 3; q++)
+    for (int m = 0; m < M; m++) x[q * M + m] = (float)(log(cur[(size_t)m * D.Npad + kk - 1 + q]) - c);
+  const double v = D.vgrid[e], dtp = D.dtprop[task];
+  const double tau = D.vgrid_i[e] * (D.rig_lamo[e] * D.mfp_r[on + kk]);
+  const double dsi = D.dsil[on + kk], dsim = D.dsil[on + kk - 1], dsip = D.dsil[on + kk + 1];
+  float *xs = x + 3 * M;
+  xs[0] = (float)(K * v * dtp * dsi);
+  xs[1] = (float)(K * dtp / tau);
+  xs[2] = (float)log(dsim / dsi);
+  xs[3] = (float)log(dsi / dsip);
+  xs[4] = (float)v;
+  xs[5] = (float)((double)K / D.nsteps[task]);
+  float *y = Y + (size_t)r * M;
+  for (int m = 0; m < M; m++) y[m] = (float)(log(aft[(size_t)m * D.Npad + kk]) - c);
+}
+
+}  // namespace ep
+
+using namespace ep;
+
+extern "C" {
+
+// n candidate samples of K-subcycle streaming on the current nodes and f
+// (FP64 storage); X [n][3M+6], Y [n][M] on the host, valid[n] marks the
+// rows that qualified.  f and the step state are unchanged.
+int ep_surrogate_samples(int32_t K, int32_t n, uint64_t seed, float *X, float *Y, int32_t *valid) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
+  if (g.fp32) return fail(EP_ERR_UNSUPPORTED, "samples need FP64 storage");
+  if (K < 1 || K > 64 || n < 1 || !X || !Y || !valid) return fail(EP_ERR_INVALID_ARG, "bad sample request");
+  const int M = g.d.num_mu, nin = 3 * M + 6;
+  const int keep_b = g.stream_block, keep_h = g.stream_halo;
+  g.stream_block = 256;
+  g.stream_halo = K;
+  Dev D = make_dev();
+  float *dx = nullptr, *dy = nullptr;
+  int32_t *dv = nullptr;
+  hipError_t e = hipMalloc(&dx, (size_t)n * nin * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dy, (size_t)n * M * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dv, (size_t)n * sizeof(int32_t));
+  DevDiag hd;
+  if (e == hipSuccess) e = launch_reset_diag(g.diag, g.st);
+  if (e == hipSuccess) e = launch_stream_prep(D, g.st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, g.st);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.st);
+  // pass 0 with halo K: slabs with nsteps >= K advance exactly K subcycles
+  if (e == hipSuccess) e = launch_stream_pass(D, 0, 0, 0, 0, hd.max_chunks, g.st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_samples, dim3((n + 255) / 256), dim3(256), 0, g.st, D, K, n, seed, dx, dy, dv);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(g.st);
+  if (e == hipSuccess) e = hipMemcpy(X, dx, (size_t)n * nin * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(Y, dy, (size_t)n * M * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(valid, dv, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  (void)hipFree(dv);
+  g.stream_block = keep_b;
+  g.stream_halo = keep_h;
+  g.drop_graph();
+  if (e != hipSuccess) return hip_fail(e, "surrogate samples");
+  if (hd.err_key != ~0ull) return fail(EP_ERR_INVALID_ARG, "node preparation reported an error");
+  return EP_OK;
+}
+
+}  // extern "C"

@@ -72,6 +72,9 @@ def lib():
         L.ep_mlp_loss_grad.argtypes = [FP, FP, C.c_int32, DP, FP]
         L.ep_mlp_train.argtypes = [FP, FP, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64, FP]
         L.ep_mlp_finalize.argtypes = []
+        L.ep_mlp_bench.argtypes = [C.c_int32, C.c_int32, DP]
+        L.ep_surrogate_samples.argtypes = [C.c_int32, C.c_int32, C.c_uint64, FP, FP, IP]
+        L.ep_shell_diffusion.argtypes = [IP, C.c_double]
         for name in abi.EXPORTED_SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -265,6 +268,23 @@ class EpSolver:
         _check(lib().ep_stream_flux(dptr(flux)))
         return flux
 
+    def shell_diffusion(self, nbr, kperxkpar):
+        """Enable perpendicular diffusion between neighbouring streams
+        (nbr [S, 4] from eprem_amd.cube.neighbours); kperxkpar = 0 disables."""
+        nb = None if nbr is None else np.ascontiguousarray(nbr, dtype=np.int32)
+        _check(lib().ep_shell_diffusion(iptr(nb), float(kperxkpar)))
+
+    def surrogate_samples(self, K, n, seed=1):
+        """Training pairs (x [k, 3M+6], y [k, M]) of K streaming subcycles on
+        the current state; only the qualifying rows of n candidates."""
+        M = self.dims.num_mu
+        x = np.zeros((n, 3 * M + 6), np.float32)
+        y = np.zeros((n, M), np.float32)
+        v = np.zeros(n, np.int32)
+        _check(lib().ep_surrogate_samples(int(K), int(n), int(seed), fptr(x), fptr(y), iptr(v)))
+        ok = v != 0
+        return x[ok], y[ok]
+
     def close(self):
         lib().ep_finalize()
 
@@ -334,6 +354,12 @@ class Mlp:
         _check(lib().ep_mlp_train(fptr(x), fptr(y), x.shape[0], int(epochs), int(batch), float(lr), int(seed),
                                   fptr(hist)))
         return hist
+
+    def bench(self, n, reps=5):
+        """Mean ms of one forward pass over n device-resident rows."""
+        ms = C.c_double()
+        _check(lib().ep_mlp_bench(int(n), int(reps), C.byref(ms)))
+        return ms.value
 
     def close(self):
         lib().ep_mlp_finalize()

@@ -76,7 +76,9 @@ extern "C" {
 #define EP_OP_FOCUS       0x2u   /* AdiabaticFocusing                   */
 #define EP_OP_CHANGE      0x4u   /* AdiabaticChange                     */
 #define EP_OP_SEEDCHECK   0x8u   /* seedPopulationCheck                 */
-#define EP_OP_ALL         0xFu
+#define EP_OP_SHELL       0x10u  /* DiffuseShellData (when enabled by
+                                    ep_shell_diffusion)                 */
+#define EP_OP_ALL         0x1Fu
 
 /* ------------------------------------------------------------------ */
 /* Dimensions and parameters                                           */
@@ -300,6 +302,16 @@ int ep_observe(const ep_observers *obs, double *dist, double *flux);
  * flux [num_streams][num_nodes][num_energies], code units. */
 int ep_stream_flux(double *flux);
 
+/* ---------------- shell diffusion ----------------------------------- */
+/* DiffuseShellData with the link lengths of ShellData
+ * (src/energeticParticles.c:247-555): perpendicular diffusion between the
+ * streams of each shell, every EP substep after focusing and change.
+ * kperxkpar > 0 enables it (config.kperxkpar; 0 disables), nbr[S][4] =
+ * north, east, west, south neighbour stream of every stream (the cube
+ * surface links of initNEWS, src/cubeShellInit.c:95-520).  Needs the
+ * device MHD state (ep_update_mhd) for B and every stream full. */
+int ep_shell_diffusion(const int32_t *nbr, double kperxkpar);
+
 /* ---------------- learned surrogate: multilayer perceptron --------- */
 /* No counterpart in the reference.  A fully connected network (ReLU on
  * hidden layers) whose products run on the matrix cores (bf16 operands,
@@ -315,7 +327,18 @@ int ep_mlp_loss_grad(const float *x, const float *y, int32_t n, double *loss, fl
 /* Adam minibatch training; loss_per_epoch[epochs] may be NULL */
 int ep_mlp_train(const float *x, const float *y, int32_t n, int32_t epochs, int32_t batch, float lr,
                  uint64_t seed, float *loss_per_epoch);
+/* forward pass over n device-resident rows, reps times; *ms = mean time */
+int ep_mlp_bench(int32_t n, int32_t reps, double *ms);
 int ep_mlp_finalize(void);
+
+/* Training pairs for the large-step surrogate of streaming: n random kept
+ * nodes of (stream, energy) slabs with at least K streaming subcycles per
+ * substep, on the current nodes and f (FP64 storage).  x [n][3M+6]: ln f at
+ * nodes k-1, k, k+1 minus the mean ln f at k, then K v dtProp/ds_k,
+ * K dtProp/tau_k, ln(ds_k/ds_{k-1}), ln(ds_{k+1}/ds_k), v, K/nsteps;
+ * y [n][M]: ln f at k after K subcycles of the plain path, minus the same
+ * mean; valid[n] = 1 for rows that qualified.  f is unchanged. */
+int ep_surrogate_samples(int32_t K, int32_t n, uint64_t seed, float *x, float *y, int32_t *valid);
 
 /* ---------------- misc --------------------------------------------- */
 int ep_last_error(char *buf, size_t len);

@@ -308,3 +308,23 @@ def move_nodes(nodes, state, tdel, rscale, first=1):
             old[k][s, first:] = a
     nodes.update(pos)
     return old
+
+
+def shell_diffuse(p, nodes, mhd, f, nbr, grids, dt, kperxkpar):
+    """or_shell_diffuse on copies: nodes x,y,z,rmag [S,N], mhd br,btheta,bphi
+    [S,N], f [S,N,E,M], nbr [S,4]; returns (rc, f_out, bad[4])."""
+    S, N, E, M = f.shape
+    xs = [np.ascontiguousarray(nodes[k], dtype=np.float64) for k in ("x", "y", "z", "rmag")]
+    bs = [np.ascontiguousarray(mhd[k], dtype=np.float64) for k in ("br", "btheta", "bphi")]
+    fo = np.ascontiguousarray(f, dtype=np.float64).copy()
+    nb = np.ascontiguousarray(nbr, dtype=np.int32)
+    rig = np.ascontiguousarray(grids["rigidity"], dtype=np.float64)
+    vg = np.ascontiguousarray(grids["vgrid"], dtype=np.float64)
+    bad = np.full(4, -1, np.int32)
+    L = lib()
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    L.or_shell_diffuse.argtypes = [C.POINTER(EpParams), C.c_int, C.c_int, C.c_int, C.c_int, IP] + [DP] * 9 + \
+        [C.c_double, C.c_double, DP, IP]
+    rc = L.or_shell_diffuse(C.byref(p), S, N, E, M, iptr(nb), *[dptr(a) for a in xs], *[dptr(a) for a in bs],
+                            dptr(rig), dptr(vg), dt, kperxkpar, dptr(fo), iptr(bad))
+    return rc, fo, bad

@@ -1352,3 +1352,93 @@ int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads) {
   grid_free(&g);
   return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* Shell diffusion (perpendicular diffusion between neighbouring       */
+/* streams on one shell)                                               */
+/* ------------------------------------------------------------------ */
+#define OR_THRESH 0.025
+#define OR_VERYSMALL 1.0e-33
+
+/* ShellData (src/energeticParticles.c:247-413) and DiffuseShellData
+   (:418-555) for one EP substep of length dt.  Streams s in 0..S-1 in
+   (face,row,col) order, nodes k in 0..N-1 (node 0 is skipped, like global
+   node 0); nbr[S][4] = north, east, west, south neighbour streams.
+   Positions x,y,z,rmag [S][N] in rScale units, B (r, theta, phi) [S][N];
+   f [S][N][E][M] in/out.  Returns EP_OK or EP_ERR_NAN / EP_ERR_INF with the
+   first bad cell in the reference's loop order (shell, energy, stream, mu). */
+int or_shell_diffuse(const ep_params *p, int S, int N, int E, int M, const int32_t *nbr,
+                     const double *x, const double *y, const double *z, const double *rmag,
+                     const double *br, const double *bt, const double *bp, const double *rigidity,
+                     const double *vgrid, double dt, double kperxkpar, double *f, int *bad) {
+  double *dlper = (double *)malloc(sizeof(double) * 4 * S);
+  double *delta = (double *)malloc(sizeof(double) * S * M);
+  double *mfp = (double *)malloc(sizeof(double) * S);
+  int k, e, s, d, m, rc = EP_OK;
+  for (k = 1; k < N && rc == EP_OK; k++) {
+    /* link lengths perpendicular to B (ShellData) */
+    for (s = 0; s < S; s++) {
+      const size_t i = (size_t)s * N + k;
+      const double theta = acos(z[i] / rmag[i]);
+      const double phi = atan2(y[i], x[i]);
+      const double bx = br[i] * sin(theta) * cos(phi) + bt[i] * cos(theta) * cos(phi) - bp[i] * sin(phi);
+      const double by = br[i] * sin(theta) * sin(phi) + bt[i] * cos(theta) * sin(phi) + bp[i] * cos(phi);
+      const double bz = br[i] * cos(theta) - bt[i] * sin(theta);
+      const double bmag = sqrt(br[i] * br[i] + bt[i] * bt[i] + bp[i] * bp[i]);
+      const double ebx = bx / bmag, eby = by / bmag, ebz = bz / bmag;
+      const double rx = x[i] * p->rScale, ry = y[i] * p->rScale, rz = z[i] * p->rScale;
+      for (d = 0; d < 4; d++) {
+        const size_t j = (size_t)nbr[s * 4 + d] * N + k;
+        double ex = x[j] * p->rScale - rx, ey = y[j] * p->rScale - ry, ez = z[j] * p->rScale - rz;
+        const double dl = sqrt(ex * ex + ey * ey + ez * ez);
+        double dot;
+        ex /= dl;
+        ey /= dl;
+        ez /= dl;
+        dot = ex * ebx + ey * eby + ez * ebz;
+        dlper[s * 4 + d] = dl * sqrt(1.0 - dot * dot);
+      }
+    }
+    for (e = 0; e < E && rc == EP_OK; e++) {
+      for (s = 0; s < S; s++) {
+        const size_t i = (size_t)s * N + k;
+        const double bmag = sqrt(br[i] * br[i] + bt[i] * bt[i] + bp[i] * bp[i]);
+        mfp[s] = or_mfp(p, rigidity[e], rmag[i], bmag);
+      }
+      memset(delta, 0, sizeof(double) * S * M);
+      for (s = 0; s < S; s++) {
+        const double dt_kper = dt * 0.33333333 * mfp[s] * vgrid[e] * kperxkpar;
+        double del[4];
+        const double *fs = f + (((size_t)s * N + k) * E + e) * M;
+        for (d = 0; d < 4; d++) {
+          del[d] = dt_kper / (dlper[s * 4 + d] * dlper[s * 4 + d] + OR_VERYSMALL);
+          if (del[d] > OR_THRESH) del[d] = OR_THRESH;
+        }
+        for (m = 0; m < M; m++) {
+          for (d = 0; d < 4; d++) delta[nbr[s * 4 + d] * M + m] += del[d] * fs[m];
+          delta[s * M + m] -= (del[0] + del[1] + del[2] + del[3]) * fs[m];
+        }
+      }
+      for (s = 0; s < S && rc == EP_OK; s++) {
+        double *fs = f + (((size_t)s * N + k) * E + e) * M;
+        for (m = 0; m < M; m++) {
+          fs[m] += delta[s * M + m];
+          if (isnan(fs[m]) || isinf(fs[m])) {
+            rc = isnan(fs[m]) ? EP_ERR_NAN : EP_ERR_INF;
+            if (bad) {
+              bad[0] = s;
+              bad[1] = k;
+              bad[2] = e;
+              bad[3] = m;
+            }
+            break;
+          }
+        }
+      }
+    }
+  }
+  free(dlper);
+  free(delta);
+  free(mfp);
+  return rc;
+}

@@ -166,6 +166,16 @@ void or_move_nodes(int n, double tDel, double rScale, double *x, double *y, doub
                    double *rox, double *roy, double *roz, const double *vr, const double *vt,
                    const double *vp);
 
+/* ShellData + DiffuseShellData (src/energeticParticles.c:247-555) for one
+ * EP substep dt: f [S][N][E][M] in/out, neighbour streams nbr[S][4],
+ * positions [S][N] (rScale units), B (r, theta, phi) [S][N]; node 0 is
+ * skipped.  bad[4] (may be NULL) = stream, node, energy, mu of the first
+ * NaN/Inf in the reference's check order. */
+int or_shell_diffuse(const ep_params *p, int S, int N, int E, int M, const int32_t *nbr,
+                     const double *x, const double *y, const double *z, const double *rmag,
+                     const double *br, const double *bt, const double *bp, const double *rigidity,
+                     const double *vgrid, double dt, double kperxkpar, double *f, int *bad);
+
 /* Run the whole analytic-wind simulation.  Returns EP_OK or an error. */
 int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads);
 

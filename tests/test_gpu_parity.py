@@ -529,3 +529,38 @@ def test_fused_streaming_parity(case):
                                     tiling=case["tiling"], precision=2)
     assert_diag_equal(gd, od)
     assert max_rel(fg, fo) <= RTOL
+
+
+@pytest.mark.parametrize("rows,kp", [(2, 0.5), (3, 50.0)])
+def test_shell_diffusion_parity(rows, kp):
+    """DiffuseShellData on the device (gather in source order) against the
+    oracle's scatter; kp = 50 drives most links to the THRESH cap."""
+    from eprem_amd import constants as K, cube, mas_synth
+    rs = 0.00465047
+    p = params(rScale=rs, numEpSteps=2)
+    nodes = workloads.parker_nodes(rows, rows, 40, rscale=rs, seed=71, jitter=0.1)
+    S, N = nodes["x"].shape
+    E, M = 6, 5
+    prev = mas_synth.prev_from_nodes(nodes, rs)
+    kw = dict(mhdUs=400e5 / K.C, mhdNsAu=8.3, mhdBsAu=2.5e-5 / K.MHD_B_NORM, omegaSun=0.0014)
+    nb = cube.neighbours(rows, rows)
+    dims = workloads.dims_of(nodes, E, M)
+    g = oracle.build_grids(p, E, M)
+    f0 = workloads.seeded_f(p, g, nodes, noise=0.4, seed=12)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_positions(nodes)
+        out = sol.update_mhd(prev, mode=0, **kw)
+        sol.upload_f(f0)
+        sol.shell_diffusion(nb, kp)
+        rc, _ = sol.step(abi.EP_OP_SHELL)
+        fg = sol.download_f()
+    finally:
+        sol.close()
+    assert rc == 0
+    fo = f0.copy()
+    for _ in range(p.numEpSteps):
+        rc, fo, _ = oracle.shell_diffuse(p, nodes, out, fo, nb, g, p.tDel / p.numEpSteps, kp)
+        assert rc == 0
+    assert not np.array_equal(fo, f0)
+    assert max_rel(fg, fo) <= 1e-12

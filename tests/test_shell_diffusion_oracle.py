@@ -1,0 +1,79 @@
+"""Pins of the shell-diffusion oracle (DiffuseShellData): the cube-sphere
+links are reciprocal; the operator conserves the shell total; a single
+loaded stream sends exactly del = min(THRESH, dt k_perp / dl_perp^2) of its
+content along each link, with dl_perp from the geometry; and with zero
+perpendicular coefficient nothing moves."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from eprem_amd import config, constants as K, cube, mas_synth, workloads
+
+
+def _setup(rows=2, N=5, E=3, M=4, seed=1):
+    p = config.default_params(rScale=0.00465047, tDel=1.0 / 64, numEpSteps=1, eMin=0.01, eMax=1000.0,
+                              lamo=0.1, mfpRadialPower=1.0, boundaryFunctEcutoff=1e30)
+    nodes = workloads.parker_nodes(rows, rows, N, rscale=0.00465047, seed=seed, jitter=0.1)
+    S = nodes["x"].shape[0]
+    mhd = {"br": 1e-5 / nodes["rmag"] ** 2, "btheta": 1e-7 * np.ones((S, N)), "bphi": -3e-6 / nodes["rmag"]}
+    g = oracle.build_grids(p, E, M)
+    rng = np.random.default_rng(seed)
+    f = rng.uniform(0.5, 2.0, (S, N, E, M))
+    return p, nodes, mhd, f, g, cube.neighbours(rows, rows)
+
+
+@pytest.mark.parametrize("rows", [1, 2, 3, 8, 16])
+def test_links_are_reciprocal(rows):
+    nb = cube.neighbours(rows, rows)
+    for s in range(len(nb)):
+        assert len(set(nb[s])) == 4 and s not in nb[s]
+        for j in nb[s]:
+            assert s in nb[j]
+
+
+def test_conserves_shell_total():
+    p, nodes, mhd, f, g, nb = _setup()
+    rc, fo, _ = oracle.shell_diffuse(p, nodes, mhd, f, nb, g, 1e-3, 0.5)
+    assert rc == 0
+    assert not np.array_equal(fo[:, 1:], f[:, 1:])
+    np.testing.assert_allclose(fo.sum(axis=0), f.sum(axis=0), rtol=1e-13)
+    np.testing.assert_array_equal(fo[:, 0], f[:, 0])  # node 0 is not a compute shell
+
+
+def test_single_source_link_weights():
+    p, nodes, mhd, f, g, nb = _setup(rows=2, N=3, E=2, M=1)
+    k, e, s0 = 2, 1, 5
+    f = np.zeros_like(f)
+    f[s0, k, e, 0] = 1.0
+    dt, kp = 2e-3, 0.3
+    rc, fo, _ = oracle.shell_diffuse(p, nodes, mhd, f, nb, g, dt, kp)
+    assert rc == 0
+    # independent geometry: B direction from (Br, Btheta, Bphi) at the node
+    x, y, z, rm = (nodes[q][s0, k] for q in ("x", "y", "z", "rmag"))
+    th, ph = math.acos(z / rm), math.atan2(y, x)
+    br, bt, bp = (mhd[q][s0, k] for q in ("br", "btheta", "bphi"))
+    B = np.array([br * math.sin(th) * math.cos(ph) + bt * math.cos(th) * math.cos(ph) - bp * math.sin(ph),
+                  br * math.sin(th) * math.sin(ph) + bt * math.cos(th) * math.sin(ph) + bp * math.cos(ph),
+                  br * math.cos(th) - bt * math.sin(th)])
+    eb = B / np.linalg.norm(B)
+    lam = oracle.mfp(p, g["rigidity"][e], rm, np.linalg.norm(B))
+    kperp = dt * 0.33333333 * lam * g["vgrid"][e] * kp
+    r0 = np.array([x, y, z]) * p.rScale
+    total = 0.0
+    for j in nb[s0]:
+        d = np.array([nodes[q][j, k] for q in ("x", "y", "z")]) * p.rScale - r0
+        dl = np.linalg.norm(d)
+        dlp = dl * math.sqrt(1.0 - (d / dl @ eb) ** 2)
+        want = min(0.025, kperp / (dlp * dlp + 1e-33))
+        assert fo[j, k, e, 0] == pytest.approx(want, rel=1e-12)
+        total += want
+    assert fo[s0, k, e, 0] == pytest.approx(1.0 - total, rel=1e-12)
+
+
+def test_zero_coefficient_is_identity():
+    p, nodes, mhd, f, g, nb = _setup()
+    rc, fo, _ = oracle.shell_diffuse(p, nodes, mhd, f, nb, g, 1e-3, 0.0)
+    assert rc == 0
+    np.testing.assert_array_equal(fo, f)
