@@ -36,7 +36,8 @@ EP_OP_FOCUS = 0x2
 EP_OP_CHANGE = 0x4
 EP_OP_SEEDCHECK = 0x8
 EP_OP_SHELL = 0x10
-EP_OP_ALL = 0x1F
+EP_OP_DRIFT = 0x20
+EP_OP_ALL = 0x3F
 
 
 class EpDims(C.Structure):
@@ -221,5 +222,5 @@ EXPORTED_SYMBOLS = (
     "ep_mlp_finalize",
     "ep_mlp_bench",
     "ep_surrogate_samples",
-    "ep_shell_diffusion",
+    "ep_shell_operators",
 )

@@ -80,6 +80,7 @@ static void build_grids(const ep_params &p, int E, int M) {
   g.vgrid.assign(E, 0);
   g.vgrid_i.assign(E, 0);
   g.rigidity.assign(E, 0);
+  g.pgrid.assign(E, 0);
   g.rig_lamo.assign(E, 0);
   g.plt.assign(E, 0);
   g.ext.assign(E, 0);
@@ -91,6 +92,7 @@ static void build_grids(const ep_params &p, int E, int M) {
   g.dlnp = (lnpmax - lnpmin) / (E - 1.0);
   for (int e = 0; e < E; e++) {
     const double pp = exp(lnpmin + e * g.dlnp);
+    g.pgrid[e] = pp;
     g.egrid[e] = sqrt(one + pp * pp) - one;
     g.vgrid[e] = pp / sqrt(one + pp * pp);
     g.vgrid_i[e] = sqrt(one + pp * pp) / pp;
@@ -255,7 +257,7 @@ static int decode_err(unsigned long long key, ep_diag *dg) {
     dg->err_node = a;
     dg->err_energy = b;
     dg->err_mu = c;
-  } else {  // shell diffusion: [node][energy | stream][mu] in the reference's order
+  } else {  // shell diffusion (2) and drift (3): [node][energy | stream][mu], the reference's order
     code = inf ? EP_ERR_INF : EP_ERR_NAN;
     dg->err_node = s;
     dg->err_energy = a >> 7;
@@ -309,7 +311,9 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
     if (sh.do_focus) EPQ(launch_focus(D, sub, st));
     if (e4) EPQ(hipEventRecord(e4[1], st));
     if (sh.do_change) EPQ(launch_change(D, sub, st));
-    if (sh.do_shell) EPQ(launch_shell_diffuse(D, sub, st));  // DiffuseShellData, after the stream loop
+    // DiffuseShellData, then DriftShellData, after the stream loop
+    if (sh.do_shell & 1) EPQ(launch_shell_diffuse(D, sub, st));
+    if (sh.do_shell & 2) EPQ(launch_shell_drift(D, sub, st));
     if (e4) EPQ(hipEventRecord(e4[2], st));
   }
 #undef EPQ
@@ -598,16 +602,19 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
     if (out) *out = dg;
     return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
   }
-  const int do_shell = (g.kperxkpar > 0.0) && (op_flags & EP_OP_SHELL) ? 1 : 0;
-  if (do_shell) {
+  const int do_shell = ((g.kperxkpar > 0.0) && (op_flags & EP_OP_SHELL)) ? 1 : 0;
+  const int do_drift = (g.shell_drift && (op_flags & EP_OP_DRIFT)) ? 1 : 0;
+  if (do_shell || do_drift) {
     // once per global step: node positions and B are fixed inside it
-    if (!g.mhd_state) return fail(EP_ERR_INVALID_ARG, "shell diffusion needs the device MHD state (ep_update_mhd)");
+    if (!g.mhd_state) return fail(EP_ERR_INVALID_ARG, "shell operators need the device MHD state (ep_update_mhd)");
+    if (do_drift && g.mhd_mode != 0)
+      return fail(EP_ERR_UNSUPPORTED, "drift needs the analytic-wind field (ep_update_mhd mode 0)");
     for (int32_t n : g.h_nactive)
-      if (n != g.d.num_nodes) return fail(EP_ERR_UNSUPPORTED, "shell diffusion needs every stream full");
+      if (n != g.d.num_nodes) return fail(EP_ERR_UNSUPPORTED, "shell operators need every stream full");
     HIPCHK(launch_shell_links(D, st), "shell links");
   }
-  const StepShape shape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed, do_shell,
-                        g.generation};
+  const StepShape shape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
+                        do_shell | (do_drift << 1), g.generation};
   if (g.use_graph) {
     // The substep loop is one HIP graph, rebuilt only when its shape (pass
     // count, grid sizes, operators) or the parameters change.

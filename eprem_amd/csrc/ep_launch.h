@@ -21,6 +21,7 @@ hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st);
 hipError_t launch_flip_all(const Dev &D, hipStream_t st);
 hipError_t launch_shell_links(const Dev &D, hipStream_t st);
 hipError_t launch_shell_diffuse(const Dev &D, int sub, hipStream_t st);
+hipError_t launch_shell_drift(const Dev &D, int sub, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_init_seed(const Dev &D, hipStream_t st);
 hipError_t launch_dpp_probe(const double *in, double *out, double *out2, hipStream_t st);

@@ -404,6 +404,9 @@ int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *
   if (device_ms) *device_ms = ms;
   g.nodes_set = true;
   g.mhd_state = true;
+  g.mhd_mode = mp->mode;
+  g.mhd_bsau = mp->mhdBsAu;
+  g.mhd_omega = mp->omegaSun;
   if (out) {
     double *dst[7] = {out->br, out->btheta, out->bphi, out->vr, out->vtheta, out->vphi, out->density};
     for (int i = 0; i < 7; i++) {

@@ -49,7 +49,7 @@ struct State {
     generation++;
   }
   // host copies of the grids
-  std::vector<double> egrid, vgrid, vgrid_i, rigidity, rig_lamo, plt, ext, mugrid;
+  std::vector<double> egrid, vgrid, vgrid_i, rigidity, rig_lamo, plt, ext, mugrid, pgrid;
   double dlnp = 0, dmu = 0, normJ0 = 0, normRadius = 0;
   int mpos = 0;
   std::vector<int32_t> h_nactive;
@@ -78,10 +78,14 @@ struct State {
   std::vector<void *> mas_allocs;
   double *mas_out = nullptr;   // 7 x S x Npad
   double *mas_prev = nullptr;  // 10 x S x Npad
-  // shell diffusion (off while kperxkpar == 0)
+  // cross-stream operators (diffusion off while kperxkpar == 0)
   double kperxkpar = 0.0;
+  int shell_drift = 0;
   int32_t *shell_nbr = nullptr, *shell_src = nullptr, *shell_dir = nullptr;
-  double *shell_dlper = nullptr;
+  double *shell_dlper = nullptr, *shell_dl = nullptr, *shell_curl = nullptr, *shell_factor = nullptr;
+  // analytic field of the last ep_update_mhd (mode 0), for drift
+  int mhd_mode = -1;
+  double mhd_bsau = 0.0, mhd_omega = 0.0;
   // the MHD buffer's state half holds the latest B, V, density per node
   // once ep_update_mhd has run (ep_move_nodes reads V from it)
   bool mhd_state = false;
@@ -99,10 +103,14 @@ struct State {
     int32_t *ip[3] = {shell_nbr, shell_src, shell_dir};
     for (int32_t *q : ip)
       if (q) (void)hipFree(q);
-    if (shell_dlper) (void)hipFree(shell_dlper);
+    double *dp[4] = {shell_dlper, shell_dl, shell_curl, shell_factor};
+    for (double *q : dp)
+      if (q) (void)hipFree(q);
     shell_nbr = shell_src = shell_dir = nullptr;
-    shell_dlper = nullptr;
+    shell_dlper = shell_dl = shell_curl = shell_factor = nullptr;
     kperxkpar = 0.0;
+    shell_drift = 0;
+    mhd_mode = -1;
     obs_cap = flux_cap = spawn_cap = 0;
     mhd_state = false;
     mas[0] = MasLevel();

@@ -74,7 +74,7 @@ def lib():
         L.ep_mlp_finalize.argtypes = []
         L.ep_mlp_bench.argtypes = [C.c_int32, C.c_int32, DP]
         L.ep_surrogate_samples.argtypes = [C.c_int32, C.c_int32, C.c_uint64, FP, FP, IP]
-        L.ep_shell_diffusion.argtypes = [IP, C.c_double]
+        L.ep_shell_operators.argtypes = [IP, C.c_double, C.c_int32]
         for name in abi.EXPORTED_SYMBOLS:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -268,11 +268,11 @@ class EpSolver:
         _check(lib().ep_stream_flux(dptr(flux)))
         return flux
 
-    def shell_diffusion(self, nbr, kperxkpar):
-        """Enable perpendicular diffusion between neighbouring streams
-        (nbr [S, 4] from eprem_amd.cube.neighbours); kperxkpar = 0 disables."""
+    def shell_operators(self, nbr, kperxkpar=0.0, drift=False):
+        """Perpendicular diffusion (kperxkpar > 0) and drift between
+        neighbouring streams (nbr [S, 4] from eprem_amd.cube.neighbours)."""
         nb = None if nbr is None else np.ascontiguousarray(nbr, dtype=np.int32)
-        _check(lib().ep_shell_diffusion(iptr(nb), float(kperxkpar)))
+        _check(lib().ep_shell_operators(iptr(nb), float(kperxkpar), 1 if drift else 0))
 
     def surrogate_samples(self, K, n, seed=1):
         """Training pairs (x [k, 3M+6], y [k, M]) of K streaming subcycles on

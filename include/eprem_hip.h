@@ -76,9 +76,9 @@ extern "C" {
 #define EP_OP_FOCUS       0x2u   /* AdiabaticFocusing                   */
 #define EP_OP_CHANGE      0x4u   /* AdiabaticChange                     */
 #define EP_OP_SEEDCHECK   0x8u   /* seedPopulationCheck                 */
-#define EP_OP_SHELL       0x10u  /* DiffuseShellData (when enabled by
-                                    ep_shell_diffusion)                 */
-#define EP_OP_ALL         0x1Fu
+#define EP_OP_SHELL       0x10u  /* DiffuseShellData  (when enabled by */
+#define EP_OP_DRIFT       0x20u  /* DriftShellData     ep_shell_operators) */
+#define EP_OP_ALL         0x3Fu
 
 /* ------------------------------------------------------------------ */
 /* Dimensions and parameters                                           */
@@ -302,15 +302,17 @@ int ep_observe(const ep_observers *obs, double *dist, double *flux);
  * flux [num_streams][num_nodes][num_energies], code units. */
 int ep_stream_flux(double *flux);
 
-/* ---------------- shell diffusion ----------------------------------- */
-/* DiffuseShellData with the link lengths of ShellData
- * (src/energeticParticles.c:247-555): perpendicular diffusion between the
- * streams of each shell, every EP substep after focusing and change.
- * kperxkpar > 0 enables it (config.kperxkpar; 0 disables), nbr[S][4] =
- * north, east, west, south neighbour stream of every stream (the cube
- * surface links of initNEWS, src/cubeShellInit.c:95-520).  Needs the
- * device MHD state (ep_update_mhd) for B and every stream full. */
-int ep_shell_diffusion(const int32_t *nbr, double kperxkpar);
+/* ---------------- cross-stream operators --------------------------- */
+/* DiffuseShellData with the link lengths of ShellData and DriftShellData
+ * with driftVelocity/curlBoverB2 (src/energeticParticles.c:208-801,
+ * src/flow.c:479-615): perpendicular diffusion (kperxkpar > 0, config.
+ * kperxkpar; 0 = off) and gradient/curvature drift (use_drift != 0) between
+ * the streams of each shell, every EP substep after focusing and change.
+ * nbr[S][4] = north, east, west, south neighbour stream of every stream
+ * (the cube surface links of initNEWS, src/cubeShellInit.c:95-520).  Both
+ * need every stream full and the device MHD state (ep_update_mhd); drift
+ * uses the analytic-wind field (mode 0), like the reference's curlBoverB2. */
+int ep_shell_operators(const int32_t *nbr, double kperxkpar, int32_t use_drift);
 
 /* ---------------- learned surrogate: multilayer perceptron --------- */
 /* No counterpart in the reference.  A fully connected network (ReLU on

@@ -328,3 +328,32 @@ def shell_diffuse(p, nodes, mhd, f, nbr, grids, dt, kperxkpar):
     rc = L.or_shell_diffuse(C.byref(p), S, N, E, M, iptr(nb), *[dptr(a) for a in xs], *[dptr(a) for a in bs],
                             dptr(rig), dptr(vg), dt, kperxkpar, dptr(fo), iptr(bad))
     return rc, fo, bad
+
+
+def curl_b_over_b2(rscale, mhdBsAu, omega, x, y, z, vr):
+    L = lib()
+    L.or_curl_b_over_b2.argtypes = [C.c_double] * 7 + [C.POINTER(C.c_double)]
+    L.or_curl_b_over_b2.restype = None
+    out = (C.c_double * 3)()
+    L.or_curl_b_over_b2(rscale, mhdBsAu, omega, x, y, z, vr, out)
+    return np.array(out[:])
+
+
+def shell_drift(p, nodes, curl, f, nbr, grids, dt):
+    """or_shell_drift on copies: curl [S,N,3], f [S,N,E,M]; returns
+    (rc, f_out, bad[4])."""
+    S, N, E, M = f.shape
+    xs = [np.ascontiguousarray(nodes[k], dtype=np.float64) for k in ("x", "y", "z")]
+    cu = np.ascontiguousarray(curl, dtype=np.float64)
+    fo = np.ascontiguousarray(f, dtype=np.float64).copy()
+    nb = np.ascontiguousarray(nbr, dtype=np.int32)
+    pg = np.ascontiguousarray(grids["pgrid"], dtype=np.float64)
+    vg = np.ascontiguousarray(grids["vgrid"], dtype=np.float64)
+    bad = np.full(4, -1, np.int32)
+    L = lib()
+    DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    L.or_shell_drift.argtypes = [C.POINTER(EpParams), C.c_int, C.c_int, C.c_int, C.c_int, IP] + [DP] * 6 + \
+        [C.c_double, DP, IP]
+    rc = L.or_shell_drift(C.byref(p), S, N, E, M, iptr(nb), *[dptr(a) for a in xs], dptr(cu), dptr(pg), dptr(vg),
+                          dt, dptr(fo), iptr(bad))
+    return rc, fo, bad

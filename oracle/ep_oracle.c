@@ -1442,3 +1442,154 @@ int or_shell_diffuse(const ep_params *p, int S, int N, int E, int M, const int32
   free(mfp);
   return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* Drift between neighbouring streams                                  */
+/* ------------------------------------------------------------------ */
+#define OR_Q 4.80320425e-10
+#define OR_OM (OR_MHD_B_NORM * OR_Q * OR_AU / (OR_MP * OR_C * OR_C))
+
+/* parkerB (src/flow.c:440-470) without the ideal shock */
+static void or_parker_b(double r, double theta, double vr, double mhdBsAu, double omega, double *b) {
+  b[0] = mhdBsAu / (r * r);
+  b[1] = 0.0;
+  b[2] = -1.0 * r * b[0] * (omega / (vr + OR_VERYSMALL)) * sin(theta);
+}
+
+/* curlBoverB2 (src/flow.c:479-615): curl(B/B^2) of the Parker field by
+   centred differences, cartesian position in rScale units. */
+void or_curl_b_over_b2(double rScale, double mhdBsAu, double omega, double x, double y, double z, double vr,
+                       double *curl) {
+  const double rw = 0.001, tw = OR_PI / 100.0, pw = OR_PI / 100.0;
+  double r, th, ph, A, B;
+  double rm[3], rp[3], tm[3], tp[3], pm[3], pp[3];
+  double Brm[3], Brp[3], Btm[3], Btp[3], Bpm[3], Bpp[3];
+  double mrm, mrp, mtm, mtp, mpm, mpp;
+  r = rScale * sqrt(x * x + y * y + z * z);
+  th = acos(z * rScale / r);
+  ph = atan2(y, x);
+  rp[0] = r + rw; rp[1] = th; rp[2] = ph;
+  rm[0] = r - rw; rm[1] = th; rm[2] = ph;
+  tp[0] = r; tp[1] = th + tw; tp[2] = ph;
+  if (tp[1] > OR_PI) {
+    tp[1] = 2.0 * OR_PI - tp[1];
+    tp[2] += OR_PI;
+    if (tp[2] > (2.0 * OR_PI)) tp[2] -= (2.0 * OR_PI);
+  }
+  tm[0] = r; tm[1] = th - tw; tm[2] = ph;
+  if (tm[1] < 0.0) {
+    tm[1] *= -1.0;
+    tm[2] += OR_PI;
+    if (tm[2] > (2.0 * OR_PI)) tm[2] -= (2.0 * OR_PI);
+  }
+  pp[0] = r; pp[1] = th; pp[2] = ph;
+  pp[2] += pw;
+  if (pp[2] > (2.0 * OR_PI)) pp[2] -= (2.0 * OR_PI);
+  pm[0] = r; pm[1] = th; pm[2] = ph;
+  pm[2] -= pw;
+  if (pm[2] < 0.0) pm[2] += (2.0 * OR_PI);
+  or_parker_b(rm[0], rm[1], vr, mhdBsAu, omega, Brm);
+  or_parker_b(rp[0], rp[1], vr, mhdBsAu, omega, Brp);
+  or_parker_b(tm[0], tm[1], vr, mhdBsAu, omega, Btm);
+  or_parker_b(tp[0], tp[1], vr, mhdBsAu, omega, Btp);
+  or_parker_b(pm[0], pm[1], vr, mhdBsAu, omega, Bpm);
+  or_parker_b(pp[0], pp[1], vr, mhdBsAu, omega, Bpp);
+  mrm = sqrt(Brm[0] * Brm[0] + Brm[1] * Brm[1] + Brm[2] * Brm[2]);
+  mrp = sqrt(Brp[0] * Brp[0] + Brp[1] * Brp[1] + Brp[2] * Brp[2]);
+  mtm = sqrt(Btm[0] * Btm[0] + Btm[1] * Btm[1] + Btm[2] * Btm[2]);
+  mtp = sqrt(Btp[0] * Btp[0] + Btp[1] * Btp[1] + Btp[2] * Btp[2]);
+  mpm = sqrt(Bpm[0] * Bpm[0] + Bpm[1] * Bpm[1] + Bpm[2] * Bpm[2]);
+  mpp = sqrt(Bpp[0] * Bpp[0] + Bpp[1] * Bpp[1] + Bpp[2] * Bpp[2]);
+  A = (0.5 / tw) * (Btp[2] * sin(tp[1]) / (mtp * mtp) - Btm[2] * sin(tm[1]) / (mtm * mtm));
+  B = (0.5 / pw) * (Bpp[1] / (mpp * mpp) - Bpm[1] / (mpm * mpm));
+  curl[0] = (1.0 / (r * sin(th))) * (A - B);
+  A = (1.0 / sin(th)) * (0.5 / pw) * (Bpp[0] / (mpp * mpp) - Bpm[0] / (mpm * mpm));
+  B = (0.5 / rw) * (rp[0] * Brp[2] / (mrp * mrp) - rm[0] * Brm[2] / (mrm * mrm));
+  curl[1] = (1.0 / r) * (A - B);
+  A = (0.5 / rw) * (rp[0] * Brp[1] / (mrp * mrp) - rm[0] * Brm[1] / (mrm * mrm));
+  B = (0.5 / tw) * (Btp[0] / (mtp * mtp) - Btm[0] / (mtm * mtm));
+  curl[2] = (1.0 / r) * (A - B);
+}
+
+/* sphToCartVector (src/geometry.c:89-125) */
+static void or_sph_to_cart(const double *v, double x, double y, double z, double *out) {
+  const double rmag = sqrt(x * x + y * y + z * z);
+  if ((rmag - fabs(z)) < OR_VERYSMALL) {
+    out[0] = 0.0;
+    out[1] = 0.0;
+  } else {
+    out[0] = x / rmag * v[0] + x * z / (rmag * sqrt(rmag * rmag - z * z)) * v[1] - y / sqrt(rmag * rmag - z * z) * v[2];
+    out[1] = y / rmag * v[0] + y * z / (rmag * sqrt(rmag * rmag - z * z)) * v[1] + x / sqrt(rmag * rmag - z * z) * v[2];
+  }
+  out[2] = z / rmag * v[0] - sqrt(rmag * rmag - z * z) / rmag * v[1];
+}
+
+/* DriftShellData (src/energeticParticles.c:560-801) with driftVelocity
+   (:208-245) for one EP substep dt; curl [S][N][3] = curlBoverB2 per node,
+   pgrid[E] = momentum (exp(lnpmin + e dlnp)); other arguments as
+   or_shell_diffuse. */
+int or_shell_drift(const ep_params *p, int S, int N, int E, int M, const int32_t *nbr, const double *x,
+                   const double *y, const double *z, const double *curl, const double *pgrid,
+                   const double *vgrid, double dt, double *f, int *bad) {
+  double *delta = (double *)malloc(sizeof(double) * S * M);
+  double *dl = (double *)malloc(sizeof(double) * 4 * S);
+  const double om1 = p->charge * OR_OM / p->mass;
+  int k, e, s, d, m, rc = EP_OK;
+  for (k = 1; k < N && rc == EP_OK; k++) {
+    for (s = 0; s < S; s++) { /* ShellData link lengths (AU) */
+      const size_t i = (size_t)s * N + k;
+      for (d = 0; d < 4; d++) {
+        const size_t j = (size_t)nbr[s * 4 + d] * N + k;
+        const double ex = x[j] * p->rScale - x[i] * p->rScale, ey = y[j] * p->rScale - y[i] * p->rScale,
+                     ez = z[j] * p->rScale - z[i] * p->rScale;
+        dl[s * 4 + d] = sqrt(ex * ex + ey * ey + ez * ez);
+      }
+    }
+    for (e = 0; e < E && rc == EP_OK; e++) {
+      const double factor = (1.0 / om1) * (pgrid[e] * vgrid[e] / 3.0);
+      memset(delta, 0, sizeof(double) * S * M);
+      for (s = 0; s < S; s++) {
+        const size_t i = (size_t)s * N + k;
+        const double *fs = f + ((size_t)i * E + e) * M;
+        double vds[3], vd[3];
+        vds[0] = factor * curl[i * 3 + 0];
+        vds[1] = factor * curl[i * 3 + 1];
+        vds[2] = factor * curl[i * 3 + 2];
+        or_sph_to_cart(vds, x[i], y[i], z[i], vd);
+        for (d = 0; d < 4; d++) {
+          const int t = nbr[s * 4 + d];
+          const size_t j = (size_t)t * N + k;
+          const double ex = x[j] - x[i], ey = y[j] - y[i], ez = z[j] - z[i];
+          const double vdp = (vd[0] * ex + vd[1] * ey + vd[2] * ez) * p->rScale / dl[s * 4 + d];
+          if (vdp > 0.0) {
+            double del = vdp * dt / dl[s * 4 + d];
+            if (del > OR_THRESH) del = OR_THRESH;
+            for (m = 0; m < M; m++) {
+              delta[t * M + m] += del * fs[m];
+              delta[s * M + m] -= del * fs[m];
+            }
+          }
+        }
+      }
+      for (s = 0; s < S && rc == EP_OK; s++) {
+        double *fs = f + (((size_t)s * N + k) * E + e) * M;
+        for (m = 0; m < M; m++) {
+          fs[m] += delta[s * M + m];
+          if (isnan(fs[m]) || isinf(fs[m])) {
+            rc = isnan(fs[m]) ? EP_ERR_NAN : EP_ERR_INF;
+            if (bad) {
+              bad[0] = s;
+              bad[1] = k;
+              bad[2] = e;
+              bad[3] = m;
+            }
+            break;
+          }
+        }
+      }
+    }
+  }
+  free(delta);
+  free(dl);
+  return rc;
+}

@@ -176,6 +176,19 @@ int or_shell_diffuse(const ep_params *p, int S, int N, int E, int M, const int32
                      const double *br, const double *bt, const double *bp, const double *rigidity,
                      const double *vgrid, double dt, double kperxkpar, double *f, int *bad);
 
+/* curlBoverB2 (src/flow.c:479-615): curl(B/B^2) of the analytic (Parker)
+ * field by centred differences at a cartesian node position (rScale
+ * units) with radial flow speed vr; curl[3] = (r, theta, phi). */
+void or_curl_b_over_b2(double rScale, double mhdBsAu, double omega, double x, double y, double z, double vr,
+                       double *curl);
+
+/* DriftShellData (src/energeticParticles.c:560-801) for one EP substep dt:
+ * curl [S][N][3] per node, pgrid[E] momentum per energy; otherwise as
+ * or_shell_diffuse. */
+int or_shell_drift(const ep_params *p, int S, int N, int E, int M, const int32_t *nbr, const double *x,
+                   const double *y, const double *z, const double *curl, const double *pgrid,
+                   const double *vgrid, double dt, double *f, int *bad);
+
 /* Run the whole analytic-wind simulation.  Returns EP_OK or an error. */
 int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads);
 

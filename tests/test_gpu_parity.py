@@ -552,7 +552,7 @@ def test_shell_diffusion_parity(rows, kp):
         sol.set_positions(nodes)
         out = sol.update_mhd(prev, mode=0, **kw)
         sol.upload_f(f0)
-        sol.shell_diffusion(nb, kp)
+        sol.shell_operators(nb, kp)
         rc, _ = sol.step(abi.EP_OP_SHELL)
         fg = sol.download_f()
     finally:
@@ -563,4 +563,59 @@ def test_shell_diffusion_parity(rows, kp):
         rc, fo, _ = oracle.shell_diffuse(p, nodes, out, fo, nb, g, p.tDel / p.numEpSteps, kp)
         assert rc == 0
     assert not np.array_equal(fo, f0)
+    assert max_rel(fg, fo) <= 1e-12
+
+
+def _drift_oracle(p, nodes, out, kw, f0, nb, g, kp, drift):
+    """Oracle reference for the cross-stream operators: per substep
+    DiffuseShellData (kp > 0), then DriftShellData with curlBoverB2 of the
+    analytic field at every node."""
+    S, N = nodes["x"].shape
+    curl = np.zeros((S, N, 3))
+    if drift:
+        for s in range(S):
+            for k in range(1, N):
+                curl[s, k] = oracle.curl_b_over_b2(p.rScale, kw["mhdBsAu"], kw["omegaSun"], nodes["x"][s, k],
+                                                   nodes["y"][s, k], nodes["z"][s, k], out["vr"][s, k])
+    fo = f0.copy()
+    dt = p.tDel / p.numEpSteps
+    for _ in range(p.numEpSteps):
+        if kp > 0:
+            rc, fo, _ = oracle.shell_diffuse(p, nodes, out, fo, nb, g, dt, kp)
+            assert rc == 0
+        if drift:
+            rc, fo, _ = oracle.shell_drift(p, nodes, curl, fo, nb, g, dt)
+            assert rc == 0
+    return fo
+
+
+@pytest.mark.parametrize("kp,charge", [(0.0, 1.0), (0.5, 1.0), (0.0, -1.0)])
+def test_shell_drift_parity(kp, charge):
+    """DriftShellData (gradient/curvature drift from curlBoverB2 of the
+    analytic Parker field) on the device, alone and after diffusion,
+    against the oracle's scatter; charge -1 reverses the drift."""
+    from eprem_amd import constants as K, cube, mas_synth
+    rs = 0.00465047
+    p = params(rScale=rs, numEpSteps=2, charge=[charge])
+    nodes = workloads.parker_nodes(3, 3, 40, rscale=rs, seed=73, jitter=0.1)
+    E, M = 6, 5
+    prev = mas_synth.prev_from_nodes(nodes, rs)
+    kw = dict(mhdUs=400e5 / K.C, mhdNsAu=8.3, mhdBsAu=2.5e-5 / K.MHD_B_NORM, omegaSun=0.0014)
+    nb = cube.neighbours(3, 3)
+    dims = workloads.dims_of(nodes, E, M)
+    g = oracle.build_grids(p, E, M)
+    f0 = workloads.seeded_f(p, g, nodes, noise=0.4, seed=13)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_positions(nodes)
+        out = sol.update_mhd(prev, mode=0, **kw)
+        sol.upload_f(f0)
+        sol.shell_operators(nb, kp, drift=True)
+        rc, _ = sol.step(abi.EP_OP_SHELL | abi.EP_OP_DRIFT)
+        fg = sol.download_f()
+    finally:
+        sol.close()
+    assert rc == 0
+    fo = _drift_oracle(p, nodes, out, kw, f0, nb, g, kp, True)
+    assert max_rel(fo, f0) > 1e-6
     assert max_rel(fg, fo) <= 1e-12

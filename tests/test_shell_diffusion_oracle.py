@@ -77,3 +77,52 @@ def test_zero_coefficient_is_identity():
     rc, fo, _ = oracle.shell_diffuse(p, nodes, mhd, f, nb, g, 1e-3, 0.0)
     assert rc == 0
     np.testing.assert_array_equal(fo, f)
+
+
+def _parker_F(r, th, A, om, V):
+    br = A / r ** 2
+    bp = -1.0 * r * br * (om / (V + 1e-33)) * math.sin(th)
+    b2 = br * br + bp * bp
+    return br / b2, 0.0, bp / b2
+
+
+def test_curl_of_b_over_b2_matches_analytic():
+    """curl(B/B^2) of the Parker field: the oracle's centred differences
+    against the spherical curl evaluated with fine differences
+    (axisymmetric field: curl_r = d_theta(sin th F_phi)/(r sin th),
+    curl_th = -d_r(r F_phi)/r, curl_phi = -d_theta(F_r)/r)."""
+    rs, A, om, V = 0.00465047, 1e-5, 0.0014, 0.0013
+    for (x, y, z) in ((100.0, 50.0, 30.0), (-20.0, 80.0, -60.0), (5.0, -3.0, 200.0)):
+        c = oracle.curl_b_over_b2(rs, A, om, x, y, z, V)
+        r = rs * math.sqrt(x * x + y * y + z * z)
+        th = math.acos(z * rs / r)
+        h = 1e-6
+        sF = lambda t: math.sin(t) * _parker_F(r, t, A, om, V)[2]
+        cr = (sF(th + h) - sF(th - h)) / (2 * h) / (r * math.sin(th))
+        rF = lambda rr: rr * _parker_F(rr, th, A, om, V)[2]
+        ct = -((rF(r * (1 + h)) - rF(r * (1 - h))) / (2 * h * r)) / r
+        Fr = lambda t: _parker_F(r, t, A, om, V)[0]
+        cp = -((Fr(th + h) - Fr(th - h)) / (2 * h)) / r
+        want = np.array([cr, ct, cp])
+        # the reference's stencil (0.001 AU, pi/100) carries an O(h^2)
+        # truncation of a few 1e-3; a wrong sign or a dropped term is O(1)
+        np.testing.assert_allclose(c, want, rtol=1e-2, atol=1e-6 * np.abs(want).max())
+
+
+def test_drift_conserves_and_moves():
+    p, nodes, mhd, f, g, nb = _setup(rows=3, N=4, E=3, M=2)
+    S, N = nodes["x"].shape
+    curl = np.zeros((S, N, 3))
+    rc, fo, _ = oracle.shell_drift(p, nodes, curl, f, nb, g, 1e-3)
+    assert rc == 0
+    np.testing.assert_array_equal(fo, f)  # no curl: no drift
+    for s in range(S):
+        for k in range(N):
+            curl[s, k] = oracle.curl_b_over_b2(p.rScale, 1e-5, 0.0014, nodes["x"][s, k], nodes["y"][s, k],
+                                               nodes["z"][s, k], 0.0013)
+    rc, fo, _ = oracle.shell_drift(p, nodes, curl, f, nb, g, 3e-3)
+    assert rc == 0
+    assert not np.array_equal(fo[:, 1:], f[:, 1:])
+    np.testing.assert_array_equal(fo[:, 0], f[:, 0])
+    np.testing.assert_allclose(fo.sum(axis=0), f.sum(axis=0), rtol=1e-13)
+    assert (fo >= 0).all()
