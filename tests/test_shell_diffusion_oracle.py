@@ -126,3 +126,52 @@ def test_drift_conserves_and_moves():
     np.testing.assert_array_equal(fo[:, 0], f[:, 0])
     np.testing.assert_allclose(fo.sum(axis=0), f.sum(axis=0), rtol=1e-13)
     assert (fo >= 0).all()
+
+
+@pytest.mark.parametrize("charge", [1.0, -2.0])
+def test_single_source_drift_velocity(charge):
+    """One loaded stream drifts exactly del = min(THRESH, vdp dt / dl) to
+    each neighbour its drift velocity points to, where v_d is the
+    gradient/curvature drift p v c / (3 q) curl(B/B^2) of a particle of
+    charge q = charge e (Gaussian units, converted here to code units:
+    velocity in c, momentum in m_p c, B in MHD_B_NORM gauss, length in AU)
+    and vdp its projection on the link, taken with the spherical unit
+    vectors at the node."""
+    p, nodes, mhd, f, g, nb = _setup(rows=2, N=3, E=2, M=1)
+    p.charge = charge
+    S, N = nodes["x"].shape
+    k, e, s0 = 2, 1, 5
+    f = np.zeros_like(f)
+    f[s0, k, e, 0] = 1.0
+    x, y, z, rm = (nodes[q][s0, k] for q in ("x", "y", "z", "rmag"))
+    th, ph = math.acos(z / rm), math.atan2(y, x)
+    er = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    et = np.array([math.cos(th) * math.cos(ph), math.cos(th) * math.sin(ph), -math.sin(th)])
+    ep = np.array([-math.sin(ph), math.cos(ph), 0.0])
+    q_e, m_p, c, au = 4.80320425e-10, 1.6726e-24, 2.99792458e10, 1.495978707e13
+    b_norm = math.sqrt(m_p) * c
+    # v_d / c = (p_code m_p c)(v_code c) c / (3 q) * curl_code / (b_norm au) / c
+    factor = g["pgrid"][e] * g["vgrid"][e] * m_p * c * c / (3.0 * charge * q_e * b_norm * au)
+    dt = 2e-3
+    r0 = np.array([x, y, z])
+    links = [np.array([nodes[q][j, k] for q in ("x", "y", "z")]) - r0 for j in nb[s0]]
+    for c_sph in ((0.0, 0.0, 1.0), (0.3, -1.0, 0.2), (1.0, 0.5, -0.7)):
+        v = factor * (c_sph[0] * er + c_sph[1] * et + c_sph[2] * ep)
+        dls = [np.linalg.norm(d) * p.rScale for d in links]
+        vdps = [v @ d * p.rScale / dl for d, dl in zip(links, dls)]
+        # scale the curl so the largest move is 1% (below THRESH)
+        scale = 0.01 / max(vd * dt / dl for vd, dl in zip(vdps, dls))
+        curl = np.zeros((S, N, 3))
+        curl[s0, k] = np.array(c_sph) * scale
+        rc, fo, _ = oracle.shell_drift(p, nodes, curl, f, nb, g, dt)
+        assert rc == 0
+        total = 0.0
+        for j, vd, dl in zip(nb[s0], vdps, dls):
+            want = max(0.0, vd * scale * dt / dl)
+            assert fo[j, k, e, 0] == pytest.approx(want, rel=1e-11, abs=1e-300)
+            total += want
+        assert 0.0 < total
+        assert fo[s0, k, e, 0] == pytest.approx(1.0 - total, rel=1e-12)
+        others = np.ones(S, bool)
+        others[list(nb[s0]) + [s0]] = False
+        assert not fo[others].any()
