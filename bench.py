@@ -31,6 +31,14 @@ HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (FMA = 2 flops)
 STREAM_FLOPS_PER_SWEEP = 8   # upwind 4 + iso sum 1 + relaxation 3 (FP64)
 
+# HBM bytes per k_stream_pass launch, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (each scaled by what the counter reads for a
+# plain 1 GiB copy kernel: FETCH x 2.000, WRITE x 1.000), keyed by
+# (workload, block, halo, precision mode); the commands and per-kernel
+# figures are in profiles/k_stream_pass_c2_hbm_counters.md.  Other tilings
+# report null.
+STREAM_TRAFFIC = {("c2", 256, 16, 2): 6.262e9}
+
 CONFIGS = {
     # rows, cols, N, E, M, numEpSteps
     "c1": dict(rows=2, cols=2, N=400, E=20, M=20, eps=10),
@@ -334,7 +342,8 @@ def main():
                                   "FP64, fused multiply-add streaming (within 1e-13 of the reference order)")},
         "roofline": {"bound": "hbm", "kernel": "k_stream_pass", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": STREAM_TRAFFIC.get((args.config, tiling["block"], tiling["halo"],
+                                                    args.precision)),
                      "note": "16 B per cell per Euler sweep; the kernel keeps up to `halo` sweeps "
                              "on chip, so the binding ceiling is FP64 VALU (see fp64)"},
         "fp64": {"kernel": "k_stream_pass", "achieved_tflops": fp64, "peak_tflops": FP64_PEAK_TFLOPS,
