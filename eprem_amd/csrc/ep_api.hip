@@ -163,6 +163,7 @@ Dev make_dev() {
   D.fast = g.fast;
   D.stream_block = g.stream_block;
   D.stream_halo = g.stream_halo;
+  D.stream_r = g.stream_r;
   D.mfpRadialPower = p.mfpRadialPower;
   D.mfpBPower = p.mfpBPower;
   D.mhdBsAu = p.mhdBsAu;
@@ -335,7 +336,7 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
     return fail(EP_ERR_NO_DEVICE, "no HIP device available");
   const ep_dims d = *dims;
   if (d.num_streams < 1 || d.num_streams > 16383 || d.num_nodes < 2 || d.num_nodes > 16383 ||
-      d.num_energies < 2 || d.num_energies > 64 || d.num_mu < 2 || d.num_mu > 32)
+      d.num_energies < 2 || d.num_energies > kMaxE || d.num_mu < 2 || d.num_mu > 32)
     return fail(EP_ERR_INVALID_ARG,
                 "dims out of range (streams 1..16383, nodes 2..16383, energies 2..64, mu 2..32)");
   int rc = check_params(*params);
@@ -389,13 +390,22 @@ int ep_set_params(const ep_params *params) {
   return upload_grids();
 }
 
-int ep_set_tuning(int32_t stream_block, int32_t stream_halo) {
-  if (stream_block < 64 || stream_block > 512 || stream_block % 64)
-    return fail(EP_ERR_INVALID_ARG, "stream_block must be a multiple of 64 in 64..512");
-  if (stream_halo < 1 || 2 * stream_halo >= stream_block)
-    return fail(EP_ERR_INVALID_ARG, "stream_halo must be >= 1 and < stream_block/2");
+// A streaming tile is stream_block kept nodes (the chunk, halo included),
+// carried by stream_block / lane_nodes lanes.
+static bool tiling_ok(int W, int H, int R) {
+  if (R != 1 && R != 2) return false;
+  if (W % (64 * R) || W < 64 * R || W / R > 512) return false;
+  return H >= 1 && 2 * H < W;
+}
+
+int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes) {
+  if (!tiling_ok(stream_block, stream_halo, lane_nodes))
+    return fail(EP_ERR_INVALID_ARG,
+                "stream tiling: lane_nodes 1 or 2, stream_block a multiple of 64*lane_nodes with at most "
+                "512 lanes, 1 <= stream_halo < stream_block/2");
   g.stream_block = stream_block;
   g.stream_halo = stream_halo;
+  g.stream_r = lane_nodes;
   g.drop_graph();
   return EP_OK;
 }
@@ -404,9 +414,10 @@ int ep_set_tuning(int32_t stream_block, int32_t stream_halo) {
 // times, per candidate (block, halo), every streaming pass of one substep
 // with flip = 0 (each pass reads the current slabs and writes the spare
 // buffer, so f and the step state are left as they were), keeps the
-// fastest and leaves it set.  Candidate blocks are 128..512 lanes; halos
-// keep at least half of a block's lanes in its interior.
-int ep_autotune(int32_t *best_block, int32_t *best_halo, double *best_ms) {
+// fastest and leaves it set.  Candidates: one node per lane with 128..512
+// lanes, two nodes per lane with 128..512 lanes (chunks of 256..1024
+// nodes); halos keep at least half of a chunk in its interior.
+int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_nodes, double *best_ms) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
   if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
   int rc = ensure_events(2);
@@ -424,15 +435,18 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, double *best_ms) {
     int code = decode_err(hd.err_key, &dg);
     return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
   }
-  static const int blocks[] = {128, 192, 256, 320, 384, 512};
-  static const int halos[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+  static const int lanes[] = {128, 192, 256, 320, 384, 512};
+  static const int halos[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
   double best = 1e300;
-  int bb = g.stream_block, bh = g.stream_halo;
-  for (int W : blocks) {
+  int bb = g.stream_block, bh = g.stream_halo, br = g.stream_r;
+  for (int R = 1; R <= 2; R++)
+  for (int Lw : lanes) {
+    const int W = Lw * R;
     for (int H : halos) {
       if (4 * H > W) continue;
       g.stream_block = W;
       g.stream_halo = H;
+      g.stream_r = R;
       D = make_dev();
       HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
       HIPCHK(launch_task_prep(D, st), "task_prep");
@@ -453,11 +467,13 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, double *best_ms) {
         best = t;
         bb = W;
         bh = H;
+        br = R;
       }
     }
   }
   g.stream_block = bb;
   g.stream_halo = bh;
+  g.stream_r = br;
   g.drop_graph();
   // leave the task tables consistent with the chosen tiling
   D = make_dev();
@@ -466,6 +482,7 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, double *best_ms) {
   HIPCHK(hipStreamSynchronize(st), "tune sync");
   if (best_block) *best_block = bb;
   if (best_halo) *best_halo = bh;
+  if (best_lane_nodes) *best_lane_nodes = br;
   if (best_ms) *best_ms = best;
   return EP_OK;
 }

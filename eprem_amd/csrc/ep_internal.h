@@ -32,6 +32,7 @@ constexpr double kMhdDensityNorm = 1.0;
 // without a check (src/energeticParticles.c:940, :1665, :2316); counts past
 // this bound only arise from degenerate inputs and are reported as errors.
 constexpr int kMaxSubcycles = 1 << 22;
+constexpr int kMaxE = 64;  // energies per stream (checked by ep_init)
 
 // Nodes whose focusing needs at least this many subcycles (the innermost
 // ones, where dlnB/ds is steep) are integrated with one lane per energy
@@ -79,6 +80,7 @@ struct Dev {
   // params (code units)
   double rScale, dt, dt_full;
   int numEpSteps, mfpType, useBC, stream_block, stream_halo;
+  int stream_r;  // kept nodes per lane in the streaming passes (1 or 2)
   double mfpRadialPower, mfpBPower, mhdBsAu, dsh_min, dampCooling;
   double normJ0, normRadius, xi, beta, dlnp, dmu;
   int mpos;  // first mu index with mugrid >= 0

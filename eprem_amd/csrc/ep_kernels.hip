@@ -560,6 +560,190 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
 }
 
 // ------------------------------------------------------------------------
+// k_stream_pass2: the same pass with two consecutive kept nodes per lane,
+// so a block of L lanes carries a chunk of 2L nodes.  At the same halo
+// fraction a chunk twice as wide takes twice the halo, so a slab needs half
+// as many passes (half the HBM traffic), and the barrier and edge exchange
+// of each subcycle are shared by twice the cells.  Only one of the two
+// upwind neighbours of a lane's pair crosses lanes (DPP); the other is the
+// lane's own other node.  The streaming coefficient ds_i_multiplier *
+// del_fac[m] is the same product as above, formed each subcycle from the
+// two factors when a second register copy per node would not fit (FP64,
+// M > 16), else once per pass.
+// ------------------------------------------------------------------------
+template <int M, int MPOS, typename T, bool FAST>
+__global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip, int do_seed, int substep) {
+  const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
+  const int task = s * D.E + e;
+  const int nsteps = D.nsteps[task];
+  const int h = D.halo[task];
+  const int done = pass * h;
+  if (done >= nsteps) return;
+  const int nsub = min(h, nsteps - done);
+  const bool last = (done + nsub == nsteps);
+  const int nk = D.nkept[s];
+  const int L = blockDim.x, W = 2 * L;
+  const int I = W - 2 * h;
+  if (c * I >= nk) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = L >> 6;
+  const int j0 = 2 * tid, j1 = j0 + 1;  // positions in the chunk
+  const int k0 = c * I - h + j0, k1 = k0 + 1;
+  const bool v0 = (k0 >= 0) && (k0 < nk), v1 = (k1 >= 0) && (k1 < nk);
+  const size_t on = (size_t)s * D.Npad;
+  const bool thin = D.thin[s] != 0;
+  const int node0 = v0 ? (thin ? D.list[on + k0] : k0) : 0;
+  const int node1 = v1 ? (thin ? D.list[on + k1] : k1) : 0;
+  const int cur = D.loc[task] ^ flip;
+  constexpr uint32_t B = sizeof(T);
+  const uint32_t slab_bytes = (uint32_t)(M * D.Npad * B), row = (uint32_t)D.Npad * B;
+  const rsrc_t src = slab_rsrc((const double *)(fbufT<T>(D, cur) + (size_t)task * M * D.Npad), slab_bytes);
+  const rsrc_t dst = slab_rsrc((const double *)(fbufT<T>(D, cur ^ 1) + (size_t)task * M * D.Npad), slab_bytes);
+  const uint32_t vo0 = (uint32_t)node0 * B, vo1 = (uint32_t)node1 * B;
+
+  T f0[M], f1[M];
+  T sd0 = 0, sd1 = 0;
+  if (do_seed) {
+    if (v0) sd0 = (T)seed_value(D, D.radial[on + node0], e);
+    if (v1) sd1 = (T)seed_value(D, D.radial[on + node1], e);
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    T a = 0, b = 0;
+    if (v0) a = buf_ld(src, vo0, m * row, T());
+    if (v1) b = buf_ld(src, vo1, m * row, T());
+    if (do_seed && a < sd0) a = sd0;
+    if (do_seed && b < sd1) b = sd1;
+    f0[m] = a;
+    f1[m] = b;
+  }
+  const size_t ok = (size_t)task * D.Npad;
+  const double dsm0 = v0 ? D.dsm[ok + k0] : 0.0, dsm1 = v1 ? D.dsm[ok + k1] : 0.0;
+  const T q0 = v0 ? (T)D.q[ok + k0] : (T)1, q1 = v1 ? (T)D.q[ok + k1] : (T)1;
+  const T omq0 = (T)(v0 ? 1.0 - D.q[ok + k0] : 0.0), omq1 = (T)(v1 ? 1.0 - D.q[ok + k1] : 0.0);
+  // stream ends are held: kept node 0 for mu >= 0, the last one for mu < 0
+  const double pos0 = (k0 == 0) ? 0.0 : dsm0, neg0 = (k0 == nk - 1) ? 0.0 : dsm0;
+  const double pos1 = (k1 == 0) ? 0.0 : dsm1, neg1 = (k1 == nk - 1) ? 0.0 : dsm1;
+  const bool bc0 = D.useBC && v0 && (k0 < 3), bc1 = D.useBC && v1 && (k1 < 3);
+  const T bcv0 = bc0 ? (T)D.seed3[(size_t)task * 3 + k0] : (T)0;
+  const T bcv1 = bc1 ? (T)D.seed3[(size_t)task * 3 + (bc1 ? k1 : 0)] : (T)0;
+  const T invM = (T)(1.0 / M);
+  const double *__restrict__ del = D.del_fac + (size_t)task * M;
+  constexpr bool kCoefRegs = (sizeof(T) == 4) || (M <= 16);
+  T cf0[kCoefRegs ? M : 1], cf1[kCoefRegs ? M : 1];
+  if (kCoefRegs) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      cf0[m] = (T)(((m >= MPOS) ? pos0 : neg0) * del[m]);
+      cf1[m] = (T)(((m >= MPOS) ? pos1 : neg1) * del[m]);
+    }
+  }
+  // Edge rows between the waves of the block, double-buffered by subcycle
+  // parity, as in k_stream_pass: up[r] = the mu >= 0 values the first node
+  // of lane 0 of wave r takes from its left (node 1 of lane 63 of wave
+  // r-1; wave 0 lane 0 writes its own node 0), dn[r] = the mu < 0 values
+  // node 1 of lane 63 of wave r-1 takes from its right (node 0 of lane 0
+  // of wave r; the last wave's lane 63 writes its own node 1).
+  constexpr int NP = M - MPOS;
+  __shared__ T up[2][9][NP > 0 ? NP : 1];
+  __shared__ T dn[2][9][MPOS > 0 ? MPOS : 1];
+  for (int jj = 0; jj < nsub; ++jj) {
+    if (bc0) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) f0[m] = bcv0;
+    }
+    if (bc1) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) f1[m] = bcv1;
+    }
+    const int b = jj & 1;
+    // one writer per row, each from a fixed register set (no selects)
+    if (lane == 63) {
+#pragma unroll
+      for (int m = MPOS; m < M; ++m) up[b][w + 1][m - MPOS] = f1[m];
+    }
+    if (w == 0 && lane == 0) {
+#pragma unroll
+      for (int m = MPOS; m < M; ++m) up[b][0][m - MPOS] = f0[m];
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int m = 0; m < MPOS; ++m) dn[b][w][m] = f0[m];
+    }
+    if (w == nw - 1 && lane == 63) {
+#pragma unroll
+      for (int m = 0; m < MPOS; ++m) dn[b][nw][m] = f1[m];
+    }
+    __syncthreads();
+    T sum0 = 0, sum1 = 0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const T a0 = f0[m], a1 = f1[m];
+      const T c0 = kCoefRegs ? cf0[m] : (T)(((m >= MPOS) ? pos0 : neg0) * del[m]);
+      const T c1 = kCoefRegs ? cf1[m] : (T)(((m >= MPOS) ? pos1 : neg1) * del[m]);
+      T n0, n1;
+      if (m >= MPOS) {
+        const T nb = dpp_shr1(a1, up[b][w][m - MPOS]);  // node 1 of the lane to the left
+        n0 = FAST ? fma(c0, nb - a0, a0) : a0 + c0 * (nb - a0);
+        n1 = FAST ? fma(c1, a0 - a1, a1) : a1 + c1 * (a0 - a1);
+      } else {
+        const T nb = dpp_shl1(a0, dn[b][w + 1][m]);  // node 0 of the lane to the right
+        n0 = FAST ? fma(c0, a0 - a1, a0) : a0 + c0 * (a0 - a1);
+        n1 = FAST ? fma(c1, a1 - nb, a1) : a1 + c1 * (a1 - nb);
+      }
+      f0[m] = n0;
+      f1[m] = n1;
+      sum0 = (m == 0) ? n0 : sum0 + n0;
+      sum1 = (m == 0) ? n1 : sum1 + n1;
+    }
+    const T iso0 = invM * sum0, iso1 = invM * sum1;
+    if (FAST) {
+      const T r0 = omq0 * iso0, r1 = omq1 * iso1;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        f0[m] = fma(q0, f0[m], r0);
+        f1[m] = fma(q1, f1[m], r1);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        f0[m] = iso0 + (f0[m] - iso0) * q0;
+        f1[m] = iso1 + (f1[m] - iso1) * q1;
+      }
+    }
+  }
+
+  const bool st0 = v0 && j0 >= h && j0 < h + I, st1 = v1 && j1 >= h && j1 < h + I;
+  int bad = -1, bad_inf = 0, bad_node = 0;
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    T a = f0[m], b2 = f1[m];
+    if (last) {
+      if (st0 && bad < 0 && !isfinite(a)) {
+        bad = m;
+        bad_inf = isinf(a) ? 1 : 0;
+        bad_node = node0;
+      }
+      if (a < FloorT<T>::v) a = FloorT<T>::v;
+      if (b2 < FloorT<T>::v) b2 = FloorT<T>::v;
+    }
+    if (st0) buf_st(a, dst, vo0, m * row);
+    if (st1) buf_st(b2, dst, vo1, m * row);
+  }
+  if (last && st1) {
+    // node 1's first bad mu, reported after node 0's (the key orders them)
+    int bad1 = -1, inf1 = 0;
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+      if (bad1 < 0 && !isfinite(f1[m])) {
+        bad1 = m;
+        inf1 = isinf(f1[m]) ? 1 : 0;
+      }
+    if (bad1 >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad1, node1, inf1));
+  }
+  if (bad >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad, bad_node, bad_inf));
+}
+
+// ------------------------------------------------------------------------
 // k_stream_finish: one block per (stream, energy).  Records which buffer
 // now holds the slab and copies thinned nodes from their reference node,
 // with the NaN/Inf check and DBL_MIN floor (src/energeticParticles.c:
@@ -723,8 +907,13 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   const int s = sm / D.M, m = sm % D.M;
   const int node = (blockIdx.x % nch) * 256 + threadIdx.x;
   const int n = D.n_active[s];
-  if (node < 1 || node >= n) return;
   const int E = D.E;
+  // the buffer each energy's slab sits in, read once per block (the slab
+  // addresses are then known ahead and the loads below can be issued early)
+  __shared__ uint8_t sloc[kMaxE];
+  if (threadIdx.x < E) sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+  __syncthreads();
+  if (node < 1 || node >= n) return;
   const size_t i = (size_t)s * D.Npad + node;
   const int nsub = D.nsub_c[i];
   const double dtsub = D.dtsub_c[i];
@@ -744,7 +933,8 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   const int niter = upd ? nsub : 1;  // nsub = 0 still takes exp(log f)
   auto slab_of = [&](int e) {
     const int task = s * E + e;
-    return slab_rsrc((const double *)(fbufT<T>(D, D.loc[task]) + (size_t)task * slab), slab_bytes);
+    const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]);
+    return slab_rsrc((const double *)(fbufT<T>(D, lc) + (size_t)task * slab), slab_bytes);
   };
   // With float storage, ln f between subcycles is kept in FP64 in the part
   // of the f allocations that float storage leaves free.
@@ -935,7 +1125,20 @@ hipError_t launch_seed_check(const Dev &D, hipStream_t st) {
 template <int M, int MPOS>
 static hipError_t stream_pass_t(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                                 hipStream_t st) {
-  const dim3 grid(chunks, D.E, D.S), block(D.stream_block);
+  const dim3 grid(chunks, D.E, D.S), block(D.stream_block / D.stream_r);
+  if (D.stream_r == 2) {
+    if (D.fp32)
+      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, float, true>), grid, block, 0, st, D, pass, flip, do_seed,
+                         substep);
+    else if (D.fast)
+      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, double, true>), grid, block, 0, st, D, pass, flip, do_seed,
+                         substep);
+    else
+      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, double, false>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
+    EP_CHECK_LAUNCH();
+    return hipSuccess;
+  }
   if (D.fp32)
     hipLaunchKernelGGL((k_stream_pass<M, MPOS, float, true>), grid, block, 0, st, D, pass, flip, do_seed,
                        substep);

@@ -346,17 +346,19 @@ int ep_surrogate_samples(int32_t K, int32_t n, uint64_t seed, float *x, float *y
 int ep_last_error(char *buf, size_t len);
 int ep_finalize(void);
 
-/* Tiling of the streaming kernel: nodes per block (64..512, multiple of
- * 64) and Euler subcycles per pass (halo, 1..block/2-1).  No counterpart
- * in the reference (its DiffuseStreamData loops on the CPU); set by the
- * autotuner or left at the defaults (256, 32). */
-int ep_set_tuning(int32_t stream_block, int32_t stream_halo);
+/* Tiling of the streaming kernel: kept nodes per chunk (stream_block, halo
+ * included), Euler subcycles per pass (stream_halo, 1..stream_block/2-1)
+ * and kept nodes per lane (lane_nodes, 1 or 2; stream_block must be a
+ * multiple of 64*lane_nodes with at most 512 lanes).  No counterpart in the
+ * reference (its DiffuseStreamData loops on the CPU); set by the autotuner
+ * or left at the defaults (256, 32, 1). */
+int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes);
 
 /* Tile-size search for the streaming kernel on the current nodes: times
- * every streaming pass of one substep for each candidate (block, halo)
- * without changing f or the step state, and sets the fastest.  Any output
- * pointer may be NULL.  No counterpart in the reference. */
-int ep_autotune(int32_t *stream_block, int32_t *stream_halo, double *stream_ms);
+ * every streaming pass of one substep for each candidate tiling without
+ * changing f or the step state, and sets the fastest.  Any output pointer
+ * may be NULL.  No counterpart in the reference. */
+int ep_autotune(int32_t *stream_block, int32_t *stream_halo, int32_t *lane_nodes, double *stream_ms);
 
 /* ep_step runs its substep loop as one captured HIP graph (enable = 1, the
  * default; rebuilt when the pass count, grid or parameters change) or as

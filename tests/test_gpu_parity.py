@@ -125,7 +125,9 @@ def test_single_operator_parity(flags):
     assert max_rel(fg, fo) <= RTOL
 
 
-@pytest.mark.parametrize("tiling", [(64, 1), (64, 20), (128, 8), (256, 32), (512, 64), (512, 200)])
+@pytest.mark.parametrize("tiling", [(64, 1), (64, 20), (128, 8), (256, 32), (512, 64), (512, 200),
+                                    (128, 1, 2), (128, 20, 2), (256, 7, 2), (512, 32, 2), (1024, 64, 2),
+                                    (1024, 400, 2)])
 def test_streaming_tiling_invariance(tiling):
     """Results do not depend on the chunk size or subcycles per pass."""
     p = params(numEpSteps=2)
@@ -345,6 +347,46 @@ def test_graph_replays_match_individual_launches():
     assert max_rel(outs[True][0], fo) <= RTOL
 
 
+@pytest.mark.parametrize("case", [
+    dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=0, tiling=(512, 16, 2)),
+    dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=2, tiling=(256, 9, 2)),
+    dict(N=120, E=10, M=6, thin=0.15, ragged=False, precision=0, tiling=(128, 5, 2)),
+    dict(N=200, E=10, M=7, thin=0.0, ragged=True, precision=0, tiling=(256, 11, 2)),
+    dict(N=200, E=8, M=16, thin=0.1, ragged=True, precision=2, tiling=(384, 30, 2)),
+    dict(N=257, E=8, M=32, thin=0.0, ragged=False, precision=0, tiling=(1024, 100, 2)),
+])
+def test_two_nodes_per_lane_parity(case):
+    """The streaming kernel with two kept nodes per lane (coefficients held
+    in registers for M <= 16, formed per subcycle above) against the
+    oracle: thinned and ragged streams, odd and even M, both FP64 modes,
+    several steps."""
+    p = params(numEpSteps=2)
+    nodes = workloads.parker_nodes(1, 1, case["N"], seed=61, perturb=0.2, jitter=0.2, thin_frac=case["thin"])
+    n_active = None
+    if case["ragged"]:
+        n_active = np.array([case["N"], 37, 150, 3, 99, 128], np.int32)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, case["E"], case["M"], n_active=n_active, steps=2,
+                                    tiling=case["tiling"], precision=case["precision"])
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo, n_active) <= RTOL
+
+
+def test_two_nodes_per_lane_nan_location():
+    """A NaN on either node of a lane's pair is reported at the cell the
+    reference panics on."""
+    p = params()
+    nodes = workloads.parker_nodes(1, 1, 64, seed=9, perturb=0.2)
+    g = oracle.build_grids(p, 12, 8)
+    for node in (17, 18):
+        f0 = workloads.seeded_f(p, g, nodes, noise=0.2, seed=1)
+        f0[3, node, 5, 2] = np.nan
+        fg, fo, gd, od, _, _ = run_both(p, nodes, 12, 8, f0=f0, tiling=(256, 8, 2))
+        (rg, dg), (ro, do) = gd[0], od[0]
+        assert rg == ro == abi.EP_ERR_NAN
+        for k in ("err_stream", "err_node", "err_energy", "err_mu"):
+            assert dg[k] == do[k], (node, k, dg[k], do[k])
+
+
 def test_autotune_leaves_state_and_parity():
     """The tile-size search times passes without changing f; the tiling it
     picks gives results within the parity bar."""
@@ -356,8 +398,9 @@ def test_autotune_leaves_state_and_parity():
         f0 = workloads.seeded_f(p, sol.grids(), nodes, noise=0.3, seed=6)
         sol.set_nodes(nodes)
         sol.upload_f(f0)
-        block, halo, ms = sol.autotune()
-        assert block in (128, 192, 256, 320, 384, 512) and 1 <= halo and 4 * halo <= block and ms > 0
+        block, halo, r, ms = sol.autotune()
+        assert r in (1, 2) and block // r in (128, 192, 256, 320, 384, 512) and block % r == 0
+        assert 1 <= halo and 4 * halo <= block and ms > 0
         np.testing.assert_array_equal(sol.download_f(), f0)
         _, dg = sol.step()
         fg = sol.download_f()
