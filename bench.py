@@ -34,10 +34,10 @@ STREAM_FLOPS_PER_SWEEP = 8   # upwind 4 + iso sum 1 + relaxation 3 (FP64)
 # HBM bytes per k_stream_pass launch, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (each scaled by what the counter reads for a
 # plain 1 GiB copy kernel: FETCH x 2.000, WRITE x 1.000), keyed by
-# (workload, block, halo, nodes per lane, precision mode); the commands and per-kernel
+# (workload, block, halo, nodes per lane, fused focusing, precision mode); the commands and per-kernel
 # figures are in profiles/k_stream_pass_c2_hbm_counters.md.  Other tilings
 # report null.
-STREAM_TRAFFIC = {("c2", 256, 16, 1, 2): 6.262e9}
+STREAM_TRAFFIC = {("c2", 256, 16, 1, 0, 2): 6.262e9}
 
 CONFIGS = {
     # rows, cols, N, E, M, numEpSteps
@@ -243,6 +243,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--halo", type=int, default=0)
     ap.add_argument("--lane-nodes", type=int, default=1, choices=(1, 2))
+    ap.add_argument("--fuse-focus", type=int, default=1, choices=(0, 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mas", action="store_true")
     ap.add_argument("--precision", type=int, default=2, choices=(0, 2),
@@ -270,14 +271,14 @@ def main():
     sol.set_precision(args.precision)
     log("nodes and seed population set; precision mode %d" % args.precision)
     if args.block or args.halo:
-        sol.set_tuning(args.block or 256, args.halo or 32, args.lane_nodes)
+        sol.set_tuning(args.block or 256, args.halo or 32, args.lane_nodes, args.fuse_focus)
         tiling = {"block": args.block or 256, "halo": args.halo or 32, "lane_nodes": args.lane_nodes,
-                  "source": "flags"}
+                  "fuse_focus": args.fuse_focus, "source": "flags"}
     else:
-        blk, halo, r, tune_ms = sol.autotune()
-        log("autotune: block %d halo %d lane_nodes %d" % (blk, halo, r))
-        tiling = {"block": blk, "halo": halo, "lane_nodes": r, "source": "autotune",
-                  "stream_ms_per_substep": tune_ms}
+        blk, halo, r, fz, tune_ms = sol.autotune()
+        log("autotune: block %d halo %d lane_nodes %d fuse_focus %d" % (blk, halo, r, fz))
+        tiling = {"block": blk, "halo": halo, "lane_nodes": r, "fuse_focus": fz, "source": "autotune",
+                  "stream_and_focus_ms_per_substep": tune_ms}
     # the timed steps launch kernels one by one so that HIP events bracket
     # each operator; the captured-graph form is timed afterwards
     sol.set_graph(False)
@@ -346,7 +347,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_stream_pass", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": STREAM_TRAFFIC.get((args.config, tiling["block"], tiling["halo"],
-                                                    tiling["lane_nodes"], args.precision)),
+                                                    tiling["lane_nodes"], tiling["fuse_focus"],
+                                                    args.precision)),
                      "note": "16 B per cell per Euler sweep; the kernel keeps up to `halo` sweeps "
                              "on chip, so the binding ceiling is FP64 VALU (see fp64)"},
         "fp64": {"kernel": "k_stream_pass", "achieved_tflops": fp64, "peak_tflops": FP64_PEAK_TFLOPS,

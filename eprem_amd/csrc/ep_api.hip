@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "ep_internal.h"
@@ -140,7 +141,15 @@ static int upload_grids() {
   HIPCHK(hipMemcpy(g.rig_lamo_d, g.rig_lamo.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
   HIPCHK(hipMemcpy(g.plt_d, g.plt.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
   HIPCHK(hipMemcpy(g.ext_d, g.ext.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
-  HIPCHK(hipMemcpy(g.mugrid_d, g.mugrid.data(), M * 8, hipMemcpyHostToDevice), "grid upload");
+  // mugrid_d holds mugrid, then 0.5 * (1 - mu^2) (the first factor of the
+  // focusing velocity, src/energeticParticles.c:1620, formed in its order)
+  std::vector<double> mu2(2 * M);
+  for (int m = 0; m < M; m++) {
+    const double mu = g.mugrid[m];
+    mu2[m] = mu;
+    mu2[M + m] = 0.5 * (1.0 - mu * mu);
+  }
+  HIPCHK(hipMemcpy(g.mugrid_d, mu2.data(), 2 * M * 8, hipMemcpyHostToDevice), "grid upload");
   return EP_OK;
 }
 
@@ -164,6 +173,8 @@ Dev make_dev() {
   D.stream_block = g.stream_block;
   D.stream_halo = g.stream_halo;
   D.stream_r = g.stream_r;
+  D.stream_fuse = g.stream_fuse;
+  D.change_col = g.change_col;
   D.mfpRadialPower = p.mfpRadialPower;
   D.mfpBPower = p.mfpBPower;
   D.mhdBsAu = p.mhdBsAu;
@@ -183,6 +194,7 @@ Dev make_dev() {
   D.plt = g.plt_d;
   D.ext = g.ext_d;
   D.mugrid = g.mugrid_d;
+  D.mu_c1 = g.mugrid_d + g.d.num_mu;
   D.fA = g.fA;
   D.fB = g.fB;
   D.loc = g.loc;
@@ -300,16 +312,18 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
     e = (x);              \
     if (e != hipSuccess) return e; \
   } while (0)
+  const int fuse = (sh.do_stream && sh.do_focus && D.stream_fuse) ? 1 : 0;
   for (int sub = 0; sub < sh.nsub; sub++) {
     hipEvent_t *e4 = ev ? &ev[2 + 4 * sub] : nullptr;
     if (sh.do_seed && !sh.do_stream) EPQ(launch_seed_check(D, st));
     if (sh.do_stream) {
+      // focusing of full-resolution streams can ride on each slab's last pass
       for (int pass = 0; pass < sh.passes; pass++)
-        EPQ(launch_stream_pass(D, pass, pass & 1, sh.do_seed && pass == 0, sub, sh.chunks, st));
+        EPQ(launch_stream_pass(D, pass, pass & 1, sh.do_seed && pass == 0, sub, sh.chunks, fuse, st));
       EPQ(launch_stream_finish(D, sub, st));
     }
     if (e4) EPQ(hipEventRecord(e4[0], st));
-    if (sh.do_focus) EPQ(launch_focus(D, sub, st));
+    if (sh.do_focus) EPQ(launch_focus(D, sub, fuse, st));
     if (e4) EPQ(hipEventRecord(e4[1], st));
     if (sh.do_change) EPQ(launch_change(D, sub, st));
     // DiffuseShellData, then DriftShellData, after the stream loop
@@ -353,7 +367,7 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
 #define AL(ptr, cnt) \
   if ((rc = dalloc(&(ptr), (cnt), #ptr))) { free_all(); return rc; }
   AL(g.egrid_d, E) AL(g.vgrid_d, E) AL(g.vgrid_i_d, E) AL(g.rig_lamo_d, E) AL(g.plt_d, E)
-  AL(g.ext_d, E) AL(g.mugrid_d, M)
+  AL(g.ext_d, E) AL(g.mugrid_d, 2 * M)
   // one extra slab each: room for the FP64 scratch of the mixed-precision form
   AL(g.fA, SE * M * NP + M * NP) AL(g.fB, SE * M * NP + M * NP) AL(g.loc, SE)
   AL(g.nx, SN) AL(g.ny, SN) AL(g.nz, SN) AL(g.nrmag, SN) AL(g.nbmag, SN) AL(g.ndlnb, SN)
@@ -398,14 +412,15 @@ static bool tiling_ok(int W, int H, int R) {
   return H >= 1 && 2 * H < W;
 }
 
-int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes) {
-  if (!tiling_ok(stream_block, stream_halo, lane_nodes))
+int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes, int32_t fuse_focus) {
+  if (!tiling_ok(stream_block, stream_halo, lane_nodes) || (fuse_focus != 0 && fuse_focus != 1))
     return fail(EP_ERR_INVALID_ARG,
                 "stream tiling: lane_nodes 1 or 2, stream_block a multiple of 64*lane_nodes with at most "
-                "512 lanes, 1 <= stream_halo < stream_block/2");
+                "512 lanes, 1 <= stream_halo < stream_block/2, fuse_focus 0 or 1");
   g.stream_block = stream_block;
   g.stream_halo = stream_halo;
   g.stream_r = lane_nodes;
+  g.stream_fuse = fuse_focus;
   g.drop_graph();
   return EP_OK;
 }
@@ -417,7 +432,8 @@ int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes)
 // fastest and leaves it set.  Candidates: one node per lane with 128..512
 // lanes, two nodes per lane with 128..512 lanes (chunks of 256..1024
 // nodes); halos keep at least half of a chunk in its interior.
-int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_nodes, double *best_ms) {
+int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_nodes, int32_t *best_fuse,
+                double *best_ms) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
   if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
   int rc = ensure_events(2);
@@ -435,45 +451,65 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_node
     int code = decode_err(hd.err_key, &dg);
     return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
   }
+  struct Cand {
+    int W, H, R, F;
+    double ms;
+  };
+  // time the passes of one substep and the focusing kernel they leave to
+  // do (both write only the spare buffer), best of two
+  auto time_cand = [&](Cand &c) -> int {
+    g.stream_block = c.W;
+    g.stream_halo = c.H;
+    g.stream_r = c.R;
+    g.stream_fuse = c.F;
+    D = make_dev();
+    HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
+    HIPCHK(launch_task_prep(D, st), "task_prep");
+    HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
+    HIPCHK(hipStreamSynchronize(st), "tune sync");
+    c.ms = 1e300;
+    for (int rep = 0; rep < 2; rep++) {
+      HIPCHK(hipEventRecord(g.events[0], st), "event");
+      for (int pass = 0; pass < hd.max_passes; pass++)
+        HIPCHK(launch_stream_pass(D, pass, 0, 0, 0, hd.max_chunks, c.F, st), "stream_pass");
+      HIPCHK(launch_focus(D, 0, c.F, st, 1), "focus");
+      HIPCHK(hipEventRecord(g.events[1], st), "event");
+      HIPCHK(hipEventSynchronize(g.events[1]), "tune sync");
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, g.events[0], g.events[1]), "elapsed");
+      if (ms < c.ms) c.ms = ms;
+    }
+    return EP_OK;
+  };
+  // phase 1: every tiling with the separate focusing kernel; phase 2: the
+  // fused form of the best few (its extra registers can change the order)
   static const int lanes[] = {128, 192, 256, 320, 384, 512};
   static const int halos[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
-  double best = 1e300;
-  int bb = g.stream_block, bh = g.stream_halo, br = g.stream_r;
+  std::vector<Cand> cands;
   for (int R = 1; R <= 2; R++)
-  for (int Lw : lanes) {
-    const int W = Lw * R;
-    for (int H : halos) {
-      if (4 * H > W) continue;
-      g.stream_block = W;
-      g.stream_halo = H;
-      g.stream_r = R;
-      D = make_dev();
-      HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
-      HIPCHK(launch_task_prep(D, st), "task_prep");
-      HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
-      HIPCHK(hipStreamSynchronize(st), "tune sync");
-      double t = 1e300;
-      for (int rep = 0; rep < 2; rep++) {
-        HIPCHK(hipEventRecord(g.events[0], st), "event");
-        for (int pass = 0; pass < hd.max_passes; pass++)
-          HIPCHK(launch_stream_pass(D, pass, 0, 0, 0, hd.max_chunks, st), "stream_pass");
-        HIPCHK(hipEventRecord(g.events[1], st), "event");
-        HIPCHK(hipEventSynchronize(g.events[1]), "tune sync");
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, g.events[0], g.events[1]), "elapsed");
-        if (ms < t) t = ms;
+    for (int Lw : lanes)
+      for (int H : halos) {
+        Cand c{Lw * R, H, R, 0, 0.0};
+        if (4 * H > c.W) continue;
+        if ((rc = time_cand(c))) return rc;
+        cands.push_back(c);
       }
-      if (t < best) {
-        best = t;
-        bb = W;
-        bh = H;
-        br = R;
-      }
-    }
+  std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+  const size_t nfused = std::min<size_t>(cands.size(), 6);
+  for (size_t k = 0; k < nfused; k++) {
+    Cand c = cands[k];
+    c.F = 1;
+    if ((rc = time_cand(c))) return rc;
+    cands.push_back(c);
   }
+  const Cand best_c = *std::min_element(cands.begin(), cands.end(),
+                                        [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+  const int bb = best_c.W, bh = best_c.H, br = best_c.R, bf = best_c.F;
+  const double best = best_c.ms;
   g.stream_block = bb;
   g.stream_halo = bh;
   g.stream_r = br;
+  g.stream_fuse = bf;
   g.drop_graph();
   // leave the task tables consistent with the chosen tiling
   D = make_dev();
@@ -483,6 +519,7 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_node
   if (best_block) *best_block = bb;
   if (best_halo) *best_halo = bh;
   if (best_lane_nodes) *best_lane_nodes = br;
+  if (best_fuse) *best_fuse = bf;
   if (best_ms) *best_ms = best;
   return EP_OK;
 }

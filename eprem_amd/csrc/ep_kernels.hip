@@ -407,6 +407,45 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
 }
 
 // ------------------------------------------------------------------------
+// AdiabaticFocusing of one mu row (src/energeticParticles.c:1676-1803,
+// :1806-1896), shared by k_focus and the streaming passes.
+// ------------------------------------------------------------------------
+// The focusing update of one (node, energy) mu row held in registers:
+// half-mesh velocities (src/energeticParticles.c:1613-1653), then nsub
+// upwind flux subcycles with zero-flux walls, then the NaN/Inf check
+// (first bad mu) -- written out in each kernel so that the row and the
+// velocities stay in registers.
+#define EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf)                  \
+  do {                                                                             \
+    T va_[(M) + 1];                                                                \
+    double vprev_ = 0.0;                                                           \
+    _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
+      const double mu = (D).mugrid[m];                                             \
+      const double vel = 0.5 * (1.0 - mu * mu) * ((a) + mu * (b));                 \
+      if (m > 0) va_[m] = (T)(0.5 * (vel + vprev_));                               \
+      vprev_ = vel;                                                                \
+    }                                                                              \
+    const T dmu_i_ = (T)(1.0 / (D).dmu);                                           \
+    const T dts_ = (T)(dtsub);                                                     \
+    for (int it_ = 0; it_ < (nsub); ++it_) {                                       \
+      T flux_lo = 0; /* zero-flux wall at mu = -1 */                               \
+      _Pragma("unroll") for (int m = 0; m < (M); ++m) {                            \
+        T flux_hi = 0; /* zero-flux wall at mu = +1 */                             \
+        if (m + 1 < (M)) flux_hi = va_[m + 1] * (signbit(va_[m + 1]) ? f[m + 1] : f[m]); \
+        const T dfl_ = (flux_hi - flux_lo) * dmu_i_;                                  \
+        f[m] = f[m] - dts_ * dfl_;                                                 \
+        flux_lo = flux_hi;                                                         \
+      }                                                                            \
+    }                                                                              \
+    _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
+      if ((bad) < 0 && !isfinite(f[m])) {                                          \
+        (bad) = m;                                                                 \
+        (bad_inf) = isinf(f[m]) ? 1 : 0;                                           \
+      }                                                                            \
+    }                                                                              \
+  } while (0)
+
+// ------------------------------------------------------------------------
 // k_stream_pass: DiffuseStreamData (src/energeticParticles.c:2352-2430).
 //
 // Block = one chunk of one (stream, energy) slab; a lane owns one kept
@@ -422,7 +461,7 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
 // (1-q)*iso formed once per lane and subcycle: 6 instead of 9 VALU
 // operations per cell and subcycle, at a rounding difference from the
 // reference's operation order (not bit-identical; ~1e-15 relative).
-template <int M, int MPOS, typename T, bool FAST>
+template <int M, int MPOS, typename T, bool FAST, bool FUSE>
 __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, int do_seed, int substep) {
   const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
   const int task = s * D.E + e;
@@ -542,20 +581,38 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
   }
 
   if (valid && tid >= h && tid < h + I) {
-    int bad = -1, bad_inf = 0;
+    if (last) {
+      int bad = -1, bad_inf = 0;
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-      T v = f[m];
-      if (last) {
-        if (bad < 0 && !isfinite(v)) {
+      for (int m = 0; m < M; ++m) {
+        if (bad < 0 && !isfinite(f[m])) {
           bad = m;
-          bad_inf = isinf(v) ? 1 : 0;
+          bad_inf = isinf(f[m]) ? 1 : 0;
         }
-        if (v < FloorT<T>::v) v = FloorT<T>::v;
+        if (f[m] < FloorT<T>::v) f[m] = FloorT<T>::v;
       }
-      buf_st(v, dst, voff, m * row);
+      if (bad >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad, node, bad_inf));
+      // AdiabaticFocusing of the node, which in the reference follows the
+      // stream's DiffuseStreamData, on the values still in registers (full
+      // streams only: thinned nodes are filled after the pass; nodes with
+      // many subcycles go to k_focus_heavy)
+      if (FUSE && !D.thin[s] && node >= 1) {
+        const size_t i = on + node;
+        const int nf = D.nsub_f[i];
+        if (nf < kHeavyFocus) {
+          const double a = -D.vgrid[e] * D.f_dlnbds[i] - (2.0 / D.vgrid[e]) * D.f_dupar[i];
+          const double b2 = D.f_b[i];
+          int fbad = -1, finf = 0;
+          EP_FOCUS_ROW(T, M, D, a, b2, f, nf, D.dtsub_f[i], fbad, finf);
+#pragma unroll
+          for (int m = 0; m < M; ++m)
+            if (f[m] < FloorT<T>::v) f[m] = FloorT<T>::v;
+          if (fbad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, fbad, finf));
+        }
+      }
     }
-    if (bad >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad, node, bad_inf));
+#pragma unroll
+    for (int m = 0; m < M; ++m) buf_st(f[m], dst, voff, m * row);
   }
 }
 
@@ -571,7 +628,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
 // two factors when a second register copy per node would not fit (FP64,
 // M > 16), else once per pass.
 // ------------------------------------------------------------------------
-template <int M, int MPOS, typename T, bool FAST>
+template <int M, int MPOS, typename T, bool FAST, bool FUSE>
 __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip, int do_seed, int substep) {
   const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
   const int task = s * D.E + e;
@@ -713,34 +770,64 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
   }
 
   const bool st0 = v0 && j0 >= h && j0 < h + I, st1 = v1 && j1 >= h && j1 < h + I;
-  int bad = -1, bad_inf = 0, bad_node = 0;
+  if (last) {
+    int bad0 = -1, inf0 = 0, bad1 = -1, inf1 = 0;
 #pragma unroll
-  for (int m = 0; m < M; ++m) {
-    T a = f0[m], b2 = f1[m];
-    if (last) {
-      if (st0 && bad < 0 && !isfinite(a)) {
-        bad = m;
-        bad_inf = isinf(a) ? 1 : 0;
-        bad_node = node0;
+    for (int m = 0; m < M; ++m) {
+      if (bad0 < 0 && !isfinite(f0[m])) {
+        bad0 = m;
+        inf0 = isinf(f0[m]) ? 1 : 0;
       }
-      if (a < FloorT<T>::v) a = FloorT<T>::v;
-      if (b2 < FloorT<T>::v) b2 = FloorT<T>::v;
-    }
-    if (st0) buf_st(a, dst, vo0, m * row);
-    if (st1) buf_st(b2, dst, vo1, m * row);
-  }
-  if (last && st1) {
-    // node 1's first bad mu, reported after node 0's (the key orders them)
-    int bad1 = -1, inf1 = 0;
-#pragma unroll
-    for (int m = 0; m < M; ++m)
       if (bad1 < 0 && !isfinite(f1[m])) {
         bad1 = m;
         inf1 = isinf(f1[m]) ? 1 : 0;
       }
-    if (bad1 >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad1, node1, inf1));
+      if (f0[m] < FloorT<T>::v) f0[m] = FloorT<T>::v;
+      if (f1[m] < FloorT<T>::v) f1[m] = FloorT<T>::v;
+    }
+    if (st0 && bad0 >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad0, node0, inf0));
+    if (st1 && bad1 >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad1, node1, inf1));
+    // AdiabaticFocusing of both nodes, as in k_stream_pass; lanes without
+    // a node to focus run zero subcycles (no divergent row updates)
+    if (FUSE && !thin) {
+      const double v = D.vgrid[e], two_v = 2.0 / v;
+      int nf0 = 0, nf1 = 0;
+      double a0 = 0, b0 = 0, a1 = 0, b1 = 0, dt0 = 0, dt1 = 0;
+      const bool fz0 = st0 && node0 >= 1, fz1 = st1 && node1 >= 1;
+      if (fz0) {
+        const size_t i = on + node0;
+        nf0 = D.nsub_f[i];
+        a0 = -v * D.f_dlnbds[i] - two_v * D.f_dupar[i];
+        b0 = D.f_b[i];
+        dt0 = D.dtsub_f[i];
+      }
+      if (fz1) {
+        const size_t i = on + node1;
+        nf1 = D.nsub_f[i];
+        a1 = -v * D.f_dlnbds[i] - two_v * D.f_dupar[i];
+        b1 = D.f_b[i];
+        dt1 = D.dtsub_f[i];
+      }
+      const bool h0 = nf0 >= kHeavyFocus, h1 = nf1 >= kHeavyFocus;  // k_focus_heavy
+      if (h0) nf0 = 0;
+      if (h1) nf1 = 0;
+      int fb0 = -1, fi0 = 0, fb1 = -1, fi1 = 0;
+      EP_FOCUS_ROW(T, M, D, a0, b0, f0, nf0, dt0, fb0, fi0);
+      EP_FOCUS_ROW(T, M, D, a1, b1, f1, nf1, dt1, fb1, fi1);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (f0[m] < FloorT<T>::v) f0[m] = FloorT<T>::v;
+        if (f1[m] < FloorT<T>::v) f1[m] = FloorT<T>::v;
+      }
+      if (fz0 && !h0 && fb0 >= 0) note_error(D, err_key(substep, 1, s, 0, node0, e, fb0, fi0));
+      if (fz1 && !h1 && fb1 >= 0) note_error(D, err_key(substep, 1, s, 0, node1, e, fb1, fi1));
+    }
   }
-  if (bad >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad, bad_node, bad_inf));
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    if (st0) buf_st(f0[m], dst, vo0, m * row);
+    if (st1) buf_st(f1[m], dst, vo1, m * row);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -784,43 +871,8 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
 // v*f_upwind exactly for finite f, and the zero-flux walls reproduce the
 // operator's boundary rows.
 // ------------------------------------------------------------------------
-// The focusing update of one (node, energy) mu row held in registers:
-// half-mesh velocities (src/energeticParticles.c:1613-1653), then nsub
-// upwind flux subcycles with zero-flux walls, then the NaN/Inf check
-// (first bad mu) -- written out in each kernel so that the row and the
-// velocities stay in registers.
-#define EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf)                  \
-  do {                                                                             \
-    T va_[(M) + 1];                                                                \
-    double vprev_ = 0.0;                                                           \
-    _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
-      const double mu = (D).mugrid[m];                                             \
-      const double vel = 0.5 * (1.0 - mu * mu) * ((a) + mu * (b));                 \
-      if (m > 0) va_[m] = (T)(0.5 * (vel + vprev_));                               \
-      vprev_ = vel;                                                                \
-    }                                                                              \
-    const T dmu_i_ = (T)(1.0 / (D).dmu);                                           \
-    const T dts_ = (T)(dtsub);                                                     \
-    for (int it_ = 0; it_ < (nsub); ++it_) {                                       \
-      T flux_lo = 0; /* zero-flux wall at mu = -1 */                               \
-      _Pragma("unroll") for (int m = 0; m < (M); ++m) {                            \
-        T flux_hi = 0; /* zero-flux wall at mu = +1 */                             \
-        if (m + 1 < (M)) flux_hi = va_[m + 1] * (signbit(va_[m + 1]) ? f[m + 1] : f[m]); \
-        const T f1 = (flux_hi - flux_lo) * dmu_i_;                                 \
-        f[m] = f[m] - dts_ * f1;                                                   \
-        flux_lo = flux_hi;                                                         \
-      }                                                                            \
-    }                                                                              \
-    _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
-      if ((bad) < 0 && !isfinite(f[m])) {                                          \
-        (bad) = m;                                                                 \
-        (bad_inf) = isinf(f[m]) ? 1 : 0;                                           \
-      }                                                                            \
-    }                                                                              \
-  } while (0)
-
 template <int M, typename T>
-__global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
+__global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, int spare) {
   // blockIdx.x = ((stream, energy), chunk of 256 nodes), chunk fastest:
   // the block reads 2 KB runs of each mu row, consecutive blocks the next.
   // Blocks go round-robin to the 8 XCDs, so the chunk order is rotated by
@@ -833,6 +885,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
   const int node = chunk * 256 + threadIdx.x;
   const int n = D.n_active[s];
   if (node < 1 || node >= n) return;
+  if (fused && !D.thin[s]) return;  // done in the stream's last streaming pass
   const size_t i = (size_t)s * D.Npad + node;
   const int nsub = D.nsub_f[i];
   if (nsub >= kHeavyFocus) return;  // k_focus_heavy
@@ -841,8 +894,12 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
   const double two = 2.0;
   const int task = s * D.E + e;
   constexpr uint32_t B = sizeof(T);
-  const rsrc_t slab = slab_rsrc((const double *)(fbufT<T>(D, D.loc[task]) + (size_t)task * M * D.Npad),
+  // spare = 1 (tile-size search only): write the spare buffer, report nothing
+  const int cur = D.loc[task];
+  const rsrc_t slab = slab_rsrc((const double *)(fbufT<T>(D, cur) + (size_t)task * M * D.Npad),
                                 (uint32_t)(M * D.Npad * B));
+  const rsrc_t out = slab_rsrc((const double *)(fbufT<T>(D, cur ^ spare) + (size_t)task * M * D.Npad),
+                               (uint32_t)(M * D.Npad * B));
   const uint32_t voff = (uint32_t)node * B, row = (uint32_t)D.Npad * B;
   const double v = D.vgrid[e];
   const double a = -v * dlnBds - (two / v) * DuParDt;
@@ -854,9 +911,9 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep) {
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const T val = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
-    if (nsub > 0 || val != f[m]) buf_st(val, slab, voff, m * row);
+    if (nsub > 0 || val != f[m]) buf_st(val, out, voff, m * row);
   }
-  if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
+  if (bad >= 0 && !spare) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
 }
 
 // k_focus_heavy: the nodes k_focus leaves out (nsub_f >= kHeavyFocus), one
@@ -894,7 +951,7 @@ __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
 // ln p (src/energeticParticles.c:859-1067, :1081-1176).  A lane per node,
 // a block per 256 nodes of one (stream, mu).  Each subcycle is one
 // sweep up the energy column with a sliding window (the new ln f at e
-// needs the old values at e-1, e, e+1), reading ahead PF energies to keep
+// needs the old values at e-1, e, e+1), reading ahead kPf energies to keep
 // loads in flight; intermediate ln f is kept in place in the f slab, the
 // first sweep takes the log and the last one writes exp back.  A lane
 // needs a handful of registers, so the kernel runs at full occupancy.
@@ -943,18 +1000,29 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   auto spare_of = [&](int e) { return slab_rsrc(spare_slab(D, s * E + e), (uint32_t)(slab * 8)); };
   int bad = -1, bad_inf = 0;
   for (int it = 0; it < niter; ++it) {
+    // first/fin differ between lanes (per-node subcycle counts): the slab
+    // descriptors are formed outside any branch on them, so that they stay
+    // in scalar registers
     const bool first = (it == 0), fin = (it == niter - 1);
     // the column is advanced in FP64 whatever the storage type
     auto ld = [&](int e) {
-      if (kSpare && !first) return buf_ld(spare_of(e), voff8, 0);
-      return (double)buf_ld(slab_of(e), voff, 0, T());
+      const rsrc_t rs = slab_of(e);
+      if (kSpare) {
+        const rsrc_t rp = spare_of(e);
+        const double a = buf_ld(rp, voff8, 0);
+        const double b = (double)buf_ld(rs, voff, 0, T());
+        return first ? b : a;
+      }
+      return (double)buf_ld(rs, voff, 0, T());
     };
     auto cvt = [&](double x) { return first ? log(x) : x; };
     double r0 = cvt(ld(0));
     double r1 = cvt(ld(1));
-    double p2 = (2 < E) ? ld(2) : 0.0;
-    double p3 = (3 < E) ? ld(3) : 0.0;
-    double p4 = (4 < E) ? ld(4) : 0.0;
+    // read-ahead: kPf energies beyond the window are in flight
+    constexpr int kPf = 3;
+    double pf[kPf];
+#pragma unroll
+    for (int q = 0; q < kPf; ++q) pf[q] = (2 + q < E) ? ld(2 + q) : 0.0;
     double old_prev = 0.0;  // ln f[e-1] before this subcycle
     double vel = damp * (a * vgi[0] + bb);
     double vel_prev = 0.0;
@@ -984,6 +1052,7 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
         flux_e = flux_hi;
       }
       const double nv = upd ? cur - dtsub * f1 : cur;
+      const rsrc_t rs = slab_of(e);
       if (fin) {
         T val = (T)exp(nv);
         if (bad < 0 && !isfinite(val)) {
@@ -991,20 +1060,121 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
           bad_inf = isinf(val) ? 1 : 0;
         }
         if (val < FloorT<T>::v) val = FloorT<T>::v;
-        buf_st(val, slab_of(e), voff, 0);
+        buf_st(val, rs, voff, 0);
       } else if (kSpare) {
-        buf_st(nv, spare_of(e), voff8, 0);
+        const rsrc_t rp = spare_of(e);
+        buf_st(nv, rp, voff8, 0);
       } else {
-        buf_st((T)nv, slab_of(e), voff, 0);
+        buf_st((T)nv, rs, voff, 0);
       }
       old_prev = cur;
       vel_prev = vel;
       vel = vel1;
       r0 = r1;
-      r1 = (e + 2 < E) ? cvt(p2) : 0.0;
-      p2 = p3;
-      p3 = p4;
-      p4 = (e + 5 < E) ? ld(e + 5) : 0.0;
+      r1 = (e + 2 < E) ? cvt(pf[0]) : 0.0;
+#pragma unroll
+      for (int q = 0; q + 1 < kPf; ++q) pf[q] = pf[q + 1];
+      pf[kPf - 1] = (e + 2 + kPf < E) ? ld(e + 2 + kPf) : 0.0;
+    }
+  }
+  if (bad >= 0) note_error(D, err_key(substep, 1, s, 1, node, bad, m, bad_inf));
+}
+
+// ------------------------------------------------------------------------
+// k_change_col: the same operator with the whole energy column of a lane's
+// (node, mu) cell in registers (EMAX >= E, a multiple of 8): all E loads
+// are issued at once (k_change keeps a few in flight), ln f stays in FP64
+// registers across the subcycles, and the column is written once.  Same
+// operations in the same order as k_change.
+// ------------------------------------------------------------------------
+template <typename T, int EMAX>
+__global__ __launch_bounds__(256) void k_change_col(Dev D, int substep) {
+  const int nch = (D.N + 255) >> 8;
+  const int sm = blockIdx.x / nch;
+  const int s = sm / D.M, m = sm % D.M;
+  const int node = (blockIdx.x % nch) * 256 + threadIdx.x;
+  const int n = D.n_active[s];
+  const int E = D.E;
+  __shared__ uint8_t sloc[kMaxE];
+  if (threadIdx.x < E) sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+  __syncthreads();
+  if (node < 1 || node >= n) return;
+  const size_t i = (size_t)s * D.Npad + node;
+  const int nsub = D.nsub_c[i];
+  const double dtsub = D.dtsub_c[i];
+  const double DuParDt = D.c_dupar[i], DlnNDt = D.c_dlnn[i], DlnBDt = D.c_dlnb[i];
+  const double damp = D.c_damp[i], bc0 = D.c_bc0[i], bc1 = D.c_bc1[i];
+  const double dlnp_i = 1.0 / D.dlnp;
+  const double half = 0.5, one = 1.0;
+  const size_t slab = (size_t)D.M * D.Npad;
+  constexpr uint32_t B = sizeof(T);
+  const uint32_t slab_bytes = (uint32_t)(slab * B);
+  const uint32_t voff = (uint32_t)(((size_t)m * D.Npad + node) * B);
+  const double mu = D.mugrid[m];
+  const double a = -mu * DuParDt;
+  const double bb = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
+  const double *__restrict__ vgi = D.vgrid_i;
+  auto slab_of = [&](int e) {
+    const int task = s * E + e;
+    const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]);
+    return slab_rsrc((const double *)(fbufT<T>(D, lc) + (size_t)task * slab), slab_bytes);
+  };
+  double col[EMAX];
+#pragma unroll
+  for (int e = 0; e < EMAX; ++e)
+    if (e < E) col[e] = (double)buf_ld(slab_of(e), voff, 0, T());
+#pragma unroll
+  for (int e = 0; e < EMAX; ++e)
+    if (e < E) col[e] = log(col[e]);
+  for (int it = 0; it < nsub; ++it) {
+    double old_prev = 0.0;  // ln f[e-1] before this subcycle
+    double vel = damp * (a * vgi[0] + bb);
+    double vel_prev = 0.0;
+    double flux_e = 0.0;  // flux at interface e (between e-1 and e)
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) {
+      if (e < E) {
+        const double cur = col[e];
+        const double nxt = (e + 1 < E) ? col[e + 1 < EMAX ? e + 1 : e] : 0.0;
+        const double vel1 = (e + 1 < E) ? damp * (a * vgi[e + 1] + bb) : 0.0;
+        double f1;
+        if (e == 0) {
+          const double va1 = half * (vel1 + vel);
+          if (va1 <= 0)
+            f1 = va1 * (nxt - cur) * dlnp_i;
+          else
+            f1 = va1 * (cur - bc0) * dlnp_i;
+          flux_e = va1 * (signbit(va1) ? nxt : cur);  // interface 1
+        } else if (e == E - 1) {
+          const double vr = half * (vel + vel_prev);
+          if (vr >= 0)
+            f1 = vr * (cur - old_prev) * dlnp_i;
+          else
+            f1 = vr * (bc1 - cur) * dlnp_i;
+        } else {
+          const double va1 = half * (vel1 + vel);
+          const double flux_hi = va1 * (signbit(va1) ? nxt : cur);
+          f1 = (flux_hi - flux_e) * dlnp_i;
+          flux_e = flux_hi;
+        }
+        col[e] = cur - dtsub * f1;
+        old_prev = cur;
+        vel_prev = vel;
+        vel = vel1;
+      }
+    }
+  }
+  int bad = -1, bad_inf = 0;
+#pragma unroll
+  for (int e = 0; e < EMAX; ++e) {
+    if (e < E) {
+      T val = (T)exp(col[e]);
+      if (bad < 0 && !isfinite(val)) {
+        bad = e;
+        bad_inf = isinf(val) ? 1 : 0;
+      }
+      if (val < FloorT<T>::v) val = FloorT<T>::v;
+      buf_st(val, slab_of(e), voff, 0);
     }
   }
   if (bad >= 0) note_error(D, err_key(substep, 1, s, 1, node, bad, m, bad_inf));
@@ -1122,44 +1292,51 @@ hipError_t launch_seed_check(const Dev &D, hipStream_t st) {
   return hipSuccess;
 }
 
-template <int M, int MPOS>
-static hipError_t stream_pass_t(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
-                                hipStream_t st) {
+template <int M, int MPOS, bool FUSE>
+static hipError_t stream_pass_tf(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
+                                 hipStream_t st) {
   const dim3 grid(chunks, D.E, D.S), block(D.stream_block / D.stream_r);
   if (D.stream_r == 2) {
     if (D.fp32)
-      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, float, true>), grid, block, 0, st, D, pass, flip, do_seed,
-                         substep);
-    else if (D.fast)
-      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, double, true>), grid, block, 0, st, D, pass, flip, do_seed,
-                         substep);
-    else
-      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, double, false>), grid, block, 0, st, D, pass, flip,
+      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, float, true, FUSE>), grid, block, 0, st, D, pass, flip,
                          do_seed, substep);
-    EP_CHECK_LAUNCH();
-    return hipSuccess;
+    else if (D.fast)
+      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, double, true, FUSE>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
+    else
+      hipLaunchKernelGGL((k_stream_pass2<M, MPOS, double, false, FUSE>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
+  } else {
+    if (D.fp32)
+      hipLaunchKernelGGL((k_stream_pass<M, MPOS, float, true, FUSE>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
+    else if (D.fast)
+      hipLaunchKernelGGL((k_stream_pass<M, MPOS, double, true, FUSE>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
+    else
+      hipLaunchKernelGGL((k_stream_pass<M, MPOS, double, false, FUSE>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
   }
-  if (D.fp32)
-    hipLaunchKernelGGL((k_stream_pass<M, MPOS, float, true>), grid, block, 0, st, D, pass, flip, do_seed,
-                       substep);
-  else if (D.fast)
-    hipLaunchKernelGGL((k_stream_pass<M, MPOS, double, true>), grid, block, 0, st, D, pass, flip, do_seed,
-                       substep);
-  else
-    hipLaunchKernelGGL((k_stream_pass<M, MPOS, double, false>), grid, block, 0, st, D, pass, flip, do_seed,
-                       substep);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
 
+template <int M, int MPOS>
+static hipError_t stream_pass_t(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks, int fuse,
+                                hipStream_t st) {
+  return fuse ? stream_pass_tf<M, MPOS, true>(D, pass, flip, do_seed, substep, chunks, st)
+              : stream_pass_tf<M, MPOS, false>(D, pass, flip, do_seed, substep, chunks, st);
+}
+
 #define EP_PASS_CASE(MM)                                                                 \
   case MM:                                                                               \
-    if (D.mpos == (MM) / 2) return stream_pass_t<MM, (MM) / 2>(D, pass, flip, do_seed, sub, chunks, st); \
+    if (D.mpos == (MM) / 2)                                                              \
+      return stream_pass_t<MM, (MM) / 2>(D, pass, flip, do_seed, sub, chunks, fuse, st); \
     if (D.mpos == ((MM) + 1) / 2)                                                        \
-      return stream_pass_t<MM, ((MM) + 1) / 2>(D, pass, flip, do_seed, sub, chunks, st); \
+      return stream_pass_t<MM, ((MM) + 1) / 2>(D, pass, flip, do_seed, sub, chunks, fuse, st); \
     return hipErrorInvalidValue;
 
-hipError_t launch_stream_pass(const Dev &D, int pass, int flip, int do_seed, int sub, int chunks,
+hipError_t launch_stream_pass(const Dev &D, int pass, int flip, int do_seed, int sub, int chunks, int fuse,
                               hipStream_t st) {
   switch (D.M) {
     EP_PASS_CASE(2) EP_PASS_CASE(3) EP_PASS_CASE(4) EP_PASS_CASE(5) EP_PASS_CASE(6)
@@ -1187,19 +1364,21 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
   case MM:                                                                                    \
     if (D.fp32) {                                                                             \
       hipLaunchKernelGGL((k_focus<MM, float>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
-                         st, D, sub);                                                         \
-      hipLaunchKernelGGL((k_focus_heavy<MM, float>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+                         st, D, sub, fused, spare);                                           \
+      if (!spare)                                                                             \
+        hipLaunchKernelGGL((k_focus_heavy<MM, float>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     } else {                                                                                  \
       hipLaunchKernelGGL((k_focus<MM, double>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
-                         st, D, sub);                                                         \
-      hipLaunchKernelGGL((k_focus_heavy<MM, double>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+                         st, D, sub, fused, spare);                                           \
+      if (!spare)                                                                             \
+        hipLaunchKernelGGL((k_focus_heavy<MM, double>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     }                                                                                         \
     break;
 
 // Blocks of the heavy-focusing walk: enough waves to fill the chip.
 constexpr int kHeavyGrid = 4096;
 
-hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
+hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int spare) {
   switch (D.M) {
     EP_FOCUS_CASE(2) EP_FOCUS_CASE(3) EP_FOCUS_CASE(4) EP_FOCUS_CASE(5) EP_FOCUS_CASE(6)
     EP_FOCUS_CASE(7) EP_FOCUS_CASE(8) EP_FOCUS_CASE(9) EP_FOCUS_CASE(10) EP_FOCUS_CASE(11)
@@ -1215,12 +1394,32 @@ hipError_t launch_focus(const Dev &D, int sub, hipStream_t st) {
   return hipSuccess;
 }
 
+template <int EMAX>
+static void change_col_t(const Dev &D, int sub, dim3 grid, dim3 block, hipStream_t st) {
+  if (D.fp32)
+    hipLaunchKernelGGL((k_change_col<float, EMAX>), grid, block, 0, st, D, sub);
+  else
+    hipLaunchKernelGGL((k_change_col<double, EMAX>), grid, block, 0, st, D, sub);
+}
+
 hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
   const dim3 grid((D.N + 255) / 256 * D.S * D.M), block(256);
-  if (D.fp32)
+  if (D.change_col) {
+    switch ((D.E + 7) / 8) {
+      case 1: change_col_t<8>(D, sub, grid, block, st); break;
+      case 2: change_col_t<16>(D, sub, grid, block, st); break;
+      case 3: change_col_t<24>(D, sub, grid, block, st); break;
+      case 4: change_col_t<32>(D, sub, grid, block, st); break;
+      case 5: change_col_t<40>(D, sub, grid, block, st); break;
+      case 6: change_col_t<48>(D, sub, grid, block, st); break;
+      case 7: change_col_t<56>(D, sub, grid, block, st); break;
+      default: change_col_t<64>(D, sub, grid, block, st); break;
+    }
+  } else if (D.fp32) {
     hipLaunchKernelGGL(k_change<float>, grid, block, 0, st, D, sub);
-  else
+  } else {
     hipLaunchKernelGGL(k_change<double>, grid, block, 0, st, D, sub);
+  }
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }

@@ -9,10 +9,12 @@ hipError_t launch_stream_prep(const Dev &D, hipStream_t st);
 hipError_t launch_node_prep(const Dev &D, hipStream_t st);
 hipError_t launch_seed_check(const Dev &D, hipStream_t st);
 hipError_t launch_stream_pass(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
-                              hipStream_t st);
+                              int fuse_focus, hipStream_t st);
 hipError_t launch_task_prep(const Dev &D, hipStream_t st);
 hipError_t launch_stream_finish(const Dev &D, int substep, hipStream_t st);
-hipError_t launch_focus(const Dev &D, int substep, hipStream_t st);
+// fused: focusing of full streams was done by the streaming passes;
+// spare: write the spare buffer and skip the heavy nodes (tile-size search)
+hipError_t launch_focus(const Dev &D, int substep, int fused, hipStream_t st, int spare = 0);
 hipError_t launch_change(const Dev &D, int substep, hipStream_t st);
 hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st);
 hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st);

@@ -34,7 +34,7 @@ struct State {
   ep_params p{};
   int Npad = 0;
   hipStream_t st = nullptr;
-  int stream_block = 256, stream_halo = 32, stream_r = 1;
+  int stream_block = 256, stream_halo = 32, stream_r = 1, stream_fuse = 1, change_col = 1;
   // graph of the substep loop; `generation` changes whenever kernel
   // arguments captured by value (parameters, tiling) change
   bool use_graph = true;

@@ -101,7 +101,7 @@ int ep_surrogate_samples(int32_t K, int32_t n, uint64_t seed, float *X, float *Y
   if (e == hipSuccess) e = hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, g.st);
   if (e == hipSuccess) e = hipStreamSynchronize(g.st);
   // pass 0 with halo K: slabs with nsteps >= K advance exactly K subcycles
-  if (e == hipSuccess) e = launch_stream_pass(D, 0, 0, 0, 0, hd.max_chunks, g.st);
+  if (e == hipSuccess) e = launch_stream_pass(D, 0, 0, 0, 0, hd.max_chunks, 0, g.st);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_samples, dim3((n + 255) / 256), dim3(256), 0, g.st, D, K, n, seed, dx, dy, dv);
     e = hipGetLastError();

@@ -37,7 +37,7 @@ def lib():
         DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int32)
         L.ep_init.argtypes = [C.POINTER(EpDims), C.POINTER(EpParams)]
         L.ep_set_params.argtypes = [C.POINTER(EpParams)]
-        L.ep_set_tuning.argtypes = [C.c_int32, C.c_int32, C.c_int32]
+        L.ep_set_tuning.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.ep_get_grids.argtypes = [DP, DP, DP, DP, DP, DP]
         L.ep_set_nodes.argtypes = [C.POINTER(EpNodeSoa), IP]
         L.ep_upload_f.argtypes = [DP, C.c_int32]
@@ -56,7 +56,7 @@ def lib():
         L.ep_stream_flux.argtypes = [DP]
         L.ep_set_positions.argtypes = [DP, DP, DP, DP, IP]
         L.ep_update_mhd.argtypes = [C.POINTER(abi.EpMhdParams), C.POINTER(EpMasPrev), C.POINTER(EpMasOut), DP]
-        L.ep_autotune.argtypes = [IP, IP, IP, DP]
+        L.ep_autotune.argtypes = [IP, IP, IP, IP, DP]
         L.ep_spawn_shell.argtypes = [C.POINTER(abi.EpSpawn)]
         L.ep_move_nodes.argtypes = [C.c_double]
         L.ep_get_positions.argtypes = [DP, DP, DP, DP, IP]
@@ -115,17 +115,19 @@ class EpSolver:
         d = self.dims
         return (d.num_streams, d.num_nodes, d.num_energies, d.num_mu)
 
-    def set_tuning(self, stream_block, stream_halo, lane_nodes=1):
+    def set_tuning(self, stream_block, stream_halo, lane_nodes=1, fuse_focus=1):
         """Streaming tiles of stream_block kept nodes (halo included),
-        stream_halo subcycles per pass, lane_nodes (1 or 2) nodes per lane."""
-        _check(lib().ep_set_tuning(int(stream_block), int(stream_halo), int(lane_nodes)))
+        stream_halo subcycles per pass, lane_nodes (1 or 2) nodes per lane;
+        fuse_focus: focusing done by the last streaming pass (1) or by the
+        separate kernel (0)."""
+        _check(lib().ep_set_tuning(int(stream_block), int(stream_halo), int(lane_nodes), int(fuse_focus)))
 
     def autotune(self):
         """Tile-size search for the streaming kernel; returns (block, halo,
-        lane_nodes, ms)."""
-        b, h, r, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
-        _check(lib().ep_autotune(C.byref(b), C.byref(h), C.byref(r), C.byref(ms)))
-        return b.value, h.value, r.value, ms.value
+        lane_nodes, fuse_focus, ms)."""
+        b, h, r, fz, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        _check(lib().ep_autotune(C.byref(b), C.byref(h), C.byref(r), C.byref(fz), C.byref(ms)))
+        return b.value, h.value, r.value, fz.value, ms.value
 
     def set_precision(self, mode):
         """0 / False: FP64 in the reference's operation order; 1 / True: FP32

@@ -347,18 +347,23 @@ int ep_last_error(char *buf, size_t len);
 int ep_finalize(void);
 
 /* Tiling of the streaming kernel: kept nodes per chunk (stream_block, halo
- * included), Euler subcycles per pass (stream_halo, 1..stream_block/2-1)
- * and kept nodes per lane (lane_nodes, 1 or 2; stream_block must be a
- * multiple of 64*lane_nodes with at most 512 lanes).  No counterpart in the
- * reference (its DiffuseStreamData loops on the CPU); set by the autotuner
- * or left at the defaults (256, 32, 1). */
-int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes);
+ * included), Euler subcycles per pass (stream_halo, 1..stream_block/2-1),
+ * kept nodes per lane (lane_nodes, 1 or 2; stream_block must be a multiple
+ * of 64*lane_nodes with at most 512 lanes) and fuse_focus (1: the
+ * focusing of each node of an unthinned stream is done by its last
+ * streaming pass on the values still in registers; 0: by the separate
+ * focusing kernel).  Results are the same either way.  No counterpart in
+ * the reference (its DiffuseStreamData loops on the CPU); set by the
+ * autotuner or left at the defaults (256, 32, 1, 1). */
+int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes, int32_t fuse_focus);
 
 /* Tile-size search for the streaming kernel on the current nodes: times
- * every streaming pass of one substep for each candidate tiling without
- * changing f or the step state, and sets the fastest.  Any output pointer
- * may be NULL.  No counterpart in the reference. */
-int ep_autotune(int32_t *stream_block, int32_t *stream_halo, int32_t *lane_nodes, double *stream_ms);
+ * every streaming pass of one substep and the focusing work it leaves for
+ * each candidate tiling without changing f or the step state, and sets the
+ * fastest.  Any output pointer may be NULL.  No counterpart in the
+ * reference. */
+int ep_autotune(int32_t *stream_block, int32_t *stream_halo, int32_t *lane_nodes, int32_t *fuse_focus,
+                double *stream_ms);
 
 /* ep_step runs its substep loop as one captured HIP graph (enable = 1, the
  * default; rebuilt when the pass count, grid or parameters change) or as

@@ -127,7 +127,7 @@ def test_single_operator_parity(flags):
 
 @pytest.mark.parametrize("tiling", [(64, 1), (64, 20), (128, 8), (256, 32), (512, 64), (512, 200),
                                     (128, 1, 2), (128, 20, 2), (256, 7, 2), (512, 32, 2), (1024, 64, 2),
-                                    (1024, 400, 2)])
+                                    (1024, 400, 2), (256, 32, 1, 0), (512, 32, 2, 0), (1024, 64, 2, 0)])
 def test_streaming_tiling_invariance(tiling):
     """Results do not depend on the chunk size or subcycles per pass."""
     p = params(numEpSteps=2)
@@ -371,6 +371,21 @@ def test_two_nodes_per_lane_parity(case):
     assert max_rel(fg, fo, n_active) <= RTOL
 
 
+@pytest.mark.parametrize("tiling", [(256, 16, 1, 1), (512, 32, 2, 1), (384, 24, 2, 1), (256, 16, 1, 0)])
+def test_fused_focusing_mixed_thinning(tiling):
+    """Streams without thinned nodes are focused by their last streaming
+    pass, thinned ones (filled after the pass) by the focusing kernel; a
+    cube with both kinds matches the oracle over several steps."""
+    p = params(numEpSteps=3)
+    a = workloads.parker_nodes(1, 1, 150, seed=64, perturb=0.2, jitter=0.2, thin_frac=0.15)
+    nodes = workloads.parker_nodes(1, 1, 150, seed=64, perturb=0.2, jitter=0.2)
+    for k in nodes:
+        nodes[k][0::2] = a[k][0::2]
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 12, 30, steps=2, tiling=tiling, precision=2)
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo) <= RTOL
+
+
 def test_two_nodes_per_lane_nan_location():
     """A NaN on either node of a lane's pair is reported at the cell the
     reference panics on."""
@@ -398,8 +413,9 @@ def test_autotune_leaves_state_and_parity():
         f0 = workloads.seeded_f(p, sol.grids(), nodes, noise=0.3, seed=6)
         sol.set_nodes(nodes)
         sol.upload_f(f0)
-        block, halo, r, ms = sol.autotune()
+        block, halo, r, fz, ms = sol.autotune()
         assert r in (1, 2) and block // r in (128, 192, 256, 320, 384, 512) and block % r == 0
+        assert fz in (0, 1)
         assert 1 <= halo and 4 * halo <= block and ms > 0
         np.testing.assert_array_equal(sol.download_f(), f0)
         _, dg = sol.step()
@@ -411,14 +427,16 @@ def test_autotune_leaves_state_and_parity():
     assert max_rel(fg, fo) <= RTOL
 
 
-def test_heavy_focusing_nodes_parity():
+@pytest.mark.parametrize("tiling", [(256, 32, 1, 0), (256, 32, 1, 1), (512, 32, 2, 1), (512, 32, 2, 0)])
+def test_heavy_focusing_nodes_parity(tiling):
     """Inner nodes with steep dlnB/ds need hundreds of focusing subcycles
-    and take the one-lane-per-energy kernel; the rest take the one-lane-per-
-    node kernel.  Both must match the oracle."""
+    and take the one-lane-per-energy kernel; the rest are focused by the
+    one-lane-per-node kernel or, fused, by the last streaming pass.  All
+    must match the oracle."""
     p = params(rScale=0.00465047, numEpSteps=2)
     nodes = workloads.parker_nodes(1, 1, 300, rscale=0.00465047, seed=23, perturb=0.1, smooth=20,
                                    jitter=0.02)
-    fg, fo, gd, od, _, _ = run_both(p, nodes, 20, 30)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, 20, 30, tiling=tiling)
     assert od[0][1]["max_subcycles_focusing"] >= 64  # heavy path taken
     assert_diag_equal(gd, od)
     assert max_rel(fg, fo) <= RTOL
