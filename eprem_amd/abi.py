@@ -212,6 +212,8 @@ EXPORTED_SYMBOLS = (
     "ep_get_positions",
     "ep_set_precision",
     "ep_selftest_copy",
+    "ep_fastmath_host",
+    "ep_selftest_fastmath",
     "ep_mlp_init",
     "ep_mlp_num_params",
     "ep_mlp_get_params",

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "ep_fastmath.h"
 #include "ep_internal.h"
 #include "ep_launch.h"
 #include "ep_state.h"
@@ -133,6 +134,23 @@ static int check_params(const ep_params &p) {
   return EP_OK;
 }
 
+// Tables of fm_exp / fm_log: 2^(j/64), 1/F and ln F (F = 1 + j/128) as
+// double hi + lo pairs, formed in x87 extended precision.
+static void build_fm_table(double *t) {
+  for (int j = 0; j < kExpN; j++) {
+    const long double v = exp2l((long double)j / kExpN);
+    t[kFmExpHi + j] = (double)v;
+    t[kFmExpLo + j] = (double)(v - (long double)t[kFmExpHi + j]);
+  }
+  for (int j = 0; j <= kLogN; j++) {
+    const long double F = 1.0L + (long double)j / kLogN;
+    const long double l = logl(F);
+    t[kFmLogInv + j] = (double)(1.0L / F);
+    t[kFmLogHi + j] = (double)l;
+    t[kFmLogLo + j] = (double)(l - (long double)t[kFmLogHi + j]);
+  }
+}
+
 static int upload_grids() {
   const int E = g.d.num_energies, M = g.d.num_mu;
   HIPCHK(hipMemcpy(g.egrid_d, g.egrid.data(), E * 8, hipMemcpyHostToDevice), "grid upload");
@@ -150,6 +168,9 @@ static int upload_grids() {
     mu2[M + m] = 0.5 * (1.0 - mu * mu);
   }
   HIPCHK(hipMemcpy(g.mugrid_d, mu2.data(), 2 * M * 8, hipMemcpyHostToDevice), "grid upload");
+  std::vector<double> fmt(kFmTable);
+  build_fm_table(fmt.data());
+  HIPCHK(hipMemcpy(g.fm_tab_d, fmt.data(), kFmTable * 8, hipMemcpyHostToDevice), "table upload");
   return EP_OK;
 }
 
@@ -174,7 +195,6 @@ Dev make_dev() {
   D.stream_halo = g.stream_halo;
   D.stream_r = g.stream_r;
   D.stream_fuse = g.stream_fuse;
-  D.change_col = g.change_col;
   D.mfpRadialPower = p.mfpRadialPower;
   D.mfpBPower = p.mfpBPower;
   D.mhdBsAu = p.mhdBsAu;
@@ -195,6 +215,7 @@ Dev make_dev() {
   D.ext = g.ext_d;
   D.mugrid = g.mugrid_d;
   D.mu_c1 = g.mugrid_d + g.d.num_mu;
+  D.fm_tab = g.fm_tab_d;
   D.fA = g.fA;
   D.fB = g.fB;
   D.loc = g.loc;
@@ -367,7 +388,7 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
 #define AL(ptr, cnt) \
   if ((rc = dalloc(&(ptr), (cnt), #ptr))) { free_all(); return rc; }
   AL(g.egrid_d, E) AL(g.vgrid_d, E) AL(g.vgrid_i_d, E) AL(g.rig_lamo_d, E) AL(g.plt_d, E)
-  AL(g.ext_d, E) AL(g.mugrid_d, 2 * M)
+  AL(g.ext_d, E) AL(g.mugrid_d, 2 * M) AL(g.fm_tab_d, kFmTable)
   // one extra slab each: room for the FP64 scratch of the mixed-precision form
   AL(g.fA, SE * M * NP + M * NP) AL(g.fB, SE * M * NP + M * NP) AL(g.loc, SE)
   AL(g.nx, SN) AL(g.ny, SN) AL(g.nz, SN) AL(g.nrmag, SN) AL(g.nbmag, SN) AL(g.ndlnb, SN)
@@ -775,6 +796,35 @@ int ep_selftest_dpp(const double *in64, double *shr64, double *shl64) {
 // Test hook: a plain device-to-device copy kernel of `bytes` bytes (a
 // multiple of 16), run `reps` times; *ms = mean kernel time.  The reference
 // point for HBM byte counters and bandwidth.
+// fm_log and fm_exp of n values on the host (the same code the device
+// runs), for the accuracy tests.
+int ep_fastmath_host(const double *x, int64_t n, double *log_out, double *exp_out) {
+  if (!x || n < 0 || !log_out || !exp_out) return fail(EP_ERR_INVALID_ARG, "bad arguments");
+  std::vector<double> t(kFmTable);
+  build_fm_table(t.data());
+  for (int64_t i = 0; i < n; i++) {
+    log_out[i] = fm_log(x[i], t.data());
+    exp_out[i] = fm_exp(x[i], t.data());
+  }
+  return EP_OK;
+}
+
+// The same on the device (needs ep_init for the tables).
+int ep_selftest_fastmath(const double *x, int64_t n, double *log_out, double *exp_out) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!x || n < 1 || !log_out || !exp_out) return fail(EP_ERR_INVALID_ARG, "bad arguments");
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, (size_t)n * 3 * sizeof(double)), "hipMalloc");
+  hipError_t e = hipMemcpy(d, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_fastmath_probe(g.fm_tab_d, d, n, d + n, d + 2 * n, g.st);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.st);
+  if (e == hipSuccess) e = hipMemcpy(log_out, d + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(exp_out, d + 2 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "fastmath probe");
+  return EP_OK;
+}
+
 int ep_selftest_copy(size_t bytes, int32_t reps, double *ms) {
   if (bytes == 0 || bytes % 16 || reps < 1) return fail(EP_ERR_INVALID_ARG, "bytes must be a positive multiple of 16");
   void *a = nullptr, *b = nullptr;

@@ -82,13 +82,13 @@ struct Dev {
   int numEpSteps, mfpType, useBC, stream_block, stream_halo;
   int stream_r;     // kept nodes per lane in the streaming passes (1 or 2)
   int stream_fuse;  // focusing of full streams done in their last streaming pass
-  int change_col;   // adiabatic change with the energy column in registers
   double mfpRadialPower, mfpBPower, mhdBsAu, dsh_min, dampCooling;
   double normJ0, normRadius, xi, beta, dlnp, dmu;
   int mpos;  // first mu index with mugrid >= 0
   // grids [E] / [M]
   const double *egrid, *vgrid, *vgrid_i, *rig_lamo, *plt, *ext, *mugrid;
   const double *mu_c1;  // [M] 0.5 * (1 - mu^2), the focusing velocity's factor
+  const double *fm_tab; // tables of fm_exp / fm_log (ep_fastmath.h)
   // f
   double *fA, *fB;
   uint8_t *loc;  // [S*E]

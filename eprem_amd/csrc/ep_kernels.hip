@@ -20,6 +20,7 @@
 #include <float.h>
 #include <math.h>
 
+#include "ep_fastmath.h"
 #include "ep_internal.h"
 #include "ep_launch.h"
 
@@ -956,7 +957,8 @@ __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
 // first sweep takes the log and the last one writes exp back.  A lane
 // needs a handful of registers, so the kernel runs at full occupancy.
 // ------------------------------------------------------------------------
-template <typename T>
+// FM: ln f and exp through fm_log / fm_exp (tables in LDS).
+template <typename T, bool FM>
 __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   // blockIdx.x = ((stream, mu), chunk of 256 nodes), chunk fastest
   const int nch = (D.N + 255) >> 8;
@@ -968,7 +970,10 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   // the buffer each energy's slab sits in, read once per block (the slab
   // addresses are then known ahead and the loads below can be issued early)
   __shared__ uint8_t sloc[kMaxE];
+  __shared__ double fmt[FM ? kFmTable : 1];
   if (threadIdx.x < E) sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+  if (FM)
+    for (int q = threadIdx.x; q < kFmTable; q += blockDim.x) fmt[q] = D.fm_tab[q];
   __syncthreads();
   if (node < 1 || node >= n) return;
   const size_t i = (size_t)s * D.Npad + node;
@@ -1015,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
       }
       return (double)buf_ld(rs, voff, 0, T());
     };
-    auto cvt = [&](double x) { return first ? log(x) : x; };
+    auto cvt = [&](double x) { return first ? (FM ? fm_log(x, fmt) : log(x)) : x; };
     double r0 = cvt(ld(0));
     double r1 = cvt(ld(1));
     // read-ahead: kPf energies beyond the window are in flight
@@ -1054,7 +1059,7 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
       const double nv = upd ? cur - dtsub * f1 : cur;
       const rsrc_t rs = slab_of(e);
       if (fin) {
-        T val = (T)exp(nv);
+        T val = (T)(FM ? fm_exp(nv, fmt) : exp(nv));
         if (bad < 0 && !isfinite(val)) {
           bad = e;
           bad_inf = isinf(val) ? 1 : 0;
@@ -1075,106 +1080,6 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
 #pragma unroll
       for (int q = 0; q + 1 < kPf; ++q) pf[q] = pf[q + 1];
       pf[kPf - 1] = (e + 2 + kPf < E) ? ld(e + 2 + kPf) : 0.0;
-    }
-  }
-  if (bad >= 0) note_error(D, err_key(substep, 1, s, 1, node, bad, m, bad_inf));
-}
-
-// ------------------------------------------------------------------------
-// k_change_col: the same operator with the whole energy column of a lane's
-// (node, mu) cell in registers (EMAX >= E, a multiple of 8): all E loads
-// are issued at once (k_change keeps a few in flight), ln f stays in FP64
-// registers across the subcycles, and the column is written once.  Same
-// operations in the same order as k_change.
-// ------------------------------------------------------------------------
-template <typename T, int EMAX>
-__global__ __launch_bounds__(256) void k_change_col(Dev D, int substep) {
-  const int nch = (D.N + 255) >> 8;
-  const int sm = blockIdx.x / nch;
-  const int s = sm / D.M, m = sm % D.M;
-  const int node = (blockIdx.x % nch) * 256 + threadIdx.x;
-  const int n = D.n_active[s];
-  const int E = D.E;
-  __shared__ uint8_t sloc[kMaxE];
-  if (threadIdx.x < E) sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
-  __syncthreads();
-  if (node < 1 || node >= n) return;
-  const size_t i = (size_t)s * D.Npad + node;
-  const int nsub = D.nsub_c[i];
-  const double dtsub = D.dtsub_c[i];
-  const double DuParDt = D.c_dupar[i], DlnNDt = D.c_dlnn[i], DlnBDt = D.c_dlnb[i];
-  const double damp = D.c_damp[i], bc0 = D.c_bc0[i], bc1 = D.c_bc1[i];
-  const double dlnp_i = 1.0 / D.dlnp;
-  const double half = 0.5, one = 1.0;
-  const size_t slab = (size_t)D.M * D.Npad;
-  constexpr uint32_t B = sizeof(T);
-  const uint32_t slab_bytes = (uint32_t)(slab * B);
-  const uint32_t voff = (uint32_t)(((size_t)m * D.Npad + node) * B);
-  const double mu = D.mugrid[m];
-  const double a = -mu * DuParDt;
-  const double bb = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
-  const double *__restrict__ vgi = D.vgrid_i;
-  auto slab_of = [&](int e) {
-    const int task = s * E + e;
-    const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]);
-    return slab_rsrc((const double *)(fbufT<T>(D, lc) + (size_t)task * slab), slab_bytes);
-  };
-  double col[EMAX];
-#pragma unroll
-  for (int e = 0; e < EMAX; ++e)
-    if (e < E) col[e] = (double)buf_ld(slab_of(e), voff, 0, T());
-#pragma unroll
-  for (int e = 0; e < EMAX; ++e)
-    if (e < E) col[e] = log(col[e]);
-  for (int it = 0; it < nsub; ++it) {
-    double old_prev = 0.0;  // ln f[e-1] before this subcycle
-    double vel = damp * (a * vgi[0] + bb);
-    double vel_prev = 0.0;
-    double flux_e = 0.0;  // flux at interface e (between e-1 and e)
-#pragma unroll
-    for (int e = 0; e < EMAX; ++e) {
-      if (e < E) {
-        const double cur = col[e];
-        const double nxt = (e + 1 < E) ? col[e + 1 < EMAX ? e + 1 : e] : 0.0;
-        const double vel1 = (e + 1 < E) ? damp * (a * vgi[e + 1] + bb) : 0.0;
-        double f1;
-        if (e == 0) {
-          const double va1 = half * (vel1 + vel);
-          if (va1 <= 0)
-            f1 = va1 * (nxt - cur) * dlnp_i;
-          else
-            f1 = va1 * (cur - bc0) * dlnp_i;
-          flux_e = va1 * (signbit(va1) ? nxt : cur);  // interface 1
-        } else if (e == E - 1) {
-          const double vr = half * (vel + vel_prev);
-          if (vr >= 0)
-            f1 = vr * (cur - old_prev) * dlnp_i;
-          else
-            f1 = vr * (bc1 - cur) * dlnp_i;
-        } else {
-          const double va1 = half * (vel1 + vel);
-          const double flux_hi = va1 * (signbit(va1) ? nxt : cur);
-          f1 = (flux_hi - flux_e) * dlnp_i;
-          flux_e = flux_hi;
-        }
-        col[e] = cur - dtsub * f1;
-        old_prev = cur;
-        vel_prev = vel;
-        vel = vel1;
-      }
-    }
-  }
-  int bad = -1, bad_inf = 0;
-#pragma unroll
-  for (int e = 0; e < EMAX; ++e) {
-    if (e < E) {
-      T val = (T)exp(col[e]);
-      if (bad < 0 && !isfinite(val)) {
-        bad = e;
-        bad_inf = isinf(val) ? 1 : 0;
-      }
-      if (val < FloorT<T>::v) val = FloorT<T>::v;
-      buf_st(val, slab_of(e), voff, 0);
     }
   }
   if (bad >= 0) note_error(D, err_key(substep, 1, s, 1, node, bad, m, bad_inf));
@@ -1394,32 +1299,16 @@ hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int sp
   return hipSuccess;
 }
 
-template <int EMAX>
-static void change_col_t(const Dev &D, int sub, dim3 grid, dim3 block, hipStream_t st) {
-  if (D.fp32)
-    hipLaunchKernelGGL((k_change_col<float, EMAX>), grid, block, 0, st, D, sub);
-  else
-    hipLaunchKernelGGL((k_change_col<double, EMAX>), grid, block, 0, st, D, sub);
-}
-
+// AdiabaticChange over every (stream, mu, node); the fused precision
+// modes (fast, or float storage) use the table-driven exp/log.
 hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
   const dim3 grid((D.N + 255) / 256 * D.S * D.M), block(256);
-  if (D.change_col) {
-    switch ((D.E + 7) / 8) {
-      case 1: change_col_t<8>(D, sub, grid, block, st); break;
-      case 2: change_col_t<16>(D, sub, grid, block, st); break;
-      case 3: change_col_t<24>(D, sub, grid, block, st); break;
-      case 4: change_col_t<32>(D, sub, grid, block, st); break;
-      case 5: change_col_t<40>(D, sub, grid, block, st); break;
-      case 6: change_col_t<48>(D, sub, grid, block, st); break;
-      case 7: change_col_t<56>(D, sub, grid, block, st); break;
-      default: change_col_t<64>(D, sub, grid, block, st); break;
-    }
-  } else if (D.fp32) {
-    hipLaunchKernelGGL(k_change<float>, grid, block, 0, st, D, sub);
-  } else {
-    hipLaunchKernelGGL(k_change<double>, grid, block, 0, st, D, sub);
-  }
+  if (D.fp32)
+    hipLaunchKernelGGL((k_change<float, true>), grid, block, 0, st, D, sub);
+  else if (D.fast)
+    hipLaunchKernelGGL((k_change<double, true>), grid, block, 0, st, D, sub);
+  else
+    hipLaunchKernelGGL((k_change<double, false>), grid, block, 0, st, D, sub);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
@@ -1496,6 +1385,24 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2 *src, double2 
 hipError_t launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, st, (const double2 *)src, (double2 *)dst,
                      bytes / 16);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+__global__ __launch_bounds__(256) void k_fastmath_probe(const double *tab, const double *x, int64_t n,
+                                                        double *lg, double *ex) {
+  __shared__ double t[kFmTable];
+  for (int i = threadIdx.x; i < kFmTable; i += blockDim.x) t[i] = tab[i];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  lg[i] = fm_log(x[i], t);
+  ex[i] = fm_exp(x[i], t);
+}
+
+hipError_t launch_fastmath_probe(const double *tab, const double *x, int64_t n, double *lg, double *ex,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(k_fastmath_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tab, x, n, lg, ex);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }

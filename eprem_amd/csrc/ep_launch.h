@@ -29,4 +29,6 @@ hipError_t launch_init_seed(const Dev &D, hipStream_t st);
 hipError_t launch_dpp_probe(const double *in, double *out, double *out2, hipStream_t st);
 hipError_t launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_reset_diag(DevDiag *d, hipStream_t st);
+hipError_t launch_fastmath_probe(const double *tab, const double *x, int64_t n, double *lg, double *ex,
+                                 hipStream_t st);
 }  // namespace ep

@@ -34,7 +34,7 @@ struct State {
   ep_params p{};
   int Npad = 0;
   hipStream_t st = nullptr;
-  int stream_block = 256, stream_halo = 32, stream_r = 1, stream_fuse = 1, change_col = 1;
+  int stream_block = 256, stream_halo = 32, stream_r = 1, stream_fuse = 1;
   // graph of the substep loop; `generation` changes whenever kernel
   // arguments captured by value (parameters, tiling) change
   bool use_graph = true;
@@ -56,6 +56,7 @@ struct State {
   // device buffers
   double *egrid_d = nullptr, *vgrid_d = nullptr, *vgrid_i_d = nullptr, *rig_lamo_d = nullptr,
          *plt_d = nullptr, *ext_d = nullptr, *mugrid_d = nullptr;
+  double *fm_tab_d = nullptr;  // exp/log tables of the fused precision modes
   double *fA = nullptr, *fB = nullptr;
   uint8_t *loc = nullptr;
   double *nx = nullptr, *ny = nullptr, *nz = nullptr, *nrmag = nullptr, *nbmag = nullptr,

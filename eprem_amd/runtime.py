@@ -301,6 +301,20 @@ def selftest_dpp():
     return x, a, b
 
 
+def fastmath(x, device=False):
+    """(log(x), exp(x)) through the table-driven functions of the fused
+    precision modes, on the host or on the device (device needs an
+    initialised EpSolver)."""
+    xs = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    lg, ex = np.empty_like(xs), np.empty_like(xs)
+    L = lib()
+    fn = L.ep_selftest_fastmath if device else L.ep_fastmath_host
+    DPt = C.POINTER(C.c_double)
+    fn.argtypes = [DPt, C.c_int64, DPt, DPt]
+    _check(fn(dptr(xs), len(xs), dptr(lg), dptr(ex)))
+    return lg, ex
+
+
 def copy_probe(nbytes=1 << 30, reps=20):
     """Mean time (ms) and bandwidth (GB/s, read + write) of a plain device
     copy of nbytes bytes."""

@@ -392,6 +392,14 @@ int ep_selftest_dpp(const double *in64, double *shr64, double *shl64);
  * reference point for HBM byte counters and bandwidth. */
 int ep_selftest_copy(size_t bytes, int32_t reps, double *ms);
 
+/* The table-driven exp and log of the fused precision modes (Tang's
+ * methods; the reference-order mode uses the device libm): n values of
+ * log(x) and exp(x), computed on the host (ep_fastmath_host, usable
+ * without a GPU) or on the device (ep_selftest_fastmath, after ep_init).
+ * The two give identical bits.  No counterpart in the reference. */
+int ep_fastmath_host(const double *x, int64_t n, double *log_out, double *exp_out);
+int ep_selftest_fastmath(const double *x, int64_t n, double *log_out, double *exp_out);
+
 #ifdef __cplusplus
 }
 #endif
