@@ -35,9 +35,9 @@ STREAM_FLOPS_PER_SWEEP = 8   # upwind 4 + iso sum 1 + relaxation 3 (FP64)
 # FETCH_SIZE and WRITE_SIZE (each scaled by what the counter reads for a
 # plain 1 GiB copy kernel: FETCH x 2.000, WRITE x 1.000), keyed by
 # (workload, block, halo, nodes per lane, fused focusing, precision mode); the commands and per-kernel
-# figures are in profiles/k_stream_pass_c2_hbm_counters.md.  Other tilings
+# figures are in profiles/k_stream_pass2_c2_counters.md.  Other tilings
 # report null.
-STREAM_TRAFFIC = {("c2", 256, 16, 1, 0, 2): 6.262e9}
+STREAM_TRAFFIC = {("c2", 512, 48, 2, 0, 2): 8.328e9}
 
 CONFIGS = {
     # rows, cols, N, E, M, numEpSteps

@@ -53,13 +53,7 @@ __host__ __device__ inline double fm_exp(double x, const double *t) {
   const double p = fma(r * r, q, r);  // exp(r) - 1
   const double th = t[kFmExpHi + j], tl = t[kFmExpLo + j];
   const double s = th + fma(th, p, tl);
-  double y;
-  if (m > 1000)
-    y = ldexp(ldexp(s, m - 1000), 1000);
-  else if (m < -1000)
-    y = ldexp(ldexp(s, m + 1000), -1000);
-  else
-    y = ldexp(s, m);
+  double y = ldexp(s, m);  // exact scaling, or correctly rounded into the subnormals / inf
   if (x > 709.8) y = INFINITY;
   if (x < -745.2) y = 0.0;
   return y;

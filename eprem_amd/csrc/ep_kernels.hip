@@ -1004,11 +1004,11 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   const uint32_t voff8 = (uint32_t)(((size_t)m * D.Npad + node) * 8);
   auto spare_of = [&](int e) { return slab_rsrc(spare_slab(D, s * E + e), (uint32_t)(slab * 8)); };
   int bad = -1, bad_inf = 0;
-  for (int it = 0; it < niter; ++it) {
-    // first/fin differ between lanes (per-node subcycle counts): the slab
-    // descriptors are formed outside any branch on them, so that they stay
-    // in scalar registers
-    const bool first = (it == 0), fin = (it == niter - 1);
+  // One sweep up the column; first: take ln f of the stored values, fin:
+  // write exp back (both per lane: per-node subcycle counts differ).  The
+  // slab descriptors are formed outside any branch on them, so that they
+  // stay in scalar registers.
+  auto sweep = [&](const bool first, const bool fin) __attribute__((always_inline)) {
     // the column is advanced in FP64 whatever the storage type
     auto ld = [&](int e) {
       const rsrc_t rs = slab_of(e);
@@ -1081,6 +1081,11 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
       for (int q = 0; q + 1 < kPf; ++q) pf[q] = pf[q + 1];
       pf[kPf - 1] = (e + 2 + kPf < E) ? ld(e + 2 + kPf) : 0.0;
     }
+  };
+  if (__all(niter == 1)) {
+    sweep(true, true);  // the usual case: one sweep per substep everywhere in the wave
+  } else {
+    for (int it = 0; it < niter; ++it) sweep(it == 0, it == niter - 1);
   }
   if (bad >= 0) note_error(D, err_key(substep, 1, s, 1, node, bad, m, bad_inf));
 }
