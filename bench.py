@@ -29,6 +29,7 @@ from eprem_amd import config, constants as K, mas_synth, runtime, workloads  # n
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (FMA = 2 flops)
+BF16_PEAK_TFLOPS = 2500.0    # MI355X dense bf16 MFMA
 STREAM_FLOPS_PER_SWEEP = 8   # upwind 4 + iso sum 1 + relaxation 3 (FP64)
 
 # HBM bytes per k_stream_pass launch, measured with rocprofv3 --pmc
@@ -93,7 +94,10 @@ def run_surrogate(args, c, p, dims):
     pred = net.forward(x[cut:])
     rows = 1 << 20
     ms = net.bench(rows, reps=5)
+    ms_layerwise = net.bench(rows, reps=5, layerwise=True)
     net.close()
+    sizes = [3 * c["M"] + 6, 128, 128, 128, c["M"]]
+    flops_row = 2.0 * sum(a * b for a, b in zip(sizes[:-1], sizes[1:]))
     e_net = surrogate.flux_error(pred, y[cut:])
     e_base = surrogate.flux_error(surrogate.relaxation_step(x[cut:], c["M"]), y[cut:])
     sur_rate = rows / (ms * 1e-3) * c["M"] * Ks
@@ -106,6 +110,11 @@ def run_surrogate(args, c, p, dims):
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16 MFMA, fp32 accumulate", "data": "synthetic",
         "config": {"workload": "c5: c2 shapes, MLP %d-128-128-128-%d replacing K=%d streaming subcycles"
                                % (3 * c["M"] + 6, c["M"], Ks), "pairs": int(len(x)), "train_s": train_s},
+        "network": {"kernel": "k_mlp_fwd (all layers in one kernel, weights in LDS)", "rows": rows,
+                    "ms": ms, "tflops": rows * flops_row / (ms * 1e-3) / 1e12,
+                    "mfma_frac": rows * flops_row / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+                    "layerwise_ms": ms_layerwise,
+                    "layerwise_tflops": rows * flops_row / (ms_layerwise * 1e-3) / 1e12},
         "plain_streaming": {"value": plain_rate, "unit": "cell-subcycles/s", "kernel": "k_stream_pass"},
         "surrogate_over_plain": sur_rate / plain_rate,
         "flux_error_heldout": {"network": surrogate.summary(e_net),

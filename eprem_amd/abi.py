@@ -207,6 +207,7 @@ EXPORTED_SYMBOLS = (
     "ep_update_mhd",
     "ep_autotune",
     "ep_set_graph",
+    "ep_set_node_fusion",
     "ep_spawn_shell",
     "ep_move_nodes",
     "ep_get_positions",

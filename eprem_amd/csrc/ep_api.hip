@@ -344,9 +344,17 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
       EPQ(launch_stream_finish(D, sub, st));
     }
     if (e4) EPQ(hipEventRecord(e4[0], st));
-    if (sh.do_focus) EPQ(launch_focus(D, sub, fuse, st));
-    if (e4) EPQ(hipEventRecord(e4[1], st));
-    if (sh.do_change) EPQ(launch_change(D, sub, st));
+    if (sh.node_fused) {
+      // heavy nodes are focused first; every node is then focused (unless
+      // done already) and changed in one pass over f
+      EPQ(launch_focus_heavy(D, sub, st));
+      if (e4) EPQ(hipEventRecord(e4[1], st));
+      EPQ(launch_focus_change(D, sub, fuse, 0, st));
+    } else {
+      if (sh.do_focus) EPQ(launch_focus(D, sub, fuse, st));
+      if (e4) EPQ(hipEventRecord(e4[1], st));
+      if (sh.do_change) EPQ(launch_change(D, sub, st));
+    }
     // DiffuseShellData, then DriftShellData, after the stream loop
     if (sh.do_shell & 1) EPQ(launch_shell_diffuse(D, sub, st));
     if (sh.do_shell & 2) EPQ(launch_shell_drift(D, sub, st));
@@ -464,6 +472,7 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_node
   DevDiag hd;
   HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
   HIPCHK(launch_stream_prep(D, st), "stream_prep");
+  HIPCHK(launch_node_prep(D, st), "node_prep");
   HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
   HIPCHK(hipStreamSynchronize(st), "prep sync");
   if (hd.err_key != ~0ull) {
@@ -472,12 +481,16 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_node
     int code = decode_err(hd.err_key, &dg);
     return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
   }
+  // the node stage as ep_step would run it: focusing alone, or focusing and
+  // change in one pass
+  const int tune_node_fused = (g.node_fuse && hd.max_sub_change <= 1) ? 1 : 0;
   struct Cand {
     int W, H, R, F;
     double ms;
   };
-  // time the passes of one substep and the focusing kernel they leave to
-  // do (both write only the spare buffer), best of two
+  // time the passes of one substep and the node stage after them (the
+  // focusing they leave to do, with the change when the two run as one
+  // kernel; all write only the spare buffer), best of two
   auto time_cand = [&](Cand &c) -> int {
     g.stream_block = c.W;
     g.stream_halo = c.H;
@@ -493,7 +506,10 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_node
       HIPCHK(hipEventRecord(g.events[0], st), "event");
       for (int pass = 0; pass < hd.max_passes; pass++)
         HIPCHK(launch_stream_pass(D, pass, 0, 0, 0, hd.max_chunks, c.F, st), "stream_pass");
-      HIPCHK(launch_focus(D, 0, c.F, st, 1), "focus");
+      if (tune_node_fused)
+        HIPCHK(launch_focus_change(D, 0, c.F, 1, st), "focus_change");
+      else
+        HIPCHK(launch_focus(D, 0, c.F, st, 1), "focus");
       HIPCHK(hipEventRecord(g.events[1], st), "event");
       HIPCHK(hipEventSynchronize(g.events[1]), "tune sync");
       float ms = 0;
@@ -569,6 +585,13 @@ int ep_set_precision(int32_t mode) {
 
 int ep_set_graph(int32_t enable) {
   g.use_graph = enable != 0;
+  g.drop_graph();
+  return EP_OK;
+}
+
+int ep_set_node_fusion(int32_t fused) {
+  if (fused != 0 && fused != 1) return fail(EP_ERR_INVALID_ARG, "node fusion must be 0 or 1");
+  g.node_fuse = fused;
   g.drop_graph();
   return EP_OK;
 }
@@ -688,8 +711,9 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
       if (n != g.d.num_nodes) return fail(EP_ERR_UNSUPPORTED, "shell operators need every stream full");
     HIPCHK(launch_shell_links(D, st), "shell links");
   }
+  const int node_fused = (g.node_fuse && do_focus && do_change && hd.max_sub_change <= 1) ? 1 : 0;
   const StepShape shape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
-                        do_shell | (do_drift << 1), g.generation};
+                        do_shell | (do_drift << 1), g.generation, node_fused};
   if (g.use_graph) {
     // The substep loop is one HIP graph, rebuilt only when its shape (pass
     // count, grid sizes, operators) or the parameters change.

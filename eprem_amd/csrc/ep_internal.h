@@ -153,6 +153,19 @@ __device__ inline double *spare_slab(const Dev &D, int task) {
   return ((task & 1) ? D.fB : D.fA) + off + (size_t)(task >> 1) * slab;
 }
 
+// The reference's interior interface flux v 0.5 ((1 - c) f[up] + (1 + c)
+// f[other]) with c = copysign(1, v) (src/energeticParticles.c:1830-1836,
+// :1103-1110) equals v * f[up] for finite f, but a non-finite value in the
+// other cell reaches it too (0 * Inf and 0 * NaN are NaN).  ep_poison(other,
+// u) is u exactly when `other` is finite (up to the sign of a zero flux,
+// which the update cannot see) and NaN otherwise, so that the first
+// non-finite cell is the reference's.  The wall rows (mu = 0, M-1;
+// E = 0, E-1) use the upwind cell alone, as in the reference.
+template <typename T>
+__device__ __forceinline__ T ep_poison(T other, T u) {
+  return fma(other, (T)0, u);
+}
+
 // Smallest normal value of T: the floor the reference applies as DBL_MIN.
 template <typename T>
 struct FloorT;

@@ -416,7 +416,12 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
 // upwind flux subcycles with zero-flux walls, then the NaN/Inf check
 // (first bad mu) -- written out in each kernel so that the row and the
 // velocities stay in registers.
-#define EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf)                  \
+// POISON: interior rows take the reference's NaN-propagating flux
+// (ep_poison).  Without it the update is the same for finite values; the
+// standalone focusing kernels run that form and redo a row with POISON
+// only when it ends non-finite (a non-finite value never turns finite
+// again in these updates), since only then can the two differ.
+#define EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, POISON)          \
   do {                                                                             \
     T va_[(M) + 1];                                                                \
     double vprev_ = 0.0;                                                           \
@@ -431,11 +436,17 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
     for (int it_ = 0; it_ < (nsub); ++it_) {                                       \
       T flux_lo = 0; /* zero-flux wall at mu = -1 */                               \
       _Pragma("unroll") for (int m = 0; m < (M); ++m) {                            \
-        T flux_hi = 0; /* zero-flux wall at mu = +1 */                             \
-        if (m + 1 < (M)) flux_hi = va_[m + 1] * (signbit(va_[m + 1]) ? f[m + 1] : f[m]); \
+        T flux_hi = 0, next_lo = 0; /* zero-flux wall at mu = +1 */                \
+        if (m + 1 < (M)) {                                                         \
+          const bool dn_ = signbit(va_[m + 1]);                                    \
+          const T u_ = va_[m + 1] * (dn_ ? f[m + 1] : f[m]);                        \
+          const T p_ = (POISON) ? ep_poison(dn_ ? f[m] : f[m + 1], u_) : u_;       \
+          flux_hi = (m == 0) ? u_ : p_;         /* wall row 0: upwind cell only */ \
+          next_lo = (m + 2 == (M)) ? u_ : p_;   /* wall row M-1 likewise */        \
+        }                                                                          \
         const T dfl_ = (flux_hi - flux_lo) * dmu_i_;                                  \
         f[m] = f[m] - dts_ * dfl_;                                                 \
-        flux_lo = flux_hi;                                                         \
+        flux_lo = next_lo;                                                         \
       }                                                                            \
     }                                                                              \
     _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
@@ -604,7 +615,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
           const double a = -D.vgrid[e] * D.f_dlnbds[i] - (2.0 / D.vgrid[e]) * D.f_dupar[i];
           const double b2 = D.f_b[i];
           int fbad = -1, finf = 0;
-          EP_FOCUS_ROW(T, M, D, a, b2, f, nf, D.dtsub_f[i], fbad, finf);
+          EP_FOCUS_ROW(T, M, D, a, b2, f, nf, D.dtsub_f[i], fbad, finf, true);
 #pragma unroll
           for (int m = 0; m < M; ++m)
             if (f[m] < FloorT<T>::v) f[m] = FloorT<T>::v;
@@ -813,8 +824,8 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
       if (h0) nf0 = 0;
       if (h1) nf1 = 0;
       int fb0 = -1, fi0 = 0, fb1 = -1, fi1 = 0;
-      EP_FOCUS_ROW(T, M, D, a0, b0, f0, nf0, dt0, fb0, fi0);
-      EP_FOCUS_ROW(T, M, D, a1, b1, f1, nf1, dt1, fb1, fi1);
+      EP_FOCUS_ROW(T, M, D, a0, b0, f0, nf0, dt0, fb0, fi0, true);
+      EP_FOCUS_ROW(T, M, D, a1, b1, f1, nf1, dt1, fb1, fi1, true);
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         if (f0[m] < FloorT<T>::v) f0[m] = FloorT<T>::v;
@@ -908,7 +919,13 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
 #pragma unroll
   for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row, T());
   int bad = -1, bad_inf = 0;
-  EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf);
+  EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, false);
+  if (bad >= 0) {  // redo from the stored row with the reference's NaN propagation
+#pragma unroll
+    for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row, T());
+    bad = -1;
+    EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, true);
+  }
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const T val = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
@@ -940,7 +957,13 @@ __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
 #pragma unroll
     for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
     int bad = -1, bad_inf = 0;
-    EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf);
+    EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, false);
+    if (bad >= 0) {  // redo with the reference's NaN propagation
+#pragma unroll
+      for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
+      bad = -1;
+      EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, true);
+    }
 #pragma unroll
     for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
     if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
@@ -1043,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
           f1 = va1 * (nxt - cur) * dlnp_i;
         else
           f1 = va1 * (cur - bc0) * dlnp_i;
-        flux_e = va1 * (signbit(va1) ? nxt : cur);  // interface 1
+        flux_e = ep_poison(signbit(va1) ? cur : nxt, va1 * (signbit(va1) ? nxt : cur));  // interface 1
       } else if (e == E - 1) {
         const double vr = half * (vel + vel_prev);
         if (vr >= 0)
@@ -1052,7 +1075,7 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
           f1 = vr * (bc1 - cur) * dlnp_i;
       } else {
         const double va1 = half * (vel1 + vel);
-        const double flux_hi = va1 * (signbit(va1) ? nxt : cur);
+        const double flux_hi = ep_poison(signbit(va1) ? cur : nxt, va1 * (signbit(va1) ? nxt : cur));
         f1 = (flux_hi - flux_e) * dlnp_i;
         flux_e = flux_hi;
       }
@@ -1297,6 +1320,30 @@ hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int sp
     EP_FOCUS_CASE(22) EP_FOCUS_CASE(23) EP_FOCUS_CASE(24) EP_FOCUS_CASE(25) EP_FOCUS_CASE(26)
     EP_FOCUS_CASE(27) EP_FOCUS_CASE(28) EP_FOCUS_CASE(29) EP_FOCUS_CASE(30) EP_FOCUS_CASE(31)
     EP_FOCUS_CASE(32)
+    default:
+      return hipErrorInvalidValue;
+  }
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+#define EP_HEAVY_CASE(MM)                                                                     \
+  case MM:                                                                                    \
+    if (D.fp32)                                                                               \
+      hipLaunchKernelGGL((k_focus_heavy<MM, float>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_focus_heavy<MM, double>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+    break;
+
+hipError_t launch_focus_heavy(const Dev &D, int sub, hipStream_t st) {
+  switch (D.M) {
+    EP_HEAVY_CASE(2) EP_HEAVY_CASE(3) EP_HEAVY_CASE(4) EP_HEAVY_CASE(5) EP_HEAVY_CASE(6)
+    EP_HEAVY_CASE(7) EP_HEAVY_CASE(8) EP_HEAVY_CASE(9) EP_HEAVY_CASE(10) EP_HEAVY_CASE(11)
+    EP_HEAVY_CASE(12) EP_HEAVY_CASE(13) EP_HEAVY_CASE(14) EP_HEAVY_CASE(15) EP_HEAVY_CASE(16)
+    EP_HEAVY_CASE(17) EP_HEAVY_CASE(18) EP_HEAVY_CASE(19) EP_HEAVY_CASE(20) EP_HEAVY_CASE(21)
+    EP_HEAVY_CASE(22) EP_HEAVY_CASE(23) EP_HEAVY_CASE(24) EP_HEAVY_CASE(25) EP_HEAVY_CASE(26)
+    EP_HEAVY_CASE(27) EP_HEAVY_CASE(28) EP_HEAVY_CASE(29) EP_HEAVY_CASE(30) EP_HEAVY_CASE(31)
+    EP_HEAVY_CASE(32)
     default:
       return hipErrorInvalidValue;
   }

@@ -16,6 +16,11 @@ hipError_t launch_stream_finish(const Dev &D, int substep, hipStream_t st);
 // spare: write the spare buffer and skip the heavy nodes (tile-size search)
 hipError_t launch_focus(const Dev &D, int substep, int fused, hipStream_t st, int spare = 0);
 hipError_t launch_change(const Dev &D, int substep, hipStream_t st);
+// focusing of the heavy nodes only (nsub_f >= kHeavyFocus), in place
+hipError_t launch_focus_heavy(const Dev &D, int substep, hipStream_t st);
+// focusing then change of every node in one pass (ep_node.hip); heavy
+// nodes are only changed (launch_focus_heavy first)
+hipError_t launch_focus_change(const Dev &D, int substep, int fused, int spare, hipStream_t st);
 hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st);
 hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st);
 hipError_t launch_normalize(const Dev &D, hipStream_t st);

@@ -10,8 +10,18 @@
 //
 // Parameters are one flat fp32 vector: for each layer W [in][out]
 // (row-major) then b [out].
+//
+// Inference of the surrogate's shape (input <= 128, hidden layers of 128,
+// output <= 32) runs in one kernel, k_mlp_fwd: every layer's weights sit in
+// LDS as bf16 MFMA fragments (v_mfma_f32_32x32x16_bf16), and the network
+// is evaluated transposed, C^T = W^T X^T, so that each layer's accumulators
+// (lane = sample, registers = output features) are already the next
+// layer's B operand once bias, ReLU and the bf16 rounding are applied: the
+// activations never leave registers.  The permutation of the inner index
+// this implies is folded into how the weights are packed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -156,6 +166,143 @@ __global__ __launch_bounds__(256) void k_gather_rows(const float *src, const int
   }
 }
 
+// ---- fused forward -------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+constexpr int kFH = 128;   // hidden width of the fused kernel
+constexpr int kFNT = 2;    // 32-sample tiles per wave (one A fragment feeds two MFMAs)
+constexpr int kFWaves = 4; // waves per block (one per SIMD; all share the LDS weights)
+
+// Fragment (ks, ot) of a layer: 64 lanes x 8 bf16.  Lane (i, h) = (l % 32,
+// l / 32) holds A[i][8h + q] = W[phi(ks, h, q)][32 ot + i], q = 0..7, where
+// phi maps the MFMA's inner index to the layer's input feature:
+//   first layer (input rows from memory): phi = 16 ks + 8 h + q;
+//   later layers (input = previous accumulators, whose lane-half h holds
+//   features 8(r/4) + 4h + r%4 of each 32-feature tile in register r):
+//   phi = 16 ks + 8 (q/4) + 4 h + q%4.
+// Inputs past `in` and outputs past `out` are zero.
+__global__ __launch_bounds__(256) void k_mlp_pack(const float *W, int in, int out, int KS, int OT, int first,
+                                                  __bf16 *dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = KS * OT * 64 * 8;
+  if (t >= total) return;
+  const int q = t & 7, lane = (t >> 3) & 63, f = t >> 9;
+  const int ks = f / OT, ot = f % OT;
+  const int i = lane & 31, h = lane >> 5;
+  const int k = first ? 16 * ks + 8 * h + q : 16 * ks + 8 * (q >> 2) + 4 * h + (q & 3);
+  const int j = 32 * ot + i;
+  const float v = (k < in && j < out) ? W[(size_t)k * out + j] : 0.0f;
+  dst[t] = (__bf16)v;
+}
+
+// One layer on kFNT sample tiles: acc[t][ot] = sum_ks A(ks, ot) B[t][ks].
+template <int KS, int OT>
+__device__ __forceinline__ void mlp_layer(const bf16x8_t *__restrict__ frag, const bf16x8_t (&b)[kFNT][KS],
+                                          f32x16_t (&acc)[kFNT][OT], int lane) {
+#pragma unroll
+  for (int t = 0; t < kFNT; ++t)
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[t][ot][r] = 0.0f;
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot) {
+      const bf16x8_t a = frag[(ks * OT + ot) * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < kFNT; ++t) acc[t][ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[t][ks], acc[t][ot], 0, 0, 0);
+    }
+}
+
+// Bias, ReLU and bf16 rounding of a hidden layer's accumulators into the
+// next layer's B operands: K-step 2 ot + u takes registers 8u..8u+7 of
+// tile ot (features 32 ot + 16 u + 8 (q/4) + 4 h + q%4).
+__device__ __forceinline__ void mlp_relu_to_b(const f32x16_t (&acc)[kFNT][kFH / 32], const float *bias, int h,
+                                              bf16x8_t (&b)[kFNT][kFH / 16]) {
+#pragma unroll
+  for (int t = 0; t < kFNT; ++t)
+#pragma unroll
+    for (int ot = 0; ot < kFH / 32; ++ot)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        bf16x8_t v;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int r = 8 * u + q;
+          float z = acc[t][ot][r] + bias[32 * ot + 8 * (r >> 2) + 4 * h + (r & 3)];
+          z = z > 0.0f ? z : 0.0f;
+          v[q] = (__bf16)z;
+        }
+        b[t][2 * ot + u] = v;
+      }
+}
+
+// Y[n][out] = network(X[n][in]); a persistent grid of kFWaves-wave blocks,
+// each wave taking kFNT x 32 rows at a time.  LDS: the packed fragments of
+// every layer, then the biases (nhid x 128 + 32 floats).
+template <int KS0>
+__global__ __launch_bounds__(64 * kFWaves) void k_mlp_fwd(const float *__restrict__ X, float *__restrict__ Y,
+                                                          int n, int in, int out, int nhid,
+                                                          const uint4 *__restrict__ packed, int packed16,
+                                                          int bias16) {
+  extern __shared__ uint4 lds_w[];
+  for (int t = threadIdx.x; t < packed16; t += blockDim.x) lds_w[t] = packed[t];
+  __syncthreads();
+  const float *lbias = (const float *)(lds_w + bias16);
+  const bf16x8_t *frag0 = (const bf16x8_t *)lds_w;
+  const int lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5;
+  const int wave = blockIdx.x * kFWaves + (threadIdx.x >> 6);
+  const int nwave = gridDim.x * kFWaves;
+  const bool vec = (in & 3) == 0;
+  for (int base = wave * 32 * kFNT; base < n; base += nwave * 32 * kFNT) {
+    bf16x8_t b0[kFNT][KS0];
+#pragma unroll
+    for (int t = 0; t < kFNT; ++t) {
+      const int row = base + 32 * t + col;
+      const float *xr = X + (size_t)row * in;
+#pragma unroll
+      for (int ks = 0; ks < KS0; ++ks) {
+        const int k0 = 16 * ks + 8 * h;
+        float v[8];
+        if (row < n && vec && k0 + 8 <= in) {
+          const float4 lo = *(const float4 *)(xr + k0), hi = *(const float4 *)(xr + k0 + 4);
+          v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
+          v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = (row < n && k0 + q < in) ? xr[k0 + q] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) b0[t][ks][q] = (__bf16)v[q];
+      }
+    }
+    f32x16_t acc[kFNT][kFH / 32];
+    mlp_layer<KS0, kFH / 32>(frag0, b0, acc, lane);
+    bf16x8_t b[kFNT][kFH / 16];
+    mlp_relu_to_b(acc, lbias, h, b);
+    const bf16x8_t *fr = frag0 + KS0 * (kFH / 32) * 64;
+    for (int l = 1; l < nhid; ++l) {
+      mlp_layer<kFH / 16, kFH / 32>(fr, b, acc, lane);
+      fr += (kFH / 16) * (kFH / 32) * 64;
+      mlp_relu_to_b(acc, lbias + l * kFH, h, b);
+    }
+    f32x16_t o[kFNT][1];
+    mlp_layer<kFH / 16, 1>(fr, b, o, lane);
+    const float *ob = lbias + nhid * kFH;
+#pragma unroll
+    for (int t = 0; t < kFNT; ++t) {
+      const int row = base + 32 * t + col;
+      if (row >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int j = 8 * (r >> 2) + 4 * h + (r & 3);
+        if (j < out) Y[(size_t)row * out + j] = o[t][0][r] + ob[j];
+      }
+    }
+  }
+}
+
 struct Mlp {
   bool init = false;
   std::vector<int> sizes;       // layer widths, input first
@@ -169,6 +316,18 @@ struct Mlp {
   std::vector<float *> dact;    // gradients w.r.t. act[l]
   float *ybuf = nullptr;
   double *loss = nullptr;
+  // fused forward: packed bf16 fragments + biases, rebuilt when the
+  // parameters change
+  uint4 *packed = nullptr;
+  int packed16 = 0, bias16 = 0;
+  bool packed_ok = false;
+  bool fusable() const {
+    const int L = (int)sizes.size();
+    if (L < 3 || L > 6 || sizes[0] > 128 || sizes[L - 1] > 32) return false;
+    for (int l = 1; l < L - 1; l++)
+      if (sizes[l] != kFH) return false;
+    return true;
+  }
   void release() {
     for (float *p : act) (void)hipFree(p);
     for (float *p : dact) (void)hipFree(p);
@@ -186,6 +345,10 @@ struct Mlp {
     param = grad = adam_m = adam_v = nullptr;
     if (loss) (void)hipFree(loss);
     loss = nullptr;
+    if (packed) (void)hipFree(packed);
+    packed = nullptr;
+    packed16 = bias16 = 0;
+    packed_ok = false;
     init = false;
   }
 };
@@ -219,6 +382,78 @@ static hipError_t gemm(const float *A, const float *B, float *C, const float *bi
   const dim3 grid((M + 31) / 32, (N + 31) / 32, (K + chunk - 1) / chunk);
   hipLaunchKernelGGL(k_gemm, grid, dim3(64), 0, st, g);
   return hipGetLastError();
+}
+
+// Packs the parameters for k_mlp_fwd (once per parameter change).
+static hipError_t mlp_pack(hipStream_t st) {
+  const int L = (int)mlp.sizes.size();
+  const int in = mlp.sizes[0], nhid = L - 2;
+  const int KS0 = (in + 15) / 16;
+  const int frags = KS0 * (kFH / 32) + (nhid - 1) * (kFH / 16) * (kFH / 32) + (kFH / 16);
+  const int bias_floats = nhid * kFH + 32;
+  const int w16 = frags * 64;  // 16-byte units
+  const int total16 = w16 + bias_floats / 4;
+  hipError_t e = hipSuccess;
+  if (!mlp.packed) {  // sized for this network; ep_mlp_init frees it
+    e = hipMalloc(&mlp.packed, (size_t)total16 * 16);
+    if (e != hipSuccess) return e;
+  }
+  __bf16 *dst = (__bf16 *)mlp.packed;
+  float *bdst = (float *)(mlp.packed + w16);
+  for (int l = 1; l < L; l++) {
+    const int lin = mlp.sizes[l - 1], lout = mlp.sizes[l];
+    const int KS = (l == 1) ? KS0 : kFH / 16;
+    const int OT = (l == L - 1) ? 1 : kFH / 32;
+    const int cnt = KS * OT * 64 * 8;
+    hipLaunchKernelGGL(k_mlp_pack, dim3((cnt + 255) / 256), dim3(256), 0, st, mlp.param + mlp.woff[l - 1], lin,
+                       lout, KS, OT, l == 1 ? 1 : 0, dst);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dst += cnt;
+    e = hipMemcpyAsync(bdst + (l - 1) * kFH, mlp.param + mlp.boff[l - 1], lout * sizeof(float),
+                       hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  mlp.packed16 = total16;
+  mlp.bias16 = w16;
+  mlp.packed_ok = true;
+  return hipSuccess;
+}
+
+template <int KS0>
+static hipError_t fwd_fused_t(const float *X, float *Y, int n, hipStream_t st) {
+  const int L = (int)mlp.sizes.size();
+  const size_t lds = (size_t)mlp.packed16 * 16;  // <= 99 KB of the 160 KB per CU
+  hipError_t e = hipFuncSetAttribute((const void *)k_mlp_fwd<KS0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int rows_per_block = kFWaves * 32 * kFNT;
+  const int blocks = std::max(1, std::min(cus, (n + rows_per_block - 1) / rows_per_block));
+  hipLaunchKernelGGL(k_mlp_fwd<KS0>, dim3(blocks), dim3(64 * kFWaves), lds, st, X, Y, n, mlp.sizes[0],
+                     mlp.sizes[L - 1], L - 2, (const uint4 *)mlp.packed, mlp.packed16, mlp.bias16);
+  return hipGetLastError();
+}
+
+// Forward through the fused kernel: X [n][in] -> Y [n][out], device memory.
+static hipError_t forward_fused(const float *X, float *Y, int n, hipStream_t st) {
+  if (!mlp.packed_ok) {
+    hipError_t e = mlp_pack(st);
+    if (e != hipSuccess) return e;
+  }
+  switch ((mlp.sizes[0] + 15) / 16) {
+    case 1: return fwd_fused_t<1>(X, Y, n, st);
+    case 2: return fwd_fused_t<2>(X, Y, n, st);
+    case 3: return fwd_fused_t<3>(X, Y, n, st);
+    case 4: return fwd_fused_t<4>(X, Y, n, st);
+    case 5: return fwd_fused_t<5>(X, Y, n, st);
+    case 6: return fwd_fused_t<6>(X, Y, n, st);
+    case 7: return fwd_fused_t<7>(X, Y, n, st);
+    case 8: return fwd_fused_t<8>(X, Y, n, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // Forward pass over n rows already in act[0].
@@ -334,6 +569,7 @@ int ep_mlp_get_params(float *p) {
 int ep_mlp_set_params(const float *p) {
   if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
   hipError_t e = hipMemcpy(mlp.param, p, mlp.nparam * sizeof(float), hipMemcpyHostToDevice);
+  mlp.packed_ok = false;
   if (e != hipSuccess) return hip_fail(e, "MLP params upload");
   return EP_OK;
 }
@@ -345,7 +581,7 @@ int ep_mlp_forward(const float *X, int32_t n, float *Y) {
   if (rc) return rc;
   const int L = (int)mlp.sizes.size();
   hipError_t e = hipMemcpy(mlp.act[0], X, (size_t)n * mlp.sizes[0] * sizeof(float), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = forward(n, 0);
+  if (e == hipSuccess) e = mlp.fusable() ? forward_fused(mlp.act[0], mlp.act[L - 1], n, 0) : forward(n, 0);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess)
     e = hipMemcpy(Y, mlp.act[L - 1], (size_t)n * mlp.sizes[L - 1] * sizeof(float), hipMemcpyDeviceToHost);
@@ -413,6 +649,7 @@ int ep_mlp_train(const float *X, const float *Y, int32_t n, int32_t epochs, int3
       e = forward(batch, 0);
       if (e == hipSuccess) e = backward(batch, 0);
       mlp.step++;
+      mlp.packed_ok = false;
       const float c1 = 1.0f - powf(b1, (float)mlp.step), c2 = 1.0f - powf(b2, (float)mlp.step);
       hipLaunchKernelGGL(k_adam, dim3(256), dim3(256), 0, 0, mlp.param, mlp.grad, mlp.adam_m, mlp.adam_v,
                          mlp.nparam, lr, b1, b2, c1, c2, eps);
@@ -429,10 +666,11 @@ int ep_mlp_train(const float *X, const float *Y, int32_t n, int32_t epochs, int3
   return EP_OK;
 }
 
-// Forward pass over n rows already resident on the device (whatever the
-// work buffer holds), reps times; *ms = mean time per pass.  Measures the
-// network alone, without host transfers.
-int ep_mlp_bench(int32_t n, int32_t reps, double *ms) {
+// Forward pass over n rows already resident on the device (zeros), reps
+// times; *ms = mean time per pass.  Measures the network alone, without
+// host transfers: the fused kernel when the network has its shape, or
+// (layerwise = 1) the one-product-per-layer kernels.
+int ep_mlp_bench(int32_t n, int32_t reps, int32_t layerwise, double *ms) {
   if (!mlp.init) return fail(EP_ERR_NOT_INIT, "ep_mlp_init first");
   if (n < 1 || reps < 1) return fail(EP_ERR_INVALID_ARG, "bad rows");
   int rc = mlp_reserve(n);
@@ -441,9 +679,13 @@ int ep_mlp_bench(int32_t n, int32_t reps, double *ms) {
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, 0, mlp.act[0], (size_t)n * mlp.sizes[0]);
-  hipError_t e = forward(n, 0);
+  const int L = (int)mlp.sizes.size();
+  auto fwd = [&]() {
+    return (mlp.fusable() && !layerwise) ? forward_fused(mlp.act[0], mlp.act[L - 1], n, 0) : forward(n, 0);
+  };
+  hipError_t e = fwd();
   if (e == hipSuccess) e = hipEventRecord(e0, 0);
-  for (int r = 0; r < reps && e == hipSuccess; r++) e = forward(n, 0);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = fwd();
   if (e == hipSuccess) e = hipEventRecord(e1, 0);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float t = 0;

@@ -1,0 +1,313 @@
+// ep_node.hip -- AdiabaticFocusing and AdiabaticChange of a node in one
+// pass over f.
+//
+// The reference applies both operators node by node: for each stream,
+// AdiabaticFocusing (src/energeticParticles.c:1533-1803, upwind operator
+// :1806-1896) and then AdiabaticChange (:805-1077, upwind operator
+// :1081-1176), each reading and writing the node's energy x mu tile.  Both
+// are local to a node, so this kernel runs them back to back on each node
+// while its tile streams through registers: f is read once and written once
+// per substep instead of twice each.
+//
+// Layout: a lane pair per node.  Lanes l and l+32 of a wave hold the lower
+// and upper halves of the mu row (MH = ceil(M/2) cells each) of node
+// (l mod 32); a 256-lane block covers 128 consecutive nodes of one stream.
+// Energies are walked upwards.  For each energy the kernel loads the row,
+// runs the node's focusing subcycles (the one interface between the halves
+// is exchanged across the lane pair each subcycle), takes ln f, and then
+// finishes the adiabatic change of the energy below it: with one change
+// subcycle (the usual case) the new ln f at e needs the focused ln f at
+// e-1, e and e+1 and the flux through the interface below e, which is all a
+// lane keeps between energies (ln f and flux of the previous energy,
+// 2 * MH doubles).
+//
+// Arithmetic is operation for operation the one of k_focus and k_change
+// (same association, same casts), so the results are bit-identical to the
+// two-kernel form.  The kernel is used when every node of the step needs at
+// most one change subcycle; nodes with kHeavyFocus or more focusing
+// subcycles are focused beforehand by k_focus_heavy and only changed here.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <type_traits>
+
+#include "ep_fastmath.h"
+#include "ep_internal.h"
+#include "ep_launch.h"
+
+namespace ep {
+
+template <typename T>
+__device__ __forceinline__ T pair_swap(T v) {
+  return __shfl_xor(v, 32, 64);
+}
+
+// FM: ln f and exp through fm_log / fm_exp (tables in LDS).
+// fused: focusing of full-resolution streams was done by their last
+// streaming pass.  spare: read the current buffers and write the other one
+// (tile-size search; f and the step state are left as they were).
+template <int M, typename T, bool FM>
+__global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fused, int spare) {
+  constexpr int MH = (M + 1) / 2;
+  // blockIdx.x = (stream, chunk of 128 nodes), chunk fastest; the chunk
+  // order is rotated by stream so that the inner chunks (most focusing
+  // subcycles) spread over the XCDs instead of landing on the same one
+  const int nch = (D.N + 127) >> 7;
+  const int s = blockIdx.x / nch;
+  const int chunk = (int)((blockIdx.x % nch + (unsigned)s) % nch);
+  const int lane = threadIdx.x & 63;
+  const int hf = lane >> 5;
+  const int node = chunk * 128 + (threadIdx.x >> 6) * 32 + (lane & 31);
+  const int E = D.E;
+  __shared__ uint8_t sloc[kMaxE];
+  // per mu (padded to 2 MH + 1 with zeros): mu, mu^2, 0.5 (1 - mu^2)
+  __shared__ double smu[2 * MH + 2], smu2[2 * MH + 2], shc[2 * MH + 2];
+  __shared__ double fmt[FM ? kFmTable : 1];
+  if (threadIdx.x < E) sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+  if (threadIdx.x < 2 * MH + 2) {
+    const int m = threadIdx.x;
+    const double mu = (m < M) ? D.mugrid[m] : 0.0;
+    smu[m] = mu;
+    smu2[m] = mu * mu;
+    shc[m] = 0.5 * (1.0 - mu * mu);
+  }
+  if (FM)
+    for (int q = threadIdx.x; q < kFmTable; q += blockDim.x) fmt[q] = D.fm_tab[q];
+  __syncthreads();
+  const int n = D.n_active[s];
+  if (node < 1 || node >= n) return;  // both lanes of a pair leave together
+  const size_t i = (size_t)s * D.Npad + node;
+  const int m0 = hf * MH;
+  // focusing coefficients (k_node_prep)
+  const int nsub_f = D.nsub_f[i];
+  const bool foc = !(fused && !D.thin[s]) && nsub_f < kHeavyFocus;
+  const double dtsub_f = D.dtsub_f[i];
+  const double dlnBds = D.f_dlnbds[i], fDuPar = D.f_dupar[i], fb = D.f_b[i];
+  // change coefficients
+  const int nsub_c = D.nsub_c[i];
+  const double dtsub_c = D.dtsub_c[i];
+  const double cDuPar = D.c_dupar[i], DlnNDt = D.c_dlnn[i], DlnBDt = D.c_dlnb[i];
+  const double damp = D.c_damp[i], bc0 = D.c_bc0[i], bc1 = D.c_bc1[i];
+  const double nmb = DlnNDt - DlnBDt;
+  const bool upd = nsub_c > 0;
+  const double dlnp_i = 1.0 / D.dlnp;
+  const double half = 0.5, two = 2.0;
+  const double *__restrict__ vgi = D.vgrid_i;
+  constexpr uint32_t B = sizeof(T);
+  const uint32_t slab_bytes = (uint32_t)((size_t)M * D.Npad * B);
+  const uint32_t row = (uint32_t)D.Npad * B;
+  const uint32_t voff = (uint32_t)(((size_t)m0 * D.Npad + node) * B);
+  auto slab_of = [&](int e, int flip) {
+    const int task = s * E + e;
+    const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]) ^ flip;
+    return slab_rsrc((const double *)(fbufT<T>(D, lc) + (size_t)task * M * D.Npad), slab_bytes);
+  };
+  // slot j of this lane is mu cell m0 + j; the upper half's last slot is
+  // padding when M is odd
+  auto live = [&](int j) { return (M % 2 == 0) || (j + 1 < MH) || (hf == 0); };
+  // velocity of the change in ln p at (energy e, slot j), as in k_change:
+  // damp * (a * (1/v) + bb), a = -mu DuPar/dt, bb = mu^2 (DlnN - DlnB)/dt
+  // + 0.5 (1 - mu^2) DlnB/dt
+  auto cvel = [&](int e, int j) {
+    const int m = m0 + j;
+    const double a = -smu[m] * cDuPar;
+    const double bb = smu2[m] * nmb + shc[m] * DlnBDt;
+    return damp * (a * vgi[e] + bb);
+  };
+  double lprev[MH], flux[MH];  // ln f at e-1 after focusing; flux below e-1
+  int fbad_e = -1, fbad_m = 0, fbad_inf = 0;
+  int cbad_e = -1, cbad_m = 0, cbad_inf = 0;
+  // z + 0 * x[j] over the live slots: NaN iff one of them is NaN or Inf
+  // (one operation per cell, no branch)
+  auto nonfinite = [&](const T *x) {
+    T z = 0;
+#pragma unroll
+    for (int j = 0; j < MH; ++j) z = live(j) ? fma(x[j], (T)0, z) : z;
+    return !isfinite(z);
+  };
+  // exp back, NaN/Inf check, floor and store of one energy's row; the
+  // floor is fmax, equal to the reference's `f < DBL_MIN ? DBL_MIN : f` for
+  // every non-NaN value (a NaN has been reported by then)
+  auto put_row = [&](int e, const double *nv, const rsrc_t &out) {
+    T val[MH];
+#pragma unroll
+    for (int j = 0; j < MH; ++j) val[j] = (T)(FM ? fm_exp(nv[j], fmt) : exp(nv[j]));
+    if (nonfinite(val) && cbad_e < 0) {
+      for (int j = 0; j < MH; ++j)
+        if (live(j) && !isfinite(val[j])) {
+          cbad_e = e;
+          cbad_m = m0 + j;
+          cbad_inf = isinf(val[j]) ? 1 : 0;
+          break;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MH; ++j)
+      if (live(j)) buf_st(fmax(val[j], FloorT<T>::v), out, voff, j * row);
+  };
+  for (int e = 0; e < E; ++e) {
+    const rsrc_t in = slab_of(e, 0);
+    T x[MH];
+#pragma unroll
+    for (int j = 0; j < MH; ++j) x[j] = live(j) ? buf_ld(in, voff, j * row, T()) : T(1);
+    if (foc) {
+      // ---- AdiabaticFocusing of (node, e): half-mesh velocities vb[j] at
+      // the interface below slot j (vb[MH]: above the last slot)
+      const double v = D.vgrid[e];
+      const double a = -v * dlnBds - (two / v) * fDuPar;
+      T vb[MH + 1];
+      double vp = (m0 > 0) ? shc[m0 - 1] * (a + smu[m0 - 1] * fb) : 0.0;
+#pragma unroll
+      for (int j = 0; j <= MH; ++j) {
+        const int m = m0 + j;
+        const double vel = shc[m] * (a + smu[m] * fb);
+        vb[j] = (T)(half * (vel + vp));
+        vp = vel;
+      }
+      const T dmu_i = (T)(1.0 / D.dmu);
+      const T dts = (T)dtsub_f;
+      // the nsub_f subcycles of the row; P: interior rows take the
+      // reference's NaN-propagating flux (ep_poison), the wall rows (mu
+      // cells 0 and M-1) the upwind one
+      auto focus_row = [&](auto poison) __attribute__((always_inline)) {
+        constexpr bool P = decltype(poison)::value;
+        for (int it = 0; it < nsub_f; ++it) {
+          // the flux through mu interface MH-1 | MH, from both halves' cells
+          const T recv = pair_swap(hf ? x[0] : x[MH - 1]);
+          const T flo = hf ? recv : x[MH - 1];
+          const T fhi = hf ? x[0] : recv;
+          const T vmid = hf ? vb[0] : vb[MH];
+          const T umid = vmid * (signbit(vmid) ? fhi : flo);
+          const T pmid = P ? ep_poison(signbit(vmid) ? flo : fhi, umid) : umid;
+          // lower half: row MH-1 is a wall row only when MH = 1; upper
+          // half: row MH is one only when MH = M-1
+          T flux_lo = hf ? ((MH == M - 1) ? umid : pmid) : (T)0;  // zero-flux wall at mu = -1
+#pragma unroll
+          for (int j = 0; j < MH; ++j) {
+            T flux_hi, next_lo;
+            if (j + 1 < MH) {
+              const int m = m0 + j;
+              const bool dn = signbit(vb[j + 1]);
+              const T u = vb[j + 1] * (dn ? x[j + 1] : x[j]);
+              const T pz = P ? ep_poison(dn ? x[j] : x[j + 1], u) : u;
+              const bool in = m + 1 < M;
+              flux_hi = in ? ((m == 0) ? u : pz) : (T)0;  // zero-flux wall at mu = +1
+              next_lo = (m + 2 == M) ? u : pz;
+            } else {
+              flux_hi = hf ? (T)0 : ((MH == 1) ? umid : pmid);
+              next_lo = (T)0;
+            }
+            const T dfl = (flux_hi - flux_lo) * dmu_i;
+            x[j] = x[j] - dts * dfl;
+            flux_lo = next_lo;
+          }
+        }
+      };
+      focus_row(std::false_type());
+      // The two forms agree while every value stays finite, and a value
+      // that turns non-finite stays so: only a row that ends non-finite is
+      // redone, from its stored values (not yet overwritten: this pass
+      // writes energy e-1 after loading e), by both lanes of the pair.
+      int bad_row = nonfinite(x) ? 1 : 0;
+      if (bad_row | pair_swap(bad_row)) {
+#pragma unroll
+        for (int j = 0; j < MH; ++j) x[j] = live(j) ? buf_ld(in, voff, j * row, T()) : T(1);
+        focus_row(std::true_type());
+        bad_row = nonfinite(x) ? 1 : 0;
+      }
+      if (bad_row && fbad_e < 0) {
+        for (int j = 0; j < MH; ++j)
+          if (live(j) && !isfinite(x[j])) {
+            fbad_e = e;
+            fbad_m = m0 + j;
+            fbad_inf = isinf(x[j]) ? 1 : 0;
+            break;
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < MH; ++j) x[j] = fmax(x[j], FloorT<T>::v);
+    }
+    // ---- AdiabaticChange, one sweep: finish energy e-1 (and e = E-1)
+    double lx[MH];
+#pragma unroll
+    for (int j = 0; j < MH; ++j) lx[j] = FM ? fm_log((double)x[j], fmt) : log((double)x[j]);
+    if (e >= 1) {
+      const int ep = e - 1;
+      const rsrc_t out = slab_of(ep, spare);
+      double nv[MH];
+#pragma unroll
+      for (int j = 0; j < MH; ++j) {
+        const double cur = lprev[j], nxt = lx[j];
+        const double va1 = half * (cvel(e, j) + cvel(ep, j));
+        double f1;
+        if (ep == 0) {
+          if (va1 <= 0)
+            f1 = va1 * (nxt - cur) * dlnp_i;
+          else
+            f1 = va1 * (cur - bc0) * dlnp_i;
+          flux[j] = ep_poison(signbit(va1) ? cur : nxt, va1 * (signbit(va1) ? nxt : cur));
+        } else {
+          const double flux_hi = ep_poison(signbit(va1) ? cur : nxt, va1 * (signbit(va1) ? nxt : cur));
+          f1 = (flux_hi - flux[j]) * dlnp_i;
+          flux[j] = flux_hi;
+        }
+        nv[j] = upd ? cur - dtsub_c * f1 : cur;
+      }
+      put_row(ep, nv, out);
+      if (e == E - 1) {
+        const rsrc_t out1 = slab_of(e, spare);
+#pragma unroll
+        for (int j = 0; j < MH; ++j) {
+          const double cur = lx[j], old_prev = lprev[j];
+          const double vr = half * (cvel(e, j) + cvel(ep, j));
+          double f1;
+          if (vr >= 0)
+            f1 = vr * (cur - old_prev) * dlnp_i;
+          else
+            f1 = vr * (bc1 - cur) * dlnp_i;
+          nv[j] = upd ? cur - dtsub_c * f1 : cur;
+        }
+        put_row(e, nv, out1);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MH; ++j) lprev[j] = lx[j];
+  }
+  if (!spare) {
+    if (fbad_e >= 0) note_error(D, err_key(substep, 1, s, 0, node, fbad_e, fbad_m, fbad_inf));
+    if (cbad_e >= 0) note_error(D, err_key(substep, 1, s, 1, node, cbad_e, cbad_m, cbad_inf));
+  }
+}
+
+template <int M>
+static void focus_change_t(const Dev &D, int sub, int fused, int spare, hipStream_t st) {
+  const dim3 grid((D.N + 127) / 128 * D.S), block(256);
+  if (D.fp32)
+    hipLaunchKernelGGL((k_focus_change<M, float, true>), grid, block, 0, st, D, sub, fused, spare);
+  else if (D.fast)
+    hipLaunchKernelGGL((k_focus_change<M, double, true>), grid, block, 0, st, D, sub, fused, spare);
+  else
+    hipLaunchKernelGGL((k_focus_change<M, double, false>), grid, block, 0, st, D, sub, fused, spare);
+}
+
+#define EP_FC_CASE(MM)                         \
+  case MM:                                     \
+    focus_change_t<MM>(D, sub, fused, spare, st); \
+    break;
+
+hipError_t launch_focus_change(const Dev &D, int sub, int fused, int spare, hipStream_t st) {
+  switch (D.M) {
+    EP_FC_CASE(2) EP_FC_CASE(3) EP_FC_CASE(4) EP_FC_CASE(5) EP_FC_CASE(6)
+    EP_FC_CASE(7) EP_FC_CASE(8) EP_FC_CASE(9) EP_FC_CASE(10) EP_FC_CASE(11)
+    EP_FC_CASE(12) EP_FC_CASE(13) EP_FC_CASE(14) EP_FC_CASE(15) EP_FC_CASE(16)
+    EP_FC_CASE(17) EP_FC_CASE(18) EP_FC_CASE(19) EP_FC_CASE(20) EP_FC_CASE(21)
+    EP_FC_CASE(22) EP_FC_CASE(23) EP_FC_CASE(24) EP_FC_CASE(25) EP_FC_CASE(26)
+    EP_FC_CASE(27) EP_FC_CASE(28) EP_FC_CASE(29) EP_FC_CASE(30) EP_FC_CASE(31)
+    EP_FC_CASE(32)
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ep

@@ -21,10 +21,11 @@ struct MasLevel {
 struct StepShape {
   int passes, chunks, nsub, do_stream, do_focus, do_change, do_seed, do_shell;
   unsigned generation;
+  int node_fused;  // focusing and change in one kernel (k_focus_change)
   bool operator==(const StepShape &o) const {
     return passes == o.passes && chunks == o.chunks && nsub == o.nsub && do_stream == o.do_stream &&
            do_focus == o.do_focus && do_change == o.do_change && do_seed == o.do_seed &&
-           do_shell == o.do_shell && generation == o.generation;
+           do_shell == o.do_shell && generation == o.generation && node_fused == o.node_fused;
   }
 };
 
@@ -35,6 +36,7 @@ struct State {
   int Npad = 0;
   hipStream_t st = nullptr;
   int stream_block = 256, stream_halo = 32, stream_r = 1, stream_fuse = 1;
+  int node_fuse = 1;  // allow k_focus_change when every node needs <= 1 change subcycle
   // graph of the substep loop; `generation` changes whenever kernel
   // arguments captured by value (parameters, tiling) change
   bool use_graph = true;

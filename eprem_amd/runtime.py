@@ -61,6 +61,7 @@ def lib():
         L.ep_move_nodes.argtypes = [C.c_double]
         L.ep_get_positions.argtypes = [DP, DP, DP, DP, IP]
         L.ep_set_graph.argtypes = [C.c_int32]
+        L.ep_set_node_fusion.argtypes = [C.c_int32]
         L.ep_set_precision.argtypes = [C.c_int32]
         L.ep_selftest_copy.argtypes = [C.c_size_t, C.c_int32, DP]
         FP = C.POINTER(C.c_float)
@@ -72,7 +73,7 @@ def lib():
         L.ep_mlp_loss_grad.argtypes = [FP, FP, C.c_int32, DP, FP]
         L.ep_mlp_train.argtypes = [FP, FP, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64, FP]
         L.ep_mlp_finalize.argtypes = []
-        L.ep_mlp_bench.argtypes = [C.c_int32, C.c_int32, DP]
+        L.ep_mlp_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, DP]
         L.ep_surrogate_samples.argtypes = [C.c_int32, C.c_int32, C.c_uint64, FP, FP, IP]
         L.ep_shell_operators.argtypes = [IP, C.c_double, C.c_int32]
         for name in abi.EXPORTED_SYMBOLS:
@@ -137,6 +138,12 @@ class EpSolver:
 
     def set_graph(self, enable):
         _check(lib().ep_set_graph(1 if enable else 0))
+
+    def set_node_fusion(self, fused):
+        """1: focusing and adiabatic change of each node in one pass over f
+        whenever every node needs at most one change subcycle (default);
+        0: always the two separate kernels.  Results are bit-identical."""
+        _check(lib().ep_set_node_fusion(1 if fused else 0))
 
     def set_params(self, params):
         self.params = params
@@ -374,10 +381,11 @@ class Mlp:
                                   fptr(hist)))
         return hist
 
-    def bench(self, n, reps=5):
-        """Mean ms of one forward pass over n device-resident rows."""
+    def bench(self, n, reps=5, layerwise=False):
+        """Mean ms of one forward pass over n device-resident rows (the fused
+        kernel, or with layerwise=True one product kernel per layer)."""
         ms = C.c_double()
-        _check(lib().ep_mlp_bench(int(n), int(reps), C.byref(ms)))
+        _check(lib().ep_mlp_bench(int(n), int(reps), 1 if layerwise else 0, C.byref(ms)))
         return ms.value
 
     def close(self):

@@ -329,8 +329,11 @@ int ep_mlp_loss_grad(const float *x, const float *y, int32_t n, double *loss, fl
 /* Adam minibatch training; loss_per_epoch[epochs] may be NULL */
 int ep_mlp_train(const float *x, const float *y, int32_t n, int32_t epochs, int32_t batch, float lr,
                  uint64_t seed, float *loss_per_epoch);
-/* forward pass over n device-resident rows, reps times; *ms = mean time */
-int ep_mlp_bench(int32_t n, int32_t reps, double *ms);
+/* forward pass over n device-resident rows, reps times; *ms = mean time.
+ * layerwise = 0: the fused one-kernel forward when the network has its
+ * shape (input <= 128, hidden layers of 128, output <= 32); 1: one product
+ * kernel per layer */
+int ep_mlp_bench(int32_t n, int32_t reps, int32_t layerwise, double *ms);
 int ep_mlp_finalize(void);
 
 /* Training pairs for the large-step surrogate of streaming: n random kept
@@ -358,9 +361,10 @@ int ep_finalize(void);
 int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes, int32_t fuse_focus);
 
 /* Tile-size search for the streaming kernel on the current nodes: times
- * every streaming pass of one substep and the focusing work it leaves for
- * each candidate tiling without changing f or the step state, and sets the
- * fastest.  Any output pointer may be NULL.  No counterpart in the
+ * every streaming pass of one substep and the node stage after it (the
+ * focusing the passes leave to do, with the adiabatic change when the two
+ * run as one kernel) for each candidate tiling without changing f or the
+ * step state, and sets the fastest.  Any output pointer may be NULL.  No counterpart in the
  * reference. */
 int ep_autotune(int32_t *stream_block, int32_t *stream_halo, int32_t *lane_nodes, int32_t *fuse_focus,
                 double *stream_ms);
@@ -370,6 +374,14 @@ int ep_autotune(int32_t *stream_block, int32_t *stream_halo, int32_t *lane_nodes
  * individual launches with per-operator timing in ep_diag (enable = 0).
  * No counterpart in the reference. */
 int ep_set_graph(int32_t enable);
+
+/* Node-stage kernels: fused = 1 (default) runs AdiabaticFocusing and
+ * AdiabaticChange of each node in one pass over f (k_focus_change) in
+ * every step whose nodes all need at most one change subcycle; fused = 0
+ * always runs them as two kernels.  Results are bit-identical either way.
+ * No counterpart in the reference (its two operators loop over the nodes of
+ * a stream on the CPU, src/energeticParticles.c:163-177). */
+int ep_set_node_fusion(int32_t fused);
 
 /* Precision of the hot path: 0 = FP64 in the reference's operation order
  * (default; the parity path), 1 = mixed: f stored as float with streaming,
