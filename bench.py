@@ -36,9 +36,10 @@ STREAM_FLOPS_PER_SWEEP = 8   # upwind 4 + iso sum 1 + relaxation 3 (FP64)
 # FETCH_SIZE and WRITE_SIZE (each scaled by what the counter reads for a
 # plain 1 GiB copy kernel: FETCH x 2.000, WRITE x 1.000), keyed by
 # (workload, block, halo, nodes per lane, fused focusing, precision mode); the commands and per-kernel
-# figures are in profiles/k_stream_pass2_c2_counters.md.  Other tilings
+# figures are in profiles/k_stream_pass2_c2_counters.md (two nodes per lane)
+# and profiles/k_stream_passR_c2_kernel_trace.md (three).  Other tilings
 # report null.
-STREAM_TRAFFIC = {("c2", 512, 48, 2, 0, 2): 8.328e9}
+STREAM_TRAFFIC = {("c2", 512, 48, 2, 0, 2): 8.328e9, ("c2", 768, 48, 3, 0, 2): 8.038e9}
 
 CONFIGS = {
     # rows, cols, N, E, M, numEpSteps
@@ -251,7 +252,7 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--halo", type=int, default=0)
-    ap.add_argument("--lane-nodes", type=int, default=1, choices=(1, 2))
+    ap.add_argument("--lane-nodes", type=int, default=1, choices=(1, 2, 3))
     ap.add_argument("--fuse-focus", type=int, default=1, choices=(0, 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mas", action="store_true")
@@ -327,7 +328,8 @@ def main():
     wall = t1 - t0
     value = acc["cell_updates"] / wall
 
-    # dominant kernel: the streaming passes (k_stream_pass)
+    # dominant kernel: the streaming passes
+    kname = {1: "k_stream_pass", 2: "k_stream_pass2", 3: "k_stream_passR<R=3>"}[tiling["lane_nodes"]]
     st_s = acc["stream_ms"] / 1e3
     launches = max(1, acc["stream_launches"])
     alg_bytes_per_launch = 16.0 * acc["stream_sweeps"] / launches
@@ -353,17 +355,20 @@ def main():
                    "path": "plain FP64 (no surrogate)",
                    "arithmetic": ("FP64, reference operation order" if args.precision == 0 else
                                   "FP64, fused multiply-add streaming (within 1e-13 of the reference order)")},
-        "roofline": {"bound": "hbm", "kernel": "k_stream_pass", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": STREAM_TRAFFIC.get((args.config, tiling["block"], tiling["halo"],
                                                     tiling["lane_nodes"], tiling["fuse_focus"],
                                                     args.precision)),
                      "note": "16 B per cell per Euler sweep; the kernel keeps up to `halo` sweeps "
                              "on chip, so the binding ceiling is FP64 VALU (see fp64)"},
-        "fp64": {"kernel": "k_stream_pass", "achieved_tflops": fp64, "peak_tflops": FP64_PEAK_TFLOPS,
+        "fp64": {"kernel": kname, "achieved_tflops": fp64, "peak_tflops": FP64_PEAK_TFLOPS,
                  "frac": fp64 / FP64_PEAK_TFLOPS},
         "breakdown_ms_per_step": {k: acc[k] / args.steps for k in
                                   ("device_ms", "prep_ms", "stream_ms", "focus_ms", "change_ms")},
+        "node_stage": ("focus_ms: heavy-node focusing (k_focus_heavy); change_ms: focusing and adiabatic "
+                       "change in one pass (k_focus_change)") if dg["max_subcycles_change"] <= 1 else
+                      "focus_ms: k_focus + k_focus_heavy; change_ms: k_change",
         "sweeps_per_step": {k: acc[k] // args.steps for k in ("stream_sweeps", "focus_sweeps",
                                                               "change_sweeps")},
         "interp_ms": mas_ms,

@@ -436,7 +436,7 @@ int ep_set_params(const ep_params *params) {
 // A streaming tile is stream_block kept nodes (the chunk, halo included),
 // carried by stream_block / lane_nodes lanes.
 static bool tiling_ok(int W, int H, int R) {
-  if (R != 1 && R != 2) return false;
+  if (R < 1 || R > 3) return false;
   if (W % (64 * R) || W < 64 * R || W / R > 512) return false;
   return H >= 1 && 2 * H < W;
 }
@@ -444,7 +444,7 @@ static bool tiling_ok(int W, int H, int R) {
 int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes, int32_t fuse_focus) {
   if (!tiling_ok(stream_block, stream_halo, lane_nodes) || (fuse_focus != 0 && fuse_focus != 1))
     return fail(EP_ERR_INVALID_ARG,
-                "stream tiling: lane_nodes 1 or 2, stream_block a multiple of 64*lane_nodes with at most "
+                "stream tiling: lane_nodes 1, 2 or 3, stream_block a multiple of 64*lane_nodes with at most "
                 "512 lanes, 1 <= stream_halo < stream_block/2, fuse_focus 0 or 1");
   g.stream_block = stream_block;
   g.stream_halo = stream_halo;
@@ -523,7 +523,7 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_node
   static const int lanes[] = {128, 192, 256, 320, 384, 512};
   static const int halos[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
   std::vector<Cand> cands;
-  for (int R = 1; R <= 2; R++)
+  for (int R = 1; R <= 3; R++)
     for (int Lw : lanes)
       for (int H : halos) {
         Cand c{Lw * R, H, R, 0, 0.0};

@@ -843,6 +843,205 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
 }
 
 // ------------------------------------------------------------------------
+// k_stream_passR: the same pass with R consecutive kept nodes per lane
+// (instantiated for R = 3), a block of L lanes carrying a chunk of R L
+// nodes.  Against two nodes per lane, one DPP move serves three cells
+// instead of two and the chunk is wider at the same halo, for fewer
+// registers of headroom (occupancy 2 instead of 3 at M = 30 in FP64); the
+// tile-size search decides.  Same arithmetic per cell as k_stream_pass2.
+// ------------------------------------------------------------------------
+template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
+__global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip, int do_seed, int substep) {
+  const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
+  const int task = s * D.E + e;
+  const int nsteps = D.nsteps[task];
+  const int h = D.halo[task];
+  const int done = pass * h;
+  if (done >= nsteps) return;
+  const int nsub = min(h, nsteps - done);
+  const bool last = (done + nsub == nsteps);
+  const int nk = D.nkept[s];
+  const int L = blockDim.x, W = R * L;
+  const int I = W - 2 * h;
+  if (c * I >= nk) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = L >> 6;
+  const size_t on = (size_t)s * D.Npad;
+  const bool thin = D.thin[s] != 0;
+  const int cur = D.loc[task] ^ flip;
+  constexpr uint32_t B = sizeof(T);
+  const uint32_t slab_bytes = (uint32_t)(M * D.Npad * B), row = (uint32_t)D.Npad * B;
+  const rsrc_t src = slab_rsrc((const double *)(fbufT<T>(D, cur) + (size_t)task * M * D.Npad), slab_bytes);
+  const rsrc_t dst = slab_rsrc((const double *)(fbufT<T>(D, cur ^ 1) + (size_t)task * M * D.Npad), slab_bytes);
+  int jj_[R], kk[R], nd[R];
+  bool vv[R];
+  uint32_t vo[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    jj_[r] = R * tid + r;  // position in the chunk
+    kk[r] = c * I - h + jj_[r];
+    vv[r] = (kk[r] >= 0) && (kk[r] < nk);
+    nd[r] = vv[r] ? (thin ? D.list[on + kk[r]] : kk[r]) : 0;
+    vo[r] = (uint32_t)nd[r] * B;
+  }
+  T f[R][M];
+  T sd[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) sd[r] = (do_seed && vv[r]) ? (T)seed_value(D, D.radial[on + nd[r]], e) : (T)0;
+#pragma unroll
+  for (int m = 0; m < M; ++m)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      T a = 0;
+      if (vv[r]) a = buf_ld(src, vo[r], m * row, T());
+      if (do_seed && a < sd[r]) a = sd[r];
+      f[r][m] = a;
+    }
+  const size_t ok = (size_t)task * D.Npad;
+  double pos[R], neg[R];
+  T q[R], omq[R], bcv[R];
+  bool bc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const double dsm = vv[r] ? D.dsm[ok + kk[r]] : 0.0;
+    q[r] = vv[r] ? (T)D.q[ok + kk[r]] : (T)1;
+    omq[r] = (T)(vv[r] ? 1.0 - D.q[ok + kk[r]] : 0.0);
+    // stream ends are held: kept node 0 for mu >= 0, the last one for mu < 0
+    pos[r] = (kk[r] == 0) ? 0.0 : dsm;
+    neg[r] = (kk[r] == nk - 1) ? 0.0 : dsm;
+    bc[r] = D.useBC && vv[r] && (kk[r] < 3);
+    bcv[r] = bc[r] ? (T)D.seed3[(size_t)task * 3 + (bc[r] ? kk[r] : 0)] : (T)0;
+  }
+  const T invM = (T)(1.0 / M);
+  const double *__restrict__ del = D.del_fac + (size_t)task * M;
+  // edge rows between the waves of the block, double-buffered by subcycle
+  // parity: up[r] = the mu >= 0 values the first node of lane 0 of wave r
+  // takes from its left (last node of lane 63 of wave r-1; wave 0 lane 0
+  // writes its own first node), dn[r] = the mu < 0 values the last node of
+  // lane 63 of wave r-1 takes from its right (first node of lane 0 of wave
+  // r; the last wave's lane 63 writes its own last node)
+  constexpr int NP = M - MPOS;
+  __shared__ T up[2][9][NP > 0 ? NP : 1];
+  __shared__ T dn[2][9][MPOS > 0 ? MPOS : 1];
+  for (int jj = 0; jj < nsub; ++jj) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (bc[r]) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) f[r][m] = bcv[r];
+      }
+    const int b = jj & 1;
+    if (lane == 63) {
+#pragma unroll
+      for (int m = MPOS; m < M; ++m) up[b][w + 1][m - MPOS] = f[R - 1][m];
+    }
+    if (w == 0 && lane == 0) {
+#pragma unroll
+      for (int m = MPOS; m < M; ++m) up[b][0][m - MPOS] = f[0][m];
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int m = 0; m < MPOS; ++m) dn[b][w][m] = f[0][m];
+    }
+    if (w == nw - 1 && lane == 63) {
+#pragma unroll
+      for (int m = 0; m < MPOS; ++m) dn[b][nw][m] = f[R - 1][m];
+    }
+    __syncthreads();
+    T sum[R];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      T a[R], cf[R], nn[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        a[r] = f[r][m];
+        cf[r] = (T)(((m >= MPOS) ? pos[r] : neg[r]) * del[m]);
+      }
+      if (m >= MPOS) {
+        const T nb = dpp_shr1(a[R - 1], up[b][w][m - MPOS]);  // last node of the lane to the left
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const T left = (r == 0) ? nb : a[r > 0 ? r - 1 : 0];
+          nn[r] = FAST ? fma(cf[r], left - a[r], a[r]) : a[r] + cf[r] * (left - a[r]);
+        }
+      } else {
+        const T nb = dpp_shl1(a[0], dn[b][w + 1][m]);  // first node of the lane to the right
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const T right = (r == R - 1) ? nb : a[r < R - 1 ? r + 1 : 0];
+          nn[r] = FAST ? fma(cf[r], a[r] - right, a[r]) : a[r] + cf[r] * (a[r] - right);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        f[r][m] = nn[r];
+        sum[r] = (m == 0) ? nn[r] : sum[r] + nn[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const T iso = invM * sum[r];
+      if (FAST) {
+        const T rr = omq[r] * iso;
+#pragma unroll
+        for (int m = 0; m < M; ++m) f[r][m] = fma(q[r], f[r][m], rr);
+      } else {
+#pragma unroll
+        for (int m = 0; m < M; ++m) f[r][m] = iso + (f[r][m] - iso) * q[r];
+      }
+    }
+  }
+
+  bool st[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) st[r] = vv[r] && jj_[r] >= h && jj_[r] < h + I;
+  if (last) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      int bad = -1, inf = 0;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (bad < 0 && !isfinite(f[r][m])) {
+          bad = m;
+          inf = isinf(f[r][m]) ? 1 : 0;
+        }
+        if (f[r][m] < FloorT<T>::v) f[r][m] = FloorT<T>::v;
+      }
+      if (st[r] && bad >= 0) note_error(D, err_key(substep, 0, s, 1, e, bad, nd[r], inf));
+    }
+    // AdiabaticFocusing of the lane's nodes, as in k_stream_pass2
+    if (FUSE && !thin) {
+      const double v = D.vgrid[e], two_v = 2.0 / v;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        int nf = 0;
+        double a = 0, bq = 0, dt = 0;
+        const bool fz = st[r] && nd[r] >= 1;
+        if (fz) {
+          const size_t i = on + nd[r];
+          nf = D.nsub_f[i];
+          a = -v * D.f_dlnbds[i] - two_v * D.f_dupar[i];
+          bq = D.f_b[i];
+          dt = D.dtsub_f[i];
+        }
+        const bool hv = nf >= kHeavyFocus;  // k_focus_heavy
+        if (hv) nf = 0;
+        int fb = -1, fi = 0;
+        EP_FOCUS_ROW(T, M, D, a, bq, f[r], nf, dt, fb, fi, true);
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+          if (f[r][m] < FloorT<T>::v) f[r][m] = FloorT<T>::v;
+        if (fz && !hv && fb >= 0) note_error(D, err_key(substep, 1, s, 0, nd[r], e, fb, fi));
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m)
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (st[r]) buf_st(f[r][m], dst, vo[r], m * row);
+}
+
+// ------------------------------------------------------------------------
 // k_stream_finish: one block per (stream, energy).  Records which buffer
 // now holds the slab and copies thinned nodes from their reference node,
 // with the NaN/Inf check and DBL_MIN floor (src/energeticParticles.c:
@@ -1229,7 +1428,17 @@ template <int M, int MPOS, bool FUSE>
 static hipError_t stream_pass_tf(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                                  hipStream_t st) {
   const dim3 grid(chunks, D.E, D.S), block(D.stream_block / D.stream_r);
-  if (D.stream_r == 2) {
+  if (D.stream_r == 3) {
+    if (D.fp32)
+      hipLaunchKernelGGL((k_stream_passR<M, MPOS, float, true, FUSE, 3>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
+    else if (D.fast)
+      hipLaunchKernelGGL((k_stream_passR<M, MPOS, double, true, FUSE, 3>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
+    else
+      hipLaunchKernelGGL((k_stream_passR<M, MPOS, double, false, FUSE, 3>), grid, block, 0, st, D, pass, flip,
+                         do_seed, substep);
+  } else if (D.stream_r == 2) {
     if (D.fp32)
       hipLaunchKernelGGL((k_stream_pass2<M, MPOS, float, true, FUSE>), grid, block, 0, st, D, pass, flip,
                          do_seed, substep);

@@ -227,7 +227,12 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
 #pragma unroll
       for (int j = 0; j < MH; ++j) x[j] = fmax(x[j], FloorT<T>::v);
     }
-    // ---- AdiabaticChange, one sweep: finish energy e-1 (and e = E-1)
+    // ---- AdiabaticChange, one sweep: finish energy e-1 (and e = E-1).
+    // The energy flux takes the upwind cell alone: the reference's form
+    // would also pass on a non-finite value of the other cell, but every
+    // value reaching this point has passed a NaN/Inf check whose error
+    // sorts first (streaming, or focusing of the same node, op 0 < op 1),
+    // so the first reported cell cannot differ.
     double lx[MH];
 #pragma unroll
     for (int j = 0; j < MH; ++j) lx[j] = FM ? fm_log((double)x[j], fmt) : log((double)x[j]);
@@ -245,9 +250,9 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
             f1 = va1 * (nxt - cur) * dlnp_i;
           else
             f1 = va1 * (cur - bc0) * dlnp_i;
-          flux[j] = ep_poison(signbit(va1) ? cur : nxt, va1 * (signbit(va1) ? nxt : cur));
+          flux[j] = va1 * (signbit(va1) ? nxt : cur);
         } else {
-          const double flux_hi = ep_poison(signbit(va1) ? cur : nxt, va1 * (signbit(va1) ? nxt : cur));
+          const double flux_hi = va1 * (signbit(va1) ? nxt : cur);
           f1 = (flux_hi - flux[j]) * dlnp_i;
           flux[j] = flux_hi;
         }

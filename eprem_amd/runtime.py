@@ -118,7 +118,7 @@ class EpSolver:
 
     def set_tuning(self, stream_block, stream_halo, lane_nodes=1, fuse_focus=1):
         """Streaming tiles of stream_block kept nodes (halo included),
-        stream_halo subcycles per pass, lane_nodes (1 or 2) nodes per lane;
+        stream_halo subcycles per pass, lane_nodes (1, 2 or 3) nodes per lane;
         fuse_focus: focusing done by the last streaming pass (1) or by the
         separate kernel (0)."""
         _check(lib().ep_set_tuning(int(stream_block), int(stream_halo), int(lane_nodes), int(fuse_focus)))

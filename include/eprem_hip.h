@@ -351,7 +351,7 @@ int ep_finalize(void);
 
 /* Tiling of the streaming kernel: kept nodes per chunk (stream_block, halo
  * included), Euler subcycles per pass (stream_halo, 1..stream_block/2-1),
- * kept nodes per lane (lane_nodes, 1 or 2; stream_block must be a multiple
+ * kept nodes per lane (lane_nodes, 1, 2 or 3; stream_block must be a multiple
  * of 64*lane_nodes with at most 512 lanes) and fuse_focus (1: the
  * focusing of each node of an unthinned stream is done by its last
  * streaming pass on the values still in registers; 0: by the separate

@@ -371,7 +371,32 @@ def test_two_nodes_per_lane_parity(case):
     assert max_rel(fg, fo, n_active) <= RTOL
 
 
-@pytest.mark.parametrize("tiling", [(256, 16, 1, 1), (512, 32, 2, 1), (384, 24, 2, 1), (256, 16, 1, 0)])
+@pytest.mark.parametrize("case", [
+    dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=0, tiling=(768, 48, 3)),
+    dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=2, tiling=(384, 9, 3)),
+    dict(N=120, E=10, M=6, thin=0.15, ragged=False, precision=0, tiling=(192, 5, 3)),
+    dict(N=200, E=10, M=7, thin=0.0, ragged=True, precision=0, tiling=(576, 11, 3)),
+    dict(N=200, E=8, M=16, thin=0.1, ragged=True, precision=2, tiling=(1152, 30, 3)),
+    dict(N=257, E=8, M=32, thin=0.0, ragged=False, precision=0, tiling=(1536, 100, 3)),
+    dict(N=150, E=12, M=30, thin=0.0, ragged=False, precision=2, tiling=(768, 24, 3, 1)),
+])
+def test_three_nodes_per_lane_parity(case):
+    """The streaming kernel with three kept nodes per lane against the
+    oracle: thinned and ragged streams, odd and even M, both FP64 modes,
+    chunks of one to eight waves, focusing in the last pass, two steps."""
+    p = params(numEpSteps=2)
+    nodes = workloads.parker_nodes(1, 1, case["N"], seed=62, perturb=0.2, jitter=0.2, thin_frac=case["thin"])
+    n_active = None
+    if case["ragged"]:
+        n_active = np.array([case["N"], 37, 150, 3, 99, 128], np.int32)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, case["E"], case["M"], n_active=n_active, steps=2,
+                                    tiling=case["tiling"], precision=case["precision"])
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo, n_active) <= RTOL
+
+
+@pytest.mark.parametrize("tiling", [(256, 16, 1, 1), (512, 32, 2, 1), (384, 24, 2, 1), (256, 16, 1, 0),
+                                    (576, 24, 3, 1)])
 def test_fused_focusing_mixed_thinning(tiling):
     """Streams without thinned nodes are focused by their last streaming
     pass, thinned ones (filled after the pass) by the focusing kernel; a
@@ -414,7 +439,7 @@ def test_autotune_leaves_state_and_parity():
         sol.set_nodes(nodes)
         sol.upload_f(f0)
         block, halo, r, fz, ms = sol.autotune()
-        assert r in (1, 2) and block // r in (128, 192, 256, 320, 384, 512) and block % r == 0
+        assert r in (1, 2, 3) and block // r in (128, 192, 256, 320, 384, 512) and block % r == 0
         assert fz in (0, 1)
         assert 1 <= halo and 4 * halo <= block and ms > 0
         np.testing.assert_array_equal(sol.download_f(), f0)
