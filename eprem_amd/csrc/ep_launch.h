@@ -10,6 +10,15 @@ hipError_t launch_node_prep(const Dev &D, hipStream_t st);
 hipError_t launch_seed_check(const Dev &D, hipStream_t st);
 hipError_t launch_stream_pass(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                               int fuse_focus, hipStream_t st);
+// the same for M in 2..9, 10..17, 18..25, 26..32 (ep_stream_m*.hip)
+hipError_t launch_stream_pass_m2_9(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
+                                   int fuse_focus, hipStream_t st);
+hipError_t launch_stream_pass_m10_17(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
+                                     int fuse_focus, hipStream_t st);
+hipError_t launch_stream_pass_m18_25(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
+                                     int fuse_focus, hipStream_t st);
+hipError_t launch_stream_pass_m26_32(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
+                                     int fuse_focus, hipStream_t st);
 hipError_t launch_task_prep(const Dev &D, hipStream_t st);
 hipError_t launch_stream_finish(const Dev &D, int substep, hipStream_t st);
 // fused: focusing of full streams was done by the streaming passes;

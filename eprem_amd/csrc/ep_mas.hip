@@ -23,6 +23,10 @@ struct MasDev {
   const float *field[2][7];
   const float *mesh[2][21];
   int dims[2][21];
+  // uq[lv][3k + a]: the first (level, slot) of axis a whose mesh has the
+  // same points (levels and fields usually share meshes): a search is
+  // reused while the mesh it ran on stays the same
+  int uq[2][21];
 };
 
 __device__ __forceinline__ double mas_bsearch(const float *__restrict__ A, double key, int *imin,
@@ -41,13 +45,30 @@ __device__ __forceinline__ double mas_bsearch(const float *__restrict__ A, doubl
   return ((key - A[lower]) / (A[upper] - A[lower]));
 }
 
+// The bracketing indices and fraction of the last search on each axis, and
+// the mesh it ran on (MasDev::uq).
+struct MasSearch {
+  int u[3], i0[3], i1[3];
+  double fr[3];
+};
+
+// masInterpolate (src/masInterp.c:177-200) of field k at level lv, the
+// three searches (masTriLinearBinarySearch) reused from `c` when the mesh
+// of an axis is the one searched last.
 __device__ __forceinline__ double mas_field(const MasDev &M, int lv, int k, double rr, double th,
-                                            double ph) {
-  const int nr = M.dims[lv][3 * k], nt = M.dims[lv][3 * k + 1], np = M.dims[lv][3 * k + 2];
-  int r0, r1, t0, t1, p0, p1;
-  const double rd = mas_bsearch(M.mesh[lv][3 * k + 0], rr, &r0, &r1, 0, nr - 1);
-  const double td = mas_bsearch(M.mesh[lv][3 * k + 1], th, &t0, &t1, 0, nt - 1);
-  const double pd = mas_bsearch(M.mesh[lv][3 * k + 2], ph, &p0, &p1, 0, np - 1);
+                                            double ph, MasSearch &c) {
+  const int nr = M.dims[lv][3 * k], nt = M.dims[lv][3 * k + 1];
+  const double key[3] = {rr, th, ph};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int u = M.uq[lv][3 * k + a];
+    if (u != c.u[a]) {
+      c.fr[a] = mas_bsearch(M.mesh[lv][3 * k + a], key[a], &c.i0[a], &c.i1[a], 0, M.dims[lv][3 * k + a] - 1);
+      c.u[a] = u;
+    }
+  }
+  const int r0 = c.i0[0], r1 = c.i1[0], t0 = c.i0[1], t1 = c.i1[1], p0 = c.i0[2], p1 = c.i1[2];
+  const double rd = c.fr[0], td = c.fr[1], pd = c.fr[2];
   const float *__restrict__ V = M.field[lv][k];
   const size_t snr = nr, snrt = (size_t)nr * nt;
 #define VI(r, t, p) V[(size_t)(r) + snr * (t) + snrt * (p)]
@@ -94,15 +115,23 @@ __device__ void mas_get_node(const Dev &D, const MasDev &M, size_t i, double s, 
       while (ph < 0.0) ph += (2.0 * kPI);
     if (ph > (2.0 * kPI))
       while (ph > (2.0 * kPI)) ph -= (2.0 * kPI);
-    out7[2] = mas_b_apply((1.0 - s) * mas_field(M, 0, 0, rr, th, ph) + s * mas_field(M, 1, 0, rr, th, ph));
-    out7[1] = mas_b_apply((1.0 - s) * mas_field(M, 0, 1, rr, th, ph) + s * mas_field(M, 1, 1, rr, th, ph));
-    out7[0] = mas_b_apply((1.0 - s) * mas_field(M, 0, 2, rr, th, ph) + s * mas_field(M, 1, 2, rr, th, ph));
-    out7[5] = mas_v_apply((1.0 - s) * mas_field(M, 0, 3, rr, th, ph) + s * mas_field(M, 1, 3, rr, th, ph));
-    out7[4] = mas_v_apply((1.0 - s) * mas_field(M, 0, 4, rr, th, ph) + s * mas_field(M, 1, 4, rr, th, ph));
-    double vr = mas_v_apply((1.0 - s) * mas_field(M, 0, 5, rr, th, ph) + s * mas_field(M, 1, 5, rr, th, ph));
+    MasSearch c;
+    c.u[0] = c.u[1] = c.u[2] = -1;
+    double fv[2][7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      fv[0][k] = mas_field(M, 0, k, rr, th, ph, c);
+      fv[1][k] = mas_field(M, 1, k, rr, th, ph, c);
+    }
+    out7[2] = mas_b_apply((1.0 - s) * fv[0][0] + s * fv[1][0]);
+    out7[1] = mas_b_apply((1.0 - s) * fv[0][1] + s * fv[1][1]);
+    out7[0] = mas_b_apply((1.0 - s) * fv[0][2] + s * fv[1][2]);
+    out7[5] = mas_v_apply((1.0 - s) * fv[0][3] + s * fv[1][3]);
+    out7[4] = mas_v_apply((1.0 - s) * fv[0][4] + s * fv[1][4]);
+    double vr = mas_v_apply((1.0 - s) * fv[0][5] + s * fv[1][5]);
     if (vr < (vmin / 2.99792458e10)) vr = (vmin / 2.99792458e10);
     out7[3] = vr;
-    out7[6] = mas_rho_apply((1.0 - s) * mas_field(M, 0, 6, rr, th, ph) + s * mas_field(M, 1, 6, rr, th, ph));
+    out7[6] = mas_rho_apply((1.0 - s) * fv[0][6] + s * fv[1][6]);
   } else {
     // masWind: scale the node's previous values by r_old / r
     const double ox = prev[0 * SN + i], oy = prev[1 * SN + i], oz = prev[2 * SN + i];
@@ -218,6 +247,20 @@ static MasDev mas_dev() {
       M.dims[lv][k] = g.mas[lv].dims[k];
     }
   }
+  // canonical mesh of each (level, slot): the first one of the same axis
+  // with the same points, in the order (level 0 slots, level 1 slots)
+  for (int q = 0; q < 42; q++) {
+    const int lv = q / 21, sl = q % 21;
+    const std::vector<float> &a = g.mas[lv].hmesh[sl];
+    int u = q;
+    for (int p = 0; p < q && u == q; p++) {
+      const int lp = p / 21, sp = p % 21;
+      if (sp % 3 != sl % 3) continue;
+      const std::vector<float> &b = g.mas[lp].hmesh[sp];
+      if (a.size() == b.size() && memcmp(a.data(), b.data(), a.size() * sizeof(float)) == 0) u = p;
+    }
+    M.uq[lv][sl] = u;
+  }
   return M;
 }
 
@@ -267,6 +310,7 @@ int ep_mas_set_level(int32_t level, const float *const fields[7], const float *c
   for (int k = 0; k < 21; k++) {
     hipError_t e = hipMemcpy(L.mesh[k], meshes[k], dims[k] * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "MAS mesh upload");
+    L.hmesh[k].assign(meshes[k], meshes[k] + dims[k]);
   }
   L.set = true;
   return EP_OK;

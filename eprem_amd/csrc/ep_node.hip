@@ -151,71 +151,77 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
 #pragma unroll
     for (int j = 0; j < MH; ++j) x[j] = live(j) ? buf_ld(in, voff, j * row, T()) : T(1);
     if (foc) {
-      // ---- AdiabaticFocusing of (node, e): half-mesh velocities vb[j] at
-      // the interface below slot j (vb[MH]: above the last slot)
+      // ---- AdiabaticFocusing of (node, e).  Half-mesh velocity va at the
+      // interface below slot j (j = MH: above the last slot), split by the
+      // side the flux is taken from: vl = va when va >= 0 (+0 included),
+      // else 0; vh = va when va < 0 (or -0), else 0.  An interior face then
+      // carries fma(vh, f_above, vl * f_below): one product plus an exact
+      // zero, which is the reference's v 0.5 ((1 - c) f_above + (1 + c)
+      // f_below) (src/energeticParticles.c:1830-1836): equal to v * f_upwind
+      // for finite f and NaN when either cell is NaN or Inf.  The wall rows
+      // (mu cells 0 and M-1) take the upwind cell alone, as in the reference
+      // (:1850-1885): their faces are interface 1 (in the lower half when
+      // MH >= 2) and interface M-1 (in the upper half, above slot JWU, when
+      // M-1 > MH); otherwise they are the interface between the halves.
+      constexpr bool LW = MH >= 2;
+      constexpr bool UW = M - 1 > MH;
+      constexpr int JWU = UW ? M - 2 - MH : 0;
       const double v = D.vgrid[e];
       const double a = -v * dlnBds - (two / v) * fDuPar;
-      T vb[MH + 1];
+      T vl[MH + 1], vh[MH + 1];
+      T vw = 0;  // this lane's wall face velocity
       double vp = (m0 > 0) ? shc[m0 - 1] * (a + smu[m0 - 1] * fb) : 0.0;
 #pragma unroll
       for (int j = 0; j <= MH; ++j) {
         const int m = m0 + j;
         const double vel = shc[m] * (a + smu[m] * fb);
-        vb[j] = (T)(half * (vel + vp));
+        const T va = (T)(half * (vel + vp));
         vp = vel;
+        vl[j] = signbit(va) ? (T)0 : va;
+        vh[j] = signbit(va) ? va : (T)0;
+        if (LW && j == 1) vw = hf ? vw : va;
+        if (UW && j == JWU + 1) vw = hf ? va : vw;
       }
+      // the face between the halves (interface MH), seen the same by both
+      const T vmid = hf ? vl[0] + vh[0] : vl[MH] + vh[MH];
+      const T vlm = hf ? vl[0] : vl[MH], vhm = hf ? vh[0] : vh[MH];
       const T dmu_i = (T)(1.0 / D.dmu);
       const T dts = (T)dtsub_f;
-      // the nsub_f subcycles of the row; P: interior rows take the
-      // reference's NaN-propagating flux (ep_poison), the wall rows (mu
-      // cells 0 and M-1) the upwind one
-      auto focus_row = [&](auto poison) __attribute__((always_inline)) {
-        constexpr bool P = decltype(poison)::value;
-        for (int it = 0; it < nsub_f; ++it) {
-          // the flux through mu interface MH-1 | MH, from both halves' cells
-          const T recv = pair_swap(hf ? x[0] : x[MH - 1]);
-          const T flo = hf ? recv : x[MH - 1];
-          const T fhi = hf ? x[0] : recv;
-          const T vmid = hf ? vb[0] : vb[MH];
-          const T umid = vmid * (signbit(vmid) ? fhi : flo);
-          const T pmid = P ? ep_poison(signbit(vmid) ? flo : fhi, umid) : umid;
-          // lower half: row MH-1 is a wall row only when MH = 1; upper
-          // half: row MH is one only when MH = M-1
-          T flux_lo = hf ? ((MH == M - 1) ? umid : pmid) : (T)0;  // zero-flux wall at mu = -1
-#pragma unroll
-          for (int j = 0; j < MH; ++j) {
-            T flux_hi, next_lo;
-            if (j + 1 < MH) {
-              const int m = m0 + j;
-              const bool dn = signbit(vb[j + 1]);
-              const T u = vb[j + 1] * (dn ? x[j + 1] : x[j]);
-              const T pz = P ? ep_poison(dn ? x[j] : x[j + 1], u) : u;
-              const bool in = m + 1 < M;
-              flux_hi = in ? ((m == 0) ? u : pz) : (T)0;  // zero-flux wall at mu = +1
-              next_lo = (m + 2 == M) ? u : pz;
-            } else {
-              flux_hi = hf ? (T)0 : ((MH == 1) ? umid : pmid);
-              next_lo = (T)0;
-            }
-            const T dfl = (flux_hi - flux_lo) * dmu_i;
-            x[j] = x[j] - dts * dfl;
-            flux_lo = next_lo;
-          }
+      for (int it = 0; it < nsub_f; ++it) {
+        const T recv = pair_swap(hf ? x[0] : x[MH - 1]);
+        const T flo = hf ? recv : x[MH - 1];  // cell MH-1
+        const T fhi = hf ? x[0] : recv;       // cell MH
+        const T pmid = fma(vhm, fhi, vlm * flo);
+        T umid = pmid;
+        if (MH == 1 || MH == M - 1) umid = vmid * (signbit(vmid) ? fhi : flo);
+        // the wall face inside this lane's half (lower: interface 1 between
+        // slots 0 and 1; upper: interface M-1 above slot JWU)
+        T uw = 0;
+        if (LW || UW) {
+          const T xa = hf ? x[JWU] : x[0], xb = hf ? x[JWU + 1 < MH ? JWU + 1 : 0] : x[MH > 1 ? 1 : 0];
+          uw = vw * (signbit(vw) ? xb : xa);
         }
-      };
-      focus_row(std::false_type());
-      // The two forms agree while every value stays finite, and a value
-      // that turns non-finite stays so: only a row that ends non-finite is
-      // redone, from its stored values (not yet overwritten: this pass
-      // writes energy e-1 after loading e), by both lanes of the pair.
-      int bad_row = nonfinite(x) ? 1 : 0;
-      if (bad_row | pair_swap(bad_row)) {
+        T flux_lo = hf ? ((MH == M - 1) ? umid : pmid) : (T)0;  // zero-flux wall at mu = -1
 #pragma unroll
-        for (int j = 0; j < MH; ++j) x[j] = live(j) ? buf_ld(in, voff, j * row, T()) : T(1);
-        focus_row(std::true_type());
-        bad_row = nonfinite(x) ? 1 : 0;
+        for (int j = 0; j < MH; ++j) {
+          T flux_hi, next_lo;
+          if (j + 1 < MH) {
+            const T pf = fma(vh[j + 1], x[j + 1], vl[j + 1] * x[j]);
+            flux_hi = pf;
+            if (LW && j == 0) flux_hi = hf ? pf : uw;                     // row 0's upper face
+            if (j + 2 == MH && M % 2 == 1) flux_hi = hf ? (T)0 : flux_hi;  // above cell M-1 (padding)
+            next_lo = pf;
+            if (UW && j == JWU) next_lo = hf ? uw : pf;                    // row M-1's lower face
+          } else {
+            flux_hi = hf ? (T)0 : ((MH == 1) ? umid : pmid);  // zero-flux wall at mu = +1
+            next_lo = (T)0;
+          }
+          const T dfl = (flux_hi - flux_lo) * dmu_i;
+          x[j] = x[j] - dts * dfl;
+          flux_lo = next_lo;
+        }
       }
-      if (bad_row && fbad_e < 0) {
+      if (nonfinite(x) && fbad_e < 0) {
         for (int j = 0; j < MH; ++j)
           if (live(j) && !isfinite(x[j])) {
             fbad_e = e;

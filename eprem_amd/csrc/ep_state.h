@@ -14,6 +14,7 @@ struct MasLevel {
   float *field[7] = {nullptr};
   float *mesh[21] = {nullptr};
   int dims[21] = {0};
+  std::vector<float> hmesh[21];  // host copies: which meshes are the same
   bool set = false;
 };
 

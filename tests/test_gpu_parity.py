@@ -205,9 +205,13 @@ def test_ds_underflow_error():
     assert gd[0][1]["err_stream"] == od[0][1]["err_stream"] == 2
 
 
-def test_mas_interp_parity():
+@pytest.mark.parametrize("levels_differ", [False, True])
+def test_mas_interp_parity(levels_differ):
     """masTriLinear/masWind on the device vs the oracle: nodes inside and
-    outside masRadialMax, two time levels blended by s_cor, phi offset."""
+    outside masRadialMax, two time levels blended by s_cor, phi offset.
+    Some fields sit on staggered meshes (as in MAS) and, in the second
+    case, the second time level has meshes of its own for one field (the
+    device reuses a search only while the mesh is the same)."""
     from eprem_amd import mas_synth
     rng = np.random.default_rng(11)
     r, t, ph = mas_synth.meshes(nr=40, nt=30, nphi=50, rmin=1.0, rmax=30.0)
@@ -217,8 +221,13 @@ def test_mas_interp_parity():
     m0 = list(lv0[1])
     m0[0] = (r * 1.01).astype(np.float32)
     m0[4] = (t + 0.01).astype(np.float32)
+    m1 = list(m0)
+    if levels_differ:
+        m1[9] = (r * 0.995).astype(np.float32)
+        m1[11] = (ph + 0.02).astype(np.float32)
+        m1[13] = m0[4].copy()  # same points as another slot of the axis
     lv0 = (lv0[0], m0)
-    lv1 = (lv1[0], m0)
+    lv1 = (lv1[0], m1)
     nodes = workloads.parker_nodes(1, 1, 60, tdel_days=1.0 / 512, rscale=0.00465047, seed=12, jitter=0.3)
     p = params(rScale=0.00465047)
     dims = workloads.dims_of(nodes, 4, 4)
