@@ -423,7 +423,7 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
 // v*f_upwind exactly for finite f, and the zero-flux walls reproduce the
 // operator's boundary rows.
 // ------------------------------------------------------------------------
-template <int M, typename T>
+template <int M, typename T, bool FASTF>
 __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, int spare) {
   // blockIdx.x = ((stream, energy), chunk of 256 nodes), chunk fastest:
   // the block reads 2 KB runs of each mu row, consecutive blocks the next.
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
 #pragma unroll
   for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row, T());
   int bad = -1, bad_inf = 0;
-  EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf);
+  EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, FASTF);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const T val = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
 // k_focus_heavy: the nodes k_focus leaves out (nsub_f >= kHeavyFocus), one
 // wave per node and one lane per energy, so every lane runs the node's
 // subcycle count.  A fixed grid walks the list built by k_node_prep.
-template <int M, typename T>
+template <int M, typename T, bool FASTF>
 __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
   const int count = D.diag->focus_heavy;
   const int e = threadIdx.x;
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
 #pragma unroll
     for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
     int bad = -1, bad_inf = 0;
-    EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf);
+    EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, FASTF);
 #pragma unroll
     for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
     if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
@@ -771,15 +771,20 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
 #define EP_FOCUS_CASE(MM)                                                                     \
   case MM:                                                                                    \
     if (D.fp32) {                                                                             \
-      hipLaunchKernelGGL((k_focus<MM, float>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
+      hipLaunchKernelGGL((k_focus<MM, float, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
       if (!spare)                                                                             \
-        hipLaunchKernelGGL((k_focus_heavy<MM, float>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+        hipLaunchKernelGGL((k_focus_heavy<MM, float, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+    } else if (D.fast) {                                                                      \
+      hipLaunchKernelGGL((k_focus<MM, double, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
+                         st, D, sub, fused, spare);                                           \
+      if (!spare)                                                                             \
+        hipLaunchKernelGGL((k_focus_heavy<MM, double, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     } else {                                                                                  \
-      hipLaunchKernelGGL((k_focus<MM, double>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
+      hipLaunchKernelGGL((k_focus<MM, double, false>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
       if (!spare)                                                                             \
-        hipLaunchKernelGGL((k_focus_heavy<MM, double>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+        hipLaunchKernelGGL((k_focus_heavy<MM, double, false>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     }                                                                                         \
     break;
 
@@ -805,9 +810,11 @@ hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int sp
 #define EP_HEAVY_CASE(MM)                                                                     \
   case MM:                                                                                    \
     if (D.fp32)                                                                               \
-      hipLaunchKernelGGL((k_focus_heavy<MM, float>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+      hipLaunchKernelGGL((k_focus_heavy<MM, float, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+    else if (D.fast)                                                                          \
+      hipLaunchKernelGGL((k_focus_heavy<MM, double, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     else                                                                                      \
-      hipLaunchKernelGGL((k_focus_heavy<MM, double>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+      hipLaunchKernelGGL((k_focus_heavy<MM, double, false>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     break;
 
 hipError_t launch_focus_heavy(const Dev &D, int sub, hipStream_t st) {

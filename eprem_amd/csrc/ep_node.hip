@@ -168,6 +168,9 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
       constexpr int JWU = UW ? M - 2 - MH : 0;
       const double v = D.vgrid[e];
       const double a = -v * dlnBds - (two / v) * fDuPar;
+      const T dmu_i = (T)(1.0 / D.dmu);
+      const T dts = (T)dtsub_f;
+      const T sc = dts * dmu_i;
       T vl[MH + 1], vh[MH + 1];
       T vw = 0;  // this lane's wall face velocity
       double vp = (m0 > 0) ? shc[m0 - 1] * (a + smu[m0 - 1] * fb) : 0.0;
@@ -175,7 +178,8 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
       for (int j = 0; j <= MH; ++j) {
         const int m = m0 + j;
         const double vel = shc[m] * (a + smu[m] * fb);
-        const T va = (T)(half * (vel + vp));
+        T va = (T)(half * (vel + vp));
+        if (FM) va = va * sc;  // fused modes: velocities in units of dmu / dt_sub
         vp = vel;
         vl[j] = signbit(va) ? (T)0 : va;
         vh[j] = signbit(va) ? va : (T)0;
@@ -185,8 +189,6 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
       // the face between the halves (interface MH), seen the same by both
       const T vmid = hf ? vl[0] + vh[0] : vl[MH] + vh[MH];
       const T vlm = hf ? vl[0] : vl[MH], vhm = hf ? vh[0] : vh[MH];
-      const T dmu_i = (T)(1.0 / D.dmu);
-      const T dts = (T)dtsub_f;
       for (int it = 0; it < nsub_f; ++it) {
         const T recv = pair_swap(hf ? x[0] : x[MH - 1]);
         const T flo = hf ? recv : x[MH - 1];  // cell MH-1
@@ -216,8 +218,12 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
             flux_hi = hf ? (T)0 : ((MH == 1) ? umid : pmid);  // zero-flux wall at mu = +1
             next_lo = (T)0;
           }
-          const T dfl = (flux_hi - flux_lo) * dmu_i;
-          x[j] = x[j] - dts * dfl;
+          if (FM) {
+            x[j] = x[j] - (flux_hi - flux_lo);
+          } else {
+            const T dfl = (flux_hi - flux_lo) * dmu_i;
+            x[j] = x[j] - dts * dfl;
+          }
           flux_lo = next_lo;
         }
       }
@@ -290,15 +296,264 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
   }
 }
 
+// ------------------------------------------------------------------------
+// k_focus_change4: the same pass with four lanes per node (for M >= 12).
+// Lanes l, l+16, l+32, l+48 of a wave hold mu cells q MG .. q MG + MG - 1
+// (q = l / 16, MG = ceil(M/4), the last group padded) of node (l mod 16);
+// a 256-lane block covers 64 consecutive nodes.  Each focusing subcycle
+// moves the boundary cells of the neighbouring groups in by two
+// cross-lane shifts of 16.  A lane keeps about half the state of the
+// pair layout, so the kernel runs at a higher occupancy; arithmetic and
+// results are those of k_focus_change.
+// ------------------------------------------------------------------------
+template <int M, typename T, bool FM>
+__global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int fused, int spare) {
+  constexpr int G = 4, W = 64 / G, NB = 4 * W;
+  constexpr int MG = (M + G - 1) / G;
+  static_assert(M >= 12, "four lanes per node from 12 mu cells");
+  constexpr int QL = (M - 1) / MG;       // group of cell M-1 (the last one for M >= 12)
+  constexpr int JL = (M - 1) - QL * MG;  // its slot
+  static_assert(QL == G - 1, "cell M-1 in the last group");
+  const int nch = (D.N + NB - 1) / NB;
+  const int s = blockIdx.x / nch;
+  const int chunk = (int)((blockIdx.x % nch + (unsigned)s) % nch);
+  const int lane = threadIdx.x & 63;
+  const int q = lane / W;
+  const int node = chunk * NB + (threadIdx.x >> 6) * W + (lane % W);
+  const int E = D.E;
+  __shared__ uint8_t sloc[kMaxE];
+  // per mu (padded with zeros): mu, mu^2, 0.5 (1 - mu^2)
+  __shared__ double smu[G * MG + 2], smu2[G * MG + 2], shc[G * MG + 2];
+  __shared__ double fmt[FM ? kFmTable : 1];
+  if (threadIdx.x < E) sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+  if (threadIdx.x < G * MG + 2) {
+    const int m = threadIdx.x;
+    const double mu = (m < M) ? D.mugrid[m] : 0.0;
+    smu[m] = mu;
+    smu2[m] = mu * mu;
+    shc[m] = 0.5 * (1.0 - mu * mu);
+  }
+  if (FM)
+    for (int t = threadIdx.x; t < kFmTable; t += blockDim.x) fmt[t] = D.fm_tab[t];
+  __syncthreads();
+  const int n = D.n_active[s];
+  if (node < 1 || node >= n) return;  // the four lanes of a node leave together
+  const size_t i = (size_t)s * D.Npad + node;
+  const int m0 = q * MG;
+  const int nsub_f = D.nsub_f[i];
+  const bool foc = !(fused && !D.thin[s]) && nsub_f < kHeavyFocus;
+  const double dtsub_f = D.dtsub_f[i];
+  const double dlnBds = D.f_dlnbds[i], fDuPar = D.f_dupar[i], fb = D.f_b[i];
+  const int nsub_c = D.nsub_c[i];
+  const double dtsub_c = D.dtsub_c[i];
+  const double cDuPar = D.c_dupar[i], DlnNDt = D.c_dlnn[i], DlnBDt = D.c_dlnb[i];
+  const double damp = D.c_damp[i], bc0 = D.c_bc0[i], bc1 = D.c_bc1[i];
+  const double nmb = DlnNDt - DlnBDt;
+  const bool upd = nsub_c > 0;
+  const double dlnp_i = 1.0 / D.dlnp;
+  const double half = 0.5, two = 2.0;
+  const double *__restrict__ vgi = D.vgrid_i;
+  constexpr uint32_t B = sizeof(T);
+  const uint32_t slab_bytes = (uint32_t)((size_t)M * D.Npad * B);
+  const uint32_t row = (uint32_t)D.Npad * B;
+  const uint32_t voff = (uint32_t)(((size_t)m0 * D.Npad + node) * B);
+  auto slab_of = [&](int e, int flip) {
+    const int task = s * E + e;
+    const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]) ^ flip;
+    return slab_rsrc((const double *)(fbufT<T>(D, lc) + (size_t)task * M * D.Npad), slab_bytes);
+  };
+  // slot j is mu cell m0 + j; slots past cell M-1 in the last group are
+  // padding
+  auto live = [&](int j) { return (j <= JL) || (q < G - 1); };
+  auto cvel = [&](int e, int j) {
+    const int m = m0 + j;
+    const double a = -smu[m] * cDuPar;
+    const double bb = smu2[m] * nmb + shc[m] * DlnBDt;
+    return damp * (a * vgi[e] + bb);
+  };
+  double lprev[MG], flux[MG];  // ln f at e-1 after focusing; flux below e-1
+  double cvp[MG];              // change velocity at e-1 (this energy's, next time)
+  int fbad_e = -1, fbad_m = 0, fbad_inf = 0;
+  int cbad_e = -1, cbad_m = 0, cbad_inf = 0;
+  auto nonfinite = [&](const T *x) {
+    T z = 0;
+#pragma unroll
+    for (int j = 0; j < MG; ++j) z = live(j) ? fma(x[j], (T)0, z) : z;
+    return !isfinite(z);
+  };
+  auto put_row = [&](int e, const double *nv, const rsrc_t &out) {
+    T val[MG];
+#pragma unroll
+    for (int j = 0; j < MG; ++j) val[j] = (T)(FM ? fm_exp(nv[j], fmt) : exp(nv[j]));
+    if (nonfinite(val) && cbad_e < 0) {
+      for (int j = 0; j < MG; ++j)
+        if (live(j) && !isfinite(val[j])) {
+          cbad_e = e;
+          cbad_m = m0 + j;
+          cbad_inf = isinf(val[j]) ? 1 : 0;
+          break;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MG; ++j)
+      if (live(j)) buf_st(fmax(val[j], FloorT<T>::v), out, voff, j * row);
+  };
+  for (int e = 0; e < E; ++e) {
+    const rsrc_t in = slab_of(e, 0);
+    T x[MG];
+#pragma unroll
+    for (int j = 0; j < MG; ++j) x[j] = live(j) ? buf_ld(in, voff, j * row, T()) : T(1);
+    if (foc) {
+      // ---- AdiabaticFocusing of (node, e), as in k_focus_change: face
+      // velocities split by upwind side (vl, vh), interior faces
+      // fma(vh, f_above, vl * f_below), the wall rows' faces (interface 1
+      // above cell 0, interface M-1 below cell M-1) the upwind cell alone.
+      const double v = D.vgrid[e];
+      const double a = -v * dlnBds - (two / v) * fDuPar;
+      const T dmu_i = (T)(1.0 / D.dmu);
+      const T dts = (T)dtsub_f;
+      const T sc = dts * dmu_i;
+      T vl[MG + 1], vh[MG + 1];  // face below slot j (j = MG: above the last slot)
+      double vp = (m0 > 0) ? shc[m0 - 1] * (a + smu[m0 - 1] * fb) : 0.0;
+#pragma unroll
+      for (int j = 0; j <= MG; ++j) {
+        const int m = m0 + j;
+        const double vel = shc[m] * (a + smu[m] * fb);
+        T va = (T)(half * (vel + vp));
+        if (FM) va = va * sc;  // fused modes: velocities in units of dmu / dt_sub
+        vp = vel;
+        vl[j] = signbit(va) ? (T)0 : va;
+        vh[j] = signbit(va) ? va : (T)0;
+      }
+      for (int it = 0; it < nsub_f; ++it) {
+        // cell m0-1 (last of the group below), cell m0+MG (first above)
+        const T below = __shfl_up(x[MG - 1], W, 64);
+        const T above = __shfl_down(x[0], W, 64);
+        // face below slot 0: zero-flux wall under cell 0, else interior
+        // (or, when cell M-1 is slot 0, that row's upwind-only face)
+        T flux_lo = fma(vh[0], x[0], vl[0] * below);
+        if (JL == 0) {
+          const T ub = (vl[0] + vh[0]) * (signbit(vh[0]) ? x[0] : below);
+          flux_lo = (q == QL) ? ub : flux_lo;
+        }
+        flux_lo = (q == 0) ? (T)0 : flux_lo;
+        const T ptop = fma(vh[MG], above, vl[MG] * x[MG - 1]);
+#pragma unroll
+        for (int j = 0; j < MG; ++j) {
+          T flux_hi, next_lo;
+          if (j + 1 < MG) {
+            const T pf = fma(vh[j + 1], x[j + 1], vl[j + 1] * x[j]);
+            flux_hi = pf;
+            next_lo = pf;
+            if (j == 0 || j + 1 == JL) {
+              const T uf = (vl[j + 1] + vh[j + 1]) * (signbit(vh[j + 1]) ? x[j + 1] : x[j]);
+              if (j == 0) flux_hi = (q == 0) ? uf : pf;         // row 0's upper face
+              if (j + 1 == JL) next_lo = (q == QL) ? uf : pf;   // row M-1's lower face
+            }
+          } else {
+            flux_hi = ptop;
+            next_lo = (T)0;
+          }
+          if (j >= JL) flux_hi = (q == QL) ? (T)0 : flux_hi;  // zero-flux wall above cell M-1
+          if (FM) {
+            x[j] = x[j] - (flux_hi - flux_lo);
+          } else {
+            const T dfl = (flux_hi - flux_lo) * dmu_i;
+            x[j] = x[j] - dts * dfl;
+          }
+          flux_lo = next_lo;
+        }
+      }
+      if (nonfinite(x) && fbad_e < 0) {
+        for (int j = 0; j < MG; ++j)
+          if (live(j) && !isfinite(x[j])) {
+            fbad_e = e;
+            fbad_m = m0 + j;
+            fbad_inf = isinf(x[j]) ? 1 : 0;
+            break;
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < MG; ++j) x[j] = fmax(x[j], FloorT<T>::v);
+    }
+    // ---- AdiabaticChange, one sweep: finish energy e-1 (and e = E-1),
+    // as in k_focus_change
+    double lx[MG];
+#pragma unroll
+    for (int j = 0; j < MG; ++j) lx[j] = FM ? fm_log((double)x[j], fmt) : log((double)x[j]);
+    if (e >= 1) {
+      const int ep = e - 1;
+      const rsrc_t out = slab_of(ep, spare);
+      double nv[MG];
+#pragma unroll
+      for (int j = 0; j < MG; ++j) {
+        const double cur = lprev[j], nxt = lx[j];
+        const double cve = cvel(e, j);
+        const double va1 = half * (cve + cvp[j]);
+        cvp[j] = cve;
+        double f1;
+        if (ep == 0) {
+          if (va1 <= 0)
+            f1 = va1 * (nxt - cur) * dlnp_i;
+          else
+            f1 = va1 * (cur - bc0) * dlnp_i;
+          flux[j] = va1 * (signbit(va1) ? nxt : cur);
+        } else {
+          const double flux_hi = va1 * (signbit(va1) ? nxt : cur);
+          f1 = (flux_hi - flux[j]) * dlnp_i;
+          flux[j] = flux_hi;
+        }
+        nv[j] = upd ? cur - dtsub_c * f1 : cur;
+      }
+      put_row(ep, nv, out);
+      if (e == E - 1) {
+        const rsrc_t out1 = slab_of(e, spare);
+#pragma unroll
+        for (int j = 0; j < MG; ++j) {
+          const double cur = lx[j], old_prev = lprev[j];
+          const double vr = half * (cvp[j] + cvel(ep, j));
+          double f1;
+          if (vr >= 0)
+            f1 = vr * (cur - old_prev) * dlnp_i;
+          else
+            f1 = vr * (bc1 - cur) * dlnp_i;
+          nv[j] = upd ? cur - dtsub_c * f1 : cur;
+        }
+        put_row(e, nv, out1);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MG; ++j) {
+      lprev[j] = lx[j];
+      if (e == 0) cvp[j] = cvel(0, j);
+    }
+  }
+  if (!spare) {
+    if (fbad_e >= 0) note_error(D, err_key(substep, 1, s, 0, node, fbad_e, fbad_m, fbad_inf));
+    if (cbad_e >= 0) note_error(D, err_key(substep, 1, s, 1, node, cbad_e, cbad_m, cbad_inf));
+  }
+}
+
 template <int M>
 static void focus_change_t(const Dev &D, int sub, int fused, int spare, hipStream_t st) {
-  const dim3 grid((D.N + 127) / 128 * D.S), block(256);
-  if (D.fp32)
-    hipLaunchKernelGGL((k_focus_change<M, float, true>), grid, block, 0, st, D, sub, fused, spare);
-  else if (D.fast)
-    hipLaunchKernelGGL((k_focus_change<M, double, true>), grid, block, 0, st, D, sub, fused, spare);
-  else
-    hipLaunchKernelGGL((k_focus_change<M, double, false>), grid, block, 0, st, D, sub, fused, spare);
+  if constexpr (M >= 12) {
+    // four lanes per node
+    const dim3 grid((D.N + 63) / 64 * D.S), block(256);
+    if (D.fp32)
+      hipLaunchKernelGGL((k_focus_change4<M, float, true>), grid, block, 0, st, D, sub, fused, spare);
+    else if (D.fast)
+      hipLaunchKernelGGL((k_focus_change4<M, double, true>), grid, block, 0, st, D, sub, fused, spare);
+    else
+      hipLaunchKernelGGL((k_focus_change4<M, double, false>), grid, block, 0, st, D, sub, fused, spare);
+  } else {
+    const dim3 grid((D.N + 127) / 128 * D.S), block(256);
+    if (D.fp32)
+      hipLaunchKernelGGL((k_focus_change<M, float, true>), grid, block, 0, st, D, sub, fused, spare);
+    else if (D.fast)
+      hipLaunchKernelGGL((k_focus_change<M, double, true>), grid, block, 0, st, D, sub, fused, spare);
+    else
+      hipLaunchKernelGGL((k_focus_change<M, double, false>), grid, block, 0, st, D, sub, fused, spare);
+  }
 }
 
 #define EP_FC_CASE(MM)                         \

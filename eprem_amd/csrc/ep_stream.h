@@ -65,8 +65,14 @@ __device__ __forceinline__ float dpp_shl1(float v, float old) {
 // + (1 + c) f[m-1]) (:1830-1836): equal to v * f_upwind for finite f and
 // NaN when either cell is NaN or Inf, with no select.  The wall rows
 // (cells 0 and M-1) take the upwind cell alone (:1850-1885).
-#define EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf)                  \
+// FASTF (the fused precision modes): the face velocities are scaled by
+// dt_sub / dmu once, so that a subcycle is f[m] -= F_hi - F_lo (the
+// reference forms (F_hi - F_lo) / dmu and then f - dt_sub * that).
+#define EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, FASTF)           \
   do {                                                                             \
+    const T dmu_i_ = (T)(1.0 / (D).dmu);                                           \
+    const T dts_ = (T)(dtsub);                                                     \
+    const T sc_ = (FASTF) ? dts_ * dmu_i_ : (T)1;                                  \
     T vl_[(M) + 1], vh_[(M) + 1];                                                  \
     T vw0_ = 0, vw1_ = 0; /* face velocities of the wall rows: faces 1, M-1 */     \
     double vprev_ = 0.0;                                                           \
@@ -74,7 +80,8 @@ __device__ __forceinline__ float dpp_shl1(float v, float old) {
       const double mu = (D).mugrid[m];                                             \
       const double vel = 0.5 * (1.0 - mu * mu) * ((a) + mu * (b));                 \
       if (m > 0) {                                                                 \
-        const T va_ = (T)(0.5 * (vel + vprev_));                                   \
+        T va_ = (T)(0.5 * (vel + vprev_));                                         \
+        if (FASTF) va_ = va_ * sc_;                                                \
         vl_[m] = signbit(va_) ? (T)0 : va_;                                        \
         vh_[m] = signbit(va_) ? va_ : (T)0;                                        \
         if (m == 1) vw0_ = va_;                                                    \
@@ -82,8 +89,6 @@ __device__ __forceinline__ float dpp_shl1(float v, float old) {
       }                                                                            \
       vprev_ = vel;                                                                \
     }                                                                              \
-    const T dmu_i_ = (T)(1.0 / (D).dmu);                                           \
-    const T dts_ = (T)(dtsub);                                                     \
     for (int it_ = 0; it_ < (nsub); ++it_) {                                       \
       T flux_lo = 0; /* zero-flux wall at mu = -1 */                               \
       _Pragma("unroll") for (int m = 0; m < (M); ++m) {                            \
@@ -96,8 +101,12 @@ __device__ __forceinline__ float dpp_shl1(float v, float old) {
           if (m + 2 == (M))                                                        \
             next_lo = vw1_ * (signbit(vw1_) ? f[(M) - 1] : f[(M) > 1 ? (M) - 2 : 0]); \
         }                                                                          \
-        const T dfl_ = (flux_hi - flux_lo) * dmu_i_;                                  \
-        f[m] = f[m] - dts_ * dfl_;                                                 \
+        if (FASTF) {                                                               \
+          f[m] = f[m] - (flux_hi - flux_lo);                                       \
+        } else {                                                                   \
+          const T dfl_ = (flux_hi - flux_lo) * dmu_i_;                             \
+          f[m] = f[m] - dts_ * dfl_;                                               \
+        }                                                                          \
         flux_lo = next_lo;                                                         \
       }                                                                            \
     }                                                                              \
@@ -267,7 +276,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
           const double a = -D.vgrid[e] * D.f_dlnbds[i] - (2.0 / D.vgrid[e]) * D.f_dupar[i];
           const double b2 = D.f_b[i];
           int fbad = -1, finf = 0;
-          EP_FOCUS_ROW(T, M, D, a, b2, f, nf, D.dtsub_f[i], fbad, finf);
+          EP_FOCUS_ROW(T, M, D, a, b2, f, nf, D.dtsub_f[i], fbad, finf, FAST);
 #pragma unroll
           for (int m = 0; m < M; ++m)
             if (f[m] < FloorT<T>::v) f[m] = FloorT<T>::v;
@@ -476,8 +485,8 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
       if (h0) nf0 = 0;
       if (h1) nf1 = 0;
       int fb0 = -1, fi0 = 0, fb1 = -1, fi1 = 0;
-      EP_FOCUS_ROW(T, M, D, a0, b0, f0, nf0, dt0, fb0, fi0);
-      EP_FOCUS_ROW(T, M, D, a1, b1, f1, nf1, dt1, fb1, fi1);
+      EP_FOCUS_ROW(T, M, D, a0, b0, f0, nf0, dt0, fb0, fi0, FAST);
+      EP_FOCUS_ROW(T, M, D, a1, b1, f1, nf1, dt1, fb1, fi1, FAST);
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         if (f0[m] < FloorT<T>::v) f0[m] = FloorT<T>::v;
@@ -678,7 +687,7 @@ __global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip,
         const bool hv = nf >= kHeavyFocus;  // k_focus_heavy
         if (hv) nf = 0;
         int fb = -1, fi = 0;
-        EP_FOCUS_ROW(T, M, D, a, bq, f[r], nf, dt, fb, fi);
+        EP_FOCUS_ROW(T, M, D, a, bq, f[r], nf, dt, fb, fi, FAST);
 #pragma unroll
         for (int m = 0; m < M; ++m)
           if (f[r][m] < FloorT<T>::v) f[r][m] = FloorT<T>::v;

@@ -60,6 +60,12 @@ CASES = [
     dict(N=90, E=64, M=32, seed=64),
     dict(N=70, E=2, M=2, seed=65),
     dict(N=300, E=16, M=31, seed=66),
+    # four lanes per node from M = 12: cell M-1 first in its group (13),
+    # second (17), third (12), fourth (25)
+    dict(N=150, E=12, M=13, seed=70),
+    dict(N=140, E=10, M=17, seed=71),
+    dict(N=130, E=9, M=12, seed=72),
+    dict(N=110, E=11, M=25, seed=73),
 ]
 
 
@@ -130,11 +136,13 @@ def test_one_pass_without_streaming(flags):
     assert float(np.max(np.abs(f1 - fo) / np.abs(fo))) <= RTOL
 
 
+@pytest.mark.parametrize("M", [11, 13])
 @pytest.mark.parametrize("flags", [abi.EP_OP_FOCUS | abi.EP_OP_CHANGE, abi.EP_OP_CHANGE])
 @pytest.mark.parametrize("where,value", [((2, 17, 5, 2), np.nan), ((0, 150, 0, 0), np.nan),
                                          ((4, 60, 13, 10), np.nan), ((1, 40, 7, 5), np.inf),
-                                         ((3, 90, 0, 10), np.nan), ((5, 33, 13, 0), np.inf)])
-def test_one_pass_nan_location(where, value, flags):
+                                         ((3, 90, 0, 10), np.nan), ((5, 33, 13, 0), np.inf),
+                                         ((2, 70, 3, -1), np.nan), ((1, 20, 9, -2), np.inf)])
+def test_one_pass_nan_location(where, value, flags, M):
     """A NaN or Inf planted in f: the same error code and first-panic cell
     as the oracle (and as the two-kernel form).  The reference's interior
     fluxes pass a non-finite value on from either neighbour, its wall rows
@@ -143,10 +151,10 @@ def test_one_pass_nan_location(where, value, flags):
     p = params()
     nodes = workloads.parker_nodes(1, 1, 160, rscale=0.00465047, seed=69, perturb=0.1, smooth=20,
                                    jitter=0.05)
-    E, M = 14, 11
+    E = 14
     g = oracle.build_grids(p, E, M)
     f0 = workloads.seeded_f(p, g, nodes, noise=0.2, seed=9)
-    f0[where] = value
+    f0[where] = value  # (a negative mu index counts from the upper wall)
     _, d1 = run_gpu(p, nodes, E, M, f0, 1, op_flags=flags)
     _, d2 = run_gpu(p, nodes, E, M, f0, 0, op_flags=flags)
     dims = workloads.dims_of(nodes, E, M)
