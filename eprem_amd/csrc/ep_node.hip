@@ -9,9 +9,12 @@
 // while its tile streams through registers: f is read once and written once
 // per substep instead of twice each.
 //
-// Layout: a lane pair per node.  Lanes l and l+32 of a wave hold the lower
-// and upper halves of the mu row (MH = ceil(M/2) cells each) of node
-// (l mod 32); a 256-lane block covers 128 consecutive nodes of one stream.
+// Layout below 12 mu cells (k_focus_change): a lane pair per node.  Lanes l
+// and l+32 of a wave hold the lower and upper halves of the mu row
+// (MH = ceil(M/2) cells each) of node (l mod 32); a 256-lane block covers
+// 128 consecutive nodes of one stream.  From 12 mu cells k_focus_change4
+// (further down) spreads a node over four lanes instead; the walk over the
+// energies is the same.
 // Energies are walked upwards.  For each energy the kernel loads the row,
 // runs the node's focusing subcycles (the one interface between the halves
 // is exchanged across the lane pair each subcycle), takes ln f, and then
