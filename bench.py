@@ -68,11 +68,17 @@ def run_surrogate(args, c, p, dims):
     Ks = c["surrogate"]
     xs, ys = [], []
     plain = None
+    plain_kernel = None
     for vi, (seed, cme_r) in enumerate(((20261203, 0.5), (7, 0.3), (11, 0.8))):
         nodes = bench_nodes(c, seed=seed, cme_r=cme_r)
         sol = runtime.EpSolver(dims, p)
         sol.set_nodes(nodes)
         sol.init_seed_population()
+        # the plain path as the c2 line runs it: mode 2, autotuned tiling
+        sol.set_precision(2)
+        if plain is None:
+            _, _, r, _, _ = sol.autotune()
+            plain_kernel = {1: "k_stream_pass", 2: "k_stream_pass2", 3: "k_stream_passR<R=3>"}[r]
         sol.set_graph(False)
         _, dg = sol.step()  # develop anisotropy from the isotropic seed
         if plain is None:
@@ -116,7 +122,8 @@ def run_surrogate(args, c, p, dims):
                     "mfma_frac": rows * flops_row / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
                     "layerwise_ms": ms_layerwise,
                     "layerwise_tflops": rows * flops_row / (ms_layerwise * 1e-3) / 1e12},
-        "plain_streaming": {"value": plain_rate, "unit": "cell-subcycles/s", "kernel": "k_stream_pass"},
+        "plain_streaming": {"value": plain_rate, "unit": "cell-subcycles/s", "kernel": plain_kernel,
+                            "precision": "mode 2, autotuned tiling"},
         "surrogate_over_plain": sur_rate / plain_rate,
         "flux_error_heldout": {"network": surrogate.summary(e_net),
                                "without_network_relaxation_only": surrogate.summary(e_base)},
