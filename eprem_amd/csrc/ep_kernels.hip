@@ -470,13 +470,24 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
 
 // k_focus_heavy: the nodes k_focus leaves out (nsub_f >= kHeavyFocus), one
 // wave per node and one lane per energy, so every lane runs the node's
-// subcycle count.  A fixed grid walks the list built by k_node_prep.
+// subcycle count.  A fixed grid walks the list built by k_node_prep, whose
+// entries come in runs of consecutive nodes of one stream.  A lane's loads
+// are 8 B of a different 128-B line each (f is node-fastest), and 16
+// consecutive nodes share those lines, so the list is split into one
+// contiguous range per XCD (blocks are dispatched round-robin over the 8
+// XCDs): neighbouring nodes then run at the same time on the same XCD and
+// meet their lines in its L2 instead of every XCD fetching each line.
 template <int M, typename T, bool FASTF>
 __global__ __launch_bounds__(64) void k_focus_heavy(Dev D, int substep) {
   const int count = D.diag->focus_heavy;
   const int e = threadIdx.x;
   const double two = 2.0;
-  for (int h = blockIdx.x; h < count; h += gridDim.x) {
+  constexpr int kXcd = 8;
+  const int per = gridDim.x / kXcd;  // the grid is a multiple of 8 blocks
+  const int xcd = blockIdx.x % kXcd, local = blockIdx.x / kXcd;
+  const int seg = (count + kXcd - 1) / kXcd;
+  const int end = min(count, (xcd + 1) * seg);
+  for (int h = xcd * seg + local; h < end; h += per) {
     const size_t i = (size_t)D.heavy[h];
     const int s = (int)(i / D.Npad), node = (int)(i % D.Npad);
     if (e >= D.E) continue;
@@ -788,8 +799,10 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
     }                                                                                         \
     break;
 
-// Blocks of the heavy-focusing walk: enough waves to fill the chip.
+// Blocks of the heavy-focusing walk: enough waves to fill the chip, a
+// multiple of the 8 XCDs (k_focus_heavy gives each XCD a range of the list).
 constexpr int kHeavyGrid = 4096;
+static_assert(kHeavyGrid % 8 == 0, "k_focus_heavy splits its grid over 8 XCDs");
 
 hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int spare) {
   switch (D.M) {
