@@ -316,7 +316,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs every stream on one GPU; --gpus %d is not supported "
-              "(the path would shard by stream, see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "(the path would shard by stream: each GPU would own a block of streams and "
+              "their f)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     c = CONFIGS[args.config]
     p = make_params(c)
