@@ -315,7 +315,7 @@ def main():
                          "multiply-adds in streaming (within 1e-13 of it)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs every stream on one GPU; --gpus %d is not supported "
+        print("bench.py: the library runs every stream on one GPU; --gpus %d is not supported "
               "(the path would shard by stream: each GPU would own a block of streams and "
               "their f)" % args.gpus, file=sys.stderr)
         sys.exit(2)
