@@ -80,7 +80,7 @@ struct Dev {
   // params (code units)
   double rScale, dt, dt_full;
   int numEpSteps, mfpType, useBC, stream_block, stream_halo;
-  int stream_r;     // kept nodes per lane in the streaming passes (1 or 2)
+  int stream_r;     // kept nodes per lane in the streaming passes (1, 2 or 3)
   int stream_fuse;  // focusing of full streams done in their last streaming pass
   double mfpRadialPower, mfpBPower, mhdBsAu, dsh_min, dampCooling;
   double normJ0, normRadius, xi, beta, dlnp, dmu;

@@ -12,9 +12,14 @@
 //   k_stream_coef   per kept node and energy: ds_i multiplier, exp(-dt/tau)
 //   k_node_prep     per node: focusing/change coefficients and subcycles
 // Per EP substep:
-//   k_stream_pass   DiffuseStreamData on register-resident node chunks,
-//                   temporal blocking with halos (one launch per pass)
+//   k_stream_pass*  DiffuseStreamData on register-resident node chunks,
+//                   temporal blocking with halos (one launch per pass;
+//                   the kernels are in ep_stream.h)
 //   k_stream_finish alias copies of thinned nodes, buffer bookkeeping
+//   k_focus_heavy   AdiabaticFocusing of nodes with many subcycles, a lane
+//                   per energy
+//   k_focus_change4 / k_focus_change (ep_node.hip) AdiabaticFocusing and
+//                   AdiabaticChange of a node in one pass over f, or
 //   k_focus         AdiabaticFocusing, a lane per node, mu in registers
 //   k_change        AdiabaticChange, a lane per node, energy in registers
 #include <float.h>
