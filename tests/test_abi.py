@@ -53,3 +53,14 @@ def test_missing_device_is_reported_not_silent():
     with pytest.raises(runtime.EpError) as ei:
         runtime.EpSolver((6, 8, 4, 4), abi.EpParams())
     assert ei.value.code in (abi.EP_ERR_NO_DEVICE, abi.EP_ERR_INVALID_ARG)
+
+
+def test_missing_library_is_reported_not_silent(monkeypatch, tmp_path):
+    """Without the built HIP library the binding raises instead of falling
+    back to any CPU path."""
+    monkeypatch.setattr(runtime, "_lib", None)
+    monkeypatch.setattr(runtime, "LIB_PATH", str(tmp_path / "libeprem_hip.so"))
+    with pytest.raises(RuntimeError, match="not built"):
+        runtime.lib()
+    with pytest.raises(RuntimeError, match="not built"):
+        runtime.EpSolver((6, 8, 4, 4), abi.EpParams())
