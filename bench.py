@@ -40,20 +40,12 @@ STREAM_FLOPS_PER_SWEEP = 8   # upwind 4 + iso sum 1 + relaxation 3 (FP64)
 # and profiles/k_stream_passR_c2_kernel_trace.md (three).  Other tilings
 # report null.
 STREAM_TRAFFIC = {("c2", 512, 48, 2, 0, 2): 8.328e9, ("c2", 768, 48, 3, 0, 2): 8.038e9}
+# the same for one launch of the one-pass focusing + change kernel
+# (profiles/k_focus_change4_c2_kernel_trace.md), keyed by (workload,
+# precision mode)
+NODE_TRAFFIC = {("c2", 2): 18.51e9}
 
-CONFIGS = {
-    # rows, cols, N, E, M, numEpSteps
-    "c1": dict(rows=2, cols=2, N=400, E=20, M=20, eps=10),
-    "c2": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10),
-    # batched form: 1536 streams whose active-node counts are drawn from a
-    # seeded uniform 200..4000, as shells are injected
-    "c3": dict(rows=16, cols=16, N=4000, E=64, M=32, eps=10, ragged=(200, 4000)),
-    # mixed-precision form on the c2 workload with a 35-decade power-law f
-    "c4": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10, mixed=True),
-    # learned large-step surrogate of K = 16 streaming subcycles on the c2
-    # workload, trained on pairs from the plain solver
-    "c5": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10, surrogate=16),
-}
+CONFIGS = workloads.WORKLOADS
 
 
 def run_surrogate(args, c, p, dims):
@@ -202,30 +194,15 @@ def log(msg):
 
 
 def active_counts(c, S):
-    """Per-stream active-node counts (all N unless the config is ragged)."""
-    if "ragged" not in c:
-        return None
-    lo, hi = c["ragged"]
-    return np.random.default_rng(20261203).integers(lo, hi + 1, S).astype(np.int32)
+    return workloads.workload_active_counts(c, S)
 
 
 def make_params(c):
-    return config.default_params(
-        rScale=0.00465047, tDel=1.0 / 64, numEpSteps=c["eps"], eMin=0.01, eMax=1000.0, lamo=0.1,
-        rigidityPower=1.0 / 3.0, mfpRadialPower=1.0, mhdBAu=2.5e-5, boundaryFunctGamma=2.0,
-        boundaryFunctEcutoff=1.0e30)
+    return workloads.workload_params(c)
 
 
 def bench_nodes(c, seed=20261203, cme_r=0.5):
-    """Parker spiral (400 km/s, B_r 2.5 nT at 1 AU, 25-day rotation) with a
-    smooth seeded 10% perturbation (correlation ~20 nodes), 2% spacing
-    jitter and a Gaussian CME shell at 0.5 AU (1500 km/s, density and field
-    x4, 60 deg wide, 0.05 AU thick)."""
-    return workloads.parker_nodes(
-        c["rows"], c["cols"], c["N"], tdel_days=1.0 / 64, flow_cms=400e5, b_au=2.5e-5,
-        omega=2 * math.pi * K.TAU / (25 * 86400.0), rscale=0.00465047, seed=seed, perturb=0.1,
-        smooth=20, jitter=0.02,
-        cme=dict(r_au=cme_r, width_au=0.05, speed_cms=1500e5, dens=4.0, ang_deg=60.0))
+    return workloads.workload_nodes(c, seed=seed, cme_r=cme_r)
 
 
 MAS_BYTES_PER_NODE = 504.0  # 7 fields x 2 time levels x 8 FP32 corners read + 7 FP64 written
@@ -268,34 +245,90 @@ def interp_inside(c, p, lv0, lv1, reps):
             "bytes_per_node": MAS_BYTES_PER_NODE}
 
 
+def host_cores():
+    """Threads the CPU baseline may use: the cores this process may run on
+    (its affinity set, the box's share), capped by OMP_NUM_THREADS when set;
+    with what the host reports beside it."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    n = aff
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, {"nproc": os.cpu_count(), "affinity_cores": aff, "omp_num_threads": omp, "cpu_model": model}
+
+
 def cpu_baseline(c, p, nodes, n_active=None, budget_s=10.0):
     """The FP64 oracle (OpenMP over streams) on a bounded stream subset of
     the same workload, scaled per cell-update: one global step on the
-    subset, repeated from the same state until about budget_s seconds of
-    CPU work have been timed."""
+    subset (at least one stream per thread), repeated from the same state
+    until about budget_s seconds of CPU work have been timed.  Returns the
+    baseline record and, for the parity check, the subset, its initial f
+    and the oracle's f after the step."""
     import oracle
 
-    cores = min(16, os.cpu_count() or 1)
+    cores, host = host_cores()
     E, M = c["E"], c["M"]
-    nsub = max(1, min(cores, 16))
+    nsub = max(16, cores)
     sub = {k: np.ascontiguousarray(v[:nsub]) for k, v in nodes.items()}
     na = None if n_active is None else np.ascontiguousarray(n_active[:nsub])
     dims = workloads.dims_of(sub, E, M)
     g = oracle.build_grids(p, E, M)
     f0 = workloads.seeded_f(p, g, sub, noise=0.0)
     dt, cells, reps = 0.0, 0, 0
+    f1 = None
     while reps == 0 or (dt < budget_s and reps < 200):
         t0 = time.perf_counter()
-        rc, d, _, _, _, _ = oracle.update_ep(dims, p, sub, f0, n_active=na, nthreads=cores)
+        rc, d, fo, _, _, _ = oracle.update_ep(dims, p, sub, f0, n_active=na, nthreads=cores)
         dt += time.perf_counter() - t0
         if rc != 0:
             raise RuntimeError("oracle failed: %d" % rc)
+        if f1 is None:
+            f1 = fo
         cells += d["cell_updates"]
         reps += 1
-    return {"value": cells / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": "%d of %d streams x %d nodes x %d E x %d mu, 1 global step (%d EP substeps) "
-                      "repeated %d times from the same state, %.1f s" % (
-                          nsub, 6 * c["rows"] * c["cols"], c["N"], E, M, p.numEpSteps, reps, dt)}
+    rec = {"value": cells / dt, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+           "host": host,
+           "sample": "%d of %d streams x %d nodes x %d E x %d mu, 1 global step (%d EP substeps) "
+                     "repeated %d times from the same state, %.1f s, %d OpenMP threads" % (
+                         nsub, 6 * c["rows"] * c["cols"], c["N"], E, M, p.numEpSteps, reps, dt, cores)}
+    return rec, (sub, na, f0, f1)
+
+
+def parity_check(c, p, tiling, precision, sample):
+    """Max relative error of f against the oracle after one global step on
+    the CPU baseline's streams, run on the device with the benchmark's
+    precision mode and tiling (outside the timed region)."""
+    sub, na, f0, f1 = sample
+    dims = workloads.dims_of(sub, c["E"], c["M"])
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_precision(precision)
+        sol.set_tuning(tiling["block"], tiling["halo"], tiling["lane_nodes"], tiling["fuse_focus"])
+        sol.set_nodes(sub, na)
+        sol.upload_f(f0)
+        sol.step()
+        fg = sol.download_f()
+    finally:
+        sol.close()
+    mask = np.ones(fg.shape[:2], bool)
+    if na is not None:
+        for s, n in enumerate(na):
+            mask[s, n:] = False
+    rel = np.abs(fg[mask] - f1[mask]) / np.abs(f1[mask])
+    return {"max_rel_err_f": float(rel.max()), "median_rel_err_f": float(np.median(rel)),
+            "tolerance": 1e-10, "streams": int(dims.num_streams),
+            "against": "FP64 oracle, one global step from the seed population"}
 
 
 def main():
@@ -391,6 +424,18 @@ def main():
     avg_launch_s = st_s / launches
     achieved = alg_bytes_per_launch / avg_launch_s / 1e9
     fp64 = STREAM_FLOPS_PER_SWEEP * acc["stream_sweeps"] / st_s / 1e12
+    traffic = STREAM_TRAFFIC.get((args.config, tiling["block"], tiling["halo"], tiling["lane_nodes"],
+                                  tiling["fuse_focus"], args.precision))
+    # node stage (focusing + adiabatic change of each node in one pass):
+    # HBM bound, one read and one write of every active cell per substep
+    node_roof = None
+    if dg["max_subcycles_change"] <= 1 and acc["change_ms"] > 0:
+        node_bytes = 16.0 * acc["cell_updates"]
+        node_gbs = node_bytes / (acc["change_ms"] * 1e-3) / 1e9
+        node_roof = {"bound": "hbm", "kernel": "k_focus_change4" if c["M"] >= 12 else "k_focus_change",
+                     "achieved": node_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": node_gbs / HBM_PEAK_GBS,
+                     "traffic": NODE_TRAFFIC.get((args.config, args.precision)),
+                     "algorithmic_bytes_per_launch": node_bytes / (args.steps * p.numEpSteps)}
     res = {
         "metric": "phase-space cell-updates/s (1 GPU), surrogate vs plain at <=1% flux error",
         "value": value,
@@ -410,15 +455,16 @@ def main():
                    "path": "plain FP64 (no surrogate)",
                    "arithmetic": ("FP64, reference operation order" if args.precision == 0 else
                                   "FP64, fused multiply-add streaming (within 1e-13 of the reference order)")},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": STREAM_TRAFFIC.get((args.config, tiling["block"], tiling["halo"],
-                                                    tiling["lane_nodes"], tiling["fuse_focus"],
-                                                    args.precision)),
-                     "note": "16 B per cell per Euler sweep; the kernel keeps up to `halo` sweeps "
-                             "on chip, so the binding ceiling is FP64 VALU (see fp64)"},
-        "fp64": {"kernel": kname, "achieved_tflops": fp64, "peak_tflops": FP64_PEAK_TFLOPS,
-                 "frac": fp64 / FP64_PEAK_TFLOPS},
+        "roofline": {"bound": "fp64", "kernel": kname, "achieved": fp64, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": fp64 / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "note": "FP64 vector issue binds the streaming kernel: %d algorithmic flops per "
+                             "cell-sweep (upwind 4, isotropic sum 1, relaxation 3); it keeps up to `halo` "
+                             "sweeps on chip, so its HBM traffic is far below the 16 B per cell-sweep of "
+                             "one read and one write per sweep" % STREAM_FLOPS_PER_SWEEP,
+                     "algorithmic_bytes_per_launch": alg_bytes_per_launch,
+                     "avg_launch_ms": avg_launch_s * 1e3,
+                     "hbm_gbs_measured_traffic": (traffic / avg_launch_s / 1e9) if traffic else None},
+        "node_stage_roofline": node_roof,
         "breakdown_ms_per_step": {k: acc[k] / args.steps for k in
                                   ("device_ms", "prep_ms", "stream_ms", "focus_ms", "change_ms")},
         "node_stage": ("focus_ms: heavy-node focusing (k_focus_heavy); change_ms: focusing and adiabatic "
@@ -443,7 +489,9 @@ def main():
         res["interp_inside"] = interp_inside(c, p, lv0, lv1, args.steps)
     if not args.no_cpu:
         log("timed steps done; CPU baseline")
-        res["cpu_baseline"] = cpu_baseline(c, p, nodes, n_active)
+        res["cpu_baseline"], sample = cpu_baseline(c, p, nodes, n_active)
+        log("parity of the benchmark setting on the baseline's streams")
+        res["parity"] = parity_check(c, p, tiling, args.precision, sample)
     if sol is not None:
         sol.close()
     print(json.dumps(res))

@@ -192,6 +192,8 @@ EXPORTED_SYMBOLS = (
     "ep_set_nodes",
     "ep_upload_f",
     "ep_download_f",
+    "ep_upload_f_streams",
+    "ep_download_f_streams",
     "ep_init_seed_population",
     "ep_step",
     "ep_get_node_outputs",

@@ -299,6 +299,31 @@ def ep_params(cfg):
     return p
 
 
+# Options of the reference that change the analytic-wind host loop
+# (eprem_amd.driver.ParkerRun) and that it does not carry out: with one of
+# them set, a run would silently differ from the reference's.
+#   masCouple          MAS background instead of the analytic wind
+#                      (src/eprem.c:472-480, src/flow.c:48-77)
+#   idealShock         the cone test's shock added to the wind (src/flow.c:
+#                      280-403 with config.idealShock)
+#   useShellDiffusion  DiffuseShellData / DriftShellData: the library has
+#   useDrift           them (ep_shell_operators) but the host loop does not
+#                      call them
+#   masCorRotateFake   the fake-corotation B_phi term of masWind
+#                      (src/masInterp.c:1039-1047), which the library's
+#                      masWind does not add
+UNSUPPORTED_BY_HOST_LOOP = ("masCouple", "idealShock", "useShellDiffusion", "useDrift", "masCorRotateFake")
+
+
+def check_host_loop_options(cfg):
+    """Raise ConfigError for every option the analytic-wind host loop does
+    not implement (UNSUPPORTED_BY_HOST_LOOP), naming them all."""
+    bad = [k for k in UNSUPPORTED_BY_HOST_LOOP if cfg.get(k, 0)]
+    if bad:
+        raise ConfigError("options not implemented by the analytic-wind host loop: " +
+                          ", ".join("%s=%r" % (k, cfg[k]) for k in bad))
+
+
 def default_params(**over):
     """EpParams from the reference defaults, overridden by keyword."""
     cfg = read_config("")
@@ -312,4 +337,4 @@ def stream_count(cfg):
 
 
 __all__ = ["ConfigError", "parse_libconfig", "read_config", "ep_params", "default_params",
-           "stream_count"]
+           "stream_count", "check_host_loop_options", "UNSUPPORTED_BY_HOST_LOOP"]

@@ -461,10 +461,33 @@ int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes,
 // fastest and leaves it set.  Candidates: one node per lane with 128..512
 // lanes, two nodes per lane with 128..512 lanes (chunks of 256..1024
 // nodes); halos keep at least half of a chunk in its interior.
+static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_nodes,
+                         int32_t *best_fuse, double *best_ms);
+
 int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_nodes, int32_t *best_fuse,
                 double *best_ms) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
   if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
+  // the search changes the tiling while it times candidates: on an error
+  // the tiling it found is restored and the task tables rebuilt for it, and
+  // any captured step (taken with the tiling as a kernel argument) dropped
+  const int W0 = g.stream_block, H0 = g.stream_halo, R0 = g.stream_r, F0 = g.stream_fuse;
+  g.drop_graph();
+  const int rc = autotune_impl(best_block, best_halo, best_lane_nodes, best_fuse, best_ms);
+  if (rc != EP_OK) {
+    g.stream_block = W0;
+    g.stream_halo = H0;
+    g.stream_r = R0;
+    g.stream_fuse = F0;
+    g.drop_graph();
+    Dev D = make_dev();
+    if (launch_task_prep(D, g.st) == hipSuccess) (void)hipStreamSynchronize(g.st);
+  }
+  return rc;
+}
+
+static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_nodes,
+                         int32_t *best_fuse, double *best_ms) {
   int rc = ensure_events(2);
   if (rc) return rc;
   hipStream_t st = g.st;
@@ -635,30 +658,48 @@ int ep_set_nodes(const ep_node_soa *nodes, const int32_t *n_active) {
   return EP_OK;
 }
 
-int ep_upload_f(const double *f, int32_t layout) {
+// Streams s0 .. s0+count-1 of f between the host array (reference layout,
+// those streams only) and the device.  Every slab is first brought back
+// to buffer A, so that buffer B can stage the host data.
+static int transfer_f(int32_t s0, int32_t count, double *hf, int upload) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
-  if (!f) return fail(EP_ERR_INVALID_ARG, "null f");
-  if (layout != EP_LAYOUT_REFERENCE) return fail(EP_ERR_INVALID_ARG, "unknown layout %d", layout);
-  const size_t cnt = (size_t)g.d.num_streams * g.d.num_nodes * g.d.num_energies * g.d.num_mu;
+  if (!hf) return fail(EP_ERR_INVALID_ARG, "null f");
+  if (s0 < 0 || count < 1 || s0 + count > g.d.num_streams)
+    return fail(EP_ERR_INVALID_ARG, "streams %d..%d outside 0..%d", s0, s0 + count - 1, g.d.num_streams - 1);
+  const size_t cnt = (size_t)count * g.d.num_nodes * g.d.num_energies * g.d.num_mu;
   Dev D = make_dev();
-  HIPCHK(hipMemcpy(g.fB, f, cnt * 8, hipMemcpyHostToDevice), "f upload");
-  HIPCHK(launch_from_reference(D, g.fB, g.fA, g.st), "from_reference");
-  HIPCHK(hipMemsetAsync(g.loc, 0, (size_t)g.d.num_streams * g.d.num_energies, g.st), "loc reset");
-  HIPCHK(hipStreamSynchronize(g.st), "upload sync");
+  HIPCHK(launch_normalize(D, g.st), "normalize");
+  if (upload) {
+    HIPCHK(hipStreamSynchronize(g.st), "normalize sync");
+    HIPCHK(hipMemcpy(g.fB, hf, cnt * 8, hipMemcpyHostToDevice), "f upload");
+    HIPCHK(launch_from_reference(D, g.fB, g.fA, g.st, s0, count), "from_reference");
+    HIPCHK(hipStreamSynchronize(g.st), "upload sync");
+  } else {
+    HIPCHK(launch_to_reference(D, g.fA, g.fB, g.st, s0, count), "to_reference");
+    HIPCHK(hipStreamSynchronize(g.st), "download sync");
+    HIPCHK(hipMemcpy(hf, g.fB, cnt * 8, hipMemcpyDeviceToHost), "f download");
+  }
   return EP_OK;
 }
 
-int ep_download_f(double *f, int32_t layout) {
-  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
-  if (!f) return fail(EP_ERR_INVALID_ARG, "null f");
+int ep_upload_f(const double *f, int32_t layout) {
   if (layout != EP_LAYOUT_REFERENCE) return fail(EP_ERR_INVALID_ARG, "unknown layout %d", layout);
-  const size_t cnt = (size_t)g.d.num_streams * g.d.num_nodes * g.d.num_energies * g.d.num_mu;
-  Dev D = make_dev();
-  HIPCHK(launch_normalize(D, g.st), "normalize");
-  HIPCHK(launch_to_reference(D, g.fA, g.fB, g.st), "to_reference");
-  HIPCHK(hipStreamSynchronize(g.st), "download sync");
-  HIPCHK(hipMemcpy(f, g.fB, cnt * 8, hipMemcpyDeviceToHost), "f download");
-  return EP_OK;
+  return transfer_f(0, g.init ? g.d.num_streams : 1, const_cast<double *>(f), 1);
+}
+
+int ep_download_f(double *f, int32_t layout) {
+  if (layout != EP_LAYOUT_REFERENCE) return fail(EP_ERR_INVALID_ARG, "unknown layout %d", layout);
+  return transfer_f(0, g.init ? g.d.num_streams : 1, f, 0);
+}
+
+int ep_upload_f_streams(int32_t stream0, int32_t count, const double *f, int32_t layout) {
+  if (layout != EP_LAYOUT_REFERENCE) return fail(EP_ERR_INVALID_ARG, "unknown layout %d", layout);
+  return transfer_f(stream0, count, const_cast<double *>(f), 1);
+}
+
+int ep_download_f_streams(int32_t stream0, int32_t count, double *f, int32_t layout) {
+  if (layout != EP_LAYOUT_REFERENCE) return fail(EP_ERR_INVALID_ARG, "unknown layout %d", layout);
+  return transfer_f(stream0, count, f, 0);
 }
 
 int ep_init_seed_population(void) {

@@ -665,20 +665,20 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
 // ------------------------------------------------------------------------
 // host order [s][node][e][m] (staged in `stage`) -> device [s][e][m][node]
 template <typename T>
-__global__ __launch_bounds__(256) void k_from_reference(Dev D, const double *stage, T *dstbuf) {
+__global__ __launch_bounds__(256) void k_from_reference(Dev D, const double *stage, T *dstbuf, int s0) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
-  const int e = blockIdx.y, s = blockIdx.z;
+  const int e = blockIdx.y, s = s0 + blockIdx.z;
   if (node >= D.N) return;
-  const size_t src = (((size_t)s * D.N + node) * D.E + e) * D.M;
+  const size_t src = (((size_t)blockIdx.z * D.N + node) * D.E + e) * D.M;
   for (int m = 0; m < D.M; m++) dstbuf[fidx(D, s, e, m, node)] = (T)stage[src + m];
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_to_reference(Dev D, const T *srcbuf, double *stage) {
+__global__ __launch_bounds__(256) void k_to_reference(Dev D, const T *srcbuf, double *stage, int s0) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
-  const int e = blockIdx.y, s = blockIdx.z;
+  const int e = blockIdx.y, s = s0 + blockIdx.z;
   if (node >= D.N) return;
-  const size_t dst = (((size_t)s * D.N + node) * D.E + e) * D.M;
+  const size_t dst = (((size_t)blockIdx.z * D.N + node) * D.E + e) * D.M;
   for (int m = 0; m < D.M; m++) stage[dst + m] = (double)srcbuf[fidx(D, s, e, m, node)];
 }
 
@@ -868,22 +868,26 @@ hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
   return hipSuccess;
 }
 
-hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st) {
-  const dim3 grid((D.N + 255) / 256, D.E, D.S);
+hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st, int s0,
+                                 int count) {
+  if (count < 0) count = D.S - s0;
+  const dim3 grid((D.N + 255) / 256, D.E, count);
   if (D.fp32)
-    hipLaunchKernelGGL(k_from_reference<float>, grid, dim3(256), 0, st, D, stage, (float *)dstbuf);
+    hipLaunchKernelGGL(k_from_reference<float>, grid, dim3(256), 0, st, D, stage, (float *)dstbuf, s0);
   else
-    hipLaunchKernelGGL(k_from_reference<double>, grid, dim3(256), 0, st, D, stage, dstbuf);
+    hipLaunchKernelGGL(k_from_reference<double>, grid, dim3(256), 0, st, D, stage, dstbuf, s0);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }
 
-hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st) {
-  const dim3 grid((D.N + 255) / 256, D.E, D.S);
+hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st, int s0,
+                               int count) {
+  if (count < 0) count = D.S - s0;
+  const dim3 grid((D.N + 255) / 256, D.E, count);
   if (D.fp32)
-    hipLaunchKernelGGL(k_to_reference<float>, grid, dim3(256), 0, st, D, (const float *)srcbuf, stage);
+    hipLaunchKernelGGL(k_to_reference<float>, grid, dim3(256), 0, st, D, (const float *)srcbuf, stage, s0);
   else
-    hipLaunchKernelGGL(k_to_reference<double>, grid, dim3(256), 0, st, D, srcbuf, stage);
+    hipLaunchKernelGGL(k_to_reference<double>, grid, dim3(256), 0, st, D, srcbuf, stage, s0);
   EP_CHECK_LAUNCH();
   return hipSuccess;
 }

@@ -30,8 +30,12 @@ hipError_t launch_focus_heavy(const Dev &D, int substep, hipStream_t st);
 // focusing then change of every node in one pass (ep_node.hip); heavy
 // nodes are only changed (launch_focus_heavy first)
 hipError_t launch_focus_change(const Dev &D, int substep, int fused, int spare, hipStream_t st);
-hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st);
-hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st);
+// streams s0 .. s0+count-1 (count < 0: to the last stream); the stage holds
+// them from its start
+hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstbuf, hipStream_t st, int s0 = 0,
+                                 int count = -1);
+hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st, int s0 = 0,
+                               int count = -1);
 hipError_t launch_normalize(const Dev &D, hipStream_t st);
 hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st);
 hipError_t launch_flip_all(const Dev &D, hipStream_t st);

@@ -448,6 +448,8 @@ int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *
   if (device_ms) *device_ms = ms;
   g.nodes_set = true;
   g.mhd_state = true;
+  // the drift kernels of a captured step take the analytic field by value
+  if (g.mhd_mode != mp->mode || g.mhd_bsau != mp->mhdBsAu || g.mhd_omega != mp->omegaSun) g.drop_graph();
   g.mhd_mode = mp->mode;
   g.mhd_bsau = mp->mhdBsAu;
   g.mhd_omega = mp->omegaSun;

@@ -76,6 +76,7 @@ class ParkerRun:
     """One analytic-wind run (masCouple = 0) on the device."""
 
     def __init__(self, cfg):
+        config.check_host_loop_options(cfg)
         self.cfg = cfg
         self.R, self.C = cfg["numRowsPerFace"], cfg["numColumnsPerFace"]
         self.N, self.E, self.M = cfg["numNodesPerStream"], cfg["numEnergySteps"], cfg["numMuSteps"]

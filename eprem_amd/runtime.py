@@ -42,6 +42,8 @@ def lib():
         L.ep_set_nodes.argtypes = [C.POINTER(EpNodeSoa), IP]
         L.ep_upload_f.argtypes = [DP, C.c_int32]
         L.ep_download_f.argtypes = [DP, C.c_int32]
+        L.ep_upload_f_streams.argtypes = [C.c_int32, C.c_int32, DP, C.c_int32]
+        L.ep_download_f_streams.argtypes = [C.c_int32, C.c_int32, DP, C.c_int32]
         L.ep_init_seed_population.argtypes = []
         L.ep_step.argtypes = [C.c_uint32, C.POINTER(EpDiag)]
         L.ep_get_node_outputs.argtypes = [DP, DP, DP]
@@ -173,6 +175,17 @@ class EpSolver:
     def download_f(self):
         f = np.empty(self.shape_f, dtype=np.float64)
         _check(lib().ep_download_f(dptr(f), EP_LAYOUT_REFERENCE))
+        return f
+
+    def upload_f_streams(self, s0, f):
+        """f [count][N][E][M] for streams s0 .. s0+count-1."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape[1:] == self.shape_f[1:], (f.shape, self.shape_f)
+        _check(lib().ep_upload_f_streams(int(s0), int(f.shape[0]), dptr(f), EP_LAYOUT_REFERENCE))
+
+    def download_f_streams(self, s0, count):
+        f = np.empty((count,) + tuple(self.shape_f[1:]), dtype=np.float64)
+        _check(lib().ep_download_f_streams(int(s0), int(count), dptr(f), EP_LAYOUT_REFERENCE))
         return f
 
     def init_seed_population(self):

@@ -191,3 +191,54 @@ def power_law_f(grids, nodes, M, seed=None, f_max=1.0e5, index=(-5.0, -2.0), rad
     rad = (rm / rm[:, :1]) ** radial_power                          # [S, N]
     f = f_max * rad[:, :, None] * spec[:, None, :]
     return np.repeat(f[..., None], M, axis=3)
+
+
+# The benchmark workloads (rows, cols, nodes N, energies E, mu cells M, EP
+# substeps per global step), shared by bench.py and the tests that check
+# them at full per-stream size.
+WORKLOADS = {
+    # plain method, small: 24 streams x 400 x 20 x 20
+    "c1": dict(rows=2, cols=2, N=400, E=20, M=20, eps=10),
+    # plain method at scale: 384 streams x 2000 x 50 x 30
+    "c2": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10),
+    # batched form: 1536 streams whose active-node counts are drawn from a
+    # seeded uniform 200..4000, as shells are injected; 64 x 32
+    "c3": dict(rows=16, cols=16, N=4000, E=64, M=32, eps=10, ragged=(200, 4000)),
+    # mixed-precision form on the 384 x 2000 x 50 x 30 workload with a
+    # 35-decade power-law f
+    "c4": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10, mixed=True),
+    # learned large-step surrogate on the 384 x 2000 x 50 x 30 workload
+    "c5": dict(rows=8, cols=8, N=2000, E=50, M=30, eps=10, surrogate=16),
+}
+
+
+def workload_params(c):
+    """EpParams of the benchmark workloads: lambda = 0.1 AU (R / 1 GV)^(1/3)
+    (r / 1 AU), seed spectrum E^-2 without cutoff, B_r = 2.5 nT at 1 AU,
+    10 keV - 1 GeV, tDel = 1/64 day."""
+    from . import config
+    return config.default_params(
+        rScale=0.00465047, tDel=1.0 / 64, numEpSteps=c["eps"], eMin=0.01, eMax=1000.0, lamo=0.1,
+        rigidityPower=1.0 / 3.0, mfpRadialPower=1.0, mhdBAu=2.5e-5, boundaryFunctGamma=2.0,
+        boundaryFunctEcutoff=1.0e30)
+
+
+def workload_nodes(c, seed=20261203, cme_r=0.5):
+    """Parker spiral (400 km/s, B_r 2.5 nT at 1 AU, 25-day rotation) with a
+    smooth seeded 10% perturbation (correlation ~20 nodes), 2% spacing
+    jitter and a Gaussian CME shell at cme_r AU (1500 km/s, density and
+    field x4, 60 deg wide, 0.05 AU thick)."""
+    return parker_nodes(
+        c["rows"], c["cols"], c["N"], tdel_days=1.0 / 64, flow_cms=400e5, b_au=2.5e-5,
+        omega=2 * math.pi * K.TAU / (25 * 86400.0), rscale=0.00465047, seed=seed, perturb=0.1,
+        smooth=20, jitter=0.02,
+        cme=dict(r_au=cme_r, width_au=0.05, speed_cms=1500e5, dens=4.0, ang_deg=60.0))
+
+
+def workload_active_counts(c, S):
+    """Per-stream active-node counts: None (all N) unless the workload is
+    ragged, then a seeded uniform draw over its range."""
+    if "ragged" not in c:
+        return None
+    lo, hi = c["ragged"]
+    return np.random.default_rng(20261203).integers(lo, hi + 1, S).astype(np.int32)

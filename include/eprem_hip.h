@@ -170,6 +170,15 @@ int ep_set_nodes(const ep_node_soa *nodes, const int32_t *n_active);
 int ep_upload_f(const double *f, int32_t layout);
 int ep_download_f(double *f, int32_t layout);
 
+/* The same for streams stream0 .. stream0+count-1 only: the host array
+ * holds count*num_nodes*num_energies*num_mu doubles, those streams in the
+ * reference layout.  The host's access to one stream's distribution, as
+ * the reference's gather update_stream_from_shells / scatter
+ * update_shells_from_stream give it (src/simCore.c:177-318), without
+ * moving the whole distribution over PCIe. */
+int ep_upload_f_streams(int32_t stream0, int32_t count, const double *f, int32_t layout);
+int ep_download_f_streams(int32_t stream0, int32_t count, double *f, int32_t layout);
+
 /* Fill f with the seed population sepSeedFunction(E, rmag) at every node
  * (initEnergeticParticles, src/energeticParticlesInit.c:106-147). */
 int ep_init_seed_population(void);

@@ -45,3 +45,26 @@ def test_range_check_panics():
 def test_comments_arrays_strings():
     raw = config.parse_libconfig('# c\nmass=[1.0, 4.0];  // x\n/* block\n */ masDirectory="a b"\nx=1e-3\n')
     assert raw == {"mass": [1.0, 4.0], "masDirectory": "a b", "x": 1e-3}
+
+
+@pytest.mark.parametrize("case,bad", [("cone", ["idealShock"]), ("mas_wind", ["masCouple"]),
+                                      ("mas_cme", ["masCouple"])])
+def test_host_loop_rejects_what_it_does_not_run(case, bad, golden_dir):
+    """The analytic-wind host loop refuses inputs it would run as a plain
+    Parker wind: the cone test's ideal shock and the MAS-coupled cases
+    (the check runs before any device call, so this needs no GPU)."""
+    from eprem_amd import driver
+    cfg = config.read_config(open(os.path.join(golden_dir, case, "eprem_input_file")).read())
+    with pytest.raises(config.ConfigError) as ei:
+        driver.ParkerRun(cfg)
+    for k in bad:
+        assert k in str(ei.value)
+
+
+@pytest.mark.parametrize("opt", config.UNSUPPORTED_BY_HOST_LOOP)
+def test_host_loop_rejects_each_option(opt, golden_dir):
+    cfg = config.read_config(open(os.path.join(golden_dir, "wind", "eprem_input_file")).read())
+    config.check_host_loop_options(cfg)  # the wind case itself is supported
+    cfg[opt] = 1
+    with pytest.raises(config.ConfigError, match=opt):
+        config.check_host_loop_options(cfg)

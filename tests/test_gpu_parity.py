@@ -250,6 +250,56 @@ def test_mas_interp_parity(levels_differ):
         np.testing.assert_allclose(out[k].ravel(), ref[k], rtol=1e-13, atol=0, err_msg=k)
 
 
+def test_mas_interp_real_mas_wind_data():
+    """masTriLinear on the device vs the oracle with the reference's own
+    MAS input (testsuite/mas_wind levels 1 and 2, seven fields on their own
+    staggered meshes, tests/golden/mas_grids): nodes on random radial lines
+    from 1.01 to 240 Rs (past the outer mesh point, where the radial weight
+    clamps), with a phi offset that wraps, V_r clamped at masVmin, and the
+    nodes beyond masRadialMax taking masWind."""
+    import os
+
+    from eprem_amd import constants as K
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mas_grids",
+                             "mas_wind_levels.npz"))
+    names = ("bp", "bt", "br", "vp", "vt", "vr", "rho")
+    lv = [([z["%d_%s_data" % (l, f)] for f in names], [z["%d_%s_%s" % (l, f, a)] for f in names for a in "rtp"])
+          for l in (1, 2)]
+    rng = np.random.default_rng(31)
+    S, N = 6, 200
+    rscale = 0.00465047
+    th = rng.uniform(0.05, np.pi - 0.05, S)
+    ph = rng.uniform(-np.pi, np.pi, S)
+    r_rs = np.exp(np.linspace(np.log(1.01), np.log(240.0), N))[None, :] * rng.uniform(0.98, 1.02, (S, N))
+    rau = r_rs * K.RSAU
+    nodes = {"x": rau * (np.sin(th) * np.cos(ph))[:, None] / rscale,
+             "y": rau * (np.sin(th) * np.sin(ph))[:, None] / rscale,
+             "z": rau * np.cos(th)[:, None] / rscale}
+    nodes["rmag"] = np.sqrt(nodes["x"] ** 2 + nodes["y"] ** 2 + nodes["z"] ** 2)
+    for k in ("bmag", "dlnb", "dlnn", "dupar"):
+        nodes[k] = np.ones((S, N))
+    from eprem_amd import mas_synth
+    prev = mas_synth.prev_from_nodes(nodes, rscale)
+    p = params(rScale=rscale)
+    dims = workloads.dims_of(nodes, 4, 4)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_nodes(nodes)
+        sol.mas_set_level(0, *lv[0])
+        sol.mas_set_level(1, *lv[1])
+        out = sol.interp_mhd(0.61, 2.9, 200.0, 150.0e5, prev)
+    finally:
+        sol.close()
+    ref = oracle.mas_interp(rscale, nodes["x"].ravel(), nodes["y"].ravel(), nodes["z"].ravel(),
+                            nodes["rmag"].ravel(), lv[0], lv[1], 0.61, 2.9, 200.0, 150.0e5,
+                            {k: v.ravel() for k, v in prev.items()})
+    inside = r_rs.ravel() <= 200.0
+    assert 0 < inside.sum() < inside.size
+    assert np.any(ref["vr"] == 150.0e5 / K.C)  # the clamp is exercised
+    for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density"):
+        np.testing.assert_allclose(out[k].ravel(), ref[k], rtol=1e-13, atol=0, err_msg=k)
+
+
 def test_observers_and_flux_parity():
     """Point-observer projection, inverse-distance blend and flux on the
     device equal the oracle's (same summation order over streams).  The
@@ -388,11 +438,19 @@ def test_two_nodes_per_lane_parity(case):
     dict(N=200, E=8, M=16, thin=0.1, ragged=True, precision=2, tiling=(1152, 30, 3)),
     dict(N=257, E=8, M=32, thin=0.0, ragged=False, precision=0, tiling=(1536, 100, 3)),
     dict(N=150, E=12, M=30, thin=0.0, ragged=False, precision=2, tiling=(768, 24, 3, 1)),
+    # several chunks per slab (interior W - 2h below the node count): the
+    # halos cross chunk boundaries and only interiors are stored
+    dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=0, tiling=(192, 8, 3, 0)),
+    dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=2, tiling=(192, 8, 3, 1)),
+    dict(N=300, E=10, M=30, thin=0.15, ragged=False, precision=2, tiling=(192, 12, 3, 1)),
+    dict(N=300, E=10, M=17, thin=0.1, ragged=True, precision=0, tiling=(192, 8, 3, 0)),
+    dict(N=700, E=8, M=30, thin=0.0, ragged=True, precision=2, tiling=(384, 40, 3, 1)),
 ])
 def test_three_nodes_per_lane_parity(case):
     """The streaming kernel with three kept nodes per lane against the
     oracle: thinned and ragged streams, odd and even M, both FP64 modes,
-    chunks of one to eight waves, focusing in the last pass, two steps."""
+    chunks of one to eight waves, one to several chunks per slab, focusing
+    in the last pass, two steps."""
     p = params(numEpSteps=2)
     nodes = workloads.parker_nodes(1, 1, case["N"], seed=62, perturb=0.2, jitter=0.2, thin_frac=case["thin"])
     n_active = None
