@@ -35,11 +35,12 @@ STREAM_FLOPS_PER_SWEEP = 8   # upwind 4 + iso sum 1 + relaxation 3 (FP64)
 # HBM bytes per k_stream_pass launch, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (each scaled by what the counter reads for a
 # plain 1 GiB copy kernel: FETCH x 2.000, WRITE x 1.000), keyed by
-# (workload, block, halo, nodes per lane, fused focusing, precision mode); the commands and per-kernel
+# (workload, block, halo, nodes per lane, fused focusing, precision mode,
+# schedule: 0 a launch per pass, 1 a persistent launch per substep); the commands and per-kernel
 # figures are in profiles/k_stream_pass2_c2_counters.md (two nodes per lane)
 # and profiles/k_stream_passR_c2_kernel_trace.md (three).  Other tilings
 # report null.
-STREAM_TRAFFIC = {("c2", 512, 48, 2, 0, 2): 8.328e9, ("c2", 768, 48, 3, 0, 2): 8.038e9}
+STREAM_TRAFFIC = {("c2", 512, 48, 2, 0, 2, 0): 8.328e9, ("c2", 768, 48, 3, 0, 2, 0): 8.038e9}
 # the same for one launch of the one-pass focusing + change kernel
 # (profiles/k_focus_change4_c2_kernel_trace.md), keyed by (workload,
 # precision mode)
@@ -343,6 +344,10 @@ def main():
     ap.add_argument("--fuse-focus", type=int, default=1, choices=(0, 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mas", action="store_true")
+    ap.add_argument("--no-unbatched", action="store_true")
+    ap.add_argument("--schedule", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="with --block/--halo: streaming schedule (0 a launch per pass, 1 persistent, "
+                         "2 per stream); -1 keeps the library's default")
     ap.add_argument("--precision", type=int, default=2, choices=(0, 2),
                     help="0: FP64 in the reference's operation order; 2: FP64 with fused "
                          "multiply-adds in streaming (within 1e-13 of it)")
@@ -370,6 +375,8 @@ def main():
     log("nodes and seed population set; precision mode %d" % args.precision)
     if args.block or args.halo:
         sol.set_tuning(args.block or 256, args.halo or 32, args.lane_nodes, args.fuse_focus)
+        if args.schedule >= 0:
+            sol.set_schedule(args.schedule)
         tiling = {"block": args.block or 256, "halo": args.halo or 32, "lane_nodes": args.lane_nodes,
                   "fuse_focus": args.fuse_focus, "source": "flags"}
     else:
@@ -377,9 +384,12 @@ def main():
         log("autotune: block %d halo %d lane_nodes %d fuse_focus %d" % (blk, halo, r, fz))
         tiling = {"block": blk, "halo": halo, "lane_nodes": r, "fuse_focus": fz, "source": "autotune",
                   "stream_and_focus_ms_per_substep": tune_ms}
-    # the timed steps launch kernels one by one so that HIP events bracket
-    # each operator; the captured-graph form is timed afterwards
-    sol.set_graph(False)
+    tiling["schedule"] = {0: "one launch per pass", 1: "one persistent launch per substep",
+                          2: "one launch per stream"}[sol.schedule()]
+    sched = sol.schedule()
+    # the timed steps replay the captured step graph (the library's default);
+    # the same steps with the kernels launched one by one, which lets HIP
+    # events bracket each operator, give the per-operator breakdown
 
     mas_ms = None
     if not args.no_mas:
@@ -397,27 +407,49 @@ def main():
     acc = dict(device_ms=0.0, stream_ms=0.0, focus_ms=0.0, change_ms=0.0, prep_ms=0.0,
                stream_sweeps=0, focus_sweeps=0, change_sweeps=0, stream_launches=0, cell_updates=0)
     mas_total = 0.0
+    cells = 0
     t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, dg = sol.step()
+        cells += dg["cell_updates"]
+    t1 = time.perf_counter()
+    log("timed steps done (graph)")
+    sol.set_graph(False)
+    te0 = time.perf_counter()
     for _ in range(args.steps):
         _, dg = sol.step()
         for k in acc:
             acc[k] += dg[k]
-    t1 = time.perf_counter()
+    te1 = time.perf_counter()
     sol.set_graph(True)
-    sol.step()  # capture
-    tg0 = time.perf_counter()
-    for _ in range(args.steps):
+    unbatched = None
+    if c.get("ragged") and sched == 1 and not args.no_unbatched:
+        # the unbatched form of the same persistent kernel: one launch per
+        # stream per substep, the reference's loop over streams
+        sol.set_schedule(2)
+        sol.set_graph(False)
         sol.step()
-    tg1 = time.perf_counter()
+        tu0 = time.perf_counter()
+        _, du = sol.step()
+        tu1 = time.perf_counter()
+        unbatched = {"ms_per_step": (tu1 - tu0) * 1e3, "stream_ms_per_step": du["stream_ms"],
+                     "stream_launches_per_step": du["stream_launches"],
+                     "batched_stream_ms_per_step": acc["stream_ms"] / args.steps,
+                     "batched_over_unbatched_speedup": (tu1 - tu0) * 1e3 / ((t1 - t0) * 1e3 / args.steps)}
+        sol.set_schedule(1)
+        sol.set_graph(True)
+        log("unbatched comparison done")
     if not args.no_mas:
         for _ in range(args.steps):
             mas_total += sol.interp_mhd(0.5, 0.0, 30.0, 50.0e5, prev)["device_ms"]
         mas_ms = mas_total / args.steps
     wall = t1 - t0
-    value = acc["cell_updates"] / wall
+    value = cells / wall
 
     # dominant kernel: the streaming passes
     kname = {1: "k_stream_pass", 2: "k_stream_pass2", 3: "k_stream_passR<R=3>"}[tiling["lane_nodes"]]
+    if sched:
+        kname = "k_stream_slab<R=%d>" % tiling["lane_nodes"]
     st_s = acc["stream_ms"] / 1e3
     launches = max(1, acc["stream_launches"])
     alg_bytes_per_launch = 16.0 * acc["stream_sweeps"] / launches
@@ -425,7 +457,7 @@ def main():
     achieved = alg_bytes_per_launch / avg_launch_s / 1e9
     fp64 = STREAM_FLOPS_PER_SWEEP * acc["stream_sweeps"] / st_s / 1e12
     traffic = STREAM_TRAFFIC.get((args.config, tiling["block"], tiling["halo"], tiling["lane_nodes"],
-                                  tiling["fuse_focus"], args.precision))
+                                  tiling["fuse_focus"], args.precision, sched))
     # node stage (focusing + adiabatic change of each node in one pass):
     # HBM bound, one read and one write of every active cell per substep
     node_roof = None
@@ -477,8 +509,12 @@ def main():
         "tiling": tiling,
         "max_subcycles": {k: dg[k] for k in ("max_nsteps_stream", "max_subcycles_focusing",
                                              "max_subcycles_change")},
-        "graph_ms_per_step": (tg1 - tg0) * 1e3 / args.steps,
+        "eager_ms_per_step": (te1 - te0) * 1e3 / args.steps,
+        "eager_note": "the same steps with kernels launched one by one and HIP events around each operator "
+                      "(the breakdown and roofline figures come from these)",
     }
+    if unbatched is not None:
+        res["unbatched"] = unbatched
     if n_active is not None:
         res["config"]["active_nodes"] = {"min": int(n_active.min()), "max": int(n_active.max()),
                                          "mean": float(n_active.mean())}

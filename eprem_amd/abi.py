@@ -202,6 +202,8 @@ EXPORTED_SYMBOLS = (
     "ep_last_error",
     "ep_finalize",
     "ep_set_tuning",
+    "ep_set_schedule",
+    "ep_get_schedule",
     "ep_selftest_dpp",
     "ep_observe",
     "ep_stream_flux",

@@ -195,6 +195,7 @@ Dev make_dev() {
   D.stream_halo = g.stream_halo;
   D.stream_r = g.stream_r;
   D.stream_fuse = g.stream_fuse;
+  D.stream_whole = g.stream_whole;
   D.mfpRadialPower = p.mfpRadialPower;
   D.mfpBPower = p.mfpBPower;
   D.mhdBsAu = p.mhdBsAu;
@@ -322,6 +323,13 @@ static int ensure_events(int count) {
   return EP_OK;
 }
 
+// Streaming schedule in use: the persistent slab launch needs two or three
+// nodes per lane; with one, the passes are launched one by one.
+static int effective_schedule() {
+  if (g.stream_sched != 0 && g.stream_r == 1) return 0;
+  return g.stream_sched;
+}
+
 // The EP substep loop of one global step (src/energeticParticles.c:
 // 102-199): streaming passes and bookkeeping, then focusing and adiabatic
 // change.  With `ev` (per-operator events, not used under graph capture)
@@ -339,12 +347,31 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
     if (sh.do_seed && !sh.do_stream) EPQ(launch_seed_check(D, st));
     if (sh.do_stream) {
       // focusing of full-resolution streams can ride on each slab's last pass
-      for (int pass = 0; pass < sh.passes; pass++)
-        EPQ(launch_stream_pass(D, pass, pass & 1, sh.do_seed && pass == 0, sub, sh.chunks, fuse, st));
+      if (sh.sched == 0) {
+        for (int pass = 0; pass < sh.passes; pass++)
+          EPQ(launch_stream_pass(D, pass, pass & 1, sh.do_seed && pass == 0, sub, sh.chunks, fuse, st));
+      } else if (sh.sched == 1) {
+        EPQ(launch_stream_slab(D, sh.do_seed, sub, fuse, 0, -1, st));
+      } else {
+        for (int s = 0; s < D.S; s++) EPQ(launch_stream_slab(D, sh.do_seed, sub, fuse, 0, s, st));
+      }
       EPQ(launch_stream_finish(D, sub, st));
     }
     if (e4) EPQ(hipEventRecord(e4[0], st));
-    if (sh.node_fused) {
+    if (sh.node_fused == 2) {
+      // the path is chosen on the device from the preparation's largest
+      // change subcycle count: heavy nodes are focused first on both paths,
+      // then either every node is focused and changed in one pass over f
+      // (node_gate 1) or focused and changed by two kernels (node_gate 2)
+      EPQ(launch_focus_heavy(D, sub, st));
+      if (e4) EPQ(hipEventRecord(e4[1], st));
+      Dev D1 = D, D2 = D;
+      D1.node_gate = 1;
+      D2.node_gate = 2;
+      EPQ(launch_focus_change(D1, sub, fuse, 0, st));
+      EPQ(launch_focus(D2, sub, fuse, st, 0, 0));
+      EPQ(launch_change(D2, sub, st));
+    } else if (sh.node_fused) {
       // heavy nodes are focused first; every node is then focused (unless
       // done already) and changed in one pass over f
       EPQ(launch_focus_heavy(D, sub, st));
@@ -471,7 +498,7 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_node
   // the search changes the tiling while it times candidates: on an error
   // the tiling it found is restored and the task tables rebuilt for it, and
   // any captured step (taken with the tiling as a kernel argument) dropped
-  const int W0 = g.stream_block, H0 = g.stream_halo, R0 = g.stream_r, F0 = g.stream_fuse;
+  const int W0 = g.stream_block, H0 = g.stream_halo, R0 = g.stream_r, F0 = g.stream_fuse, S0 = g.stream_sched;
   g.drop_graph();
   const int rc = autotune_impl(best_block, best_halo, best_lane_nodes, best_fuse, best_ms);
   if (rc != EP_OK) {
@@ -479,6 +506,7 @@ int ep_autotune(int32_t *best_block, int32_t *best_halo, int32_t *best_lane_node
     g.stream_halo = H0;
     g.stream_r = R0;
     g.stream_fuse = F0;
+    g.stream_sched = S0;
     g.drop_graph();
     Dev D = make_dev();
     if (launch_task_prep(D, g.st) == hipSuccess) (void)hipStreamSynchronize(g.st);
@@ -508,7 +536,7 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
   // change in one pass
   const int tune_node_fused = (g.node_fuse && hd.max_sub_change <= 1) ? 1 : 0;
   struct Cand {
-    int W, H, R, F;
+    int W, H, R, F, S;  // S: streaming schedule
     double ms;
   };
   // time the passes of one substep and the node stage after them (the
@@ -526,9 +554,13 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
     HIPCHK(hipStreamSynchronize(st), "tune sync");
     c.ms = 1e300;
     for (int rep = 0; rep < 2; rep++) {
+      if (c.S) HIPCHK(launch_reset_queue(D, st), "reset_queue");
       HIPCHK(hipEventRecord(g.events[0], st), "event");
-      for (int pass = 0; pass < hd.max_passes; pass++)
-        HIPCHK(launch_stream_pass(D, pass, 0, 0, 0, hd.max_chunks, c.F, st), "stream_pass");
+      if (c.S)
+        HIPCHK(launch_stream_slab(D, 0, 0, c.F, 1, -1, st), "stream_slab");
+      else
+        for (int pass = 0; pass < hd.max_passes; pass++)
+          HIPCHK(launch_stream_pass(D, pass, 0, 0, 0, hd.max_chunks, c.F, st), "stream_pass");
       if (tune_node_fused)
         HIPCHK(launch_focus_change(D, 0, c.F, 1, st), "focus_change");
       else
@@ -545,22 +577,31 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
   // fused form of the best few (its extra registers can change the order)
   static const int lanes[] = {128, 192, 256, 320, 384, 512};
   static const int halos[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+  // phase 1 times one node per lane with a launch per pass and two or
+  // three with the persistent slab launch; phase 2 the best few with
+  // focusing in the last pass and with the other schedule
   std::vector<Cand> cands;
   for (int R = 1; R <= 3; R++)
     for (int Lw : lanes)
       for (int H : halos) {
-        Cand c{Lw * R, H, R, 0, 0.0};
+        Cand c{Lw * R, H, R, 0, R > 1 ? 1 : 0, 0.0};
         if (4 * H > c.W) continue;
         if ((rc = time_cand(c))) return rc;
         cands.push_back(c);
       }
   std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-  const size_t nfused = std::min<size_t>(cands.size(), 6);
-  for (size_t k = 0; k < nfused; k++) {
+  const size_t nbest = std::min<size_t>(cands.size(), 6);
+  for (size_t k = 0; k < nbest; k++) {
     Cand c = cands[k];
     c.F = 1;
     if ((rc = time_cand(c))) return rc;
     cands.push_back(c);
+    if (c.R > 1 && k < 3) {
+      Cand o = cands[k];
+      o.S = 1 - o.S;
+      if ((rc = time_cand(o))) return rc;
+      cands.push_back(o);
+    }
   }
   const Cand best_c = *std::min_element(cands.begin(), cands.end(),
                                         [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
@@ -570,6 +611,7 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
   g.stream_halo = bh;
   g.stream_r = br;
   g.stream_fuse = bf;
+  g.stream_sched = best_c.S;
   g.drop_graph();
   // leave the task tables consistent with the chosen tiling
   D = make_dev();
@@ -603,6 +645,20 @@ int ep_set_precision(int32_t mode) {
   g.fp32 = fp32;
   g.fast = (mode != 0) ? 1 : 0;
   g.drop_graph();
+  return EP_OK;
+}
+
+int ep_set_schedule(int32_t schedule) {
+  if (schedule < 0 || schedule > 2)
+    return fail(EP_ERR_INVALID_ARG, "schedule must be 0 (a launch per pass), 1 (persistent) or 2 (per stream)");
+  g.stream_sched = schedule;
+  g.drop_graph();
+  return EP_OK;
+}
+
+int ep_get_schedule(int32_t *schedule) {
+  if (!schedule) return fail(EP_ERR_INVALID_ARG, "null schedule");
+  *schedule = effective_schedule();
   return EP_OK;
 }
 
@@ -711,6 +767,21 @@ int ep_init_seed_population(void) {
   return EP_OK;
 }
 
+// Everything a step launches after the preparation, when the launches do
+// not depend on what the preparation found (the persistent streaming
+// schedule): the preparation, then the substeps with device-side gating.
+static hipError_t enqueue_step(const Dev &D, const StepShape &sh, hipStream_t st, hipEvent_t *ev) {
+  hipError_t e = launch_reset_diag(D.diag, st);
+  if (e == hipSuccess) e = launch_stream_prep(D, st);
+  if (e == hipSuccess) e = launch_node_prep(D, st);
+  if (e == hipSuccess && sh.do_shell) e = launch_shell_links(D, st);
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[1], st);
+  if (e != hipSuccess) return e;
+  Dev Dg = D;
+  Dg.gated = 1;
+  return enqueue_substeps(Dg, sh, st, ev);
+}
+
 int ep_step(uint32_t op_flags, ep_diag *out) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
   if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
@@ -725,49 +796,63 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   Dev D = make_dev();
   hipStream_t st = g.st;
   std::vector<hipEvent_t> &ev = g.events;
-  HIPCHK(hipEventRecord(ev[0], st), "event");
-  HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
-  HIPCHK(launch_stream_prep(D, st), "stream_prep");
-  HIPCHK(launch_node_prep(D, st), "node_prep");
-  HIPCHK(hipEventRecord(ev[1], st), "event");
-  DevDiag hd;
-  HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
-  HIPCHK(hipStreamSynchronize(st), "prep sync");
-  ep_diag dg;
-  memset(&dg, 0, sizeof(dg));
-  dg.err_stream = dg.err_node = dg.err_energy = dg.err_mu = -1;
-  if (hd.err_key != ~0ull) {
-    int code = decode_err(hd.err_key, &dg);
-    if (out) *out = dg;
-    return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
-  }
   const int do_shell = ((g.kperxkpar > 0.0) && (op_flags & EP_OP_SHELL)) ? 1 : 0;
   const int do_drift = (g.shell_drift && (op_flags & EP_OP_DRIFT)) ? 1 : 0;
   if (do_shell || do_drift) {
-    // once per global step: node positions and B are fixed inside it
     if (!g.mhd_state) return fail(EP_ERR_INVALID_ARG, "shell operators need the device MHD state (ep_update_mhd)");
     if (do_drift && g.mhd_mode != 0)
       return fail(EP_ERR_UNSUPPORTED, "drift needs the analytic-wind field (ep_update_mhd mode 0)");
     for (int32_t n : g.h_nactive)
       if (n != g.d.num_nodes) return fail(EP_ERR_UNSUPPORTED, "shell operators need every stream full");
-    HIPCHK(launch_shell_links(D, st), "shell links");
   }
-  const int node_fused = (g.node_fuse && do_focus && do_change && hd.max_sub_change <= 1) ? 1 : 0;
-  const StepShape shape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
-                        do_shell | (do_drift << 1), g.generation, node_fused};
+  const int sched = effective_schedule();
+  // With the persistent streaming launch nothing the host launches depends
+  // on the preparation's results: the whole global step (preparation and
+  // substeps) is one sequence, captured as one graph, and the host waits
+  // once, at the end.  The per-pass schedule sizes its launches from the
+  // preparation and waits for it first.
+  const bool one_sequence = sched != 0;
+  DevDiag hd;
+  ep_diag dg;
+  memset(&dg, 0, sizeof(dg));
+  dg.err_stream = dg.err_node = dg.err_energy = dg.err_mu = -1;
+  HIPCHK(hipEventRecord(ev[0], st), "event");
+  int node_fused;
+  StepShape shape;
+  if (one_sequence) {
+    node_fused = (g.node_fuse && do_focus && do_change) ? 2 : 0;
+    shape = StepShape{0, 0, nsub, do_stream, do_focus, do_change, do_seed, do_shell | (do_drift << 1), g.generation,
+                      node_fused, sched};
+  } else {
+    HIPCHK(launch_reset_diag(g.diag, st), "reset_diag");
+    HIPCHK(launch_stream_prep(D, st), "stream_prep");
+    HIPCHK(launch_node_prep(D, st), "node_prep");
+    HIPCHK(hipEventRecord(ev[1], st), "event");
+    HIPCHK(hipMemcpyAsync(&hd, g.diag, sizeof(hd), hipMemcpyDeviceToHost, st), "diag copy");
+    HIPCHK(hipStreamSynchronize(st), "prep sync");
+    if (hd.err_key != ~0ull) {
+      int code = decode_err(hd.err_key, &dg);
+      if (out) *out = dg;
+      return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
+    }
+    if (do_shell || do_drift) HIPCHK(launch_shell_links(D, st), "shell links");
+    node_fused = (g.node_fuse && do_focus && do_change && hd.max_sub_change <= 1) ? 1 : 0;
+    shape = StepShape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
+                      do_shell | (do_drift << 1), g.generation, node_fused, sched};
+  }
   if (g.use_graph) {
-    // The substep loop is one HIP graph, rebuilt only when its shape (pass
-    // count, grid sizes, operators) or the parameters change.
+    // One HIP graph, rebuilt only when its shape (operators, schedule, pass
+    // count of the per-pass schedule) or the parameters change.
     if (!g.graph_exec || !(g.graph_shape == shape)) {
       if (g.graph_exec) (void)hipGraphExecDestroy(g.graph_exec);
       g.graph_exec = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
-      hipError_t ce = enqueue_substeps(D, shape, st, nullptr);
+      hipError_t ce = one_sequence ? enqueue_step(D, shape, st, nullptr) : enqueue_substeps(D, shape, st, nullptr);
       hipGraph_t graph = nullptr;
       hipError_t ee = hipStreamEndCapture(st, &graph);
       if (ce != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
-        return hip_fail(ce, "substep capture");
+        return hip_fail(ce, "step capture");
       }
       HIPCHK(ee, "end capture");
       hipError_t ie = hipGraphInstantiate(&g.graph_exec, graph, nullptr, nullptr, 0);
@@ -777,6 +862,8 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
     }
     HIPCHK(hipGraphLaunch(g.graph_exec, st), "graph launch");
     HIPCHK(hipEventRecord(ev[2 + 4 * (nsub - 1) + 2], st), "event");
+  } else if (one_sequence) {
+    HIPCHK(enqueue_step(D, shape, st, ev.data()), "step");
   } else {
     HIPCHK(enqueue_substeps(D, shape, st, ev.data()), "substeps");
   }
@@ -785,8 +872,10 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[2 + 4 * (nsub - 1) + 2]), "elapsed");
   dg.device_ms = ms;
-  HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]), "elapsed");
-  dg.prep_ms = ms;
+  if (!(g.use_graph && one_sequence)) {
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]), "elapsed");
+    dg.prep_ms = ms;
+  }
   for (int sub = 0; sub < nsub && !g.use_graph; sub++) {
     hipEvent_t start = (sub == 0) ? ev[1] : ev[2 + 4 * (sub - 1) + 2];
     hipEvent_t *e4 = &ev[2 + 4 * sub];
@@ -796,6 +885,13 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
     dg.focus_ms += ms;
     HIPCHK(hipEventElapsedTime(&ms, e4[1], e4[2]), "elapsed");
     dg.change_ms += ms;
+  }
+  if (hd.prep_err) {
+    // the preparation failed: the substeps did nothing (the reference
+    // panics before them)
+    int code = decode_err(hd.err_key, &dg);
+    if (out) *out = dg;
+    return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
   }
   dg.max_subcycles_change = do_change ? hd.max_sub_change : 0;
   dg.max_subcycles_focusing = do_focus ? hd.max_sub_focus : 0;
@@ -809,7 +905,7 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   dg.stream_sweeps = do_stream ? (int64_t)hd.stream_sweeps * nsub : 0;
   dg.focus_sweeps = do_focus ? (int64_t)hd.focus_sweeps * nsub : 0;
   dg.change_sweeps = do_change ? (int64_t)hd.change_sweeps * nsub : 0;
-  dg.stream_launches = do_stream ? hd.max_passes * nsub : 0;
+  dg.stream_launches = !do_stream ? 0 : sched == 0 ? hd.max_passes * nsub : sched == 1 ? nsub : nsub * g.d.num_streams;
   int code = EP_OK;
   if (hd.err_key != ~0ull) code = decode_err(hd.err_key, &dg);
   if (out) *out = dg;

@@ -70,6 +70,8 @@ struct DevDiag {
   unsigned long long err_key;       // ~0ull = no error
   unsigned long long stream_sweeps, focus_sweeps, change_sweeps;  // per substep
   int focus_heavy;  // nodes in the heavy-focusing list
+  unsigned queue;   // next slab of the persistent streaming launch
+  int prep_err;     // the step's preparation found an error (ds underflow, short stream, subcycles)
   int pad_;
 };
 
@@ -82,6 +84,7 @@ struct Dev {
   int numEpSteps, mfpType, useBC, stream_block, stream_halo;
   int stream_r;     // kept nodes per lane in the streaming passes (1, 2 or 3)
   int stream_fuse;  // focusing of full streams done in their last streaming pass
+  int stream_whole; // a stream that fits in one chunk runs all its subcycles in one pass
   double mfpRadialPower, mfpBPower, mhdBsAu, dsh_min, dampCooling;
   double normJ0, normRadius, xi, beta, dlnp, dmu;
   int mpos;  // first mu index with mugrid >= 0
@@ -113,10 +116,38 @@ struct Dev {
   DevDiag *diag;
   int fp32;  // f stored as float (mixed-precision form)
   int fast;  // streaming with fused multiply-adds (not bit-identical)
+  int gated;      // launched as part of an EP substep (see substep_off)
+  int node_gate;  // node-stage path: 0 always, 1 / 2 when every / not every node needs <= 1 change subcycle
 };
 
 __device__ inline void note_error(const Dev &D, unsigned long long key) {
   atomicMin(&D.diag->err_key, key);
+}
+
+// An error of the once-per-step preparation: the reference panics there,
+// before any substep, so the substeps of the step are skipped.
+__device__ inline void note_prep_error(const Dev &D, unsigned long long key) {
+  atomicMin(&D.diag->err_key, key);
+  atomicOr(&D.diag->prep_err, 1);
+}
+
+// Whether a kernel of an EP substep has nothing to do: the step's
+// preparation reported an error, or the kernel belongs to the node-stage
+// path the preparation did not choose (node_gate 1: the one-pass focusing
+// and change, taken when every node needs at most one change subcycle;
+// 2: the two-kernel path otherwise).  The whole global step is then one
+// sequence of launches that the host never has to inspect midway.
+__device__ inline bool substep_off(const Dev &D) {
+  if (!D.gated) return false;
+  // a plain load: the values were written by earlier launches.  (A volatile
+  // access here counts as a possible clobber of all memory, and the
+  // streaming kernels then read their per-mu coefficients with vector
+  // instead of scalar loads inside the subcycle loop: 14% slower.)
+  const DevDiag *dg = D.diag;
+  if (dg->prep_err) return true;
+  if (D.node_gate == 1) return dg->max_sub_change > 1;
+  if (D.node_gate == 2) return dg->max_sub_change <= 1;
+  return false;
 }
 
 // sepSeedFunction (src/energeticParticlesBoundary.c:67-116) with the

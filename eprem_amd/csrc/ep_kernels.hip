@@ -184,9 +184,9 @@ __global__ __launch_bounds__(256) void k_stream_prep(Dev D) {
   if (tid == 0) {
     D.dsmin[s] = dsMin;
     if (size < 2 || (size < 3 && D.useBC))
-      note_error(D, err_key(0, 0, s, 0, kErrStreamShort, 0, 0, 0));
+      note_prep_error(D, err_key(0, 0, s, 0, kErrStreamShort, 0, 0, 0));
     else if (dsMin <= 0.0)
-      note_error(D, err_key(0, 0, s, 0, kErrDsUnderflow, 0, 0, 0));
+      note_prep_error(D, err_key(0, 0, s, 0, kErrDsUnderflow, 0, 0, 0));
   }
   // min_tau over the nodes the shell stage processes (1..n-1).
   double tmin = DBL_MAX;
@@ -218,16 +218,20 @@ __global__ __launch_bounds__(256) void k_task_prep(Dev D) {
     const double dtMin = 0.4 * dsMin * D.vgrid_i[e];
     const double xs = D.dt / dtMin + 1.0;
     if (!(xs < (double)kMaxSubcycles)) {
-      note_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
+      note_prep_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
     } else {
       nsteps = (int)floor(xs);
       dtProp = D.dt / (1.0 * nsteps);
     }
   }
-  const int H = D.stream_halo, W = D.stream_block;
+  const bool whole = D.stream_whole && D.stream_block >= size;
+  const int H = whole ? nsteps : D.stream_halo, W = D.stream_block;
+  // a stream that fits in one chunk takes all its subcycles in one pass
+  // without a halo (the streaming kernels hold its two ends); a longer one
+  // is cut into chunks whose halos cover the subcycles of a pass
   const int h = nsteps < H ? nsteps : H;
   const int passes = (nsteps + h - 1) / h;
-  const int interior = W - 2 * h;
+  const int interior = whole ? W : W - 2 * h;
   const int chunks = (size + interior - 1) / interior;
   D.nsteps[t] = nsteps;
   D.halo[t] = h;
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(256) void k_node_prep(Dev D) {
     const double xs = ceil(dt / dt_stable);
     int nsub = 0;
     if (!(xs < (double)kMaxSubcycles)) {
-      note_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
+      note_prep_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
     } else {
       nsub = (int)xs;
     }
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(256) void k_node_prep(Dev D) {
     const double xs = ceil(dt / dt_stable);
     int nsub = 0;
     if (!(xs < (double)kMaxSubcycles)) {
-      note_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
+      note_prep_error(D, err_key(0, 0, s, 0, kErrSubcycles, 0, 0, 0));
     } else {
       nsub = (int)xs;
     }
@@ -366,6 +370,7 @@ __global__ __launch_bounds__(256) void k_node_prep(Dev D) {
 // ------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
+  if (substep_off(D)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.n_active[s]) return;
@@ -395,11 +400,14 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
 // ------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
+  if (substep_off(D)) return;
   const int task = blockIdx.x;
   const int s = task / D.E, e = task % D.E;
   const int loc_new = D.loc[task] ^ (D.passes[task] & 1);
   __syncthreads();
   if (threadIdx.x == 0) D.loc[task] = (uint8_t)loc_new;
+  // the slab queue of the substep's streaming launch, ready for the next
+  if (task == 0 && threadIdx.x == 0) D.diag->queue = 0u;
   if (!D.thin[s]) return;
   const int n = D.n_active[s];
   const size_t on = (size_t)s * D.Npad;
@@ -430,6 +438,7 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
 // ------------------------------------------------------------------------
 template <int M, typename T, bool FASTF>
 __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, int spare) {
+  if (substep_off(D)) return;
   // blockIdx.x = ((stream, energy), chunk of 256 nodes), chunk fastest:
   // the block reads 2 KB runs of each mu row, consecutive blocks the next.
   // Blocks go round-robin to the 8 XCDs, so the chunk order is rotated by
@@ -487,6 +496,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
 // its load latency than with one wave per SIMD (4% faster on C2).
 template <int M, typename T, bool FASTF>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_focus_heavy(Dev D, int substep) {
+  if (substep_off(D)) return;
   const int count = D.diag->focus_heavy;
   const int e = threadIdx.x;
   const double two = 2.0;
@@ -530,6 +540,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // FM: ln f and exp through fm_log / fm_exp (tables in LDS).
 template <typename T, bool FM>
 __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
+  if (substep_off(D)) return;
   // blockIdx.x = ((stream, mu), chunk of 256 nodes), chunk fastest
   const int nch = (D.N + 255) >> 8;
   const int sm = blockIdx.x / nch;
@@ -728,6 +739,7 @@ __global__ __launch_bounds__(256) void k_convert(Dev D) {
 }
 
 __global__ __launch_bounds__(256) void k_flip_all(Dev D) {
+  if (substep_off(D)) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < D.S * D.E) D.loc[t] ^= 1;
 }
@@ -778,6 +790,22 @@ hipError_t launch_stream_pass(const Dev &D, int pass, int flip, int do_seed, int
   return launch_stream_pass_m26_32(D, pass, flip, do_seed, sub, chunks, fuse, st);
 }
 
+hipError_t launch_stream_slab(const Dev &D, int do_seed, int sub, int fuse, int tune, int one, hipStream_t st) {
+  if (D.M < 2 || D.M > 32) return hipErrorInvalidValue;
+  if (D.M <= 9) return launch_stream_slab_m2_9(D, do_seed, sub, fuse, tune, one, st);
+  if (D.M <= 17) return launch_stream_slab_m10_17(D, do_seed, sub, fuse, tune, one, st);
+  if (D.M <= 25) return launch_stream_slab_m18_25(D, do_seed, sub, fuse, tune, one, st);
+  return launch_stream_slab_m26_32(D, do_seed, sub, fuse, tune, one, st);
+}
+
+__global__ void k_reset_queue(DevDiag *d) { d->queue = 0u; }
+
+hipError_t launch_reset_queue(const Dev &D, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, st, D.diag);
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
 hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
   if (D.fp32)
     hipLaunchKernelGGL(k_stream_finish<float>, dim3(D.S * D.E), dim3(256), 0, st, D, sub);
@@ -792,17 +820,17 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
     if (D.fp32) {                                                                             \
       hipLaunchKernelGGL((k_focus<MM, float, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
-      if (!spare)                                                                             \
+      if (!spare && heavy)                                                                    \
         hipLaunchKernelGGL((k_focus_heavy<MM, float, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     } else if (D.fast) {                                                                      \
       hipLaunchKernelGGL((k_focus<MM, double, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
-      if (!spare)                                                                             \
+      if (!spare && heavy)                                                                    \
         hipLaunchKernelGGL((k_focus_heavy<MM, double, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     } else {                                                                                  \
       hipLaunchKernelGGL((k_focus<MM, double, false>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
-      if (!spare)                                                                             \
+      if (!spare && heavy)                                                                    \
         hipLaunchKernelGGL((k_focus_heavy<MM, double, false>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     }                                                                                         \
     break;
@@ -812,7 +840,7 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
 constexpr int kHeavyGrid = 4096;
 static_assert(kHeavyGrid % 8 == 0, "k_focus_heavy splits its grid over 8 XCDs");
 
-hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int spare) {
+hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int spare, int heavy) {
   switch (D.M) {
     EP_FOCUS_CASE(2) EP_FOCUS_CASE(3) EP_FOCUS_CASE(4) EP_FOCUS_CASE(5) EP_FOCUS_CASE(6)
     EP_FOCUS_CASE(7) EP_FOCUS_CASE(8) EP_FOCUS_CASE(9) EP_FOCUS_CASE(10) EP_FOCUS_CASE(11)
@@ -988,6 +1016,8 @@ __global__ void k_reset_diag(DevDiag *d) {
   d->focus_sweeps = 0;
   d->change_sweeps = 0;
   d->focus_heavy = 0;
+  d->queue = 0u;
+  d->prep_err = 0;
 }
 
 hipError_t launch_reset_diag(DevDiag *d, hipStream_t st) {

@@ -19,11 +19,21 @@ hipError_t launch_stream_pass_m18_25(const Dev &D, int pass, int flip, int do_se
                                      int fuse_focus, hipStream_t st);
 hipError_t launch_stream_pass_m26_32(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                                      int fuse_focus, hipStream_t st);
+// one persistent launch per EP substep over whole slabs (ep_stream.h,
+// k_stream_slab); lane_nodes 2 or 3.  tune: state-preserving timing run;
+// one >= 0: only stream `one` (the unbatched form)
+hipError_t launch_stream_slab(const Dev &D, int do_seed, int sub, int fuse, int tune, int one, hipStream_t st);
+hipError_t launch_stream_slab_m2_9(const Dev &D, int do_seed, int sub, int fuse, int tune, int one, hipStream_t st);
+hipError_t launch_stream_slab_m10_17(const Dev &D, int do_seed, int sub, int fuse, int tune, int one, hipStream_t st);
+hipError_t launch_stream_slab_m18_25(const Dev &D, int do_seed, int sub, int fuse, int tune, int one, hipStream_t st);
+hipError_t launch_stream_slab_m26_32(const Dev &D, int do_seed, int sub, int fuse, int tune, int one, hipStream_t st);
+hipError_t launch_reset_queue(const Dev &D, hipStream_t st);
 hipError_t launch_task_prep(const Dev &D, hipStream_t st);
 hipError_t launch_stream_finish(const Dev &D, int substep, hipStream_t st);
 // fused: focusing of full streams was done by the streaming passes;
 // spare: write the spare buffer and skip the heavy nodes (tile-size search)
-hipError_t launch_focus(const Dev &D, int substep, int fused, hipStream_t st, int spare = 0);
+// heavy = 0: leave the heavy nodes to a separate launch_focus_heavy
+hipError_t launch_focus(const Dev &D, int substep, int fused, hipStream_t st, int spare = 0, int heavy = 1);
 hipError_t launch_change(const Dev &D, int substep, hipStream_t st);
 // focusing of the heavy nodes only (nsub_f >= kHeavyFocus), in place
 hipError_t launch_focus_heavy(const Dev &D, int substep, hipStream_t st);

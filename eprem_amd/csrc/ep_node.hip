@@ -51,6 +51,7 @@ __device__ __forceinline__ T pair_swap(T v) {
 // (tile-size search; f and the step state are left as they were).
 template <int M, typename T, bool FM>
 __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fused, int spare) {
+  if (substep_off(D)) return;
   constexpr int MH = (M + 1) / 2;
   // blockIdx.x = (stream, chunk of 128 nodes), chunk fastest; the chunk
   // order is rotated by stream so that the inner chunks (most focusing
@@ -311,6 +312,7 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
 // ------------------------------------------------------------------------
 template <int M, typename T, bool FM>
 __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int fused, int spare) {
+  if (substep_off(D)) return;
   constexpr int G = 4, W = 64 / G, NB = 4 * W;
   constexpr int MG = (M + G - 1) / G;
   static_assert(M >= 12, "four lanes per node from 12 mu cells");

@@ -127,6 +127,7 @@ __device__ __forceinline__ double shell_del(const Dev &D, const ShellDev &H, int
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_shell_diffuse(Dev D, ShellDev H, int substep) {
+  if (substep_off(D)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.N) return;
@@ -209,6 +210,7 @@ __device__ __forceinline__ double drift_del(const Dev &D, const ShellDev &H, int
 // subtracted one by one at its own position.
 template <typename T>
 __global__ __launch_bounds__(256) void k_shell_drift(Dev D, ShellDev H, int substep) {
+  if (substep_off(D)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.N) return;

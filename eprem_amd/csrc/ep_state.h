@@ -23,10 +23,12 @@ struct StepShape {
   int passes, chunks, nsub, do_stream, do_focus, do_change, do_seed, do_shell;
   unsigned generation;
   int node_fused;  // focusing and change in one kernel (k_focus_change)
+  int sched;       // streaming schedule (0: a launch per pass, 1: persistent, 2: per stream)
   bool operator==(const StepShape &o) const {
     return passes == o.passes && chunks == o.chunks && nsub == o.nsub && do_stream == o.do_stream &&
            do_focus == o.do_focus && do_change == o.do_change && do_seed == o.do_seed &&
-           do_shell == o.do_shell && generation == o.generation && node_fused == o.node_fused;
+           do_shell == o.do_shell && generation == o.generation && node_fused == o.node_fused &&
+           sched == o.sched;
   }
 };
 
@@ -36,7 +38,11 @@ struct State {
   ep_params p{};
   int Npad = 0;
   hipStream_t st = nullptr;
-  int stream_block = 256, stream_halo = 32, stream_r = 1, stream_fuse = 1;
+  int stream_block = 384, stream_halo = 32, stream_r = 3, stream_fuse = 1, stream_whole = 1;
+  // streaming schedule: 0 = one launch per pass over every slab, 1 = one
+  // persistent launch per substep over whole slabs (k_stream_slab), 2 = the
+  // unbatched form of 1, one launch per stream
+  int stream_sched = 1;
   int node_fuse = 1;  // allow k_focus_change when every node needs <= 1 change subcycle
   // graph of the substep loop; `generation` changes whenever kernel
   // arguments captured by value (parameters, tiling) change

@@ -21,6 +21,24 @@
 
 namespace ep {
 
+// Per-mu and per-energy tables (coefficients, grids) read the same on every
+// lane and are fixed while a kernel runs.  Read through the constant address
+// space they become scalar loads even after an atomic earlier in the kernel
+// (the slab queue), which otherwise makes the compiler fetch them with
+// vector loads inside the subcycle loop.
+typedef const __attribute__((address_space(4))) double cdouble;
+__device__ __forceinline__ const cdouble *as_const_space(const double *p) { return (const cdouble *)p; }
+
+// The same pointer, re-derived where the compiler cannot see through it:
+// taken once per subcycle, it keeps the coefficient loads inside the
+// subcycle loop (scalar loads into the same few registers) instead of
+// letting them be hoisted out of it into 30 live register pairs.
+__device__ __forceinline__ const cdouble *opaque(const cdouble *p) {
+  uint64_t v = (uint64_t)p;
+  asm volatile("" : "+s"(v));
+  return (const cdouble *)v;
+}
+
 // ------------------------------------------------------------------------
 // Cross-lane shifts along the stream (DPP wave_shr:1 / wave_shl:1 move a
 // 32-bit value one lane up/down the whole 64-lane wave; lanes without a
@@ -77,7 +95,7 @@ __device__ __forceinline__ float dpp_shl1(float v, float old) {
     T vw0_ = 0, vw1_ = 0; /* face velocities of the wall rows: faces 1, M-1 */     \
     double vprev_ = 0.0;                                                           \
     _Pragma("unroll") for (int m = 0; m < (M); ++m) {                              \
-      const double mu = (D).mugrid[m];                                             \
+      const double mu = as_const_space((D).mugrid)[m];                             \
       const double vel = 0.5 * (1.0 - mu * mu) * ((a) + mu * (b));                 \
       if (m > 0) {                                                                 \
         T va_ = (T)(0.5 * (vel + vprev_));                                         \
@@ -136,6 +154,7 @@ __device__ __forceinline__ float dpp_shl1(float v, float old) {
 // reference's operation order (not bit-identical; ~1e-15 relative).
 template <int M, int MPOS, typename T, bool FAST, bool FUSE>
 __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, int do_seed, int substep) {
+  if (substep_off(D)) return;
   const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
   const int task = s * D.E + e;
   const int nsteps = D.nsteps[task];
@@ -146,10 +165,13 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
   const bool last = (done + nsub == nsteps);
   const int nk = D.nkept[s];
   const int W = blockDim.x;
-  const int I = W - 2 * h;
+  // a stream that fits in one chunk needs no halo (its ends are held):
+  // all its subcycles run in one pass over the whole stream
+  const int hs = (D.stream_whole && nk <= W) ? 0 : h;
+  const int I = W - 2 * hs;
   if (c * I >= nk) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = W >> 6;
-  const int k = c * I - h + tid;
+  const int k = c * I - hs + tid;
   const bool valid = (k >= 0) && (k < nk);
   const size_t on = (size_t)s * D.Npad;
   const int node = valid ? (D.thin[s] ? D.list[on + k] : k) : 0;
@@ -198,7 +220,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
   // every subcycle is loop-invariant, so it is formed once per pass.
   T coef[M];
   {
-    const double *__restrict__ del = D.del_fac + (size_t)task * M;
+    const cdouble *del = as_const_space(D.del_fac + (size_t)task * M);
 #pragma unroll
     for (int m = 0; m < M; ++m) coef[m] = (T)(((m >= MPOS) ? dsm_pos : dsm_neg) * del[m]);
   }
@@ -253,7 +275,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
     }
   }
 
-  if (valid && tid >= h && tid < h + I) {
+  if (valid && tid >= hs && tid < hs + I) {
     if (last) {
       int bad = -1, bad_inf = 0;
 #pragma unroll
@@ -303,6 +325,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
 // ------------------------------------------------------------------------
 template <int M, int MPOS, typename T, bool FAST, bool FUSE>
 __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip, int do_seed, int substep) {
+  if (substep_off(D)) return;
   const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
   const int task = s * D.E + e;
   const int nsteps = D.nsteps[task];
@@ -313,11 +336,12 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
   const bool last = (done + nsub == nsteps);
   const int nk = D.nkept[s];
   const int L = blockDim.x, W = 2 * L;
-  const int I = W - 2 * h;
+  const int hs = (D.stream_whole && nk <= W) ? 0 : h;  // one chunk: no halo
+  const int I = W - 2 * hs;
   if (c * I >= nk) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = L >> 6;
   const int j0 = 2 * tid, j1 = j0 + 1;  // positions in the chunk
-  const int k0 = c * I - h + j0, k1 = k0 + 1;
+  const int k0 = c * I - hs + j0, k1 = k0 + 1;
   const bool v0 = (k0 >= 0) && (k0 < nk), v1 = (k1 >= 0) && (k1 < nk);
   const size_t on = (size_t)s * D.Npad;
   const bool thin = D.thin[s] != 0;
@@ -442,7 +466,7 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
     }
   }
 
-  const bool st0 = v0 && j0 >= h && j0 < h + I, st1 = v1 && j1 >= h && j1 < h + I;
+  const bool st0 = v0 && j0 >= hs && j0 < hs + I, st1 = v1 && j1 >= hs && j1 < hs + I;
   if (last) {
     int bad0 = -1, inf0 = 0, bad1 = -1, inf1 = 0;
 #pragma unroll
@@ -511,9 +535,13 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
 // registers of headroom (occupancy 2 instead of 3 at M = 30 in FP64); the
 // tile-size search decides.  Same arithmetic per cell as k_stream_pass2.
 // ------------------------------------------------------------------------
+//
+// stream_chunk: the work of one block on chunk c of slab (s, e) in pass
+// `pass`, reading buffer src_buf and writing buffer dst_buf; shared by the
+// one-launch-per-pass kernel and the persistent slab kernel below.
 template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
-__global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip, int do_seed, int substep) {
-  const int c = blockIdx.x, e = blockIdx.y, s = blockIdx.z;
+__device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, int pass, int src_buf, int dst_buf,
+                                             int do_seed, int substep) {
   const int task = s * D.E + e;
   const int nsteps = D.nsteps[task];
   const int h = D.halo[task];
@@ -523,23 +551,23 @@ __global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip,
   const bool last = (done + nsub == nsteps);
   const int nk = D.nkept[s];
   const int L = blockDim.x, W = R * L;
-  const int I = W - 2 * h;
+  const int hs = (D.stream_whole && nk <= W) ? 0 : h;  // one chunk: no halo
+  const int I = W - 2 * hs;
   if (c * I >= nk) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = L >> 6;
   const size_t on = (size_t)s * D.Npad;
   const bool thin = D.thin[s] != 0;
-  const int cur = D.loc[task] ^ flip;
   constexpr uint32_t B = sizeof(T);
   const uint32_t slab_bytes = (uint32_t)(M * D.Npad * B), row = (uint32_t)D.Npad * B;
-  const rsrc_t src = slab_rsrc((const double *)(fbufT<T>(D, cur) + (size_t)task * M * D.Npad), slab_bytes);
-  const rsrc_t dst = slab_rsrc((const double *)(fbufT<T>(D, cur ^ 1) + (size_t)task * M * D.Npad), slab_bytes);
+  const rsrc_t src = slab_rsrc((const double *)(fbufT<T>(D, src_buf) + (size_t)task * M * D.Npad), slab_bytes);
+  const rsrc_t dst = slab_rsrc((const double *)(fbufT<T>(D, dst_buf) + (size_t)task * M * D.Npad), slab_bytes);
   int jj_[R], kk[R], nd[R];
   bool vv[R];
   uint32_t vo[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     jj_[r] = R * tid + r;  // position in the chunk
-    kk[r] = c * I - h + jj_[r];
+    kk[r] = c * I - hs + jj_[r];
     vv[r] = (kk[r] >= 0) && (kk[r] < nk);
     nd[r] = vv[r] ? (thin ? D.list[on + kk[r]] : kk[r]) : 0;
     vo[r] = (uint32_t)nd[r] * B;
@@ -573,7 +601,7 @@ __global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip,
     bcv[r] = bc[r] ? (T)D.seed3[(size_t)task * 3 + (bc[r] ? kk[r] : 0)] : (T)0;
   }
   const T invM = (T)(1.0 / M);
-  const double *__restrict__ del = D.del_fac + (size_t)task * M;
+  const cdouble *del = as_const_space(D.del_fac + (size_t)task * M);
   // edge rows between the waves of the block, double-buffered by subcycle
   // parity: up[r] = the mu >= 0 values the first node of lane 0 of wave r
   // takes from its left (last node of lane 63 of wave r-1; wave 0 lane 0
@@ -584,6 +612,7 @@ __global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip,
   __shared__ T up[2][9][NP > 0 ? NP : 1];
   __shared__ T dn[2][9][MPOS > 0 ? MPOS : 1];
   for (int jj = 0; jj < nsub; ++jj) {
+    const cdouble *delj = opaque(del);
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (bc[r]) {
@@ -615,7 +644,7 @@ __global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip,
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         a[r] = f[r][m];
-        cf[r] = (T)(((m >= MPOS) ? pos[r] : neg[r]) * del[m]);
+        cf[r] = (T)(((m >= MPOS) ? pos[r] : neg[r]) * delj[m]);
       }
       if (m >= MPOS) {
         const T nb = dpp_shr1(a[R - 1], up[b][w][m - MPOS]);  // last node of the lane to the left
@@ -654,7 +683,7 @@ __global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip,
 
   bool st[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) st[r] = vv[r] && jj_[r] >= h && jj_[r] < h + I;
+  for (int r = 0; r < R; ++r) st[r] = vv[r] && jj_[r] >= hs && jj_[r] < hs + I;
   if (last) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -701,6 +730,134 @@ __global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip,
     for (int r = 0; r < R; ++r)
       if (st[r]) buf_st(f[r][m], dst, vo[r], m * row);
 }
+
+template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
+__global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip, int do_seed, int substep) {
+  if (substep_off(D)) return;
+  const int cur = D.loc[blockIdx.z * D.E + blockIdx.y] ^ flip;
+  stream_chunk<M, MPOS, T, FAST, FUSE, R>(D, blockIdx.x, blockIdx.y, blockIdx.z, pass, cur, cur ^ 1, do_seed,
+                                          substep);
+}
+
+// ------------------------------------------------------------------------
+// k_stream_slab: DiffuseStreamData of whole (stream, energy) slabs in one
+// persistent launch per EP substep.  Each block takes the next slab from a
+// device queue (highest energies first: they carry the most subcycles, so
+// the long slabs start early and the short ones fill in behind them) and
+// runs all of the slab's passes and chunks itself, ping-ponging between the
+// slab's two buffers; the passes of a slab depend only on that slab, so
+// the block's own barrier orders them and no other block is waited for.
+// The launch does not depend on the pass or chunk counts, which the
+// preparation kernels leave on the device.
+// tune = 1 (tile-size search): pass 0 reads the current slab and every pass
+// writes the spare one, so the work is that of a step and the state is
+// left as it was.  one_stream >= 0: the unbatched form, one launch per
+// stream whose block b takes energy E-1-b of that stream.
+// ------------------------------------------------------------------------
+template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
+__global__ __launch_bounds__(512) void k_stream_slab(Dev D, int do_seed, int substep, int tune, int one_stream) {
+  if (substep_off(D)) return;
+  __shared__ int item_sh;
+  const int S = D.S, E = D.E;
+  for (;;) {
+    int e, s;
+    if (one_stream >= 0) {
+      if ((int)blockIdx.x >= E) return;
+      e = E - 1 - (int)blockIdx.x;
+      s = one_stream;
+    } else {
+      if (threadIdx.x == 0) item_sh = (int)atomicAdd(&D.diag->queue, 1u);
+      __syncthreads();
+      const int item = item_sh;
+      __syncthreads();
+      if (item >= S * E) return;
+      e = E - 1 - item / S;
+      s = item % S;
+    }
+    const int task = s * E + e;
+    const int passes = D.passes[task], chunks = D.chunks[task], loc = D.loc[task];
+    for (int p = 0; p < passes; ++p) {
+      const int src = tune ? (p == 0 ? loc : loc ^ 1) : (loc ^ (p & 1));
+      const int dst = tune ? (loc ^ 1) : (src ^ 1);
+      for (int c = 0; c < chunks; ++c) {
+        stream_chunk<M, MPOS, T, FAST, FUSE, R>(D, c, e, s, p, src, dst, do_seed && p == 0, substep);
+        // the next chunk reuses the edge rows in LDS, and the next pass
+        // reads what this one stored (the block's own stores)
+        __syncthreads();
+      }
+    }
+    if (one_stream >= 0) return;
+  }
+}
+
+// Resident blocks of a slab-kernel instantiation for a block size (cache:
+// the instantiation's own table, one entry per block size).
+template <typename K>
+inline int resident_blocks(K kernel, int block, int *cache) {
+  const int slot = block / 64;
+  if (slot < 0 || slot > 8) return 0;
+  if (!cache[slot]) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) per_cu = 1;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    cache[slot] = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+  }
+  return cache[slot];
+}
+
+template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
+inline void launch_slab_kernel(const Dev &D, int do_seed, int substep, int tune, int one_stream, hipStream_t st) {
+  const int L = D.stream_block / R;
+  auto k = k_stream_slab<M, MPOS, T, FAST, FUSE, R>;
+  static int cache[9] = {0};
+  int grid;
+  if (one_stream >= 0) {
+    grid = D.E;
+  } else {
+    grid = resident_blocks(k, L, cache);
+    if (grid > D.S * D.E) grid = D.S * D.E;
+    if (grid < 1) grid = 1;
+  }
+  hipLaunchKernelGGL(k, dim3(grid), dim3(L), 0, st, D, do_seed, substep, tune, one_stream);
+}
+
+template <int M, int MPOS, bool FUSE, int R>
+inline void stream_slab_tfr(const Dev &D, int do_seed, int substep, int tune, int one_stream, hipStream_t st) {
+  if (D.fp32)
+    launch_slab_kernel<M, MPOS, float, true, FUSE, R>(D, do_seed, substep, tune, one_stream, st);
+  else if (D.fast)
+    launch_slab_kernel<M, MPOS, double, true, FUSE, R>(D, do_seed, substep, tune, one_stream, st);
+  else
+    launch_slab_kernel<M, MPOS, double, false, FUSE, R>(D, do_seed, substep, tune, one_stream, st);
+}
+
+// Slab kernels exist for two and three nodes per lane.
+template <int M, int MPOS>
+inline hipError_t stream_slab_t(const Dev &D, int do_seed, int substep, int fuse, int tune, int one_stream,
+                                hipStream_t st) {
+  if (D.stream_r != 2 && D.stream_r != 3) return hipErrorInvalidValue;
+  if (D.stream_r == 3) {
+    if (fuse)
+      stream_slab_tfr<M, MPOS, true, 3>(D, do_seed, substep, tune, one_stream, st);
+    else
+      stream_slab_tfr<M, MPOS, false, 3>(D, do_seed, substep, tune, one_stream, st);
+  } else {
+    if (fuse)
+      stream_slab_tfr<M, MPOS, true, 2>(D, do_seed, substep, tune, one_stream, st);
+    else
+      stream_slab_tfr<M, MPOS, false, 2>(D, do_seed, substep, tune, one_stream, st);
+  }
+  EP_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+#define EP_SLAB_CASE(MM)                                                                          \
+  case MM:                                                                                        \
+    if (D.mpos == (MM) / 2) return stream_slab_t<MM, (MM) / 2>(D, do_seed, sub, fuse, tune, one, st); \
+    if (D.mpos == ((MM) + 1) / 2)                                                                 \
+      return stream_slab_t<MM, ((MM) + 1) / 2>(D, do_seed, sub, fuse, tune, one, st);            \
+    return hipErrorInvalidValue;
 
 template <int M, int MPOS, bool FUSE>
 inline hipError_t stream_pass_tf(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,

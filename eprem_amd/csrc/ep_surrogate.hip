@@ -85,8 +85,9 @@ int ep_surrogate_samples(int32_t K, int32_t n, uint64_t seed, float *X, float *Y
   if (g.fp32) return fail(EP_ERR_UNSUPPORTED, "samples need FP64 storage");
   if (K < 1 || K > 64 || n < 1 || !X || !Y || !valid) return fail(EP_ERR_INVALID_ARG, "bad sample request");
   const int M = g.d.num_mu, nin = 3 * M + 6;
-  const int keep_b = g.stream_block, keep_h = g.stream_halo, keep_r = g.stream_r;
+  const int keep_b = g.stream_block, keep_h = g.stream_halo, keep_r = g.stream_r, keep_w = g.stream_whole;
   g.stream_block = 256;
+  g.stream_whole = 0;  // every slab advances exactly K subcycles, short streams too
   g.stream_halo = K;
   g.stream_r = 1;
   Dev D = make_dev();
@@ -116,6 +117,7 @@ int ep_surrogate_samples(int32_t K, int32_t n, uint64_t seed, float *X, float *Y
   g.stream_block = keep_b;
   g.stream_halo = keep_h;
   g.stream_r = keep_r;
+  g.stream_whole = keep_w;
   g.drop_graph();
   if (e != hipSuccess) return hip_fail(e, "surrogate samples");
   if (hd.err_key != ~0ull) return fail(EP_ERR_INVALID_ARG, "node preparation reported an error");

@@ -38,6 +38,8 @@ def lib():
         L.ep_init.argtypes = [C.POINTER(EpDims), C.POINTER(EpParams)]
         L.ep_set_params.argtypes = [C.POINTER(EpParams)]
         L.ep_set_tuning.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.ep_set_schedule.argtypes = [C.c_int32]
+        L.ep_get_schedule.argtypes = [C.POINTER(C.c_int32)]
         L.ep_get_grids.argtypes = [DP, DP, DP, DP, DP, DP]
         L.ep_set_nodes.argtypes = [C.POINTER(EpNodeSoa), IP]
         L.ep_upload_f.argtypes = [DP, C.c_int32]
@@ -124,6 +126,16 @@ class EpSolver:
         fuse_focus: focusing done by the last streaming pass (1) or by the
         separate kernel (0)."""
         _check(lib().ep_set_tuning(int(stream_block), int(stream_halo), int(lane_nodes), int(fuse_focus)))
+
+    def set_schedule(self, schedule):
+        """Streaming launches: 1 persistent per substep, 0 one per pass,
+        2 one per stream (the unbatched form)."""
+        _check(lib().ep_set_schedule(int(schedule)))
+
+    def schedule(self):
+        v = C.c_int32()
+        _check(lib().ep_get_schedule(C.byref(v)))
+        return v.value
 
     def autotune(self):
         """Tile-size search for the streaming kernel; returns (block, halo,

@@ -366,8 +366,23 @@ int ep_finalize(void);
  * streaming pass on the values still in registers; 0: by the separate
  * focusing kernel).  Results are the same either way.  No counterpart in
  * the reference (its DiffuseStreamData loops on the CPU); set by the
- * autotuner or left at the defaults (256, 32, 1, 1). */
+ * autotuner or left at the defaults (384, 32, 3, 1).  A stream whose kept
+ * nodes fit in one chunk runs all its subcycles in one pass without a
+ * halo. */
 int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes, int32_t fuse_focus);
+
+/* How the streaming passes of an EP substep are launched: 1 (default) =
+ * one persistent launch per substep whose blocks take whole (stream,
+ * energy) slabs from a device queue, highest energies first, and run all
+ * their passes and chunks (needs lane_nodes 2 or 3; with 1 the passes are
+ * launched one by one); 0 = one launch per pass over every slab; 2 = the
+ * unbatched form of 1, one launch per stream (the reference's loop over
+ * streams, src/energeticParticles.c:102-160), kept for comparison.
+ * Results are identical.  ep_get_schedule reports the schedule in effect
+ * (set by the autotuner, or by ep_set_schedule).  No counterpart in the
+ * reference. */
+int ep_set_schedule(int32_t schedule);
+int ep_get_schedule(int32_t *schedule);
 
 /* Tile-size search for the streaming kernel on the current nodes: times
  * every streaming pass of one substep and the node stage after it (the

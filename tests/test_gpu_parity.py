@@ -205,6 +205,39 @@ def test_ds_underflow_error():
     assert gd[0][1]["err_stream"] == od[0][1]["err_stream"] == 2
 
 
+@pytest.mark.parametrize("sched,graph", [(0, True), (1, True), (1, False), (2, False)])
+def test_preparation_error_leaves_f_unchanged(sched, graph):
+    """The reference panics in FindSegmentLengths before any substep.  With
+    the persistent schedule the host does not look at the preparation before
+    launching the substeps: the error is found on the device, every substep
+    kernel skips its work, and the step reports the error afterwards with f
+    as it was."""
+    p = params(dsh_min=-1.0)
+    nodes = workloads.parker_nodes(1, 1, 32, seed=10)
+    for k in ("x", "y", "z", "rmag"):
+        nodes[k][2, 11] = nodes[k][2, 10]
+    dims = workloads.dims_of(nodes, 8, 6)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_tuning(384, 16, 3, 1)
+        sol.set_schedule(sched)
+        sol.set_graph(graph)
+        f0 = workloads.seeded_f(p, sol.grids(), nodes, noise=0.3, seed=3)
+        sol.set_nodes(nodes)
+        sol.upload_f(f0)
+        rc, dg = sol.step(raise_on_error=False)
+        fg = sol.download_f()
+        # a good step afterwards runs normally
+        good = workloads.parker_nodes(1, 1, 32, seed=10)
+        sol.set_nodes(good)
+        rc2, _ = sol.step(raise_on_error=False)
+    finally:
+        sol.close()
+    assert rc == abi.EP_ERR_DS_UNDERFLOW and dg["err_stream"] == 2
+    np.testing.assert_array_equal(fg, f0)
+    assert rc2 == 0
+
+
 @pytest.mark.parametrize("levels_differ", [False, True])
 def test_mas_interp_parity(levels_differ):
     """masTriLinear/masWind on the device vs the oracle: nodes inside and
@@ -466,6 +499,50 @@ def test_three_nodes_per_lane_parity(case):
                                     tiling=case["tiling"], precision=case["precision"])
     assert_diag_equal(gd, od)
     assert max_rel(fg, fo, n_active) <= RTOL
+
+
+@pytest.mark.parametrize("sched", [0, 1, 2])
+@pytest.mark.parametrize("case", [
+    # two chunks per slab, several passes
+    dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=0, tiling=(192, 8, 3, 0)),
+    dict(N=300, E=10, M=17, thin=0.1, ragged=True, precision=2, tiling=(256, 12, 2, 1)),
+    dict(N=700, E=8, M=30, thin=0.0, ragged=True, precision=2, tiling=(384, 40, 3, 1)),
+    # every stream fits in one chunk: all subcycles in one pass, no halo
+    dict(N=120, E=12, M=20, thin=0.0, ragged=False, precision=0, tiling=(384, 16, 3, 1)),
+    dict(N=250, E=9, M=7, thin=0.15, ragged=True, precision=0, tiling=(256, 4, 2, 0)),
+])
+def test_streaming_schedules_parity(sched, case):
+    """One launch per pass (0), one persistent launch per substep whose
+    blocks take whole slabs from a queue (1) and its unbatched form, one
+    launch per stream (2), against the oracle: multi-chunk and one-chunk
+    slabs, thinned and ragged streams, both FP64 modes, two steps."""
+    p = params(numEpSteps=2)
+    nodes = workloads.parker_nodes(1, 1, case["N"], seed=63, perturb=0.2, jitter=0.2, thin_frac=case["thin"])
+    n_active = None
+    if case["ragged"]:
+        n_active = np.array([case["N"], 37, 150, 3, 99, 128], np.int32)
+    dims = workloads.dims_of(nodes, case["E"], case["M"])
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_precision(case["precision"])
+        sol.set_tuning(*case["tiling"])
+        sol.set_schedule(sched)
+        assert sol.schedule() == sched
+        f0 = workloads.seeded_f(p, sol.grids(), nodes, noise=0.3, seed=8)
+        sol.set_nodes(nodes, n_active)
+        sol.upload_f(f0)
+        gd = [sol.step(raise_on_error=False) for _ in range(2)]
+        fg = sol.download_f()
+    finally:
+        sol.close()
+    fo, od = f0, []
+    for _ in range(2):
+        rc, do, fo, _, _, _ = oracle.update_ep(dims, p, nodes, fo, n_active=n_active)
+        od.append((rc, do))
+    assert_diag_equal(gd, od)
+    assert max_rel(fg, fo, n_active) <= RTOL
+    if sched:
+        assert gd[0][1]["stream_launches"] == (2 if sched == 1 else 2 * dims.num_streams)
 
 
 @pytest.mark.parametrize("tiling", [(256, 16, 1, 1), (512, 32, 2, 1), (384, 24, 2, 1), (256, 16, 1, 0),
