@@ -206,6 +206,7 @@ EXPORTED_SYMBOLS = (
     "ep_get_schedule",
     "ep_selftest_dpp",
     "ep_observe",
+    "ep_observe_noalloc",
     "ep_stream_flux",
     "ep_set_positions",
     "ep_update_mhd",

@@ -446,6 +446,8 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
   HIPCHK(hipMemcpy(g.n_active, na.data(), S * 4, hipMemcpyHostToDevice), "n_active");
   g.h_nactive = na;
   HIPCHK(hipStreamSynchronize(g.st), "init sync");
+  double *scr = nullptr;  // the fixed scratch's address on this device
+  if ((rc = scratch_address(&scr))) { free_all(); return rc; }
   return EP_OK;
 }
 

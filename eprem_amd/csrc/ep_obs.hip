@@ -126,28 +126,42 @@ __global__ __launch_bounds__(256) void k_stream_flux(Dev D, double *flux) {
 
 using namespace ep;
 
+#define HIPCHK_OBS(x, what)                              \
+  do {                                                   \
+    hipError_t e_ = (x);                                 \
+    if (e_ != hipSuccess) return hip_fail(e_, what);     \
+  } while (0)
+
 extern "C" {
 
-int ep_observe(const ep_observers *obs, double *dist, double *flux) {
+}  // extern "C"
+
+namespace ep {
+
+// Doubles of device memory ep_observe needs for O observers: their
+// coordinates, the per-stream projections and weights, the blended
+// distribution and the flux.
+static size_t observe_need(int O, int S, int E, int M) {
+  const size_t EM = (size_t)E * M;
+  return 3 * (size_t)O + (size_t)O * S * EM + (size_t)O * S + (size_t)O * EM + (size_t)O * E;
+}
+
+static int check_observers(const ep_observers *obs) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
   if (!g.nodes_set) return fail(EP_ERR_INVALID_ARG, "ep_set_nodes first");
   if (!obs || obs->num < 1 || obs->num > 256 || !obs->r_au || !obs->theta || !obs->phi)
     return fail(EP_ERR_INVALID_ARG, "bad observer list (1..256 observers)");
-  const int O = obs->num, S = g.d.num_streams, E = g.d.num_energies, M = g.d.num_mu;
-  const size_t EM = (size_t)E * M;
-  for (int o = 0; o < O; o++)
+  for (int o = 0; o < obs->num; o++)
     if (!(obs->r_au[o] > 0) || !std::isfinite(obs->theta[o]) || !std::isfinite(obs->phi[o]))
       return fail(EP_ERR_INVALID_ARG, "observer %d: need r > 0 and finite angles", o);
-  const size_t need = 3 * O + (size_t)O * S * EM + (size_t)O * S + (size_t)O * EM + (size_t)O * E;
-  if (need > g.obs_cap) {
-    if (g.obs_buf) (void)hipFree(g.obs_buf);
-    g.obs_buf = nullptr;
-    g.obs_cap = 0;
-    hipError_t e = hipMalloc(&g.obs_buf, need * sizeof(double));
-    if (e != hipSuccess) return hip_fail(e, "observer scratch");
-    g.obs_cap = need;
-  }
-  double *b = g.obs_buf;
+  return EP_OK;
+}
+
+// The observer kernels on the device buffer b (observe_need doubles);
+// every word of b they read they have written first.
+static int observe_on(const ep_observers *obs, double *dist, double *flux, double *b) {
+  const int O = obs->num, S = g.d.num_streams, E = g.d.num_energies, M = g.d.num_mu;
+  const size_t EM = (size_t)E * M;
   ObsDev od;
   od.num = O;
   od.idw_p = obs->idw_p;
@@ -181,6 +195,68 @@ int ep_observe(const ep_observers *obs, double *dist, double *flux) {
     if (e != hipSuccess) return hip_fail(e, "observer download");
   }
   return EP_OK;
+}
+
+// Fixed scratch of ep_observe_noalloc: 32 MiB that each GPU the library
+// runs on holds from the module's load until the process ends.  In host
+// code `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress, once per device (scratch_device), and only that
+// address is handed to kernels and copies.
+constexpr size_t kScratchDoubles = (size_t)4 << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double *g_scratch_dev[kMaxDevices];
+
+// The device address of the current device's scratch: looked up with
+// hipGetSymbolAddress once per device, with that device current (ep_init
+// does it for its device; a later call on another device does it there).
+int scratch_address(double **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(EP_ERR_NO_DEVICE, "device %d out of range", dev);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = (double *)p;
+  }
+  *ptr = g_scratch_dev[dev];
+  return EP_OK;
+}
+
+}  // namespace ep
+
+extern "C" {
+
+int ep_observe(const ep_observers *obs, double *dist, double *flux) {
+  int rc = check_observers(obs);
+  if (rc) return rc;
+  const size_t need = observe_need(obs->num, g.d.num_streams, g.d.num_energies, g.d.num_mu);
+  if (need > g.obs_cap) {
+    if (g.obs_buf) (void)hipFree(g.obs_buf);
+    g.obs_buf = nullptr;
+    g.obs_cap = 0;
+    hipError_t e = hipMalloc(&g.obs_buf, need * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "observer scratch");
+    g.obs_cap = need;
+  }
+  return observe_on(obs, dist, flux, g.obs_buf);
+}
+
+int ep_observe_noalloc(const ep_observers *obs, double *dist, double *flux) {
+  int rc = check_observers(obs);
+  if (rc) return rc;
+  const size_t need = observe_need(obs->num, g.d.num_streams, g.d.num_energies, g.d.num_mu);
+  if (need > kScratchDoubles)
+    return fail(EP_ERR_INVALID_ARG,
+                "ep_observe_noalloc: %d observers x %d streams x %d x %d cells need %zu bytes of scratch, "
+                "more than its %zu",
+                obs->num, g.d.num_streams, g.d.num_energies, g.d.num_mu, need * sizeof(double),
+                kScratchDoubles * sizeof(double));
+  double *b = nullptr;
+  if ((rc = scratch_address(&b))) return rc;
+  return observe_on(obs, dist, flux, b);
 }
 
 int ep_stream_flux(double *flux) {

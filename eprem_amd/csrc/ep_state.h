@@ -133,6 +133,8 @@ struct State {
 extern State g;
 int fail(int code, const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);
+// device address of ep_observe_noalloc's fixed scratch (ep_obs.hip)
+int scratch_address(double **ptr);
 Dev make_dev();
 
 }  // namespace ep

@@ -57,6 +57,7 @@ def lib():
         L.ep_finalize.argtypes = []
         L.ep_selftest_dpp.argtypes = [DP, DP, DP]
         L.ep_observe.argtypes = [C.POINTER(abi.EpObservers), DP, DP]
+        L.ep_observe_noalloc.argtypes = [C.POINTER(abi.EpObservers), DP, DP]
         L.ep_stream_flux.argtypes = [DP]
         L.ep_set_positions.argtypes = [DP, DP, DP, DP, IP]
         L.ep_update_mhd.argtypes = [C.POINTER(abi.EpMhdParams), C.POINTER(EpMasPrev), C.POINTER(EpMasOut), DP]
@@ -288,15 +289,17 @@ class EpSolver:
         outs["device_ms"] = ms.value
         return outs
 
-    def observe(self, r_au, theta, phi, idw_p=3.0):
+    def observe(self, r_au, theta, phi, idw_p=3.0, noalloc=False):
         """Distribution [O, E, M] and omnidirectional flux [O, E] at point
-        observers (code units)."""
+        observers (code units); noalloc: through ep_observe_noalloc (fixed
+        scratch, no allocation)."""
         r, t, ph = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (r_au, theta, phi))
         o = len(r)
         ob = abi.EpObservers(o, dptr(r), dptr(t), dptr(ph), float(idw_p))
         E, M = self.dims.num_energies, self.dims.num_mu
         dist, flux = np.zeros((o, E, M)), np.zeros((o, E))
-        _check(lib().ep_observe(C.byref(ob), dptr(dist), dptr(flux)))
+        fn = lib().ep_observe_noalloc if noalloc else lib().ep_observe
+        _check(fn(C.byref(ob), dptr(dist), dptr(flux)))
         return dist, flux
 
     def stream_flux(self):

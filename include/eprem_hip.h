@@ -307,6 +307,18 @@ typedef struct ep_observers {
  * dist [num][E][M] and flux [num][E] (either may be NULL), code units. */
 int ep_observe(const ep_observers *obs, double *dist, double *flux);
 
+/* ep_observe without any allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  Its partial
+ * results (observer coordinates, per-stream projections and weights) live
+ * in a fixed 32 MiB device buffer that the library holds on each GPU it
+ * runs on until the process ends, instead of the buffer ep_observe
+ * allocates; a request needing more (3 num + num S (E M + 1) + num E (M +
+ * 1) doubles) fails with EP_ERR_INVALID_ARG.  Results are identical to
+ * ep_observe's.  Every call on a GPU shares that buffer: calls that
+ * overlap on one GPU (other host threads) race. */
+int ep_observe_noalloc(const ep_observers *obs, double *dist, double *flux);
+
 /* computeFlux (src/unifiedOutput.c:153-180) on every node:
  * flux [num_streams][num_nodes][num_energies], code units. */
 int ep_stream_flux(double *flux);
