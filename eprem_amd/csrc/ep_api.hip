@@ -127,10 +127,21 @@ static int check_params(const ep_params &p) {
     return fail(EP_ERR_INVALID_ARG, "rScale, tDel must be > 0 and numEpSteps >= 1");
   if (p.mfpType != 1 && p.mfpType != 2) return fail(EP_ERR_INVALID_ARG, "mfpType must be 1 or 2");
   if (!(p.eMin > 0) || !(p.eMax >= p.eMin)) return fail(EP_ERR_INVALID_ARG, "bad eMin/eMax");
-  if (p.adiabaticFocusAlg != 1 || p.adiabaticChangeAlg != 1)
-    return fail(EP_ERR_UNSUPPORTED,
-                "only the upwind focusing/change operators (alg 1, the reference default) are "
-                "implemented on the device");
+  if (p.adiabaticFocusAlg < 1 || p.adiabaticFocusAlg > 3 || p.adiabaticChangeAlg < 1 || p.adiabaticChangeAlg > 3)
+    return fail(EP_ERR_INVALID_ARG, "adiabaticFocusAlg / adiabaticChangeAlg must be 1, 2 or 3");
+  return EP_OK;
+}
+
+// What the operator variants need of the dimensions and storage: the
+// reference's WENO3 stencils read four cells on each line (they run past
+// their arrays below that), and the RK3 kernels work on FP64 storage.
+static int check_algs(const ep_params &p, const ep_dims &d, int fp32) {
+  if (p.adiabaticFocusAlg == 3 && d.num_mu < 4)
+    return fail(EP_ERR_UNSUPPORTED, "WENO3 focusing (adiabaticFocusAlg = 3) needs at least 4 mu cells");
+  if (p.adiabaticChangeAlg == 3 && d.num_energies < 4)
+    return fail(EP_ERR_UNSUPPORTED, "WENO3 adiabatic change (adiabaticChangeAlg = 3) needs at least 4 energies");
+  if (fp32 && (p.adiabaticFocusAlg != 1 || p.adiabaticChangeAlg != 1))
+    return fail(EP_ERR_UNSUPPORTED, "the RK3 operators (alg 2, 3) need FP64 storage (precision mode 0 or 2)");
   return EP_OK;
 }
 
@@ -196,6 +207,8 @@ Dev make_dev() {
   D.stream_r = g.stream_r;
   D.stream_fuse = g.stream_fuse;
   D.stream_whole = g.stream_whole;
+  D.focus_alg = p.adiabaticFocusAlg;
+  D.change_alg = p.adiabaticChangeAlg;
   D.mfpRadialPower = p.mfpRadialPower;
   D.mfpBPower = p.mfpBPower;
   D.mhdBsAu = p.mhdBsAu;
@@ -341,7 +354,10 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
     e = (x);              \
     if (e != hipSuccess) return e; \
   } while (0)
-  const int fuse = (sh.do_stream && sh.do_focus && D.stream_fuse) ? 1 : 0;
+  // the streaming passes and the one-pass node kernels carry the upwind
+  // operators only
+  const int fuse = (sh.do_stream && sh.do_focus && D.stream_fuse && D.focus_alg == 1) ? 1 : 0;
+  const bool rk = D.focus_alg != 1 || D.change_alg != 1;
   for (int sub = 0; sub < sh.nsub; sub++) {
     hipEvent_t *e4 = ev ? &ev[2 + 4 * sub] : nullptr;
     if (sh.do_seed && !sh.do_stream) EPQ(launch_seed_check(D, st));
@@ -358,7 +374,14 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
       EPQ(launch_stream_finish(D, sub, st));
     }
     if (e4) EPQ(hipEventRecord(e4[0], st));
-    if (sh.node_fused == 2) {
+    if (rk) {
+      // AdiabaticFocusing then AdiabaticChange, each with its own operator
+      if (sh.do_focus)
+        EPQ(D.focus_alg == 1 ? launch_focus(D, sub, fuse, st) : launch_focus_rk(D, D.focus_alg, sub, st));
+      if (e4) EPQ(hipEventRecord(e4[1], st));
+      if (sh.do_change)
+        EPQ(D.change_alg == 1 ? launch_change(D, sub, st) : launch_change_rk(D, D.change_alg, sub, st));
+    } else if (sh.node_fused == 2) {
       // the path is chosen on the device from the preparation's largest
       // change subcycle count: heavy nodes are focused first on both paths,
       // then either every node is focused and changed in one pass over f
@@ -411,6 +434,7 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
                 "dims out of range (streams 1..16383, nodes 2..16383, energies 2..64, mu 2..32)");
   int rc = check_params(*params);
   if (rc) return rc;
+  if ((rc = check_algs(*params, d, 0))) return rc;
   g.d = d;
   g.p = *params;
   g.Npad = (d.num_nodes + 63) / 64 * 64;
@@ -456,6 +480,7 @@ int ep_set_params(const ep_params *params) {
   if (!params) return fail(EP_ERR_INVALID_ARG, "null params");
   int rc = check_params(*params);
   if (rc) return rc;
+  if ((rc = check_algs(*params, g.d, g.fp32))) return rc;
   g.p = *params;
   g.drop_graph();
   build_grids(g.p, g.d.num_energies, g.d.num_mu);
@@ -638,6 +663,8 @@ int ep_set_precision(int32_t mode) {
   if (mode < 0 || mode > 2)
     return fail(EP_ERR_INVALID_ARG, "precision mode must be 0 (FP64), 1 (mixed) or 2 (FP64, fused)");
   const int fp32 = (mode == 1) ? 1 : 0;
+  int rc = check_algs(g.p, g.d, fp32);
+  if (rc) return rc;
   if (fp32 != g.fp32) {
     Dev D = make_dev();
     HIPCHK(launch_normalize(D, g.st), "normalize");
@@ -822,7 +849,7 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   int node_fused;
   StepShape shape;
   if (one_sequence) {
-    node_fused = (g.node_fuse && do_focus && do_change) ? 2 : 0;
+    node_fused = (g.node_fuse && do_focus && do_change && p.adiabaticFocusAlg == 1 && p.adiabaticChangeAlg == 1) ? 2 : 0;
     shape = StepShape{0, 0, nsub, do_stream, do_focus, do_change, do_seed, do_shell | (do_drift << 1), g.generation,
                       node_fused, sched};
   } else {
@@ -838,7 +865,8 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
       return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
     }
     if (do_shell || do_drift) HIPCHK(launch_shell_links(D, st), "shell links");
-    node_fused = (g.node_fuse && do_focus && do_change && hd.max_sub_change <= 1) ? 1 : 0;
+    node_fused = (g.node_fuse && do_focus && do_change && hd.max_sub_change <= 1 && p.adiabaticFocusAlg == 1 &&
+                  p.adiabaticChangeAlg == 1) ? 1 : 0;
     shape = StepShape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
                       do_shell | (do_drift << 1), g.generation, node_fused, sched};
   }

@@ -116,6 +116,7 @@ struct Dev {
   DevDiag *diag;
   int fp32;  // f stored as float (mixed-precision form)
   int fast;  // streaming with fused multiply-adds (not bit-identical)
+  int focus_alg, change_alg;  // 1 upwind Euler, 2 RK3 upwind, 3 RK3 WENO3
   int gated;      // launched as part of an EP substep (see substep_off)
   int node_gate;  // node-stage path: 0 always, 1 / 2 when every / not every node needs <= 1 change subcycle
 };

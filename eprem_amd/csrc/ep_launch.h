@@ -35,6 +35,10 @@ hipError_t launch_stream_finish(const Dev &D, int substep, hipStream_t st);
 // heavy = 0: leave the heavy nodes to a separate launch_focus_heavy
 hipError_t launch_focus(const Dev &D, int substep, int fused, hipStream_t st, int spare = 0, int heavy = 1);
 hipError_t launch_change(const Dev &D, int substep, hipStream_t st);
+// focusing / change with the RK3 integrator, alg 2 (upwind) or 3 (WENO3),
+// on every node (ep_rk.hip; FP64 storage)
+hipError_t launch_focus_rk(const Dev &D, int alg, int substep, hipStream_t st);
+hipError_t launch_change_rk(const Dev &D, int alg, int substep, hipStream_t st);
 // focusing of the heavy nodes only (nsub_f >= kHeavyFocus), in place
 hipError_t launch_focus_heavy(const Dev &D, int substep, hipStream_t st);
 // focusing then change of every node in one pass (ep_node.hip); heavy

@@ -111,8 +111,9 @@ typedef struct ep_params {
          boundaryFunctBeta, boundaryFunctEcutoff, boundaryFunctR0;
   int32_t useParallelDiffusion, useAdiabaticFocus, useAdiabaticChange,
           checkSeedPopulation, useEPBoundary, useBoundaryFunction;
-  int32_t adiabaticFocusAlg;  /* 1 = upwind (the only one on the GPU)      */
-  int32_t adiabaticChangeAlg; /* 1 = upwind (the only one on the GPU)      */
+  int32_t adiabaticFocusAlg;  /* 1 upwind Euler, 2 RK3 upwind, 3 RK3 WENO3  */
+  int32_t adiabaticChangeAlg; /* (3 needs >= 4 mu cells / energies; 2 and 3
+                                 need FP64 storage)                        */
 } ep_params;
 
 /* Per-node inputs, each array [num_streams][num_nodes] (node fastest).

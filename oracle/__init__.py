@@ -36,6 +36,10 @@ def lib():
         L.or_update_ep.restype = C.c_int
         L.or_get_stream_list.argtypes = [C.POINTER(EpParams), C.c_int, DP, DP, DP, IP, IP, DP, DP, DP]
         L.or_get_stream_list.restype = C.c_int
+        L.or_focus_operator.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, DP, DP, DP]
+        L.or_focus_operator.restype = None
+        L.or_change_operator.argtypes = [C.POINTER(EpParams), C.c_int, C.c_int, C.c_double, DP, DP, DP]
+        L.or_change_operator.restype = None
         L.or_mas_bsearch.argtypes = [C.POINTER(C.c_float), C.c_double, IP, IP, C.c_int, C.c_int]
         L.or_mas_bsearch.restype = C.c_double
         FPA = C.POINTER(C.c_float) * 7
@@ -193,6 +197,30 @@ def run_parker(cfg, max_dumps=64, max_steps=4096, nthreads=0, raw=False):
     res["rc"] = rc
     res["diag"] = [diags[i].as_dict() for i in range(o.num_steps)]
     return res
+
+
+def focus_operator(alg, f, v_avg, dmu):
+    """d(v f)/dmu of the focusing operator (alg 1 upwind, 3 WENO3) for
+    f [E][M] and half-mesh velocities v_avg [E][M+1]."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    va = np.ascontiguousarray(v_avg, dtype=np.float64)
+    E, M = f.shape
+    assert va.shape == (E, M + 1)
+    f1 = np.zeros((E, M))
+    lib().or_focus_operator(int(alg), E, M, float(dmu), dptr(f), dptr(va), dptr(f1))
+    return f1
+
+
+def change_operator(p, lnf, v_avg, rmag):
+    """The adiabatic-change operator of p.adiabaticChangeAlg on ln f [E][M]
+    with v_avg [E+1][M] at node radius rmag."""
+    lnf = np.ascontiguousarray(lnf, dtype=np.float64)
+    va = np.ascontiguousarray(v_avg, dtype=np.float64)
+    E, M = lnf.shape
+    assert va.shape == (E + 1, M)
+    f1 = np.zeros((E, M))
+    lib().or_change_operator(C.byref(p), E, M, float(rmag), dptr(lnf), dptr(va), dptr(f1))
+    return f1
 
 
 def mas_interp(rScale, x, y, z, rmag, level0, level1, s_cor, phi_offset, mas_radial_max, mas_vmin,

@@ -407,6 +407,125 @@ static void focus_operator_upwind(int E, int M, double dmu, double *f1,
   }
 }
 
+/* The WENO3 flux of one face from the split fluxes LN (left cells) and LP
+ * (right cells) around it: lnm2, lnm1, ln0 = LN at the cells two left, one
+ * left and right of the face; lpm1, lp0, lpp1 = LP at the cells left, right
+ * and two right of the face (AdiabaticFocusing_Operator_WENO3 and
+ * AdiabaticChange_Operator_WENO3 share it, src/energeticParticles.c:
+ * 2070-2103, :1321-1354). */
+static double weno3_face(double lnm2, double lnm1, double ln0, double lpm1, double lp0, double lpp1) {
+  const double D_C_CPt = 1 / 2.0, D_C_MCt = 1 / 2.0, D_M_Tt = 1 / 3.0, D_CP_Tt = 2 / 3.0,
+               D_P_Tt = 1 / 3.0, D_MC_Tt = 2 / 3.0;
+  const double weno_eps = 10.0 * sqrt(DBL_MIN);
+  double p0m = (1.0 + D_C_MCt) * lnm1 - D_C_MCt * lnm2;
+  double p1m = D_C_MCt * lnm1 + D_C_CPt * ln0;
+  double p0p = (1.0 + D_C_CPt) * lp0 - D_C_CPt * lpp1;
+  double p1p = D_C_CPt * lp0 + D_C_MCt * lpm1;
+  double B0m = 4.0 * pow((D_C_MCt * (lnm1 - lnm2)), 2);
+  double B1m = 4.0 * pow((D_C_CPt * (ln0 - lnm1)), 2);
+  double B0p = 4.0 * pow((D_C_CPt * (lpp1 - lp0)), 2);
+  double B1p = 4.0 * pow((D_C_MCt * (lp0 - lpm1)), 2);
+  double w0m = D_P_Tt / pow((weno_eps + B0m), 2);
+  double w1m = D_MC_Tt / pow((weno_eps + B1m), 2);
+  double w0p = D_M_Tt / pow((weno_eps + B0p), 2);
+  double w1p = D_CP_Tt / pow((weno_eps + B1p), 2);
+  double wm_sum = w0m + w1m, wp_sum = w0p + w1p;
+  double um = (w0m / wm_sum) * p0m + (w1m / wm_sum) * p1m;
+  double up = (w0p / wp_sum) * p0p + (w1p / wp_sum) * p1p;
+  return up + um;
+}
+
+/* alpha of the WENO3 operators at cell i of a line of n cells whose
+ * half-mesh velocities va[0..n] are extended by copies at both ends: the
+ * largest |va| over faces i-2..i+3 that exist, in the reference's order
+ * (src/energeticParticles.c:1961-2017 for mu, :1218-1274 for energy). */
+static double weno3_alpha(const double *va, int stride, int i, int n) {
+  double v = fabs(va[i * stride]);
+#define OR_AL(j) \
+  if (fabs(va[(j) * stride]) > v) v = fabs(va[(j) * stride]);
+  if (i >= 2 && i < n - 2) {
+    OR_AL(i - 1) OR_AL(i - 2) OR_AL(i + 1) OR_AL(i + 2) OR_AL(i + 3)
+  } else if (i == 0) {
+    OR_AL(i + 1) OR_AL(i + 2) OR_AL(i + 3)
+  } else if (i == 1) {
+    OR_AL(i - 1) OR_AL(i + 1) OR_AL(i + 2) OR_AL(i + 3)
+  } else if (i == n - 2) {
+    OR_AL(i + 1) OR_AL(i + 2) OR_AL(i - 1) OR_AL(i - 2)
+  } else {
+    OR_AL(i + 1) OR_AL(i - 1) OR_AL(i - 2)
+  }
+#undef OR_AL
+  return v;
+}
+
+/* AdiabaticFocusing_Operator_WENO3 (src/energeticParticles.c:1899-2144)
+ * for one energy row (v_avg / flux [M+1], f / f1 [M]); needs M >= 4.
+ * ln / lp / al are scratch rows of M. */
+static void focus_row_weno3(int M, double dmu, double *f1, const double *f, const double *va, double *flux,
+                            double *ln, double *lp, double *al) {
+  const double upwind = 1.0;
+  int m;
+  double cce;
+  for (m = 0; m < M; m++) al[m] = weno3_alpha(va, 1, m, M);
+  for (m = 0; m < M; m++) {
+    lp[m] = 0.5 * f[m] * va[m + 1] - al[m];
+    ln[m] = 0.5 * f[m] * va[m] + al[m];
+  }
+  for (m = 2; m < M - 1; m++) flux[m] = weno3_face(ln[m - 2], ln[m - 1], ln[m], lp[m - 1], lp[m], lp[m + 1]);
+  m = 1;
+  cce = copysign(upwind, va[m]);
+  flux[m] = va[m] * 0.5 * ((1.0 - cce) * f[m] + (1.0 + cce) * f[m - 1]);
+  m = M - 1;
+  cce = copysign(upwind, va[m]);
+  flux[m] = va[m] * 0.5 * ((1.0 - cce) * f[m] + (1.0 + cce) * f[m - 1]);
+  for (m = 1; m < M - 1; m++) f1[m] = (flux[m + 1] - flux[m]) / dmu;
+  if (va[1] <= 0)
+    f1[0] = va[1] * f[1] / dmu;
+  else
+    f1[0] = va[1] * f[0] / dmu;
+  if (va[M - 1] >= 0)
+    f1[M - 1] = va[M - 1] * (-f[M - 2]) / dmu;
+  else
+    f1[M - 1] = va[M - 1] * (-f[M - 1]) / dmu;
+}
+
+/* The focusing operator of alg (1 upwind, 3 WENO3) on every energy row.
+ * scratch: 3 M doubles. */
+static void focus_operator(int alg, int E, int M, double dmu, double *f1, const double *f, const double *v_avg,
+                           double *flux, double *scratch) {
+  int e;
+  if (alg != 3) {
+    focus_operator_upwind(E, M, dmu, f1, f, v_avg, flux);
+    return;
+  }
+  for (e = 0; e < E; e++)
+    focus_row_weno3(M, dmu, f1 + e * M, f + e * M, v_avg + e * (M + 1), flux + e * (M + 1), scratch,
+                    scratch + M, scratch + 2 * M);
+}
+
+/* The optimal third-order strong-stability-preserving Runge-Kutta step of
+ * AdiabaticFocusing / AdiabaticChange for alg 2 and 3 (src/energeticParticles.c
+ * :1677-1743, :952-1018): s1 = f - dt L(f); s1 = 3/4 f + 1/4 s1 - 1/4 dt L(s1);
+ * f = 1/3 f + 2/3 s1 - 2/3 dt L(s1), in the reference's operation order. */
+static void rk3_stage(int stage, int n, double dt, double *f, double *s1, const double *f1) {
+  int i;
+  if (stage == 0)
+    for (i = 0; i < n; i++) s1[i] = f[i] - dt * f1[i];
+  else if (stage == 1)
+    for (i = 0; i < n; i++) s1[i] = (3.0 / 4.0) * f[i] + (1.0 / 4.0) * s1[i] - (1.0 / 4.0) * dt * f1[i];
+  else
+    for (i = 0; i < n; i++) f[i] = (1.0 / 3.0) * f[i] + (2.0 / 3.0) * s1[i] - (2.0 / 3.0) * dt * f1[i];
+}
+
+/* Test hooks: one application of the focusing operator (alg 1 upwind, 3
+ * WENO3) to f [E][M] with half-mesh velocities v_avg [E][M+1], and of the
+ * change operator to ln f [E][M] with v_avg [E+1][M] at radius rmag. */
+void or_focus_operator(int alg, int E, int M, double dmu, const double *f, const double *v_avg, double *f1) {
+  double *flux = (double *)malloc(sizeof(double) * (E * (M + 1) + 3 * M));
+  focus_operator(alg, E, M, dmu, f1, f, v_avg, flux, flux + E * (M + 1));
+  free(flux);
+}
+
 /* One node: returns N_subcycles (0 if vel_abs_max == 0). */
 static int focus_node(const ep_params *p, const or_grid *g, double dt,
                       double ds_node, double bplus, double bminus,
@@ -443,9 +562,21 @@ static int focus_node(const ep_params *p, const or_grid *g, double dt,
   dt_stable = safety_factor * g->dmu / vel_abs_max;
   N = (int)ceil(dt / dt_stable);
   dt_subcycle = dt / N;
-  for (s = 0; s < N; s++) {
-    focus_operator_upwind(E, M, g->dmu, f1, f, v_avg, flux);
-    for (e = 0; e < E * M; e++) f[e] = f[e] - dt_subcycle * f1[e];
+  if (p->adiabaticFocusAlg == 2 || p->adiabaticFocusAlg == 3) {
+    /* s1 and the WENO3 scratch sit after the E (M + 1) flux entries */
+    double *s1 = flux + E * (M + 1), *scr = s1 + E * M;
+    for (s = 0; s < N; s++) {
+      int st;
+      for (st = 0; st < 3; st++) {
+        focus_operator(p->adiabaticFocusAlg, E, M, g->dmu, f1, st == 0 ? f : s1, v_avg, flux, scr);
+        rk3_stage(st, E * M, dt_subcycle, f, s1, f1);
+      }
+    }
+  } else {
+    for (s = 0; s < N; s++) {
+      focus_operator_upwind(E, M, g->dmu, f1, f, v_avg, flux);
+      for (e = 0; e < E * M; e++) f[e] = f[e] - dt_subcycle * f1[e];
+    }
   }
   for (e = 0; e < E * M; e++) fn[e] = f[e];
   return N;
@@ -489,6 +620,64 @@ static void change_operator_upwind(const ep_params *p, const or_grid *g,
   }
 }
 
+/* AdiabaticChange_Operator_WENO3 (src/energeticParticles.c:1179-1427) for
+ * one mu column m (v_avg / flux [E+1][M], f / f1 [E][M]); needs E >= 4.
+ * ln / lp / al are scratch columns of E. */
+static void change_col_weno3(const ep_params *p, const or_grid *g, double rmag, int m, double *f1,
+                             const double *f, const double *v_avg, double *flux, double *ln, double *lp,
+                             double *al) {
+  const int E = g->E, M = g->M;
+  const double upwind = 1.0;
+  int e;
+  double cce;
+  for (e = 0; e < E; e++) al[e] = weno3_alpha(v_avg + m, M, e, E);
+  for (e = 0; e < E; e++) {
+    lp[e] = 0.5 * f[e * M + m] * v_avg[(e + 1) * M + m] - al[e];
+    ln[e] = 0.5 * f[e * M + m] * v_avg[e * M + m] + al[e];
+  }
+  for (e = 2; e < E - 1; e++)
+    flux[e * M + m] = weno3_face(ln[e - 2], ln[e - 1], ln[e], lp[e - 1], lp[e], lp[e + 1]);
+  e = 1;
+  cce = copysign(upwind, v_avg[e * M + m]);
+  flux[e * M + m] = v_avg[e * M + m] * 0.5 * ((1.0 - cce) * f[e * M + m] + (1.0 + cce) * f[(e - 1) * M + m]);
+  e = E - 1;
+  cce = copysign(upwind, v_avg[e * M + m]);
+  flux[e * M + m] = v_avg[e * M + m] * 0.5 * ((1.0 - cce) * f[e * M + m] + (1.0 + cce) * f[(e - 1) * M + m]);
+  for (e = 1; e < E - 1; e++) f1[e * M + m] = (flux[(e + 1) * M + m] - flux[e * M + m]) / g->dlnp;
+  {
+    double vl = v_avg[1 * M + m], vr = v_avg[(E - 1) * M + m];
+    if (vl <= 0)
+      f1[0 * M + m] = vl * (f[1 * M + m] - f[0 * M + m]) / g->dlnp;
+    else
+      f1[0 * M + m] = vl * (f[0 * M + m] - log(or_seed(p, g->egrid[0], rmag))) / g->dlnp;
+    if (vr >= 0)
+      f1[(E - 1) * M + m] = vr * (f[(E - 1) * M + m] - f[(E - 2) * M + m]) / g->dlnp;
+    else
+      f1[(E - 1) * M + m] = vr * (log(or_seed(p, g->egrid[E - 1], rmag)) - f[(E - 1) * M + m]) / g->dlnp;
+  }
+}
+
+static void change_operator(const ep_params *p, const or_grid *g, double rmag, double *f1, const double *f,
+                            const double *v_avg, double *flux, double *scratch) {
+  int m;
+  if (p->adiabaticChangeAlg != 3) {
+    change_operator_upwind(p, g, rmag, f1, f, v_avg, flux);
+    return;
+  }
+  for (m = 0; m < g->M; m++)
+    change_col_weno3(p, g, rmag, m, f1, f, v_avg, flux, scratch, scratch + g->E, scratch + 2 * g->E);
+}
+
+void or_change_operator(const ep_params *p, int E, int M, double rmag, const double *f, const double *v_avg,
+                        double *f1) {
+  or_grid g;
+  double *flux = (double *)malloc(sizeof(double) * ((E + 1) * M + 3 * E));
+  grid_alloc(&g, p, E, M);
+  change_operator(p, &g, rmag, f1, f, v_avg, flux, flux + (E + 1) * M);
+  grid_free(&g);
+  free(flux);
+}
+
 static int change_node(const ep_params *p, const or_grid *g, double dt,
                        double rmag, double dlnn, double dlnb, double dupar,
                        double *fn, double *f, double *f1, double *v_avg,
@@ -521,9 +710,20 @@ static int change_node(const ep_params *p, const or_grid *g, double dt,
   dt_stable = safety_factor * g->dlnp / vel_abs_max;
   N = (int)ceil(dt / dt_stable);
   dt_subcycle = dt / N;
-  for (s = 0; s < N; s++) {
-    change_operator_upwind(p, g, rmag, f1, f, v_avg, flux);
-    for (e = 0; e < E * M; e++) f[e] = f[e] - dt_subcycle * f1[e];
+  if (p->adiabaticChangeAlg == 2 || p->adiabaticChangeAlg == 3) {
+    double *s1 = flux + (E + 1) * (M + 1), *scr = s1 + E * M;
+    for (s = 0; s < N; s++) {
+      int st;
+      for (st = 0; st < 3; st++) {
+        change_operator(p, g, rmag, f1, st == 0 ? f : s1, v_avg, flux, scr);
+        rk3_stage(st, E * M, dt_subcycle, f, s1, f1);
+      }
+    }
+  } else {
+    for (s = 0; s < N; s++) {
+      change_operator_upwind(p, g, rmag, f1, f, v_avg, flux);
+      for (e = 0; e < E * M; e++) f[e] = f[e] - dt_subcycle * f1[e];
+    }
   }
   for (e = 0; e < E * M; e++) fn[e] = exp(f[e]);
   return N;
@@ -543,7 +743,10 @@ int or_update_ep(const ep_dims *d, const ep_params *p, const int *n_active,
   const size_t cells_node = (size_t)E * M;
   or_grid g;
   or_err err;
-  double *ds = (double *)malloc(sizeof(double) * SN);
+  double *ds;
+  /* the reference's WENO3 stencils read past their arrays below 4 cells */
+  if ((p->adiabaticFocusAlg == 3 && M < 4) || (p->adiabaticChangeAlg == 3 && E < 4)) return EP_ERR_UNSUPPORTED;
+  ds = (double *)malloc(sizeof(double) * SN);
   double *bplus = (double *)malloc(sizeof(double) * SN);
   double *bminus = (double *)malloc(sizeof(double) * SN);
   double dt = p->tDel / (1.0 * p->numEpSteps);
@@ -616,7 +819,8 @@ int or_update_ep(const ep_dims *d, const ep_params *p, const int *n_active,
       double *f1 = (double *)malloc(sizeof(double) * cells_node);
       double *vel = (double *)malloc(sizeof(double) * cells_node);
       double *v_avg = (double *)malloc(sizeof(double) * (E + 1) * (M + 1));
-      double *flux = (double *)malloc(sizeof(double) * (E + 1) * (M + 1));
+      /* half-mesh fluxes, then the RK3 stage and the WENO3 scratch */
+      double *flux = (double *)malloc(sizeof(double) * ((E + 1) * (M + 1) + E * M + 3 * (E + M + 2)));
       or_err lerr;
       int k, e, m;
       memset(&lerr, 0, sizeof(lerr));

@@ -189,6 +189,16 @@ int or_shell_drift(const ep_params *p, int S, int N, int E, int M, const int32_t
                    const double *y, const double *z, const double *curl, const double *pgrid,
                    const double *vgrid, double dt, double *f, int *bad);
 
+/* One application of the focusing operator (alg 1 upwind, 3 WENO3;
+ * src/energeticParticles.c:1806-1896, :1899-2144) to f [E][M] with the
+ * half-mesh velocities v_avg [E][M+1]: f1 = d(v f)/dmu [E][M]. */
+void or_focus_operator(int alg, int E, int M, double dmu, const double *f, const double *v_avg, double *f1);
+/* The adiabatic-change operator of p->adiabaticChangeAlg (1/2 upwind, 3
+ * WENO3; :1081-1176, :1179-1427) on ln f [E][M] with v_avg [E+1][M] at
+ * node radius rmag (inflow boundary = ln of the seed spectrum). */
+void or_change_operator(const ep_params *p, int E, int M, double rmag, const double *f, const double *v_avg,
+                        double *f1);
+
 /* Run the whole analytic-wind simulation.  Returns EP_OK or an error. */
 int or_run_parker(const or_sim_config *c, or_sim_output *out, int nthreads);
 

@@ -66,6 +66,33 @@ def max_rel(a, b, n_active=None):
     return float(np.max(np.abs(a - b) / np.abs(b)))
 
 
+def active_mask(shape, n_active):
+    mask = np.zeros(shape, bool)
+    for s, n in enumerate(n_active):
+        mask[s, :n] = True
+    return mask
+
+
+def ulp_spread(dims, p, nodes, f0, n_active, steps, samples=4, seed=11):
+    """Per-cell largest move of the oracle's answer over runs whose node
+    inputs (coordinates, |B|, the MHD differences) each differ from the
+    given ones by about one ulp."""
+    rng = np.random.default_rng(seed)
+    fo = f0
+    for _ in range(steps):
+        fo = oracle.update_ep(dims, p, nodes, fo, n_active=n_active)[2]
+    spread = np.zeros_like(fo)
+    for _ in range(samples):
+        nd = {k: (v * (1.0 + 2.2e-16 * rng.standard_normal(v.shape))
+                  if isinstance(v, np.ndarray) and v.dtype == np.float64 else v)
+              for k, v in nodes.items()}
+        fp = f0
+        for _ in range(steps):
+            fp = oracle.update_ep(dims, p, nd, fp, n_active=n_active)[2]
+        spread = np.maximum(spread, np.abs(fp - fo))
+    return spread
+
+
 def assert_diag_equal(gd, od):
     assert len(gd) == len(od)
     for (rg, dg), (ro, do) in zip(gd, od):
@@ -499,6 +526,82 @@ def test_three_nodes_per_lane_parity(case):
                                     tiling=case["tiling"], precision=case["precision"])
     assert_diag_equal(gd, od)
     assert max_rel(fg, fo, n_active) <= RTOL
+
+
+@pytest.mark.parametrize("algs", [(2, 1), (3, 1), (1, 2), (1, 3), (2, 2), (3, 3)])
+@pytest.mark.parametrize("case", [
+    dict(N=80, E=12, M=8, thin=0.0, precision=0),
+    dict(N=60, E=20, M=30, thin=0.0, precision=2),
+    dict(N=70, E=9, M=7, thin=0.15, precision=0),
+])
+def test_rk3_operator_parity(algs, case):
+    """AdiabaticFocusing / AdiabaticChange with the three-stage Runge-Kutta
+    integrator and the upwind (2) or WENO3 (3) operator against the oracle,
+    with streaming on, odd and even mu counts, thinned streams, two steps."""
+    falg, calg = algs
+    # The WENO3 focusing operator subtracts alpha from the split fluxes
+    # (LP = f v / 2 - alpha, src/energeticParticles.c:2021-2025) instead of
+    # multiplying it into them.  Two consequences for a parity test:
+    # - it carries no dissipation, and over hundreds of subcycles it
+    #   amplifies rounding without bound (at tDel = 1/64 the oracle's own
+    #   answer moves by 1e292 under one ulp of change in its inputs), so
+    #   its cases run at a step of a few subcycles;
+    # - the alphas, which scale with the velocities and not with f, cancel
+    #   in every face flux, and the operator drives some cells to zero by
+    #   cancellation (then to the floor); pitch-angle scattering spreads
+    #   what is left of them along the row.  There the answer depends on
+    #   the last ulps of the node inputs: one ulp of change in the node
+    #   coordinates moves the oracle's own answer by up to 2e-5 in such
+    #   cells.  Device logarithms and fused multiply-adds differ from the
+    #   host's by about that much, so each cell is held to RTOL plus four
+    #   times the oracle's own spread over runs whose node inputs differ
+    #   by an ulp, and the share of cells where that spread exceeds RTOL
+    #   must stay small.
+    weno_focus = falg == 3
+    tdel, perturb = (1.0 / 25600, 0.05) if weno_focus else (1.0 / 64, 0.2)
+    p = params(numEpSteps=2, adiabaticFocusAlg=falg, adiabaticChangeAlg=calg, tDel=tdel)
+    nodes = workloads.parker_nodes(1, 1, case["N"], seed=65, perturb=perturb, jitter=0.2, thin_frac=case["thin"])
+    n_active = np.array([case["N"], 37, case["N"] - 5, 12, 40, 3], np.int32)
+    fg, fo, gd, od, _, _ = run_both(p, nodes, case["E"], case["M"], n_active=n_active, steps=2,
+                                    precision=case["precision"])
+    assert_diag_equal(gd, od)
+    if od[-1][0] != 0:
+        # both report the same failure (WENO3 change of WENO3-focused rows
+        # overflows its smoothness weights to NaN); f is not defined after it
+        return
+    if not weno_focus:
+        assert max_rel(fg, fo, n_active) <= RTOL
+        return
+    dims = workloads.dims_of(nodes, case["E"], case["M"])
+    f0 = workloads.seeded_f(p, oracle.build_grids(p, case["E"], case["M"]), nodes, noise=0.3, seed=2)
+    spread = ulp_spread(dims, p, nodes, f0, n_active, steps=2)
+    mask = active_mask(fo.shape, n_active)
+    # the envelope stays narrow: the bound below is RTOL in most cells and
+    # the median cell's envelope is at the 1e-13 level
+    rel_spread = (spread / np.abs(fo))[mask]
+    assert np.mean(rel_spread > RTOL) < 0.3
+    assert np.median(rel_spread) < 1e-12
+    excess = np.abs(fg - fo) / (RTOL * np.abs(fo) + 4.0 * spread)
+    assert float(np.max(excess[mask])) <= 1.0
+
+
+def test_rk3_operator_limits():
+    """WENO3 needs four cells per line and the RK3 kernels FP64 storage:
+    both are refused on the host before any device work."""
+    nodes = workloads.parker_nodes(1, 1, 20, seed=66)
+    with pytest.raises(runtime.EpError) as ei:
+        runtime.EpSolver(workloads.dims_of(nodes, 8, 3), params(adiabaticFocusAlg=3))
+    assert ei.value.code == abi.EP_ERR_UNSUPPORTED
+    with pytest.raises(runtime.EpError) as ei:
+        runtime.EpSolver(workloads.dims_of(nodes, 3, 8), params(adiabaticChangeAlg=3))
+    assert ei.value.code == abi.EP_ERR_UNSUPPORTED
+    sol = runtime.EpSolver(workloads.dims_of(nodes, 8, 6), params(adiabaticChangeAlg=2))
+    try:
+        with pytest.raises(runtime.EpError) as ei:
+            sol.set_precision(1)
+        assert ei.value.code == abi.EP_ERR_UNSUPPORTED
+    finally:
+        sol.close()
 
 
 @pytest.mark.parametrize("sched", [0, 1, 2])
