@@ -215,6 +215,7 @@ EXPORTED_SYMBOLS = (
     "ep_set_node_fusion",
     "ep_spawn_shell",
     "ep_move_nodes",
+    "ep_move_nodes_mas",
     "ep_get_positions",
     "ep_set_precision",
     "ep_selftest_copy",

@@ -94,14 +94,16 @@ __device__ __forceinline__ double mas_v_apply(double x) {
 }
 __device__ __forceinline__ double mas_rho_apply(double x) { return x * (1.6726e-16 / 1.6726e-24) / 1.0; }
 
-// masGetNode for one node: masTriLinear inside mas_radial_max, masWind
-// outside.  out7 = {Br, Btheta, Bphi, Vr, Vtheta, Vphi, density}.
-__device__ void mas_get_node(const Dev &D, const MasDev &M, size_t i, double s, double phi_offset,
-                             double radial_max, double vmin, const double *prev, size_t SN,
-                             double out7[7]) {
+// masGetNode at the position (px, py, pz) [rScale units] for node i:
+// masTriLinear inside mas_radial_max, else masWind of node i (its previous
+// old position, its rmag and MHD state, whatever position is asked).
+// out7 = {Br, Btheta, Bphi, Vr, Vtheta, Vphi, density}.
+__device__ void mas_at(const Dev &D, const MasDev &M, double px, double py, double pz, size_t i, double s,
+                       double phi_offset, double radial_max, double vmin, const double *prev, size_t SN,
+                       double out7[7]) {
   const double rs = D.rScale;
   // cartToSphPosAu (src/geometry.c:131-160, :35-45)
-  const double ax = D.x[i] * rs, ay = D.y[i] * rs, az = D.z[i] * rs;
+  const double ax = px * rs, ay = py * rs, az = pz * rs;
   const double pr = sqrt(ax * ax + ay * ay + az * az);
   const double pth = acos(az / pr);
   double pph = atan2(ay, ax);
@@ -145,6 +147,72 @@ __device__ void mas_get_node(const Dev &D, const MasDev &M, size_t i, double s, 
     out7[5] = prev[8 * SN + i] * oneOverR;
     out7[6] = prev[9 * SN + i] * oneOverR * oneOverR;
   }
+}
+
+// masGetNode for node i at its own position.
+__device__ __forceinline__ void mas_get_node(const Dev &D, const MasDev &M, size_t i, double s, double phi_offset,
+                                             double radial_max, double vmin, const double *prev, size_t SN,
+                                             double out7[7]) {
+  mas_at(D, M, D.x[i], D.y[i], D.z[i], i, s, phi_offset, radial_max, vmin, prev, SN, out7);
+}
+
+// sphToCartVector (src/geometry.c:89-125): (r, theta, phi) components v at
+// position (x, y, z) to Cartesian.
+__device__ __forceinline__ void sph_to_cart(const double *v, double x, double y, double z, double *out) {
+  const double rmag = sqrt(x * x + y * y + z * z);
+  if ((rmag - fabs(z)) < 1.0e-33) {
+    out[0] = 0.0;
+    out[1] = 0.0;
+  } else {
+    out[0] = x / rmag * v[0] + x * z / (rmag * sqrt(rmag * rmag - z * z)) * v[1] - y / sqrt(rmag * rmag - z * z) * v[2];
+    out[1] = y / rmag * v[0] + y * z / (rmag * sqrt(rmag * rmag - z * z)) * v[1] + x / sqrt(rmag * rmag - z * z) * v[2];
+  }
+  out[2] = z / rmag * v[0] - sqrt(rmag * rmag - z * z) / rmag * v[1];
+}
+
+// masMoveNodes (src/masInterp.c:1344-1393): every node but the inner one
+// takes the classical RK4 step of rungeKuttaFlow (:1305-1338), whose four
+// velocities are vMas (:1279-1296) -- masGetNode at the stage position,
+// made Cartesian there.  Stage positions r0, r0 + k1/2, r0 + k2/2, r0 + k3;
+// r1 = r0 + (k1 + 2 k2 + 2 k3 + k4) / 6 in the reference's operation
+// order.  The node's old position (read by masWind at every stage) is
+// replaced by its position before the move only after the four stages.
+// state: the MHD buffer ([0..2] old position, [3..9] B, V, density).
+__global__ __launch_bounds__(256) void k_move_nodes_mas(Dev D, MasDev M, double *x, double *y, double *z,
+                                                        double *rmag, double *state, double tdel, double s,
+                                                        double phi_offset, double radial_max, double vmin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int st = blockIdx.y;
+  if (k >= D.n_active[st]) return;
+  const size_t i = (size_t)st * D.Npad + k;
+  const size_t SN = (size_t)D.S * D.Npad;
+  const double r0[3] = {x[i], y[i], z[i]};
+  if (k > 0) {
+    const double sixth = 1.0 / 6.0, two = 2.0, half = 0.5;
+    const double h = tdel / D.rScale;
+    double kk[4][3];
+#pragma unroll
+    for (int sg = 0; sg < 4; ++sg) {
+      double pos[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) pos[q] = sg == 0 ? r0[q] : (sg < 3 ? r0[q] + kk[sg - 1][q] * half : r0[q] + kk[2][q]);
+      double o7[7], vc[3];
+      mas_at(D, M, pos[0], pos[1], pos[2], i, s, phi_offset, radial_max, vmin, state, SN, o7);
+      sph_to_cart(o7 + 3, pos[0], pos[1], pos[2], vc);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) kk[sg][q] = vc[q] * h;
+    }
+    const double nx = r0[0] + sixth * (kk[0][0] + two * kk[1][0] + two * kk[2][0] + kk[3][0]);
+    const double ny = r0[1] + sixth * (kk[0][1] + two * kk[1][1] + two * kk[2][1] + kk[3][1]);
+    const double nz = r0[2] + sixth * (kk[0][2] + two * kk[1][2] + two * kk[2][2] + kk[3][2]);
+    x[i] = nx;
+    y[i] = ny;
+    z[i] = nz;
+    rmag[i] = sqrt((nx * nx) + (ny * ny) + (nz * nz));
+  }
+  state[i] = r0[0];
+  state[SN + i] = r0[1];
+  state[2 * SN + i] = r0[2];
 }
 
 __global__ __launch_bounds__(256) void k_mas_interp(Dev D, MasDev M, double s, double phi_offset,
@@ -462,6 +530,25 @@ int ep_update_mhd(const ep_mhd_params *mp, const ep_mas_prev *prev, ep_mas_out *
       if (e != hipSuccess) return hip_fail(e, "MHD output download");
     }
   }
+  return EP_OK;
+}
+
+int ep_move_nodes_mas(const ep_mas_params *mp, double tdel) {
+  if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
+  if (!g.mhd_state) return fail(EP_ERR_INVALID_ARG, "ep_update_mhd first (node MHD state)");
+  if (!mp) return fail(EP_ERR_INVALID_ARG, "null MAS params");
+  if (!(tdel >= 0) || !std::isfinite(tdel)) return fail(EP_ERR_INVALID_ARG, "tdel must be finite and >= 0");
+  if (!(fabs(mp->phi_offset) <= 1.0e3) || !std::isfinite(mp->s_cor) || !std::isfinite(mp->mas_radial_max))
+    return fail(EP_ERR_INVALID_ARG, "phi_offset must be finite with |phi_offset| <= 1000");
+  if (!g.mas[0].set || !g.mas[1].set) return fail(EP_ERR_INVALID_ARG, "set both MAS levels first");
+  MasDev M = mas_dev();
+  Dev D = make_dev();
+  hipLaunchKernelGGL(k_move_nodes_mas, dim3((g.d.num_nodes + 255) / 256, g.d.num_streams), dim3(256), 0, g.st, D,
+                     M, g.nx, g.ny, g.nz, g.nrmag, g.mas_prev, tdel, mp->s_cor, mp->phi_offset, mp->mas_radial_max,
+                     mp->mas_vmin);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(g.st);
+  if (e != hipSuccess) return hip_fail(e, "move nodes (MAS)");
   return EP_OK;
 }
 

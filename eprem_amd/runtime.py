@@ -64,6 +64,7 @@ def lib():
         L.ep_autotune.argtypes = [IP, IP, IP, IP, DP]
         L.ep_spawn_shell.argtypes = [C.POINTER(abi.EpSpawn)]
         L.ep_move_nodes.argtypes = [C.c_double]
+        L.ep_move_nodes_mas.argtypes = [C.POINTER(EpMasParams), C.c_double]
         L.ep_get_positions.argtypes = [DP, DP, DP, DP, IP]
         L.ep_set_graph.argtypes = [C.c_int32]
         L.ep_set_node_fusion.argtypes = [C.c_int32]
@@ -260,6 +261,12 @@ class EpSolver:
 
     def move_nodes(self, tdel):
         _check(lib().ep_move_nodes(float(tdel)))
+
+    def move_nodes_mas(self, tdel, s_cor, phi_offset, mas_radial_max, mas_vmin=0.0):
+        """masMoveNodes on the device: RK4 through the MAS levels set with
+        set_mas_level, masWind of the device-held node state outside."""
+        mp = EpMasParams(s_cor, phi_offset, mas_radial_max, mas_vmin)
+        _check(lib().ep_move_nodes_mas(C.byref(mp), float(tdel)))
 
     def positions(self):
         S, N = self.dims.num_streams, self.dims.num_nodes

@@ -285,6 +285,17 @@ int ep_spawn_shell(const ep_spawn *sp);
  * next ep_update_mhd.  tdel in code units. */
 int ep_move_nodes(double tdel);
 
+/* masMoveNodes (src/masInterp.c:1344-1393, rungeKuttaFlow :1305-1338,
+ * vMas :1279-1296): every node but node 0 takes one classical RK4 step of
+ * length tdel through the MAS velocity, each stage interpolated from the
+ * two levels set with ep_mas_set_level at s_cor (as ep_update_mhd, mode 1)
+ * at the stage position; outside mas_radial_max masWind of the node's
+ * device-held state (its old position and MHD values from the last
+ * ep_update_mhd).  The position before the move becomes the node's old
+ * position.  Call it where the reference calls masMoveNodes: after the
+ * spawn and before the next ep_update_mhd.  tdel in code units. */
+int ep_move_nodes_mas(const ep_mas_params *mp, double tdel);
+
 /* Host copies of the device node positions and active counts (any pointer
  * may be NULL). */
 int ep_get_positions(double *x, double *y, double *z, double *rmag, int32_t *n_active);

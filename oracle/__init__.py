@@ -52,6 +52,10 @@ def lib():
         L.or_stream_flux.argtypes = [C.POINTER(EpDims), DP, DP, DP]
         L.or_move_nodes.argtypes = [C.c_int, C.c_double, C.c_double] + [DP] * 10
         L.or_move_nodes.restype = None
+        FPP_ = C.POINTER(C.POINTER(C.c_float))
+        L.or_move_nodes_mas.argtypes = ([C.c_int, C.c_double, C.c_double] + [DP] * 7 +
+                                        [C.POINTER(DP), FPP_, FPP_, IP, FPP_, FPP_, IP] + [C.c_double] * 4)
+        L.or_move_nodes_mas.restype = None
         L.or_update_mhd.argtypes = ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, DP, DP, DP, DP, IP,
                                      C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_float)), IP,
@@ -334,6 +338,57 @@ def move_nodes(nodes, state, tdel, rscale, first=1):
             pos[k][s, first:] = a
         for k, a in zip(("rold_x", "rold_y", "rold_z"), args[4:7]):
             old[k][s, first:] = a
+    nodes.update(pos)
+    return old
+
+
+def _mas_level(level, keep):
+    """(fields[7], meshes[21]) as the C pointer tables and the dims array."""
+    FPP = C.POINTER(C.POINTER(C.c_float))
+    fields = [np.ascontiguousarray(a, dtype=np.float32) for a in level[0]]
+    meshes = [np.ascontiguousarray(a, dtype=np.float32) for a in level[1]]
+    dims = np.array([len(m) for m in meshes], np.int32)
+    fa = (C.POINTER(C.c_float) * 7)(*[fptr(a) for a in fields])
+    ma = (C.POINTER(C.c_float) * 21)(*[fptr(a) for a in meshes])
+    keep.extend(fields + meshes + [dims, fa, ma])
+    return C.cast(fa, FPP), C.cast(ma, FPP), iptr(dims)
+
+
+MHD_NAMES = ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")
+
+
+def move_nodes_mas(nodes, state, tdel, rscale, level0, level1, s_cor, phi_offset, mas_radial_max, mas_vmin,
+                   n_active=None, first=1):
+    """or_move_nodes_mas (masMoveNodes) on [S, N] node arrays in place:
+    nodes first..n_active-1 of every stream move through the MAS field.
+    state: each node's previous old position (rold_x, rold_y, rold_z) and
+    MHD state (MHD_NAMES), [S, N].  Returns the new old positions: the
+    positions before the move (the unmoved ones for nodes before
+    `first`)."""
+    S, N = nodes["x"].shape
+    na = np.full(S, N, np.int32) if n_active is None else np.asarray(n_active, np.int32)
+    pos = {k: np.ascontiguousarray(nodes[k], dtype=np.float64) for k in ("x", "y", "z", "rmag")}
+    old = {k: np.ascontiguousarray(state[k], dtype=np.float64).copy() for k in ("rold_x", "rold_y", "rold_z")}
+    for k, c in (("rold_x", "x"), ("rold_y", "y"), ("rold_z", "z")):
+        old[k][:, :first] = pos[c][:, :first]
+    keep = []
+    f0, m0, d0 = _mas_level(level0, keep)
+    f1, m1, d1 = _mas_level(level1, keep)
+    for s in range(S):
+        n = int(na[s]) - first
+        if n <= 0:
+            continue
+        sl = slice(first, first + n)
+        args = [np.ascontiguousarray(a[s, sl]) for a in (pos["x"], pos["y"], pos["z"], pos["rmag"],
+                                                         old["rold_x"], old["rold_y"], old["rold_z"])]
+        pv = [np.ascontiguousarray(np.asarray(state[k], dtype=np.float64)[s, sl]) for k in MHD_NAMES]
+        pa = (C.POINTER(C.c_double) * 7)(*[dptr(a) for a in pv])
+        lib().or_move_nodes_mas(n, tdel, rscale, *[dptr(a) for a in args], pa, f0, m0, d0, f1, m1, d1,
+                                s_cor, phi_offset, mas_radial_max, mas_vmin)
+        for k, a in zip(("x", "y", "z", "rmag"), args[:4]):
+            pos[k][s, sl] = a
+        for k, a in zip(("rold_x", "rold_y", "rold_z"), args[4:7]):
+            old[k][s, sl] = a
     nodes.update(pos)
     return old
 

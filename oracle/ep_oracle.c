@@ -944,6 +944,66 @@ static double mas_field(const float *V, const float *const meshes[21],
   return mas_interpolate(V, r0, r1, t0, t1, p0, p1, rd, td, pd, nr, nt);
 }
 
+/* masGetNode (src/masInterp.c:31-68) at a position (x, y, z) in rScale
+   units: masTriLinear inside mas_radial_max, else masWind for the node
+   whose state is given (src/masInterp.c:1007-1065, no fake co-rotation),
+   which scales that node's previous values by |rOld| / node_rmag whatever
+   the position asked.  prev7 = {br, bt, bp, vr, vt, vp, density} of the
+   node; out7 the same order. */
+static void mas_get_node(double rScale, double x, double y, double z,
+                         const float *const fields0[7], const float *const meshes0[21],
+                         const int *dims0, const float *const fields1[7],
+                         const float *const meshes1[21], const int *dims1, double s,
+                         double phi_offset, double mas_radial_max, double mas_vmin, double rox,
+                         double roy, double roz, double node_rmag, const double *prev7,
+                         double *out7) {
+  /* cartToSphPosAu (src/geometry.c:131-160, :35-45) */
+  double ax = x * rScale, ay = y * rScale, az = z * rScale;
+  double pr = sqrt(ax * ax + ay * ay + az * az);
+  double pth = acos(az / pr);
+  double pph = atan2(ay, ax);
+  while (pph < 0.0) pph += (2.0 * OR_PI);
+  if (pph < 0.0)
+    while (pph < 0.0) pph += 2 * OR_PI;
+  if (pph > 2 * OR_PI)
+    while (pph > 2 * OR_PI) pph -= 2 * OR_PI;
+  if (pr <= (mas_radial_max * OR_RSAU)) {
+    /* masTriLinear (src/masInterp.c:539-741) */
+    double rr = pr / OR_RSAU, th = pth, ph = pph - phi_offset;
+    double v;
+    if (ph < 0.0)
+      while (ph < 0.0) ph += (2.0 * OR_PI);
+    if (ph > (2.0 * OR_PI))
+      while (ph > (2.0 * OR_PI)) ph -= (2.0 * OR_PI);
+    out7[2] = ((1.0 - s) * mas_field(fields0[0], meshes0, dims0, 0, rr, th, ph) +
+               s * mas_field(fields1[0], meshes1, dims1, 0, rr, th, ph)) * OR_MAS_B_CONVERT;
+    out7[1] = ((1.0 - s) * mas_field(fields0[1], meshes0, dims0, 1, rr, th, ph) +
+               s * mas_field(fields1[1], meshes1, dims1, 1, rr, th, ph)) * OR_MAS_B_CONVERT;
+    out7[0] = ((1.0 - s) * mas_field(fields0[2], meshes0, dims0, 2, rr, th, ph) +
+               s * mas_field(fields1[2], meshes1, dims1, 2, rr, th, ph)) * OR_MAS_B_CONVERT;
+    out7[5] = ((1.0 - s) * mas_field(fields0[3], meshes0, dims0, 3, rr, th, ph) +
+               s * mas_field(fields1[3], meshes1, dims1, 3, rr, th, ph)) * OR_MAS_V_CONVERT;
+    out7[4] = ((1.0 - s) * mas_field(fields0[4], meshes0, dims0, 4, rr, th, ph) +
+               s * mas_field(fields1[4], meshes1, dims1, 4, rr, th, ph)) * OR_MAS_V_CONVERT;
+    v = ((1.0 - s) * mas_field(fields0[5], meshes0, dims0, 5, rr, th, ph) +
+         s * mas_field(fields1[5], meshes1, dims1, 5, rr, th, ph)) * OR_MAS_V_CONVERT;
+    if (v < (mas_vmin / OR_C)) v = (mas_vmin / OR_C);
+    out7[3] = v;
+    out7[6] = ((1.0 - s) * mas_field(fields0[6], meshes0, dims0, 6, rr, th, ph) +
+               s * mas_field(fields1[6], meshes1, dims1, 6, rr, th, ph)) * OR_MAS_RHO_CONVERT;
+  } else {
+    double rOldmag = sqrt(rox * rox + roy * roy + roz * roz);
+    double oneOverR = rOldmag / node_rmag;
+    out7[0] = prev7[0] * oneOverR * oneOverR;
+    out7[1] = prev7[1] * oneOverR;
+    out7[2] = prev7[2] * oneOverR;
+    out7[3] = prev7[3];
+    out7[4] = prev7[4] * oneOverR;
+    out7[5] = prev7[5] * oneOverR;
+    out7[6] = prev7[6] * oneOverR * oneOverR;
+  }
+}
+
 void or_mas_interp(int n, double rScale, const double *x, const double *y,
                    const double *z, const double *rmag,
                    const float *const fields0[7], const float *const meshes0[21],
@@ -959,53 +1019,19 @@ void or_mas_interp(int n, double rScale, const double *x, const double *y,
                    double *vr, double *vt, double *vp, double *dens) {
   int i;
   for (i = 0; i < n; i++) {
-    /* cartToSphPosAu (src/geometry.c:131-160, :35-45) */
-    double ax = x[i] * rScale, ay = y[i] * rScale, az = z[i] * rScale;
-    double pr = sqrt(ax * ax + ay * ay + az * az);
-    double pth = acos(az / pr);
-    double pph = atan2(ay, ax);
-    while (pph < 0.0) pph += (2.0 * OR_PI);
-    if (pph < 0.0)
-      while (pph < 0.0) pph += 2 * OR_PI;
-    if (pph > 2 * OR_PI)
-      while (pph > 2 * OR_PI) pph -= 2 * OR_PI;
-    if (pr <= (mas_radial_max * OR_RSAU)) {
-      /* masTriLinear (src/masInterp.c:539-741) */
-      double rr = pr / OR_RSAU, th = pth, ph = pph - phi_offset;
-      double v;
-      if (ph < 0.0)
-        while (ph < 0.0) ph += (2.0 * OR_PI);
-      if (ph > (2.0 * OR_PI))
-        while (ph > (2.0 * OR_PI)) ph -= (2.0 * OR_PI);
-      bp[i] = ((1.0 - s) * mas_field(fields0[0], meshes0, dims0, 0, rr, th, ph) +
-               s * mas_field(fields1[0], meshes1, dims1, 0, rr, th, ph)) * OR_MAS_B_CONVERT;
-      bt[i] = ((1.0 - s) * mas_field(fields0[1], meshes0, dims0, 1, rr, th, ph) +
-               s * mas_field(fields1[1], meshes1, dims1, 1, rr, th, ph)) * OR_MAS_B_CONVERT;
-      br[i] = ((1.0 - s) * mas_field(fields0[2], meshes0, dims0, 2, rr, th, ph) +
-               s * mas_field(fields1[2], meshes1, dims1, 2, rr, th, ph)) * OR_MAS_B_CONVERT;
-      vp[i] = ((1.0 - s) * mas_field(fields0[3], meshes0, dims0, 3, rr, th, ph) +
-               s * mas_field(fields1[3], meshes1, dims1, 3, rr, th, ph)) * OR_MAS_V_CONVERT;
-      vt[i] = ((1.0 - s) * mas_field(fields0[4], meshes0, dims0, 4, rr, th, ph) +
-               s * mas_field(fields1[4], meshes1, dims1, 4, rr, th, ph)) * OR_MAS_V_CONVERT;
-      v = ((1.0 - s) * mas_field(fields0[5], meshes0, dims0, 5, rr, th, ph) +
-           s * mas_field(fields1[5], meshes1, dims1, 5, rr, th, ph)) * OR_MAS_V_CONVERT;
-      if (v < (mas_vmin / OR_C)) v = (mas_vmin / OR_C);
-      vr[i] = v;
-      dens[i] = ((1.0 - s) * mas_field(fields0[6], meshes0, dims0, 6, rr, th, ph) +
-                 s * mas_field(fields1[6], meshes1, dims1, 6, rr, th, ph)) * OR_MAS_RHO_CONVERT;
-    } else {
-      /* masWind (src/masInterp.c:1007-1065), no fake co-rotation */
-      double ox = rold_x[i], oy = rold_y[i], oz = rold_z[i];
-      double rOldmag = sqrt(ox * ox + oy * oy + oz * oz);
-      double oneOverR = rOldmag / rmag[i];
-      br[i] = prev_br[i] * oneOverR * oneOverR;
-      bt[i] = prev_bt[i] * oneOverR;
-      bp[i] = prev_bp[i] * oneOverR;
-      vr[i] = prev_vr[i];
-      vt[i] = prev_vt[i] * oneOverR;
-      vp[i] = prev_vp[i] * oneOverR;
-      dens[i] = prev_d[i] * oneOverR * oneOverR;
-    }
+    const double prev7[7] = {prev_br[i], prev_bt[i], prev_bp[i], prev_vr[i],
+                             prev_vt[i], prev_vp[i], prev_d[i]};
+    double o[7];
+    mas_get_node(rScale, x[i], y[i], z[i], fields0, meshes0, dims0, fields1, meshes1, dims1, s,
+                 phi_offset, mas_radial_max, mas_vmin, rold_x[i], rold_y[i], rold_z[i], rmag[i],
+                 prev7, o);
+    br[i] = o[0];
+    bt[i] = o[1];
+    bp[i] = o[2];
+    vr[i] = o[3];
+    vt[i] = o[4];
+    vp[i] = o[5];
+    dens[i] = o[6];
   }
 }
 
@@ -1726,6 +1752,48 @@ static void or_sph_to_cart(const double *v, double x, double y, double z, double
     out[1] = y / rmag * v[0] + y * z / (rmag * sqrt(rmag * rmag - z * z)) * v[1] + x / sqrt(rmag * rmag - z * z) * v[2];
   }
   out[2] = z / rmag * v[0] - sqrt(rmag * rmag - z * z) / rmag * v[1];
+}
+
+/* masMoveNodes (src/masInterp.c:1344-1393) for n nodes: the classical RK4
+   of rungeKuttaFlow (:1305-1338) with vMas (:1279-1296), the MAS velocity
+   of masGetNode at each stage position made Cartesian there
+   (sphToCartVector).  Outside the MAS domain masGetNode takes masWind of
+   the node as it stands before the move (its position, its previous old
+   position and MHD state), the same at every stage.  The node's position
+   before the move becomes its old position.  prev = the node's MHD state
+   {br, bt, bp, vr, vt, vp, density}, each [n]. */
+void or_move_nodes_mas(int n, double tDel, double rScale, double *x, double *y, double *z,
+                       double *rmag, double *rox, double *roy, double *roz,
+                       const double *const prev[7], const float *const fields0[7],
+                       const float *const meshes0[21], const int *dims0,
+                       const float *const fields1[7], const float *const meshes1[21],
+                       const int *dims1, double s, double phi_offset, double mas_radial_max,
+                       double mas_vmin) {
+  const double sixth = 1.0 / 6.0, two = 2.0, half = 0.5;
+  const double h = tDel / rScale;
+  int i, q, st;
+  for (i = 0; i < n; i++) {
+    const double r0[3] = {x[i], y[i], z[i]};
+    const double ox = rox[i], oy = roy[i], oz = roz[i], nrm = rmag[i];
+    double p7[7], kk[4][3], pos[3], o7[7], vc[3];
+    for (q = 0; q < 7; q++) p7[q] = prev[q][i];
+    for (st = 0; st < 4; st++) {
+      /* stage positions r0, r0 + k1/2, r0 + k2/2, r0 + k3 */
+      for (q = 0; q < 3; q++)
+        pos[q] = st == 0 ? r0[q] : (st < 3 ? r0[q] + kk[st - 1][q] * half : r0[q] + kk[2][q]);
+      mas_get_node(rScale, pos[0], pos[1], pos[2], fields0, meshes0, dims0, fields1, meshes1, dims1, s,
+                   phi_offset, mas_radial_max, mas_vmin, ox, oy, oz, nrm, p7, o7);
+      or_sph_to_cart(o7 + 3, pos[0], pos[1], pos[2], vc);
+      for (q = 0; q < 3; q++) kk[st][q] = vc[q] * h;
+    }
+    rox[i] = r0[0];
+    roy[i] = r0[1];
+    roz[i] = r0[2];
+    x[i] = r0[0] + sixth * (kk[0][0] + two * kk[1][0] + two * kk[2][0] + kk[3][0]);
+    y[i] = r0[1] + sixth * (kk[0][1] + two * kk[1][1] + two * kk[2][1] + kk[3][1]);
+    z[i] = r0[2] + sixth * (kk[0][2] + two * kk[1][2] + two * kk[2][2] + kk[3][2]);
+    rmag[i] = sqrt((x[i] * x[i]) + (y[i] * y[i]) + (z[i] * z[i]));
+  }
 }
 
 /* DriftShellData (src/energeticParticles.c:560-801) with driftVelocity

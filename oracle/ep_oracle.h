@@ -166,6 +166,16 @@ void or_move_nodes(int n, double tDel, double rScale, double *x, double *y, doub
                    double *rox, double *roy, double *roz, const double *vr, const double *vt,
                    const double *vp);
 
+/* masMoveNodes for n nodes (RK4 through the MAS field, see the source);
+ * prev[7] = the nodes' MHD state {br, bt, bp, vr, vt, vp, density}. */
+void or_move_nodes_mas(int n, double tDel, double rScale, double *x, double *y, double *z,
+                       double *rmag, double *rox, double *roy, double *roz,
+                       const double *const prev[7], const float *const fields0[7],
+                       const float *const meshes0[21], const int *dims0,
+                       const float *const fields1[7], const float *const meshes1[21],
+                       const int *dims1, double s, double phi_offset, double mas_radial_max,
+                       double mas_vmin);
+
 /* ShellData + DiffuseShellData (src/energeticParticles.c:247-555) for one
  * EP substep dt: f [S][N][E][M] in/out, neighbour streams nbr[S][4],
  * positions [S][N] (rScale units), B (r, theta, phi) [S][N]; node 0 is

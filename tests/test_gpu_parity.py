@@ -804,6 +804,96 @@ def test_device_node_pipeline_parity():
     assert max_rel(fg, fo, na) <= RTOL
 
 
+def test_device_mas_node_pipeline_parity():
+    """Global steps of a MAS-coupled run on the device -- spawn/ripple,
+    masMoveNodes (RK4 through the two MAS levels inside mas_radial_max,
+    masWind of the node's own state outside), updateMhd from MAS, EP
+    update -- against the oracle doing the same steps.  The nodes span
+    1-54 Rs, so both branches of masGetNode act at every RK4 stage."""
+    from eprem_amd import constants as K, mas_synth
+    rs = 0.00465047
+    p = params(rScale=rs, numEpSteps=2)
+    E, M = 8, 6
+    nodes = workloads.parker_nodes(1, 1, 70, tdel_days=1.0 / 64, rscale=rs, seed=33, jitter=0.1)
+    S, N = nodes["x"].shape
+    n_active = np.full(S, N, np.int32)
+    n_active[1::2] = N - 4
+    prev = mas_synth.prev_from_nodes(nodes, rs)
+    r, t, ph = mas_synth.meshes(nr=120, nt=90, nphi=180)
+    lv0 = mas_synth.level(r, t, ph, cme_r=8.0, seed=5)
+    lv1 = mas_synth.level(r, t, ph, cme_r=12.0, seed=5)
+    mas = dict(s_cor=0.3, phi_offset=0.2, mas_radial_max=30.0, mas_vmin=200e5)
+    inside = nodes["rmag"] * rs <= mas["mas_radial_max"] * K.RSAU
+    assert 0.2 < inside.mean() < 0.8
+    tdel = p.tDel
+    g = oracle.build_grids(p, E, M)
+    f = workloads.seeded_f(p, g, nodes, noise=0.2, seed=9)
+    spawn_r = nodes["rmag"][:, 0].copy()
+    zen0 = np.arccos(nodes["z"][:, 0] / spawn_r)
+    azi0 = np.arctan2(nodes["y"][:, 0], nodes["x"][:, 0])
+    steps = 3
+
+    def spawn_pos(j):
+        azi = azi0 + 0.01 * (j + 1)
+        return (spawn_r * np.sin(zen0) * np.cos(azi), spawn_r * np.sin(zen0) * np.sin(azi),
+                spawn_r * np.cos(zen0), spawn_r.copy())
+
+    dims = workloads.dims_of(nodes, E, M)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.mas_set_level(0, *lv0)
+        sol.mas_set_level(1, *lv1)
+        sol.set_positions(nodes, n_active)
+        sol.update_mhd(prev, mode=1, **mas)
+        sol.upload_f(f)
+        for j in range(steps):
+            sol.spawn_shell(*spawn_pos(j))
+            sol.move_nodes_mas(tdel, **mas)
+            sol.update_mhd(None, mode=1, **mas)
+            rc, _ = sol.step(raise_on_error=False)
+            assert rc == 0
+        pos_g = sol.positions()
+        fg = sol.download_f()
+    finally:
+        sol.close()
+
+    o_nodes = {k: nodes[k].copy() for k in ("x", "y", "z", "rmag")}
+    lv = dict(level0=lv0, level1=lv1, **mas)
+    state = oracle.update_mhd(dict(o_nodes), prev, 1, rs, **lv)
+    rold = {k: prev[k].copy() for k in ("rold_x", "rold_y", "rold_z")}
+    na = n_active.copy()
+    fo = f.copy()
+    for j in range(steps):
+        sx, sy, sz, sr = spawn_pos(j)
+        for k, h in zip(("x", "y", "z", "rmag"), (sx, sy, sz, sr)):
+            o_nodes[k] = _ripple(o_nodes[k], na, h)
+        for k, h in zip(("rold_x", "rold_y", "rold_z"), (sx, sy, sz)):
+            rold[k] = _ripple(rold[k], na, h)
+        for k in oracle.MHD_NAMES:
+            state[k] = _ripple(state[k], na, state[k][:, 0])
+        seed0 = np.array([[oracle.seed(p, g["egrid"][e], sr[s]) for e in range(E)] for s in range(S)])
+        fo = _ripple(fo, na, np.repeat(seed0[:, :, None], M, axis=2))
+        na = np.minimum(na + 1, N)
+        st = {k: state[k] for k in oracle.MHD_NAMES}
+        st.update(rold)
+        rold = oracle.move_nodes_mas(o_nodes, st, tdel, rs, lv0, lv1, mas["s_cor"], mas["phi_offset"],
+                                     mas["mas_radial_max"], mas["mas_vmin"], n_active=na, first=1)
+        pv = {k: state[k] for k in oracle.MHD_NAMES}
+        pv.update(rold)
+        state = oracle.update_mhd(dict(o_nodes), pv, 1, rs, **lv)
+        full = dict(o_nodes)
+        full.update({k: state[k] for k in ("bmag", "dlnb", "dlnn", "dupar")})
+        rc, _, fo, _, _, _ = oracle.update_ep(dims, p, full, fo, n_active=na)
+        assert rc == 0
+    np.testing.assert_array_equal(pos_g["n_active"], na)
+    act = np.arange(N)[None, :] < na[:, None]
+    # the nodes moved, and differently from a straight radial step
+    assert np.max(np.abs(o_nodes["rmag"][act] - _ripple(nodes["rmag"], n_active, spawn_r)[act])) > 0
+    for k in ("x", "y", "z", "rmag"):
+        np.testing.assert_allclose(pos_g[k][act], o_nodes[k][act], rtol=1e-13, atol=1e-11, err_msg=k)
+    assert max_rel(fg, fo, na) <= RTOL
+
+
 def test_mixed_precision_error_bounds():
     """FP32 streaming/focusing with FP64 adiabatic change against the FP64
     oracle on a 35-decade power-law distribution: relative error of f and
