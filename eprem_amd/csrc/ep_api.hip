@@ -233,6 +233,7 @@ Dev make_dev() {
   D.fA = g.fA;
   D.fB = g.fB;
   D.loc = g.loc;
+  D.fhead = g.fhead;
   D.x = g.nx;
   D.y = g.ny;
   D.z = g.nz;
@@ -437,7 +438,8 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
   if ((rc = check_algs(*params, d, 0))) return rc;
   g.d = d;
   g.p = *params;
-  g.Npad = (d.num_nodes + 63) / 64 * 64;
+  // every f row keeps at least 64 free slots for the spawn head (fidx)
+  g.Npad = (d.num_nodes + 64 + 63) / 64 * 64;
   HIPCHK(hipStreamCreate(&g.st), "hipStreamCreate");
   const size_t S = d.num_streams, E = d.num_energies, M = d.num_mu, NP = g.Npad;
   const size_t SN = S * NP, SE = S * E;
@@ -449,7 +451,7 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
   AL(g.egrid_d, E) AL(g.vgrid_d, E) AL(g.vgrid_i_d, E) AL(g.rig_lamo_d, E) AL(g.plt_d, E)
   AL(g.ext_d, E) AL(g.mugrid_d, 2 * M) AL(g.fm_tab_d, kFmTable)
   // one extra slab each: room for the FP64 scratch of the mixed-precision form
-  AL(g.fA, SE * M * NP + M * NP) AL(g.fB, SE * M * NP + M * NP) AL(g.loc, SE)
+  AL(g.fA, SE * M * NP + M * NP) AL(g.fB, SE * M * NP + M * NP) AL(g.loc, SE) AL(g.fhead, 1)
   AL(g.nx, SN) AL(g.ny, SN) AL(g.nz, SN) AL(g.nrmag, SN) AL(g.nbmag, SN) AL(g.ndlnb, SN)
   AL(g.ndlnn, SN) AL(g.ndupar, SN) AL(g.n_active, S)
   AL(g.ds, SN) AL(g.bplus, SN) AL(g.bminus, SN) AL(g.mfp_r, SN) AL(g.radial, SN)
@@ -464,6 +466,8 @@ int ep_init(const ep_dims *dims, const ep_params *params) {
   if ((rc = upload_grids())) { free_all(); return rc; }
   // every f starts at DBL_MIN (initEnergeticParticlesGrids), in buffer A
   HIPCHK(hipMemset(g.loc, 0, SE), "hipMemset");
+  g.fhead_h = (int)(NP - d.num_nodes);  // the rows start at their far end
+  HIPCHK(hipMemcpy(g.fhead, &g.fhead_h, 4, hipMemcpyHostToDevice), "f head");
   HIPCHK(launch_fill(g.fA, SE * M * NP, kDblMin, g.st), "fill");
   HIPCHK(launch_fill(g.fB, SE * M * NP, kDblMin, g.st), "fill");
   std::vector<int32_t> na(S, d.num_nodes);

@@ -95,6 +95,7 @@ struct Dev {
   // f
   double *fA, *fB;
   uint8_t *loc;  // [S*E]
+  int32_t *fhead;  // [1] the slot of node 0 in every f row
   // nodes [S*Npad]
   const double *x, *y, *z, *rmag, *bmag, *dlnb, *dlnn, *dupar;
   const int32_t *n_active;  // [S]
@@ -163,8 +164,13 @@ __device__ inline double seed_value(const Dev &D, double radial, int e) {
   return normf;
 }
 
+// f rows: each (stream, energy, mu) row has Npad >= N + 64 slots and holds
+// the stream's nodes 0..N-1 from slot *D.fhead on.  A shell spawn moves the
+// head back one slot and writes the new node 0 there instead of moving
+// every node outward; only when the head reaches slot 0 are the rows
+// copied back to the far end (once every Npad - N spawns, ep_nodes.hip).
 __device__ inline size_t fidx(const Dev &D, int s, int e, int m, int node) {
-  return (((size_t)s * D.E + e) * D.M + m) * (size_t)D.Npad + node;
+  return (((size_t)s * D.E + e) * D.M + m) * (size_t)D.Npad + D.fhead[0] + node;
 }
 
 __device__ inline double *fbuf(const Dev &D, int which) { return which ? D.fB : D.fA; }
@@ -175,6 +181,14 @@ template <typename T>
 __device__ inline T *fbufT(const Dev &D, int which) {
   return reinterpret_cast<T *>(which ? D.fB : D.fA);
 }
+
+// The f slab of task t in buffer `which`, from the head: node k of mu row m
+// at [m * Npad + k].
+template <typename T>
+__device__ inline T *fslab(const Dev &D, int which, int task) {
+  return fbufT<T>(D, which) + (size_t)task * D.M * D.Npad + D.fhead[0];
+}
+__device__ inline double *fslab(const Dev &D, int which, int task) { return fslab<double>(D, which, task); }
 
 // FP64 scratch slab of one (stream, energy) when f is stored as float: the
 // upper halves of the two f allocations, which float storage leaves unused

@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
   if (k >= D.n_active[s]) return;
   const int t = s * D.E + e;
   const T sd = (T)seed_value(D, D.radial[(size_t)s * D.Npad + k], e);
-  T *f = fbufT<T>(D, D.loc[t]) + (size_t)t * D.M * D.Npad + k;
+  T *f = fslab<T>(D, D.loc[t], t) + k;
   for (int m = 0; m < D.M; m++) {
     const T v = f[(size_t)m * D.Npad];
     if (v < sd) f[(size_t)m * D.Npad] = sd;
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
   if (!D.thin[s]) return;
   const int n = D.n_active[s];
   const size_t on = (size_t)s * D.Npad;
-  T *f = fbufT<T>(D, loc_new) + (size_t)task * D.M * D.Npad;
+  T *f = fslab<T>(D, loc_new, task);
   for (int idx = threadIdx.x; idx < n * D.M; idx += blockDim.x) {
     const int m = idx / n, shell = idx % n;
     const int r = D.ref[on + shell];
@@ -462,10 +462,8 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
   constexpr uint32_t B = sizeof(T);
   // spare = 1 (tile-size search only): write the spare buffer, report nothing
   const int cur = D.loc[task];
-  const rsrc_t slab = slab_rsrc((const double *)(fbufT<T>(D, cur) + (size_t)task * M * D.Npad),
-                                (uint32_t)(M * D.Npad * B));
-  const rsrc_t out = slab_rsrc((const double *)(fbufT<T>(D, cur ^ spare) + (size_t)task * M * D.Npad),
-                               (uint32_t)(M * D.Npad * B));
+  const rsrc_t slab = slab_rsrc((const double *)fslab<T>(D, cur, task), (uint32_t)(M * D.Npad * B));
+  const rsrc_t out = slab_rsrc((const double *)fslab<T>(D, cur ^ spare, task), (uint32_t)(M * D.Npad * B));
   const uint32_t voff = (uint32_t)node * B, row = (uint32_t)D.Npad * B;
   const double v = D.vgrid[e];
   const double a = -v * dlnBds - (two / v) * DuParDt;
@@ -513,7 +511,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const double dtsub = D.dtsub_f[i];
     const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], b = D.f_b[i];
     const int task = s * D.E + e;
-    T *fp = fbufT<T>(D, D.loc[task]) + (size_t)task * M * D.Npad + node;
+    T *fp = fslab<T>(D, D.loc[task], task) + node;
     const double v = D.vgrid[e];
     const double a = -v * dlnBds - (two / v) * DuParDt;
     T f[M];
@@ -577,7 +575,7 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   auto slab_of = [&](int e) {
     const int task = s * E + e;
     const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]);
-    return slab_rsrc((const double *)(fbufT<T>(D, lc) + (size_t)task * slab), slab_bytes);
+    return slab_rsrc((const double *)fslab<T>(D, lc, task), slab_bytes);
   };
   // With float storage, ln f between subcycles is kept in FP64 in the part
   // of the f allocations that float storage leaves free.
@@ -721,7 +719,7 @@ __global__ __launch_bounds__(256) void k_init_seed(Dev D) {
   const double rad = pow(D.rmag[i] / D.normRadius, -D.beta);
   const T sd = (T)seed_value(D, rad, e);
   const int t = s * D.E + e;
-  T *f = fbufT<T>(D, D.loc[t]) + (size_t)t * D.M * D.Npad + k;
+  T *f = fslab<T>(D, D.loc[t], t) + k;
   for (int m = 0; m < D.M; m++) f[(size_t)m * D.Npad] = sd;
 }
 

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
   auto slab_of = [&](int e, int flip) {
     const int task = s * E + e;
     const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]) ^ flip;
-    return slab_rsrc((const double *)(fbufT<T>(D, lc) + (size_t)task * M * D.Npad), slab_bytes);
+    return slab_rsrc((const double *)fslab<T>(D, lc, task), slab_bytes);
   };
   // slot j of this lane is mu cell m0 + j; the upper half's last slot is
   // padding when M is odd
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
   auto slab_of = [&](int e, int flip) {
     const int task = s * E + e;
     const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]) ^ flip;
-    return slab_rsrc((const double *)(fbufT<T>(D, lc) + (size_t)task * M * D.Npad), slab_bytes);
+    return slab_rsrc((const double *)fslab<T>(D, lc, task), slab_bytes);
   };
   // slot j is mu cell m0 + j; slots past cell M-1 in the last group are
   // padding

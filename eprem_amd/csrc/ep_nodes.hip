@@ -6,9 +6,15 @@
 //                 node k takes node k-1's record, node 0 the new shell.
 //                 One block per stream shifts its arrays in place, from the
 //                 outer end inward, a block-sized chunk at a time.
-// k_ripple_f      the same shift of f, slab by slab into the spare buffer
-//                 (k_flip_loc then makes it current); node 0 gets the
-//                 spawned shell's distribution.
+// k_spawn_f       the same shift of f without moving it: every f row holds
+//                 the nodes from a head slot on (fidx), so the new node 0
+//                 goes into the slot before the head and the head moves
+//                 back by one; the outermost node of a full stream simply
+//                 falls off the end.
+// k_ripple_f      when the head has reached slot 0: the shift as a copy of
+//                 every slab into the spare buffer, to the far end of its
+//                 rows (k_flip_loc then makes it current) -- once every
+//                 Npad - N spawns.
 // k_move_nodes    moveNodes with delrFlow for the analytic wind
 //                 (src/simCore.c:736-785, src/flow.c:169-178): r += tDel V/rScale
 //                 with V made Cartesian at the node (src/flow.c:355-403).
@@ -66,27 +72,49 @@ __global__ __launch_bounds__(256) void k_ripple_nodes(Dev D, RippleArgs A) {
   if (threadIdx.x == 0) A.n_active[s] = n_new;
 }
 
-// dst[m][k] = src[m][k-1]; node 0 takes spawn_f[s][e][m], or the seed
-// spectrum at the new node's radius when spawn_f is null.
-// Source and copy use the same element type, so a slab occupies the same
-// offsets in both buffers and slabs never overlap.
+// The distribution of the new node 0 of stream s, energy e, mu m:
+// spawn_f[s][e][m], or the seed spectrum at the new node's radius when
+// spawn_f is null.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ripple_f(Dev D, const double *spawn_f) {
+__device__ inline T spawn_value(const Dev &D, const double *spawn_f, int s, int e, int m) {
+  if (spawn_f) return (T)spawn_f[((size_t)s * D.E + e) * D.M + m];
+  const double radial = pow(D.rmag[(size_t)s * D.Npad] / D.normRadius, -D.beta);
+  return (T)seed_value(D, radial, e);
+}
+
+// New head slot new_head = old head - 1 >= 0: node 0 of every row written
+// there, in the current buffer; the head itself is advanced by the first
+// lane (no lane here reads it).
+template <typename T>
+__global__ __launch_bounds__(256) void k_spawn_f(Dev D, const double *spawn_f, int new_head) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;  // (energy, mu) of stream blockIdx.y
+  const int s = blockIdx.y;
+  if (q == 0 && s == 0) D.fhead[0] = new_head;
+  if (q >= D.E * D.M) return;
+  const int e = q / D.M, m = q % D.M;
+  const int task = s * D.E + e;
+  T *row = fbufT<T>(D, D.loc[task]) + ((size_t)task * D.M + m) * D.Npad;
+  row[new_head] = spawn_value<T>(D, spawn_f, s, e, m);
+}
+
+// Head at slot 0: dst[m][new_head + k] = src[m][k - 1] (old head 0) into
+// the spare buffer, node 0 from the spawn; new_head = Npad - N - 1, so a
+// full stream's nodes end at slot Npad - 2.  Source and copy use the same
+// element type, so a slab occupies the same offsets in both buffers.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ripple_f(Dev D, const double *spawn_f, int new_head) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
+  if (k == 0 && e == 0 && s == 0) D.fhead[0] = new_head;
   if (k >= D.n_active[s]) return;  // n_active already advanced
   const int task = s * D.E + e;
   const int cur = D.loc[task];
   const T *src = fbufT<T>(D, cur) + (size_t)task * D.M * D.Npad;
-  T *dst = fbufT<T>(D, cur ^ 1) + (size_t)task * D.M * D.Npad;
+  T *dst = fbufT<T>(D, cur ^ 1) + (size_t)task * D.M * D.Npad + new_head;
   if (k > 0) {
     for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad + k] = src[(size_t)m * D.Npad + k - 1];
-  } else if (spawn_f) {
-    for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad] = (T)spawn_f[(size_t)task * D.M + m];
   } else {
-    const double radial = pow(D.rmag[(size_t)s * D.Npad] / D.normRadius, -D.beta);
-    const T sd = (T)seed_value(D, radial, e);
-    for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad] = sd;
+    for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad] = spawn_value<T>(D, spawn_f, s, e, m);
   }
 }
 
@@ -177,12 +205,26 @@ int ep_spawn_shell(const ep_spawn *sp) {
   A.rold = 8;
   Dev D = make_dev();
   hipLaunchKernelGGL(k_ripple_nodes, dim3(S), dim3(256), 0, g.st, D, A);
-  const dim3 grid((g.d.num_nodes + 255) / 256, E, S);
-  if (D.fp32)
-    hipLaunchKernelGGL(k_ripple_f<float>, grid, dim3(256), 0, g.st, D, spawn_f);
-  else
-    hipLaunchKernelGGL(k_ripple_f<double>, grid, dim3(256), 0, g.st, D, spawn_f);
-  hipLaunchKernelGGL(k_flip_loc, dim3((S * E + 255) / 256), dim3(256), 0, g.st, D);
+  if (g.fhead_h > 0) {
+    // the new node 0 goes into the slot before the head
+    const int nh = g.fhead_h - 1;
+    const dim3 grid((E * M + 255) / 256, S);
+    if (D.fp32)
+      hipLaunchKernelGGL(k_spawn_f<float>, grid, dim3(256), 0, g.st, D, spawn_f, nh);
+    else
+      hipLaunchKernelGGL(k_spawn_f<double>, grid, dim3(256), 0, g.st, D, spawn_f, nh);
+    g.fhead_h = nh;
+  } else {
+    // no slot left before the rows: copy them to the far end
+    const int nh = g.Npad - g.d.num_nodes - 1;
+    const dim3 grid((g.d.num_nodes + 255) / 256, E, S);
+    if (D.fp32)
+      hipLaunchKernelGGL(k_ripple_f<float>, grid, dim3(256), 0, g.st, D, spawn_f, nh);
+    else
+      hipLaunchKernelGGL(k_ripple_f<double>, grid, dim3(256), 0, g.st, D, spawn_f, nh);
+    hipLaunchKernelGGL(k_flip_loc, dim3((S * E + 255) / 256), dim3(256), 0, g.st, D);
+    g.fhead_h = nh;
+  }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(g.st);
   if (e != hipSuccess) return hip_fail(e, "spawn kernels");

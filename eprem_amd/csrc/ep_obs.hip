@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_obs_project(Dev D, ObsDev O) {
   for (int c = threadIdx.x; c < EM; c += blockDim.x) {
     const int e = c / D.M, m = c % D.M;
     const int task = s * D.E + e;
-    const T *fb = fbufT<T>(D, D.loc[task]) + (size_t)task * D.M * D.Npad + (size_t)m * D.Npad;
+    const T *fb = fslab<T>(D, D.loc[task], task) + (size_t)m * D.Npad;
     const double fi = (double)fb[in], fo = (double)fb[out];
     double v;
     if ((sf >= 0.0) && (sf <= 1.0))
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_stream_flux(Dev D, double *flux) {
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.N) return;
   const int task = s * D.E + e;
-  const T *fb = fbufT<T>(D, D.loc[task]) + (size_t)task * D.M * D.Npad + k;
+  const T *fb = fslab<T>(D, D.loc[task], task) + k;
   double iso = 0.0;
   for (int m = 0; m < D.M; m++) iso += (double)fb[(size_t)m * D.Npad];
   iso /= D.M;

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(64) void k_focus_rk(Dev D, int substep) {
   const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], b = D.f_b[i];
   const double half = 0.5, one = 1.0, two = 2.0;
   const int task = s * D.E + e;
-  double *fp = fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad + node;
+  double *fp = fslab(D, D.loc[task], task) + node;
   const double v = D.vgrid[e];
   const double a = -v * dlnBds - (two / v) * DuParDt;
   double f[M], s1[M], va[M + 1];
@@ -276,14 +276,14 @@ __global__ __launch_bounds__(256) void k_change_rk(Dev D, int substep) {
   const double a = -mu * DuParDt;
   const double bb = mu * mu * (DlnNDt - DlnBDt) + half * (one - mu * mu) * DlnBDt;
   const double *vgi = D.vgrid_i;
-  const size_t slab = (size_t)D.M * D.Npad, cell = (size_t)m * D.Npad + node;
+  const size_t cell = (size_t)m * D.Npad + node;
   auto cur = [&](int e) {
     const int t = s * E + e;
-    return fbuf(D, D.loc[t]) + (size_t)t * slab + cell;
+    return fslab(D, D.loc[t], t) + cell;
   };
   auto spare = [&](int e) {
     const int t = s * E + e;
-    return fbuf(D, D.loc[t] ^ 1) + (size_t)t * slab + cell;
+    return fslab(D, D.loc[t] ^ 1, t) + cell;
   };
   auto vel = [&](int e) { return damp * (a * vgi[e] + bb); };
   auto va = [&](int j) {

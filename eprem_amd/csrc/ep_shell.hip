@@ -132,8 +132,8 @@ __global__ __launch_bounds__(256) void k_shell_diffuse(Dev D, ShellDev H, int su
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.N) return;
   const int task = s * D.E + e;
-  const T *fs = fbufT<T>(D, D.loc[task]) + (size_t)task * D.M * D.Npad;
-  T *out = fbufT<T>(D, D.loc[task] ^ 1) + (size_t)task * D.M * D.Npad;
+  const T *fs = fslab<T>(D, D.loc[task], task);
+  T *out = fslab<T>(D, D.loc[task] ^ 1, task);
   if (k == 0) {  // not a compute shell: carried over
     for (int m = 0; m < D.M; m++) out[(size_t)m * D.Npad] = fs[(size_t)m * D.Npad];
     return;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_shell_diffuse(Dev D, ShellDev H, int su
     if (j < 0) break;
     const int d = H.dir[s * kMaxIn + q];
     const int tj = j * D.E + e;
-    srcf[q] = fbufT<T>(D, D.loc[tj]) + (size_t)tj * D.M * D.Npad + k;
+    srcf[q] = fslab<T>(D, D.loc[tj], tj) + k;
     if (d < 4) {
       w[q] = shell_del(D, H, j, k, e, d);
     } else {
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(256) void k_shell_drift(Dev D, ShellDev H, int subs
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.N) return;
   const int task = s * D.E + e;
-  const T *fs = fbufT<T>(D, D.loc[task]) + (size_t)task * D.M * D.Npad;
-  T *out = fbufT<T>(D, D.loc[task] ^ 1) + (size_t)task * D.M * D.Npad;
+  const T *fs = fslab<T>(D, D.loc[task], task);
+  T *out = fslab<T>(D, D.loc[task] ^ 1, task);
   if (k == 0) {
     for (int m = 0; m < D.M; m++) out[(size_t)m * D.Npad] = fs[(size_t)m * D.Npad];
     return;
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void k_shell_drift(Dev D, ShellDev H, int subs
     if (j < 0) break;
     const int d = H.dir[s * kMaxIn + q];
     const int tj = j * D.E + e;
-    const T *fp = fbufT<T>(D, D.loc[tj]) + (size_t)tj * D.M * D.Npad + k;
+    const T *fp = fslab<T>(D, D.loc[tj], tj) + k;
     if (d < 4) {
       const double del = drift_del(D, H, j, k, e, d);
       if (del > 0.0) {

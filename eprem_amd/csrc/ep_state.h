@@ -68,6 +68,8 @@ struct State {
   double *fm_tab_d = nullptr;  // exp/log tables of the fused precision modes
   double *fA = nullptr, *fB = nullptr;
   uint8_t *loc = nullptr;
+  int32_t *fhead = nullptr;  // [1] the slot of node 0 in every f row (fidx)
+  int fhead_h = 0;           // host copy
   double *nx = nullptr, *ny = nullptr, *nz = nullptr, *nrmag = nullptr, *nbmag = nullptr,
          *ndlnb = nullptr, *ndlnn = nullptr, *ndupar = nullptr;
   int32_t *n_active = nullptr;

@@ -181,8 +181,8 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
   const int cur = D.loc[task] ^ flip;
   constexpr uint32_t B = sizeof(T);
   const uint32_t slab_bytes = (uint32_t)(M * D.Npad * B), row = (uint32_t)D.Npad * B;
-  const rsrc_t src = slab_rsrc((const double *)(fbufT<T>(D, cur) + (size_t)task * M * D.Npad), slab_bytes);
-  const rsrc_t dst = slab_rsrc((const double *)(fbufT<T>(D, cur ^ 1) + (size_t)task * M * D.Npad), slab_bytes);
+  const rsrc_t src = slab_rsrc((const double *)fslab<T>(D, cur, task), slab_bytes);
+  const rsrc_t dst = slab_rsrc((const double *)fslab<T>(D, cur ^ 1, task), slab_bytes);
   const uint32_t voff = (uint32_t)node * B;
 
   T f[M];
@@ -350,8 +350,8 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
   const int cur = D.loc[task] ^ flip;
   constexpr uint32_t B = sizeof(T);
   const uint32_t slab_bytes = (uint32_t)(M * D.Npad * B), row = (uint32_t)D.Npad * B;
-  const rsrc_t src = slab_rsrc((const double *)(fbufT<T>(D, cur) + (size_t)task * M * D.Npad), slab_bytes);
-  const rsrc_t dst = slab_rsrc((const double *)(fbufT<T>(D, cur ^ 1) + (size_t)task * M * D.Npad), slab_bytes);
+  const rsrc_t src = slab_rsrc((const double *)fslab<T>(D, cur, task), slab_bytes);
+  const rsrc_t dst = slab_rsrc((const double *)fslab<T>(D, cur ^ 1, task), slab_bytes);
   const uint32_t vo0 = (uint32_t)node0 * B, vo1 = (uint32_t)node1 * B;
 
   T f0[M], f1[M];
@@ -559,8 +559,8 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
   const bool thin = D.thin[s] != 0;
   constexpr uint32_t B = sizeof(T);
   const uint32_t slab_bytes = (uint32_t)(M * D.Npad * B), row = (uint32_t)D.Npad * B;
-  const rsrc_t src = slab_rsrc((const double *)(fbufT<T>(D, src_buf) + (size_t)task * M * D.Npad), slab_bytes);
-  const rsrc_t dst = slab_rsrc((const double *)(fbufT<T>(D, dst_buf) + (size_t)task * M * D.Npad), slab_bytes);
+  const rsrc_t src = slab_rsrc((const double *)fslab<T>(D, src_buf, task), slab_bytes);
+  const rsrc_t dst = slab_rsrc((const double *)fslab<T>(D, dst_buf, task), slab_bytes);
   int jj_[R], kk[R], nd[R];
   bool vv[R];
   uint32_t vo[R];

@@ -48,8 +48,8 @@ __global__ __launch_bounds__(256) void k_samples(Dev D, int K, int n, uint64_t s
   valid[r] = ok;
   if (!ok) return;
   const size_t on = (size_t)s * D.Npad;
-  const double *cur = fbuf(D, D.loc[task]) + (size_t)task * M * D.Npad;
-  const double *aft = fbuf(D, D.loc[task] ^ 1) + (size_t)task * M * D.Npad;
+  const double *cur = fslab(D, D.loc[task], task);
+  const double *aft = fslab(D, D.loc[task] ^ 1, task);
   float *x = X + (size_t)r * nin;
   double c = 0.0;
   for (int m = 0; m < M; m++) c += log(cur[(size_t)m * D.Npad + kk]);

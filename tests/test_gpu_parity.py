@@ -731,11 +731,70 @@ def _ripple(a, n_active, head):
     return out
 
 
-def test_device_node_pipeline_parity():
+@pytest.mark.parametrize("precision", [0, 1])
+def test_spawn_head_moves_f_like_the_ripple(precision):
+    """140 shell spawns with given spawn distributions: the device's f
+    (head slot moved back per spawn, rows copied to the far end when the
+    head reaches slot 0) equals the reference's ripple of every node one
+    shell outward, bit for bit, for filling and full streams, in FP64 and
+    FP32 storage; an EP step afterwards still matches the oracle."""
+    p = params(numEpSteps=1)
+    E, M = 5, 4
+    nodes = workloads.parker_nodes(1, 1, 70, seed=35, jitter=0.1)
+    S, N = nodes["x"].shape
+    n_active = np.array([70, 40, 10, 69, 70, 2], np.int32)
+    rng = np.random.default_rng(36)
+    f = rng.uniform(0.5, 2.0, (S, N, E, M))
+    dims = workloads.dims_of(nodes, E, M)
+    sol = runtime.EpSolver(dims, p)
+    cast = (lambda a: a.astype(np.float32).astype(np.float64)) if precision else (lambda a: a)
+    try:
+        sol.set_nodes(nodes, n_active)
+        sol.set_precision(precision)
+        sol.upload_f(f)
+        fo, na = cast(f), n_active.copy()
+        # spawn positions chosen so that the last 70 spawns lay out a stream
+        final = workloads.parker_nodes(1, 1, N, seed=37, jitter=0.1)
+        for j in range(140):
+            sp = rng.uniform(0.5, 2.0, (S, E, M))
+            k = min(139 - j, N - 1)
+            sol.spawn_shell(*[final[q][:, k].copy() for q in ("x", "y", "z", "rmag")], f=sp)
+            fo = _ripple(fo, na, cast(sp))
+            na = np.minimum(na + 1, N)
+            if j % 35 == 34:
+                fg = sol.download_f()
+                mask = active_mask(fo.shape[:2], na)
+                np.testing.assert_array_equal(fg[mask], fo[mask], err_msg="spawn %d" % j)
+        assert np.array_equal(sol.positions()["n_active"], na)
+        if precision == 0:
+            # the EP update reads the moved rows: one step against the oracle
+            full = {k: _ripple_n(nodes[k], n_active, 140) for k in nodes}
+            full.update({q: final[q] for q in ("x", "y", "z", "rmag")})
+            rc, _ = sol.step(raise_on_error=False)
+            fg = sol.download_f()
+            rco, _, fe, _, _, _ = oracle.update_ep(dims, p, full, fo, n_active=na)
+            assert rc == rco == 0
+            assert max_rel(fg, fe, na) <= RTOL
+    finally:
+        sol.close()
+
+
+def _ripple_n(a, n_active, times):
+    """`times` ripples of a node array whose new node 0 repeats node 0."""
+    na = np.array(n_active)
+    for _ in range(times):
+        a = _ripple(a, na, a[:, 0])
+        na = np.minimum(na + 1, a.shape[1])
+    return a
+
+
+@pytest.mark.parametrize("steps", [3, 130])
+def test_device_node_pipeline_parity(steps):
     """Global steps with node motion on the device -- spawn/ripple, move,
     updateMhd from the device-held state, EP update -- against the oracle
     doing the same steps (or_move_nodes is the function the oracle's
-    analytic-wind pipeline uses, pinned by the wind digests)."""
+    analytic-wind pipeline uses, pinned by the wind digests).  130 steps
+    run the f rows' head down to slot 0 and through the copy back."""
     from eprem_amd import constants as K, mas_synth
     rs = 0.00465047
     p = params(rScale=rs, numEpSteps=2)
@@ -752,7 +811,6 @@ def test_device_node_pipeline_parity():
     spawn_r = nodes["rmag"][:, 0].copy()
     zen0 = np.arccos(nodes["z"][:, 0] / spawn_r)
     azi0 = np.arctan2(nodes["y"][:, 0], nodes["x"][:, 0])
-    steps = 3
 
     def spawn_pos(j):
         azi = azi0 + 0.01 * (j + 1)
