@@ -158,6 +158,18 @@ class EpMasPrev(C.Structure):
                                    "vr", "vtheta", "vphi", "density")]
 
 
+class EpSurrogateStats(C.Structure):
+    _fields_ = [
+        ("network_slabs", C.c_int64),
+        ("fallback_slabs", C.c_int64),
+        ("plain_slabs", C.c_int64),
+        ("network_cell_subcycles", C.c_int64),
+        ("rejected_stream_substeps", C.c_int32),
+        ("pad_", C.c_int32),
+        ("max_residual", C.c_double),
+    ]
+
+
 class EpMasOut(C.Structure):
     _fields_ = [(n, _DP) for n in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density")]
 
@@ -231,5 +243,8 @@ EXPORTED_SYMBOLS = (
     "ep_mlp_finalize",
     "ep_mlp_bench",
     "ep_surrogate_samples",
+    "ep_set_surrogate",
+    "ep_get_surrogate_stats",
+    "ep_surrogate_probe",
     "ep_shell_operators",
 )

@@ -64,6 +64,7 @@ static void free_all() {
   if (g.st) (void)hipStreamDestroy(g.st);
   g.st = nullptr;
   g.mas_free();
+  g.sur = SurState();
   g.init = false;
   g.nodes_set = false;
 }
@@ -278,6 +279,7 @@ Dev make_dev() {
   D.c_bc0 = g.c_bc0;
   D.c_bc1 = g.c_bc1;
   D.diag = g.diag;
+  D.fphi = nullptr;
   return D;
 }
 
@@ -357,39 +359,48 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
   } while (0)
   // the streaming passes and the one-pass node kernels carry the upwind
   // operators only
-  const int fuse = (sh.do_stream && sh.do_focus && D.stream_fuse && D.focus_alg == 1) ? 1 : 0;
+  // learned-surrogate form: the network's applications come first, then
+  // the streaming kernels run what is left of each slab (Ds) and the node
+  // stage focuses each row over the fraction of the substep it still needs
+  // (Dn); the seed floor then runs as its own kernel, before the network
+  const bool sur = sh.sur_levels > 0;
+  const int fuse = (!sur && sh.do_stream && sh.do_focus && D.stream_fuse && D.focus_alg == 1) ? 1 : 0;
   const bool rk = D.focus_alg != 1 || D.change_alg != 1;
+  const Dev Ds = sur ? sur_stream_dev(D) : D;
+  const Dev Dn = sur ? sur_node_dev(D) : D;
+  const int seed_in_pass = (sh.do_seed && !sur) ? 1 : 0;
   for (int sub = 0; sub < sh.nsub; sub++) {
     hipEvent_t *e4 = ev ? &ev[2 + 4 * sub] : nullptr;
-    if (sh.do_seed && !sh.do_stream) EPQ(launch_seed_check(D, st));
+    if (sh.do_seed && (!sh.do_stream || sur)) EPQ(launch_seed_check(D, st));
     if (sh.do_stream) {
+      if (sur) EPQ(launch_sur_substep(D, sub, sh.sur_levels - 1, st));
       // focusing of full-resolution streams can ride on each slab's last pass
       if (sh.sched == 0) {
         for (int pass = 0; pass < sh.passes; pass++)
-          EPQ(launch_stream_pass(D, pass, pass & 1, sh.do_seed && pass == 0, sub, sh.chunks, fuse, st));
+          EPQ(launch_stream_pass(Ds, pass, pass & 1, seed_in_pass && pass == 0, sub, sh.chunks, fuse, st));
       } else if (sh.sched == 1) {
-        EPQ(launch_stream_slab(D, sh.do_seed, sub, fuse, 0, -1, st));
+        EPQ(launch_stream_slab(Ds, seed_in_pass, sub, fuse, 0, -1, st));
       } else {
-        for (int s = 0; s < D.S; s++) EPQ(launch_stream_slab(D, sh.do_seed, sub, fuse, 0, s, st));
+        for (int s = 0; s < D.S; s++) EPQ(launch_stream_slab(Ds, seed_in_pass, sub, fuse, 0, s, st));
       }
-      EPQ(launch_stream_finish(D, sub, st));
+      EPQ(launch_stream_finish(Ds, sub, st));
     }
     if (e4) EPQ(hipEventRecord(e4[0], st));
     if (rk) {
       // AdiabaticFocusing then AdiabaticChange, each with its own operator
       if (sh.do_focus)
-        EPQ(D.focus_alg == 1 ? launch_focus(D, sub, fuse, st) : launch_focus_rk(D, D.focus_alg, sub, st));
+        EPQ(D.focus_alg == 1 ? launch_focus(Dn, sub, fuse, st) : launch_focus_rk(Dn, D.focus_alg, sub, st));
       if (e4) EPQ(hipEventRecord(e4[1], st));
       if (sh.do_change)
-        EPQ(D.change_alg == 1 ? launch_change(D, sub, st) : launch_change_rk(D, D.change_alg, sub, st));
+        EPQ(D.change_alg == 1 ? launch_change(Dn, sub, st) : launch_change_rk(Dn, D.change_alg, sub, st));
     } else if (sh.node_fused == 2) {
       // the path is chosen on the device from the preparation's largest
       // change subcycle count: heavy nodes are focused first on both paths,
       // then either every node is focused and changed in one pass over f
       // (node_gate 1) or focused and changed by two kernels (node_gate 2)
-      EPQ(launch_focus_heavy(D, sub, st));
+      EPQ(launch_focus_heavy(Dn, sub, st));
       if (e4) EPQ(hipEventRecord(e4[1], st));
-      Dev D1 = D, D2 = D;
+      Dev D1 = Dn, D2 = Dn;
       D1.node_gate = 1;
       D2.node_gate = 2;
       EPQ(launch_focus_change(D1, sub, fuse, 0, st));
@@ -398,13 +409,13 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
     } else if (sh.node_fused) {
       // heavy nodes are focused first; every node is then focused (unless
       // done already) and changed in one pass over f
-      EPQ(launch_focus_heavy(D, sub, st));
+      EPQ(launch_focus_heavy(Dn, sub, st));
       if (e4) EPQ(hipEventRecord(e4[1], st));
-      EPQ(launch_focus_change(D, sub, fuse, 0, st));
+      EPQ(launch_focus_change(Dn, sub, fuse, 0, st));
     } else {
-      if (sh.do_focus) EPQ(launch_focus(D, sub, fuse, st));
+      if (sh.do_focus) EPQ(launch_focus(Dn, sub, fuse, st));
       if (e4) EPQ(hipEventRecord(e4[1], st));
-      if (sh.do_change) EPQ(launch_change(D, sub, st));
+      if (sh.do_change) EPQ(launch_change(Dn, sub, st));
     }
     // DiffuseShellData, then DriftShellData, after the stream loop
     if (sh.do_shell & 1) EPQ(launch_shell_diffuse(D, sub, st));
@@ -838,13 +849,19 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
     for (int32_t n : g.h_nactive)
       if (n != g.d.num_nodes) return fail(EP_ERR_UNSUPPORTED, "shell operators need every stream full");
   }
+  if ((rc = sur_check_step())) return rc;
+  const bool sur = g.sur.K > 0 && do_stream;
+  if (sur && (do_shell || do_drift))
+    return fail(EP_ERR_UNSUPPORTED, "surrogate: shell diffusion and drift are not supported with it");
   const int sched = effective_schedule();
   // With the persistent streaming launch nothing the host launches depends
   // on the preparation's results: the whole global step (preparation and
   // substeps) is one sequence, captured as one graph, and the host waits
   // once, at the end.  The per-pass schedule sizes its launches from the
   // preparation and waits for it first.
-  const bool one_sequence = sched != 0;
+  // The surrogate form sizes its network launches from the preparation's
+  // largest subcycle count and takes the per-pass path.
+  const bool one_sequence = sched != 0 && !sur;
   DevDiag hd;
   ep_diag dg;
   memset(&dg, 0, sizeof(dg));
@@ -869,10 +886,12 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
       return fail(code, "%s (stream %d)", err_text(code), dg.err_stream);
     }
     if (do_shell || do_drift) HIPCHK(launch_shell_links(D, st), "shell links");
+    if (sur) HIPCHK(launch_sur_prep(D, st), "surrogate scalars");
     node_fused = (g.node_fuse && do_focus && do_change && hd.max_sub_change <= 1 && p.adiabaticFocusAlg == 1 &&
                   p.adiabaticChangeAlg == 1) ? 1 : 0;
     shape = StepShape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
-                      do_shell | (do_drift << 1), g.generation, node_fused, sched};
+                      do_shell | (do_drift << 1), g.generation, node_fused, sched,
+                      sur ? 1 + hd.max_nsteps / g.sur.K : 0};
   }
   if (g.use_graph) {
     // One HIP graph, rebuilt only when its shape (operators, schedule, pass

@@ -120,7 +120,39 @@ struct Dev {
   int focus_alg, change_alg;  // 1 upwind Euler, 2 RK3 upwind, 3 RK3 WENO3
   int gated;      // launched as part of an EP substep (see substep_off)
   int node_gate;  // node-stage path: 0 always, 1 / 2 when every / not every node needs <= 1 change subcycle
+  // fraction of the substep each (stream, energy) row is focused over
+  // [S*E]; null = the whole substep.  The learned-surrogate form sets it
+  // for rows whose first subcycles the network has already advanced.
+  const double *fphi;
 };
+
+// Temporal-blocking tiling of the streaming passes of one (stream, energy)
+// slab with `size` kept nodes and `nsteps` subcycles: halo h (subcycles
+// per pass), passes and chunks.  A stream that fits in one chunk takes all
+// its subcycles in one pass without a halo (the streaming kernels hold its
+// two ends); a longer one is cut into chunks whose halos cover the
+// subcycles of a pass.  nsteps = 0 (nothing left to stream): no pass.
+__device__ inline void task_tiling(const Dev &D, int size, int nsteps, int *h_out, int *passes_out,
+                                   int *chunks_out) {
+  if (nsteps <= 0) {
+    *h_out = 1;
+    *passes_out = 0;
+    *chunks_out = 1;
+    return;
+  }
+  const bool whole = D.stream_whole && D.stream_block >= size;
+  const int H = whole ? nsteps : D.stream_halo, W = D.stream_block;
+  const int h = nsteps < H ? nsteps : H;
+  const int interior = whole ? W : W - 2 * h;
+  *h_out = h;
+  *passes_out = (nsteps + h - 1) / h;
+  *chunks_out = (size + interior - 1) / interior;
+}
+
+// Focusing subcycle length of row (task) of a node whose subcycle is dtsub.
+__device__ inline double focus_dtsub(const Dev &D, int task, double dtsub) {
+  return D.fphi ? dtsub * D.fphi[task] : dtsub;
+}
 
 __device__ inline void note_error(const Dev &D, unsigned long long key) {
   atomicMin(&D.diag->err_key, key);

@@ -224,15 +224,8 @@ __global__ __launch_bounds__(256) void k_task_prep(Dev D) {
       dtProp = D.dt / (1.0 * nsteps);
     }
   }
-  const bool whole = D.stream_whole && D.stream_block >= size;
-  const int H = whole ? nsteps : D.stream_halo, W = D.stream_block;
-  // a stream that fits in one chunk takes all its subcycles in one pass
-  // without a halo (the streaming kernels hold its two ends); a longer one
-  // is cut into chunks whose halos cover the subcycles of a pass
-  const int h = nsteps < H ? nsteps : H;
-  const int passes = (nsteps + h - 1) / h;
-  const int interior = whole ? W : W - 2 * h;
-  const int chunks = (size + interior - 1) / interior;
+  int h, passes, chunks;
+  task_tiling(D, size, nsteps, &h, &passes, &chunks);
   D.nsteps[t] = nsteps;
   D.halo[t] = h;
   D.passes[t] = passes;
@@ -471,7 +464,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
 #pragma unroll
   for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row, T());
   int bad = -1, bad_inf = 0;
-  EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, FASTF);
+  EP_FOCUS_ROW(T, M, D, a, b, f, nsub, focus_dtsub(D, task, dtsub), bad, bad_inf, FASTF);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const T val = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
@@ -518,7 +511,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 #pragma unroll
     for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
     int bad = -1, bad_inf = 0;
-    EP_FOCUS_ROW(T, M, D, a, b, f, nsub, dtsub, bad, bad_inf, FASTF);
+    EP_FOCUS_ROW(T, M, D, a, b, f, nsub, focus_dtsub(D, task, dtsub), bad, bad_inf, FASTF);
 #pragma unroll
     for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
     if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));

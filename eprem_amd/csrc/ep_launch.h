@@ -63,4 +63,14 @@ hipError_t launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream
 hipError_t launch_reset_diag(DevDiag *d, hipStream_t st);
 hipError_t launch_fastmath_probe(const double *tab, const double *x, int64_t n, double *lg, double *ex,
                                  hipStream_t st);
+// learned-surrogate form (ep_sur.hip): the network applications of one
+// substep (levels = the largest application count), their residuals and
+// the accept / fall-back decision; the Dev copies the plain streaming and
+// node stage then run with
+hipError_t launch_sur_substep(const Dev &D, int substep, int levels, hipStream_t st);
+// the network's scalar inputs of every kept node, once per global step
+hipError_t launch_sur_prep(const Dev &D, hipStream_t st);
+Dev sur_stream_dev(const Dev &D);
+Dev sur_node_dev(const Dev &D);
+int sur_check_step();
 }  // namespace ep

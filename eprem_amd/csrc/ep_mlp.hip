@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "ep_internal.h"
+#include "ep_mlp_core.h"
 #include "ep_state.h"
 
 namespace ep {
@@ -167,12 +168,6 @@ __global__ __launch_bounds__(256) void k_gather_rows(const float *src, const int
 }
 
 // ---- fused forward -------------------------------------------------------
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-
-constexpr int kFH = 128;   // hidden width of the fused kernel
-constexpr int kFNT = 2;    // 32-sample tiles per wave (one A fragment feeds two MFMAs)
-constexpr int kFWaves = 4; // waves per block (one per SIMD; all share the LDS weights)
-
 // Fragment (ks, ot) of a layer: 64 lanes x 8 bf16.  Lane (i, h) = (l % 32,
 // l / 32) holds A[i][8h + q] = W[phi(ks, h, q)][32 ot + i], q = 0..7, where
 // phi maps the MFMA's inner index to the layer's input feature:
@@ -193,49 +188,6 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float *W, int in, int ou
   const int j = 32 * ot + i;
   const float v = (k < in && j < out) ? W[(size_t)k * out + j] : 0.0f;
   dst[t] = (__bf16)v;
-}
-
-// One layer on kFNT sample tiles: acc[t][ot] = sum_ks A(ks, ot) B[t][ks].
-template <int KS, int OT>
-__device__ __forceinline__ void mlp_layer(const bf16x8_t *__restrict__ frag, const bf16x8_t (&b)[kFNT][KS],
-                                          f32x16_t (&acc)[kFNT][OT], int lane) {
-#pragma unroll
-  for (int t = 0; t < kFNT; ++t)
-#pragma unroll
-    for (int ot = 0; ot < OT; ++ot)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[t][ot][r] = 0.0f;
-#pragma unroll
-  for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-    for (int ot = 0; ot < OT; ++ot) {
-      const bf16x8_t a = frag[(ks * OT + ot) * 64 + lane];
-#pragma unroll
-      for (int t = 0; t < kFNT; ++t) acc[t][ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[t][ks], acc[t][ot], 0, 0, 0);
-    }
-}
-
-// Bias, ReLU and bf16 rounding of a hidden layer's accumulators into the
-// next layer's B operands: K-step 2 ot + u takes registers 8u..8u+7 of
-// tile ot (features 32 ot + 16 u + 8 (q/4) + 4 h + q%4).
-__device__ __forceinline__ void mlp_relu_to_b(const f32x16_t (&acc)[kFNT][kFH / 32], const float *bias, int h,
-                                              bf16x8_t (&b)[kFNT][kFH / 16]) {
-#pragma unroll
-  for (int t = 0; t < kFNT; ++t)
-#pragma unroll
-    for (int ot = 0; ot < kFH / 32; ++ot)
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        bf16x8_t v;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int r = 8 * u + q;
-          float z = acc[t][ot][r] + bias[32 * ot + 8 * (r >> 2) + 4 * h + (r & 3)];
-          z = z > 0.0f ? z : 0.0f;
-          v[q] = (__bf16)z;
-        }
-        b[t][2 * ot + u] = v;
-      }
 }
 
 // Y[n][out] = network(X[n][in]); a persistent grid of kFWaves-wave blocks,
@@ -417,6 +369,20 @@ static hipError_t mlp_pack(hipStream_t st) {
   mlp.packed16 = total16;
   mlp.bias16 = w16;
   mlp.packed_ok = true;
+  return hipSuccess;
+}
+
+hipError_t mlp_fused_weights(hipStream_t st, const uint4 **packed, int *packed16, int *bias16,
+                             std::vector<int> *sizes) {
+  if (!mlp.init || !mlp.fusable()) return hipErrorInvalidValue;
+  if (!mlp.packed_ok) {
+    hipError_t e = mlp_pack(st);
+    if (e != hipSuccess) return e;
+  }
+  *packed = (const uint4 *)mlp.packed;
+  *packed16 = mlp.packed16;
+  *bias16 = mlp.bias16;
+  *sizes = mlp.sizes;
   return hipSuccess;
 }
 

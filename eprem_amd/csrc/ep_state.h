@@ -18,17 +18,45 @@ struct MasLevel {
   bool set = false;
 };
 
+// Counters of the learned-surrogate form over one ep_step (device side).
+struct SurStats {
+  unsigned long long network_slabs, fallback_slabs, plain_slabs, network_cell_subcycles;
+  unsigned long long max_res_bits;  // positive doubles order like their bits
+  unsigned rejected_stream_substeps, pad_;
+};
+
+// The learned-surrogate form of the EP update (ep_sur.hip): its device
+// buffers, allocated by ep_set_surrogate and released with the others.
+struct SurState {
+  int K = 0;          // streaming subcycles per network application (0: off)
+  double tol = 1e-3;  // residual above which a stream falls back to the plain path
+  int G = 0;          // k_sur_apply work items per slab
+  int stride = 0;     // kept nodes per sampled node of the plain check
+  int NS = 0;         // most sampled nodes of a slab
+  double *fC = nullptr;              // third f buffer, the layout of fA/fB
+  int32_t *A = nullptr;              // [S*E] network applications this substep
+  int32_t *reject = nullptr;         // [S] stream falls back this substep
+  double *errsum = nullptr;          // [S*E] summed sampled errors of one application
+  unsigned long long *res = nullptr; // [S*E] largest residual of the substep (bits)
+  int32_t *nrem = nullptr, *hrem = nullptr, *prem = nullptr, *crem = nullptr;  // [S*E] plain remainder
+  double *phi = nullptr;             // [S*E] fraction of the substep left to plain focusing
+  float *scal = nullptr;             // [S*E*6*Npad] scalar inputs of every kept node
+  SurStats *stats = nullptr;
+  const void *packed = nullptr;      // network weights the captured graph reads
+};
+
 // What a captured substep graph depends on besides device memory.
 struct StepShape {
   int passes, chunks, nsub, do_stream, do_focus, do_change, do_seed, do_shell;
   unsigned generation;
   int node_fused;  // focusing and change in one kernel (k_focus_change)
   int sched;       // streaming schedule (0: a launch per pass, 1: persistent, 2: per stream)
+  int sur_levels;  // learned-surrogate form: 1 + network application levels; 0 = off
   bool operator==(const StepShape &o) const {
     return passes == o.passes && chunks == o.chunks && nsub == o.nsub && do_stream == o.do_stream &&
            do_focus == o.do_focus && do_change == o.do_change && do_seed == o.do_seed &&
            do_shell == o.do_shell && generation == o.generation && node_fused == o.node_fused &&
-           sched == o.sched;
+           sched == o.sched && sur_levels == o.sur_levels;
   }
 };
 
@@ -101,6 +129,7 @@ struct State {
   // the MHD buffer's state half holds the latest B, V, density per node
   // once ep_update_mhd has run (ep_move_nodes reads V from it)
   bool mhd_state = false;
+  SurState sur;  // learned-surrogate form (ep_sur.hip)
   // point observers, stream flux and shell-spawn staging (allocated on
   // first use)
   double *obs_buf = nullptr, *flux_buf = nullptr, *spawn_buf = nullptr;

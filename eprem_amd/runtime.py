@@ -80,7 +80,10 @@ def lib():
         L.ep_mlp_train.argtypes = [FP, FP, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64, FP]
         L.ep_mlp_finalize.argtypes = []
         L.ep_mlp_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, DP]
-        L.ep_surrogate_samples.argtypes = [C.c_int32, C.c_int32, C.c_uint64, FP, FP, IP]
+        L.ep_surrogate_samples.argtypes = [C.c_int32, C.c_int32, C.c_uint64, FP, FP, IP, IP]
+        L.ep_set_surrogate.argtypes = [C.c_int32, C.c_double]
+        L.ep_get_surrogate_stats.argtypes = [C.POINTER(abi.EpSurrogateStats)]
+        L.ep_surrogate_probe.argtypes = [C.c_int32, DP]
         L.ep_shell_operators.argtypes = [IP, C.c_double, C.c_int32]
         for name in abi.EXPORTED_SYMBOLS:
             getattr(L, name).restype = C.c_int
@@ -321,16 +324,40 @@ class EpSolver:
         nb = None if nbr is None else np.ascontiguousarray(nbr, dtype=np.int32)
         _check(lib().ep_shell_operators(iptr(nb), float(kperxkpar), 1 if drift else 0))
 
-    def surrogate_samples(self, K, n, seed=1):
-        """Training pairs (x [k, 3M+6], y [k, M]) of K streaming subcycles on
-        the current state; only the qualifying rows of n candidates."""
+    def surrogate_samples(self, K, n, seed=1, where=False):
+        """Training pairs (x [k, 3M+6], y [k, M]) of K streaming subcycles and
+        the focusing of the same time on the current state; only the
+        qualifying rows of n candidates (with where: also their (stream,
+        energy, node) [k, 3])."""
         M = self.dims.num_mu
         x = np.zeros((n, 3 * M + 6), np.float32)
         y = np.zeros((n, M), np.float32)
         v = np.zeros(n, np.int32)
-        _check(lib().ep_surrogate_samples(int(K), int(n), int(seed), fptr(x), fptr(y), iptr(v)))
+        w = np.zeros((n, 3), np.int32)
+        _check(lib().ep_surrogate_samples(int(K), int(n), int(seed), fptr(x), fptr(y), iptr(v), iptr(w)))
         ok = v != 0
+        if where:
+            return x[ok], y[ok], w[ok]
         return x[ok], y[ok]
+
+    def set_surrogate(self, K, tol=1e-3):
+        """Learned-surrogate form of step(): the network of the current Mlp
+        replaces K streaming + focusing subcycles of every slab with at least
+        K, with a per-stream plain fallback above the residual tol; K = 0
+        turns it off."""
+        _check(lib().ep_set_surrogate(int(K), float(tol)))
+
+    def surrogate_stats(self):
+        st = abi.EpSurrogateStats()
+        _check(lib().ep_get_surrogate_stats(C.byref(st)))
+        return {n: getattr(st, n) for n, _ in abi.EpSurrogateStats._fields_ if n != "pad_"}
+
+    def surrogate_probe(self, mode):
+        """mode 0: one network application made current; 1: the plain K
+        subcycles into the spare buffer.  Returns the largest residual."""
+        r = C.c_double()
+        _check(lib().ep_surrogate_probe(int(mode), C.byref(r)))
+        return r.value
 
     def close(self):
         lib().ep_finalize()

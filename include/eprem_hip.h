@@ -369,14 +369,54 @@ int ep_mlp_train(const float *x, const float *y, int32_t n, int32_t epochs, int3
 int ep_mlp_bench(int32_t n, int32_t reps, int32_t layerwise, double *ms);
 int ep_mlp_finalize(void);
 
-/* Training pairs for the large-step surrogate of streaming: n random kept
- * nodes of (stream, energy) slabs with at least K streaming subcycles per
- * substep, on the current nodes and f (FP64 storage).  x [n][3M+6]: ln f at
- * nodes k-1, k, k+1 minus the mean ln f at k, then K v dtProp/ds_k,
- * K dtProp/tau_k, ln(ds_k/ds_{k-1}), ln(ds_{k+1}/ds_k), v, K/nsteps;
- * y [n][M]: ln f at k after K subcycles of the plain path, minus the same
- * mean; valid[n] = 1 for rows that qualified.  f is unchanged. */
-int ep_surrogate_samples(int32_t K, int32_t n, uint64_t seed, float *x, float *y, int32_t *valid);
+/* Training pairs for the large-step surrogate: n random kept nodes of
+ * (stream, energy) slabs of full-resolution streams with at least K
+ * streaming subcycles per substep, on the current nodes and f (FP64
+ * storage).  x [n][3M+6]: ln f at nodes k-1, k, k+1 minus the mean ln f at
+ * k, then T v/ds_k, T/tau_k, ln(ds_k/ds_{k-1}), ln(ds_{k+1}/ds_k),
+ * asinh(a T/dmu), asinh(b T/dmu) with T = K dtProp and a, b the focusing
+ * velocity's terms; y [n][M]: ln f at k after K streaming subcycles of the
+ * plain path (DiffuseStreamData, src/energeticParticles.c:2352-2430) and
+ * the upwind focusing over T (:1806-1896), minus the same mean; valid[n] =
+ * 1 for rows that qualified; where [n][3] (may be NULL): each row's
+ * (stream, energy, node).  f is unchanged. */
+int ep_surrogate_samples(int32_t K, int32_t n, uint64_t seed, float *x, float *y, int32_t *valid,
+                         int32_t *where);
+
+/* Learned-surrogate form of ep_step.  No counterpart in the reference:
+ * it replaces, per EP substep, K CFL-limited subcycles of streaming with
+ * relaxation (DiffuseStreamData, src/energeticParticles.c:2237-2490) plus
+ * the adiabatic focusing of the same time (AdiabaticFocusing, :1533-1896)
+ * by one application of the network set with ep_mlp_init /
+ * ep_mlp_set_params / ep_mlp_train (3M+6 inputs as in
+ * ep_surrogate_samples, hidden layers of 128, M outputs), on every
+ * (stream, energy) slab with at least K subcycles.  A stream whose
+ * conservation residual after an application exceeds tol (or whose
+ * network output is not finite) falls back to the plain path for that
+ * substep.  K = 0 turns it off.  Needs FP64 storage and the upwind
+ * operators; allocates a third f buffer on first use. */
+int ep_set_surrogate(int32_t K, double tol);
+
+typedef struct ep_surrogate_stats {
+  int64_t network_slabs;          /* slab-substeps advanced by the network and accepted */
+  int64_t fallback_slabs;         /* slab-substeps of rejected streams, redone by the plain path */
+  int64_t plain_slabs;            /* slab-substeps with fewer than K subcycles (plain path) */
+  int64_t network_cell_subcycles; /* cell-subcycles of streaming + focusing the accepted applications cover */
+  int32_t rejected_stream_substeps; /* (stream, substep) pairs that fell back */
+  int32_t pad_;
+  double max_residual;            /* largest residual of any application */
+} ep_surrogate_stats;
+
+/* Counters of the last ep_step in the surrogate form. */
+int ep_get_surrogate_stats(ep_surrogate_stats *out);
+
+/* Test probes of the surrogate form on the current nodes and f: mode 0
+ * applies the network once to every slab with at least K subcycles and
+ * makes the result current (the other slabs keep theirs); mode 1 runs the
+ * plain path for exactly K streaming subcycles into the spare buffer and
+ * leaves f as it was.  Either way *max_residual receives the largest
+ * conservation residual of that one application. */
+int ep_surrogate_probe(int32_t mode, double *max_residual);
 
 /* ---------------- misc --------------------------------------------- */
 int ep_last_error(char *buf, size_t len);

@@ -1,6 +1,7 @@
 """Learned large-step surrogate: the training pairs equal the oracle's K
-streaming subcycles, and the trained network beats the large step without
-it (relaxation only) on held-out nodes."""
+streaming subcycles followed by the focusing of the same time, and the
+trained network beats the large step without it (relaxation only) on
+held-out nodes."""
 import math
 
 import numpy as np
@@ -20,17 +21,17 @@ def _params(**kw):
 
 
 def test_samples_match_oracle_subcycles():
-    """With one energy (eMin = eMax) and dt chosen so that a substep is
-    exactly Ks streaming subcycles, the sample targets are the oracle's
-    streaming step."""
+    """With two close energies and dt chosen so that a substep is exactly Ks
+    streaming subcycles on one stream, the sample targets of the (stream,
+    energy) slabs with Ks subcycles are the oracle's streaming then focusing
+    over the substep."""
     Ks, E, M = 8, 2, 12
     nodes = workloads.parker_nodes(1, 1, 120, rscale=0.00465047, seed=61, jitter=0.2)
-    p = _params(eMin=10.0, eMax=10.0, numEpSteps=1)
+    p = _params(eMin=10.0, eMax=10.02, numEpSteps=1)
     g = oracle.build_grids(p, E, M)
     rs = p.rScale
     seg = np.sqrt(np.diff(nodes["x"] * rs, axis=1) ** 2 + np.diff(nodes["y"] * rs, axis=1) ** 2
                   + np.diff(nodes["z"] * rs, axis=1) ** 2)
-    dtmin = 0.4 * seg.min() * g["vgrid_i"][0]      # every stream shares the smallest dsMin bound
     dsmin_s = seg.min(axis=1)
     # dt in code units giving nsteps = floor(dt/dtMin_s + 1) = Ks on the
     # stream with the smallest dsMin
@@ -43,29 +44,23 @@ def test_samples_match_oracle_subcycles():
     try:
         sol.set_nodes(nodes)
         sol.upload_f(f0)
-        x, y = sol.surrogate_samples(Ks, 4000, seed=3)
+        x, y, where = sol.surrogate_samples(Ks, 4000, seed=3, where=True)
     finally:
         sol.close()
     assert len(x) > 100
-    rc, dg, fo, _, _, _ = oracle.update_ep(dims, p, nodes, f0, op_flags=abi.EP_OP_STREAM)
+    rc, dg, fo, _, _, _ = oracle.update_ep(dims, p, nodes, f0, op_flags=abi.EP_OP_STREAM | abi.EP_OP_FOCUS)
     assert rc == 0
-    # locate each sample's (stream, node) by matching its features against f0
     lf0 = np.log(f0)
     lfo = np.log(fo)
     matched = 0
-    for s in range(dims.num_streams):
-        nsteps = math.floor(dt / (0.4 * dsmin_s[s] * g["vgrid_i"][0]) + 1)
-        if nsteps != Ks:
+    for r in range(len(x)):
+        s, e, k = where[r]
+        if math.floor(dt / (0.4 * dsmin_s[s] * g["vgrid_i"][e]) + 1) != Ks:
             continue
-        for e in range(E):
-            c = lf0[s, :, e, :].mean(axis=1)                 # [N]
-            feat_mid = lf0[s, :, e, :] - c[:, None]
-            tgt = lfo[s, :, e, :] - c[:, None]
-            for r in range(len(x)):
-                k = np.nonzero(np.all(np.abs(feat_mid[1:-1] - x[r, M:2 * M]) < 1e-6, axis=1))[0]
-                if len(k) == 1:
-                    np.testing.assert_allclose(y[r], tgt[1 + k[0]], rtol=0, atol=2e-5)
-                    matched += 1
+        c = lf0[s, k, e, :].mean()
+        np.testing.assert_allclose(x[r, :3 * M].reshape(3, M), lf0[s, k - 1:k + 2, e, :] - c, rtol=0, atol=2e-5)
+        np.testing.assert_allclose(y[r], lfo[s, k, e, :] - c, rtol=0, atol=2e-5)
+        matched += 1
     assert matched > 50
 
 
