@@ -12,7 +12,10 @@
 // nodes 0-2 to the seed before every subcycle) and the last node takes the
 // relaxation-only large step.  The network reads, per node, ln f over the
 // M mu cells of the three nodes minus the mean ln f at node k, and six
-// scalars (sur_scalars, ep_sur.h); it returns ln f at node k minus the same mean.
+// scalars (sur_scalars, ep_sur.h); it returns the correction ln f - ln f_L
+// to the large step f_L = iso + (f - iso) q^K (the exact relaxation over
+// K dtProp, no transport), which is formed in FP64: the network's bf16
+// arithmetic only touches the correction.
 //
 // Residual: the plain path judges the network.  After every application,
 // at sampled kept nodes c of each slab (about one per `stride` nodes) one
@@ -204,6 +207,9 @@ __global__ __launch_bounds__(64 * kSurWaves) void k_sur_apply(Dev D, SurDev S, i
     const int A = S.A[task];
     if (A <= level) continue;
     const int s = task / E;
+    // a stream rejected at an earlier application recomputes its substep on
+    // the plain path: the rest of its applications are not needed
+    if (level > 0 && S.reject[s]) continue;
     const int nk = D.nkept[s];
     const int k0 = 1 + kSurSeg * g;
     if (k0 > nk - 2) continue;
@@ -230,7 +236,6 @@ __global__ __launch_bounds__(64 * kSurWaves) void k_sur_apply(Dev D, SurDev S, i
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    double cmean[NT];
     bool valid[NT];
     int kk[NT];
     bf16x8_t b0[NT][KS0];
@@ -252,7 +257,6 @@ __global__ __launch_bounds__(64 * kSurWaves) void k_sur_apply(Dev D, SurDev S, i
       csum += __shfl_xor(csum, 32);
       const double c = csum / (double)M;
       const float cf = (float)c;
-      cmean[t] = c;
 #pragma unroll
       for (int ks = 0; ks < KS0; ++ks) {
         bf16x8_t v;
@@ -288,17 +292,30 @@ __global__ __launch_bounds__(64 * kSurWaves) void k_sur_apply(Dev D, SurDev S, i
     f32x16_mlp_t o[NT][1];
     mlp_layer_lean<kFH / 16, 1, NT>(fr, b, o, lane);
     const float *ob = lbias + nhid * kFH;
-    // ---- new rows
+    // ---- new rows: the large step (exact relaxation over K dtProp,
+    // f -> iso + (f - iso) q^K, in FP64) times exp of the network's
+    // correction
     bool bad = false;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
+      double fk[16];
+      double part = 0.0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int j = 8 * (r >> 2) + 4 * h + (r & 3);
+        fk[r] = (valid[t] && j < M) ? src[(size_t)j * Np + kk[t]] : 0.0;
+        part += fk[r];
+      }
+      part += __shfl_xor(part, 32);
       if (!valid[t]) continue;
+      const double iso = part / (double)M;
+      const double qK = pow(D.q[ok + kk[t]], (double)K);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int j = 8 * (r >> 2) + 4 * h + (r & 3);
         if (j < M) {
-          const double z = (double)(o[t][0][r] + ob[j]) + cmean[t];
-          double fv = sur_exp(z);
+          const double base = iso + (fk[r] - iso) * qK;
+          double fv = base * sur_exp((double)(o[t][0][r] + ob[j]));
           bad = bad || !isfinite(fv);
           fv = fmax(fv, kDblMin);
           dst[(size_t)j * Np + kk[t]] = fv;
@@ -347,6 +364,7 @@ __global__ __launch_bounds__(256) void k_sur_check(Dev D, SurDev S, int level, i
   const int A = S.A[task];
   if (A <= level) return;
   const int s = task / D.E, e = task % D.E;
+  if (level > 0 && S.reject[s]) return;
   const int nk = D.nkept[s];
   const int ns = sur_samples(S, nk);
   if (j >= ns) return;

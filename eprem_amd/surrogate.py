@@ -2,15 +2,19 @@
 
 One network application advances the mu row of one kept node of one
 (stream, energy) slab by K streaming-and-relaxation subcycles of the plain
-path (DiffuseStreamData, src/energeticParticles.c:2352-2430).  Inputs and
-targets come from `EpSolver.surrogate_samples` (ln f at the node and its
-two neighbours relative to the node's mean ln f, plus six scalars; target
-ln f after K subcycles); the network is `runtime.Mlp` (matrix-core
+path (DiffuseStreamData, src/energeticParticles.c:2352-2430) and the
+focusing of the same time (:1683-1896).  Inputs and targets come from
+`EpSolver.surrogate_samples` (ln f at the node and its two neighbours
+relative to the node's mean ln f, plus six scalars; target ln f after the
+K subcycles and the focusing); the network is `runtime.Mlp` (matrix-core
 products, bf16 operands, fp32 accumulation), trained on the device.
 
-The method without the network is the large step that keeps only what a
-single node can do alone: the exact relaxation of the anisotropy over
-K dtProp, f -> iso + (f - iso) exp(-K dtProp / tau), with no transport.
+The large step is what a single node can do alone: the exact relaxation
+of the anisotropy over K dtProp, f -> iso + (f - iso) exp(-K dtProp / tau),
+with no transport; it is also the method without the network.  The
+network is trained on, and in the EP update applies, the correction
+ln f - ln f_large (`corrections`, `predict`), so that its bf16 arithmetic
+only touches a small quantity.
 Errors are stated on the omnidirectional flux 2 E <f>_mu at the node
 (computeFlux, src/unifiedOutput.c:153-180), the quantity the accuracy
 gate is defined on.
@@ -30,6 +34,19 @@ def relaxation_step(x, M):
     iso = f.mean(axis=1, keepdims=True)
     q = np.exp(-x[:, 3 * M + 1].astype(np.float64))[:, None]
     return np.log(iso + (f - iso) * q)
+
+
+def corrections(x, y, M):
+    """Training targets of the corrected large step: ln f after the plain
+    subcycles minus ln f of the large step (both relative to the node's
+    mean ln f, so the mean cancels)."""
+    return (np.asarray(y, np.float64) - relaxation_step(x, M)).astype(np.float32)
+
+
+def predict(net, x, M):
+    """ln f (relative to the node's mean ln f) of the corrected large step:
+    the large step plus the network's correction."""
+    return relaxation_step(x, M) + net.forward(x).astype(np.float64)
 
 
 def flux_error(pred, y):

@@ -86,9 +86,10 @@ def test_trained_network_beats_large_step_without_it():
     n = len(x)
     assert n > 20000
     cut = int(0.8 * n)
-    net, hist = surrogate.train(x[:cut], y[:cut], epochs=15, batch=1024, lr=2e-3)
+    net, hist = surrogate.train(x[:cut], surrogate.corrections(x[:cut], y[:cut], M), epochs=15, batch=1024,
+                                lr=2e-3)
     try:
-        pred = net.forward(x[cut:])
+        pred = surrogate.predict(net, x[cut:], M)
     finally:
         net.close()
     assert hist[-1] < hist[0]
