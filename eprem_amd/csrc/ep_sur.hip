@@ -10,9 +10,10 @@
 // :1683-1762, :1806-1896) of every kept node k = 1..n-2 from the rows of
 // nodes k-1, k, k+1; node 0 keeps its boundary row (the reference resets
 // nodes 0-2 to the seed before every subcycle) and the last node takes the
-// relaxation-only large step.  The network reads, per node, ln f over the
-// M mu cells of the three nodes minus the mean ln f at node k, and six
-// scalars (sur_scalars, ep_sur.h); it returns the correction ln f - ln f_L
+// relaxation-only large step.  The network reads, per node and mu cell,
+// ln f(k-1) - ln f(k), ln f(k) minus its mean over mu and ln f(k+1) -
+// ln f(k), and six scalars (sur_scalars, ep_sur.h); it returns the
+// correction ln f - ln f_L
 // to the large step f_L = iso + (f - iso) q^K (the exact relaxation over
 // K dtProp, no transport), which is formed in FP64: the network's bf16
 // arithmetic only touches the correction.
@@ -222,14 +223,17 @@ __global__ __launch_bounds__(64 * kSurWaves) void k_sur_apply(Dev D, SurDev S, i
     // ---- inputs.  ln f of the window of kept nodes k0-1 .. k0+kSurSeg
     // (every sample's three nodes) goes to this wave's LDS rows [m][node];
     // then each lane reads its samples' inputs back from LDS.
+    // (stored relative to ln f of the window's node k0-1, mu 0, so that
+    // FP32 keeps the small differences between neighbouring nodes)
     constexpr int WN = kSurSeg + 2;
     float *lw = lds_f + (threadIdx.x >> 6) * (32 * WN);
+    const double lref = sur_ln(src[(size_t)(k0 - 1)]);
     for (int idx = lane; idx < M * WN; idx += 64) {
       const int m = idx / WN, j = idx - m * WN;
       const int node = k0 - 1 + j;
       float lf = 0.0f;
       if (node <= nk - 1) {
-        lf = (float)sur_ln(src[(size_t)m * Np + node]);
+        lf = (float)(sur_ln(src[(size_t)m * Np + node]) - lref);
       }
       lw[idx] = lf;
     }
@@ -266,9 +270,11 @@ __global__ __launch_bounds__(64 * kSurWaves) void k_sur_apply(Dev D, SurDev S, i
           float r = 0.0f;
           if (valid[t]) {
             if (phi < nin) {
+              // ln f(k-1) - ln f(k), ln f(k) - mean, ln f(k+1) - ln f(k)
               const int off = (phi >= M) + (phi >= 2 * M);
               const int m = phi - off * M;
-              r = lw[m * WN + jc - 1 + off] - cf;
+              const float own = lw[m * WN + jc];
+              r = (off == 1) ? own - cf : lw[m * WN + jc - 1 + off] - own;
             } else if (phi < nin + 6) {
               r = scal[(size_t)(phi - nin) * Np + k];
             }

@@ -10,8 +10,9 @@
 // spare f buffer, so f and the step state are unchanged), then the
 // focusing of the sampled row.
 //
-// Inputs of a sample (3M + 6): ln f at the kept nodes k-1, k, k+1 over
-// the M mu cells, each minus the mean ln f at node k, then the six
+// Inputs of a sample (3M + 6), per mu cell: ln f(k-1) - ln f(k), then
+// ln f(k) minus its mean over mu, then ln f(k+1) - ln f(k) (neighbours as
+// differences, which bf16 keeps to its relative precision), then the six
 // scalars of sur_scalars (ep_sur.h).  Target (M): ln f at node k after
 // the K subcycles and the focusing, minus the same mean.
 #include <hip/hip_runtime.h>
@@ -60,8 +61,12 @@ __global__ __launch_bounds__(256) void k_samples(Dev D, int K, int n, uint64_t s
   double c = 0.0;
   for (int m = 0; m < M; m++) c += log(cur[(size_t)m * D.Npad + kk]);
   c /= M;
-  for (int q = 0; q < 3; q++)
-    for (int m = 0; m < M; m++) x[q * M + m] = (float)(log(cur[(size_t)m * D.Npad + kk - 1 + q]) - c);
+  for (int m = 0; m < M; m++) {
+    const double lk = log(cur[(size_t)m * D.Npad + kk]);
+    x[m] = (float)(log(cur[(size_t)m * D.Npad + kk - 1]) - lk);
+    x[M + m] = (float)(lk - c);
+    x[2 * M + m] = (float)(log(cur[(size_t)m * D.Npad + kk + 1]) - lk);
+  }
   double xs[6];
   sur_scalars(D, s, e, kk, K, xs);
   for (int j = 0; j < 6; j++) x[3 * M + j] = (float)xs[j];

@@ -4,9 +4,10 @@ One network application advances the mu row of one kept node of one
 (stream, energy) slab by K streaming-and-relaxation subcycles of the plain
 path (DiffuseStreamData, src/energeticParticles.c:2352-2430) and the
 focusing of the same time (:1683-1896).  Inputs and targets come from
-`EpSolver.surrogate_samples` (ln f at the node and its two neighbours
-relative to the node's mean ln f, plus six scalars; target ln f after the
-K subcycles and the focusing); the network is `runtime.Mlp` (matrix-core
+`EpSolver.surrogate_samples` (per mu cell: ln f at the previous node minus
+ln f at the node, ln f at the node minus its mean over mu, ln f at the
+next node minus ln f at the node; plus six scalars; target ln f after the
+K subcycles and the focusing, relative to the same mean); the network is `runtime.Mlp` (matrix-core
 products, bf16 operands, fp32 accumulation), trained on the device.
 
 The large step is what a single node can do alone: the exact relaxation

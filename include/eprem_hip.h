@@ -372,8 +372,8 @@ int ep_mlp_finalize(void);
 /* Training pairs for the large-step surrogate: n random kept nodes of
  * (stream, energy) slabs of full-resolution streams with at least K
  * streaming subcycles per substep, on the current nodes and f (FP64
- * storage).  x [n][3M+6]: ln f at nodes k-1, k, k+1 minus the mean ln f at
- * k, then T v/ds_k, T/tau_k, ln(ds_k/ds_{k-1}), ln(ds_{k+1}/ds_k),
+ * storage).  x [n][3M+6]: per mu cell ln f(k-1) - ln f(k), ln f(k) minus
+ * its mean over mu, ln f(k+1) - ln f(k), then T v/ds_k, T/tau_k, ln(ds_k/ds_{k-1}), ln(ds_{k+1}/ds_k),
  * asinh(a T/dmu), asinh(b T/dmu) with T = K dtProp and a, b the focusing
  * velocity's terms; y [n][M]: ln f at k after K streaming subcycles of the
  * plain path (DiffuseStreamData, src/energeticParticles.c:2352-2430) and

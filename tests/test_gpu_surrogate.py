@@ -58,7 +58,9 @@ def test_samples_match_oracle_subcycles():
         if math.floor(dt / (0.4 * dsmin_s[s] * g["vgrid_i"][e]) + 1) != Ks:
             continue
         c = lf0[s, k, e, :].mean()
-        np.testing.assert_allclose(x[r, :3 * M].reshape(3, M), lf0[s, k - 1:k + 2, e, :] - c, rtol=0, atol=2e-5)
+        lk = lf0[s, k, e, :]
+        want = np.stack([lf0[s, k - 1, e, :] - lk, lk - c, lf0[s, k + 1, e, :] - lk])
+        np.testing.assert_allclose(x[r, :3 * M].reshape(3, M), want, rtol=0, atol=2e-5)
         np.testing.assert_allclose(y[r], lfo[s, k, e, :] - c, rtol=0, atol=2e-5)
         matched += 1
     assert matched > 50
