@@ -887,8 +887,10 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
     }
     if (do_shell || do_drift) HIPCHK(launch_shell_links(D, st), "shell links");
     if (sur) HIPCHK(launch_sur_prep(D, st), "surrogate scalars");
+    // the surrogate form focuses each row over its own fraction of the
+    // substep, which the two-kernel node stage (k_focus) carries
     node_fused = (g.node_fuse && do_focus && do_change && hd.max_sub_change <= 1 && p.adiabaticFocusAlg == 1 &&
-                  p.adiabaticChangeAlg == 1) ? 1 : 0;
+                  p.adiabaticChangeAlg == 1 && !sur) ? 1 : 0;
     shape = StepShape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
                       do_shell | (do_drift << 1), g.generation, node_fused, sched,
                       sur ? 1 + hd.max_nsteps / g.sur.K : 0};

@@ -122,7 +122,9 @@ struct Dev {
   int node_gate;  // node-stage path: 0 always, 1 / 2 when every / not every node needs <= 1 change subcycle
   // fraction of the substep each (stream, energy) row is focused over
   // [S*E]; null = the whole substep.  The learned-surrogate form sets it
-  // for rows whose first subcycles the network has already advanced.
+  // for rows whose first subcycles the network has already advanced; the
+  // focusing kernels that take it are k_focus and k_focus_heavy (the
+  // surrogate form runs the two-kernel node stage).
   const double *fphi;
 };
 

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
       const double v = D.vgrid[e];
       const double a = -v * dlnBds - (two / v) * fDuPar;
       const T dmu_i = (T)(1.0 / D.dmu);
-      const T dts = (T)focus_dtsub(D, s * E + e, dtsub_f);
+      const T dts = (T)dtsub_f;
       const T sc = dts * dmu_i;
       T vl[MH + 1], vh[MH + 1];
       T vw = 0;  // this lane's wall face velocity
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
       const double v = D.vgrid[e];
       const double a = -v * dlnBds - (two / v) * fDuPar;
       const T dmu_i = (T)(1.0 / D.dmu);
-      const T dts = (T)focus_dtsub(D, s * E + e, dtsub_f);
+      const T dts = (T)dtsub_f;
       const T sc = dts * dmu_i;
       T vl[MG + 1], vh[MG + 1];  // face below slot j (j = MG: above the last slot)
       double vp = (m0 > 0) ? shc[m0 - 1] * (a + smu[m0 - 1] * fb) : 0.0;
