@@ -232,6 +232,9 @@ def run_mixed(args, c, p, nodes, dims):
     f32, flux32, flu32, wall, cells = run(True, True)
     f64, flux64, flu64, wall64, cells64 = run(False, False)
     sol.close()
+    oracle_cmp = mixed_vs_oracle(args, c, p, nodes, grids, f0, tdel_days)
+    log("mixed precision against the oracle done")
+    interp_cmp = None if args.no_mas else interp_fp32_vs_oracle(c, p)
     # float storage floors at FLT_MIN: the error of f is stated over the
     # cells inside the workload's range (f >= 1e-30)
     live = f64 >= 1e-30
@@ -251,10 +254,111 @@ def run_mixed(args, c, p, nodes, dims):
         "config": {"workload": "c4: c2 shapes, mixed precision, power-law f spanning %.1e..%.1e"
                                % (f0.min(), f0.max()), "path": "mixed precision"},
         "fp64_same_steps": {"value": cells64 / wall64, "ms_per_step": wall64 * 1e3 / args.steps},
-        "error_vs_fp64": {"f": {"max": float(rel.max()), "median": float(np.median(rel)),
-                                "cells_at_or_above_1e-30": live_frac},
-                          "observer_flux": err(flux32, flux64), "observer_fluence": err(flu32, flu64)},
+        "precision": "f stored as float; streaming (upwind), focusing and MAS interpolation in FP32; "
+                     "relaxation (the pitch-angle part) and the adiabatic change in FP64",
+        "error_vs_fp64_device_all_streams": {
+            "f": {"max": float(rel.max()), "median": float(np.median(rel)), "cells_at_or_above_1e-30": live_frac},
+            "observer_flux": err(flux32, flux64), "observer_fluence": err(flu32, flu64)},
+        "error_vs_fp64_oracle": oracle_cmp,
+        "interpolation_fp32_vs_oracle": interp_cmp,
     }
+
+
+def mixed_vs_oracle(args, c, p, nodes, grids, f0, tdel_days, nsub=16):
+    """The mixed-precision form against the FP64 oracle itself on the first
+    nsub streams of the workload, over the same steps: relative error of f
+    (cells >= 1e-30), of the omnidirectional flux at the three observers
+    (both blended over the same nsub streams) and of its fluence."""
+    import oracle
+
+    cores, _ = host_cores()
+    sub = {k: np.ascontiguousarray(v[:nsub]) for k, v in nodes.items()}
+    sdims = workloads.dims_of(sub, c["E"], c["M"])
+    f0s = np.ascontiguousarray(f0[:nsub])
+    nsteps = args.warmup + args.steps
+    obs = (OBSERVERS["r_au"], OBSERVERS["theta"], OBSERVERS["phi"])
+    fo = f0s.copy()
+    flu_o = np.zeros((3, c["E"]))
+    flux_o = None
+    for _ in range(nsteps):
+        rc, _, fo, _, _, _ = oracle.update_ep(sdims, p, sub, fo, nthreads=cores)
+        if rc != 0:
+            raise RuntimeError("oracle failed: %d" % rc)
+        _, flux_o = oracle.observers(sdims, sub, fo, grids["egrid"], p.rScale, *obs)
+        flu_o += flux_o * tdel_days
+    ss = runtime.EpSolver(sdims, p)
+    try:
+        ss.set_nodes(sub)
+        ss.upload_f(f0s)
+        ss.set_precision(1)
+        flu_m = np.zeros((3, c["E"]))
+        flux_m = None
+        for _ in range(nsteps):
+            ss.step()
+            _, flux_m = ss.observe(*obs)
+            flu_m += flux_m * tdel_days
+        fm = ss.download_f()
+    finally:
+        ss.close()
+    live = fo >= 1e-30
+    rel = np.abs(fm[live] - fo[live]) / fo[live]
+
+    def err(a, b):
+        r = np.abs(a - b) / np.abs(b)
+        return {"max": float(r.max()), "median": float(np.median(r))}
+
+    return {"streams": nsub, "global_steps": nsteps,
+            "f": {"max": float(rel.max()), "median": float(np.median(rel)),
+                  "cells_at_or_above_1e-30": float(live.mean())},
+            "observer_flux": err(flux_m, flux_o), "observer_fluence": err(flu_m, flu_o)}
+
+
+def interp_fp32_vs_oracle(c, p, nsub=16):
+    """MAS trilinear interpolation in FP32 (the mixed-precision form) and in
+    FP64, each against the oracle's masTriLinear, on radial streams from 1.2
+    to 29 Rs (all inside the synthetic MAS domain): relative error of each
+    of the seven outputs, scaled by that output's largest magnitude."""
+    import oracle
+
+    lv0, lv1 = mas_synth.levels(seed=20261203)
+    S = 6 * c["rows"] * c["cols"]
+    N = c["N"]
+    i = np.arange(S) + 0.5
+    ct = 1.0 - 2.0 * i / S
+    ph = np.pi * (1.0 + 5.0 ** 0.5) * i
+    rs = K.RSAU / p.rScale
+    r = np.linspace(1.2, 29.0, N)[None, :] * rs * np.ones((S, 1))
+    st = np.sqrt(1.0 - ct ** 2)
+    x, y, z = r * (st * np.cos(ph))[:, None], r * (st * np.sin(ph))[:, None], r * ct[:, None]
+    nodes = dict(bench_nodes(c), x=x, y=y, z=z, rmag=r)
+    dims = workloads.dims_of(nodes, 2, 2)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_nodes(nodes)
+        sol.mas_set_level(0, *lv0)
+        sol.mas_set_level(1, *lv1)
+        prev = mas_synth.prev_from_nodes(nodes, p.rScale)
+        out = {}
+        for mode in (0, 1):
+            sol.set_precision(mode)
+            out[mode] = sol.interp_mhd(0.5, 0.0, 30.0, 50.0e5, prev)
+    finally:
+        sol.close()
+    sub = slice(0, nsub)
+    ref = oracle.mas_interp(p.rScale, x[sub].ravel(), y[sub].ravel(), z[sub].ravel(), r[sub].ravel(), lv0, lv1,
+                            0.5, 0.0, 30.0, 50.0e5)
+    res = {}
+    for name, mode in (("fp32", 1), ("fp64", 0)):
+        comp = {}
+        for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density"):
+            a = np.asarray(out[mode][k])[sub].ravel()
+            b = ref[k]
+            scale = np.abs(b).max()
+            comp[k] = float(np.abs(a - b).max() / scale) if scale > 0 else float(np.abs(a).max())
+        res[name] = comp
+    res["nodes_compared"] = int(nsub * N)
+    res["note"] = "max |device - oracle| / max |oracle| per output (max |device| where the oracle's output is 0)"
+    return res
 
 
 def log(msg):

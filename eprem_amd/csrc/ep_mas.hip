@@ -27,10 +27,14 @@ struct MasDev {
   // same points (levels and fields usually share meshes): a search is
   // reused while the mesh it ran on stays the same
   int uq[2][21];
+  // the mixed-precision form: search fractions, trilinear stencil and time
+  // blend in FP32 (the unit conversions and the node state stay FP64)
+  int fp32;
 };
 
-__device__ __forceinline__ double mas_bsearch(const float *__restrict__ A, double key, int *imin,
-                                              int *imax, int lower, int upper) {
+template <typename R>
+__device__ __forceinline__ R mas_bsearch(const float *__restrict__ A, R key, int *imin, int *imax, int lower,
+                                         int upper) {
   while (upper != (lower + 1)) {
     const int imid = (lower + upper) / 2;
     if (A[imid] < key)
@@ -40,9 +44,10 @@ __device__ __forceinline__ double mas_bsearch(const float *__restrict__ A, doubl
   }
   *imin = lower;
   *imax = upper;
-  if (A[lower] > key) return 0.0;
-  if (A[upper] < key) return 1.0;
-  return ((key - A[lower]) / (A[upper] - A[lower]));
+  if (A[lower] > key) return (R)0;
+  if (A[upper] < key) return (R)1;
+  // the denominator is an FP32 difference, as in the reference
+  return ((key - (R)A[lower]) / (R)(A[upper] - A[lower]));
 }
 
 // The bracketing indices and fraction of the last search on each axis, and
@@ -55,31 +60,50 @@ struct MasSearch {
 // masInterpolate (src/masInterp.c:177-200) of field k at level lv, the
 // three searches (masTriLinearBinarySearch) reused from `c` when the mesh
 // of an axis is the one searched last.
-__device__ __forceinline__ double mas_field(const MasDev &M, int lv, int k, double rr, double th,
-                                            double ph, MasSearch &c) {
+// R: the arithmetic (double; float in the mixed-precision form).
+template <typename R>
+__device__ __forceinline__ R mas_field(const MasDev &M, int lv, int k, double rr, double th, double ph,
+                                       MasSearch &c) {
   const int nr = M.dims[lv][3 * k], nt = M.dims[lv][3 * k + 1];
-  const double key[3] = {rr, th, ph};
+  const R key[3] = {(R)rr, (R)th, (R)ph};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int u = M.uq[lv][3 * k + a];
     if (u != c.u[a]) {
-      c.fr[a] = mas_bsearch(M.mesh[lv][3 * k + a], key[a], &c.i0[a], &c.i1[a], 0, M.dims[lv][3 * k + a] - 1);
+      c.fr[a] = mas_bsearch<R>(M.mesh[lv][3 * k + a], key[a], &c.i0[a], &c.i1[a], 0, M.dims[lv][3 * k + a] - 1);
       c.u[a] = u;
     }
   }
   const int r0 = c.i0[0], r1 = c.i1[0], t0 = c.i0[1], t1 = c.i1[1], p0 = c.i0[2], p1 = c.i1[2];
-  const double rd = c.fr[0], td = c.fr[1], pd = c.fr[2];
+  const R rd = (R)c.fr[0], td = (R)c.fr[1], pd = (R)c.fr[2];
+  const R one = 1;
   const float *__restrict__ V = M.field[lv][k];
   const size_t snr = nr, snrt = (size_t)nr * nt;
-#define VI(r, t, p) V[(size_t)(r) + snr * (t) + snrt * (p)]
-  const double c00 = VI(r0, t0, p0) * (1.0 - rd) + VI(r1, t0, p0) * rd;
-  const double c10 = VI(r0, t0, p1) * (1.0 - rd) + VI(r1, t0, p1) * rd;
-  const double c01 = VI(r0, t1, p0) * (1.0 - rd) + VI(r1, t1, p0) * rd;
-  const double c11 = VI(r0, t1, p1) * (1.0 - rd) + VI(r1, t1, p1) * rd;
+#define VI(r, t, p) ((R)V[(size_t)(r) + snr * (t) + snrt * (p)])
+  const R c00 = VI(r0, t0, p0) * (one - rd) + VI(r1, t0, p0) * rd;
+  const R c10 = VI(r0, t0, p1) * (one - rd) + VI(r1, t0, p1) * rd;
+  const R c01 = VI(r0, t1, p0) * (one - rd) + VI(r1, t1, p0) * rd;
+  const R c11 = VI(r0, t1, p1) * (one - rd) + VI(r1, t1, p1) * rd;
 #undef VI
-  const double c0 = c00 * (1.0 - pd) + c10 * pd;
-  const double c1 = c01 * (1.0 - pd) + c11 * pd;
-  return (c0 * (1.0 - td) + c1 * td);
+  const R c0 = c00 * (one - pd) + c10 * pd;
+  const R c1 = c01 * (one - pd) + c11 * pd;
+  return (c0 * (one - td) + c1 * td);
+}
+
+// The 7 fields at both levels, blended in time: fields in the order of
+// MasLevel (Bp, Bt, Br, Vp, Vt, Vr, rho), before the unit conversions.
+template <typename R>
+__device__ __forceinline__ void mas_blend(const MasDev &M, double rr, double th, double ph, double s,
+                                          double out[7]) {
+  MasSearch c;
+  c.u[0] = c.u[1] = c.u[2] = -1;
+  const R sr = (R)s, one = 1;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const R a = mas_field<R>(M, 0, k, rr, th, ph, c);
+    const R b = mas_field<R>(M, 1, k, rr, th, ph, c);
+    out[k] = (double)((one - sr) * a + sr * b);
+  }
 }
 
 // MAS unit conversions (src/baseTypes.h:107-115).  The reference's macros
@@ -117,23 +141,20 @@ __device__ void mas_at(const Dev &D, const MasDev &M, double px, double py, doub
       while (ph < 0.0) ph += (2.0 * kPI);
     if (ph > (2.0 * kPI))
       while (ph > (2.0 * kPI)) ph -= (2.0 * kPI);
-    MasSearch c;
-    c.u[0] = c.u[1] = c.u[2] = -1;
-    double fv[2][7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-      fv[0][k] = mas_field(M, 0, k, rr, th, ph, c);
-      fv[1][k] = mas_field(M, 1, k, rr, th, ph, c);
-    }
-    out7[2] = mas_b_apply((1.0 - s) * fv[0][0] + s * fv[1][0]);
-    out7[1] = mas_b_apply((1.0 - s) * fv[0][1] + s * fv[1][1]);
-    out7[0] = mas_b_apply((1.0 - s) * fv[0][2] + s * fv[1][2]);
-    out7[5] = mas_v_apply((1.0 - s) * fv[0][3] + s * fv[1][3]);
-    out7[4] = mas_v_apply((1.0 - s) * fv[0][4] + s * fv[1][4]);
-    double vr = mas_v_apply((1.0 - s) * fv[0][5] + s * fv[1][5]);
+    double bl[7];
+    if (M.fp32)
+      mas_blend<float>(M, rr, th, ph, s, bl);
+    else
+      mas_blend<double>(M, rr, th, ph, s, bl);
+    out7[2] = mas_b_apply(bl[0]);
+    out7[1] = mas_b_apply(bl[1]);
+    out7[0] = mas_b_apply(bl[2]);
+    out7[5] = mas_v_apply(bl[3]);
+    out7[4] = mas_v_apply(bl[4]);
+    double vr = mas_v_apply(bl[5]);
     if (vr < (vmin / 2.99792458e10)) vr = (vmin / 2.99792458e10);
     out7[3] = vr;
-    out7[6] = mas_rho_apply((1.0 - s) * fv[0][6] + s * fv[1][6]);
+    out7[6] = mas_rho_apply(bl[6]);
   } else {
     // masWind: scale the node's previous values by r_old / r
     const double ox = prev[0 * SN + i], oy = prev[1 * SN + i], oz = prev[2 * SN + i];
@@ -329,6 +350,7 @@ static MasDev mas_dev() {
     }
     M.uq[lv][sl] = u;
   }
+  M.fp32 = g.fp32;
   return M;
 }
 

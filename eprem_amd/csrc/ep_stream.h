@@ -197,14 +197,16 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
   }
   const size_t ok = (size_t)task * D.Npad;
   const double dsm = valid ? D.dsm[ok + k] : 0.0;
-  const T q = valid ? (T)D.q[ok + k] : (T)1;
-  const T omq = (T)(valid ? 1.0 - D.q[ok + k] : 0.0);
+  // the relaxation (the pitch-angle part) runs in FP64 also when f is
+  // stored as float
+  const double q = valid ? D.q[ok + k] : 1.0;
+  const double omq = valid ? 1.0 - D.q[ok + k] : 0.0;
   // Stream ends are held: kept node 0 for mu >= 0, the last one for mu < 0.
   const double dsm_pos = (k == 0) ? 0.0 : dsm;
   const double dsm_neg = (k == nk - 1) ? 0.0 : dsm;
   const bool bc = D.useBC && valid && (k < 3);
   const T bcv = bc ? (T)D.seed3[(size_t)task * 3 + k] : (T)0;
-  const T invM = (T)(1.0 / M);
+  const double invM = 1.0 / M;
   // Edge exchange between the waves of the block, double-buffered by
   // subcycle parity.  up[r] holds the mu >= 0 values lane 0 of wave r
   // takes as its upstream neighbour: lane 63 of wave r-1 writes row r, and
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
     for (int m = MPOS; m < M; ++m) eu[m - MPOS] = up[b][w][m - MPOS];
 #pragma unroll
     for (int m = 0; m < MPOS; ++m) ed[m] = dn[b][w + 1][m];
-    T sum = 0;
+    double sum = 0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const T fm = f[m];
@@ -262,16 +264,16 @@ __global__ __launch_bounds__(512) void k_stream_pass(Dev D, int pass, int flip, 
         fn = FAST ? fma(coef[m], fm - nb, fm) : fm + coef[m] * (fm - nb);
       }
       f[m] = fn;
-      sum = sum + fn;
+      sum = sum + (double)fn;
     }
-    const T iso = invM * sum;
+    const double iso = invM * sum;
     if (FAST) {
-      const T r = omq * iso;
+      const double r = omq * iso;
 #pragma unroll
-      for (int m = 0; m < M; ++m) f[m] = fma(q, f[m], r);
+      for (int m = 0; m < M; ++m) f[m] = (T)fma(q, (double)f[m], r);
     } else {
 #pragma unroll
-      for (int m = 0; m < M; ++m) f[m] = iso + (f[m] - iso) * q;
+      for (int m = 0; m < M; ++m) f[m] = (T)(iso + ((double)f[m] - iso) * q);
     }
   }
 
@@ -372,15 +374,16 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
   }
   const size_t ok = (size_t)task * D.Npad;
   const double dsm0 = v0 ? D.dsm[ok + k0] : 0.0, dsm1 = v1 ? D.dsm[ok + k1] : 0.0;
-  const T q0 = v0 ? (T)D.q[ok + k0] : (T)1, q1 = v1 ? (T)D.q[ok + k1] : (T)1;
-  const T omq0 = (T)(v0 ? 1.0 - D.q[ok + k0] : 0.0), omq1 = (T)(v1 ? 1.0 - D.q[ok + k1] : 0.0);
+  // the relaxation runs in FP64 also when f is stored as float
+  const double q0 = v0 ? D.q[ok + k0] : 1.0, q1 = v1 ? D.q[ok + k1] : 1.0;
+  const double omq0 = v0 ? 1.0 - D.q[ok + k0] : 0.0, omq1 = v1 ? 1.0 - D.q[ok + k1] : 0.0;
   // stream ends are held: kept node 0 for mu >= 0, the last one for mu < 0
   const double pos0 = (k0 == 0) ? 0.0 : dsm0, neg0 = (k0 == nk - 1) ? 0.0 : dsm0;
   const double pos1 = (k1 == 0) ? 0.0 : dsm1, neg1 = (k1 == nk - 1) ? 0.0 : dsm1;
   const bool bc0 = D.useBC && v0 && (k0 < 3), bc1 = D.useBC && v1 && (k1 < 3);
   const T bcv0 = bc0 ? (T)D.seed3[(size_t)task * 3 + k0] : (T)0;
   const T bcv1 = bc1 ? (T)D.seed3[(size_t)task * 3 + (bc1 ? k1 : 0)] : (T)0;
-  const T invM = (T)(1.0 / M);
+  const double invM = 1.0 / M;
   const double *__restrict__ del = D.del_fac + (size_t)task * M;
   constexpr bool kCoefRegs = (sizeof(T) == 4) || (M <= 16);
   T cf0[kCoefRegs ? M : 1], cf1[kCoefRegs ? M : 1];
@@ -428,7 +431,7 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
       for (int m = 0; m < MPOS; ++m) dn[b][nw][m] = f1[m];
     }
     __syncthreads();
-    T sum0 = 0, sum1 = 0;
+    double sum0 = 0, sum1 = 0;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const T a0 = f0[m], a1 = f1[m];
@@ -446,22 +449,22 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
       }
       f0[m] = n0;
       f1[m] = n1;
-      sum0 = (m == 0) ? n0 : sum0 + n0;
-      sum1 = (m == 0) ? n1 : sum1 + n1;
+      sum0 = (m == 0) ? (double)n0 : sum0 + (double)n0;
+      sum1 = (m == 0) ? (double)n1 : sum1 + (double)n1;
     }
-    const T iso0 = invM * sum0, iso1 = invM * sum1;
+    const double iso0 = invM * sum0, iso1 = invM * sum1;
     if (FAST) {
-      const T r0 = omq0 * iso0, r1 = omq1 * iso1;
+      const double r0 = omq0 * iso0, r1 = omq1 * iso1;
 #pragma unroll
       for (int m = 0; m < M; ++m) {
-        f0[m] = fma(q0, f0[m], r0);
-        f1[m] = fma(q1, f1[m], r1);
+        f0[m] = (T)fma(q0, (double)f0[m], r0);
+        f1[m] = (T)fma(q1, (double)f1[m], r1);
       }
     } else {
 #pragma unroll
       for (int m = 0; m < M; ++m) {
-        f0[m] = iso0 + (f0[m] - iso0) * q0;
-        f1[m] = iso1 + (f1[m] - iso1) * q1;
+        f0[m] = (T)(iso0 + ((double)f0[m] - iso0) * q0);
+        f1[m] = (T)(iso1 + ((double)f1[m] - iso1) * q1);
       }
     }
   }
@@ -587,20 +590,21 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
     }
   const size_t ok = (size_t)task * D.Npad;
   double pos[R], neg[R];
-  T q[R], omq[R], bcv[R];
+  double q[R], omq[R];  // the relaxation runs in FP64 also when f is stored as float
+  T bcv[R];
   bool bc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const double dsm = vv[r] ? D.dsm[ok + kk[r]] : 0.0;
-    q[r] = vv[r] ? (T)D.q[ok + kk[r]] : (T)1;
-    omq[r] = (T)(vv[r] ? 1.0 - D.q[ok + kk[r]] : 0.0);
+    q[r] = vv[r] ? D.q[ok + kk[r]] : 1.0;
+    omq[r] = vv[r] ? 1.0 - D.q[ok + kk[r]] : 0.0;
     // stream ends are held: kept node 0 for mu >= 0, the last one for mu < 0
     pos[r] = (kk[r] == 0) ? 0.0 : dsm;
     neg[r] = (kk[r] == nk - 1) ? 0.0 : dsm;
     bc[r] = D.useBC && vv[r] && (kk[r] < 3);
     bcv[r] = bc[r] ? (T)D.seed3[(size_t)task * 3 + (bc[r] ? kk[r] : 0)] : (T)0;
   }
-  const T invM = (T)(1.0 / M);
+  const double invM = 1.0 / M;
   const cdouble *del = as_const_space(D.del_fac + (size_t)task * M);
   // edge rows between the waves of the block, double-buffered by subcycle
   // parity: up[r] = the mu >= 0 values the first node of lane 0 of wave r
@@ -637,7 +641,7 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
       for (int m = 0; m < MPOS; ++m) dn[b][nw][m] = f[R - 1][m];
     }
     __syncthreads();
-    T sum[R];
+    double sum[R];
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       T a[R], cf[R], nn[R];
@@ -664,19 +668,19 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         f[r][m] = nn[r];
-        sum[r] = (m == 0) ? nn[r] : sum[r] + nn[r];
+        sum[r] = (m == 0) ? (double)nn[r] : sum[r] + (double)nn[r];
       }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const T iso = invM * sum[r];
+      const double iso = invM * sum[r];
       if (FAST) {
-        const T rr = omq[r] * iso;
+        const double rr = omq[r] * iso;
 #pragma unroll
-        for (int m = 0; m < M; ++m) f[r][m] = fma(q[r], f[r][m], rr);
+        for (int m = 0; m < M; ++m) f[r][m] = (T)fma(q[r], (double)f[r][m], rr);
       } else {
 #pragma unroll
-        for (int m = 0; m < M; ++m) f[r][m] = iso + (f[r][m] - iso) * q[r];
+        for (int m = 0; m < M; ++m) f[r][m] = (T)(iso + ((double)f[r][m] - iso) * q[r]);
       }
     }
   }

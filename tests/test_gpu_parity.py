@@ -1000,6 +1000,51 @@ def test_mixed_precision_error_bounds():
     assert rel_flux.max() < 1e-4, rel_flux.max()
 
 
+def test_mixed_precision_interpolation():
+    """In the mixed-precision form the MAS trilinear interpolation runs in
+    FP32 (search fractions, stencil, time blend; unit conversions in FP64):
+    every output within a few FP32 roundings of the oracle's FP64
+    masTriLinear, scaled by the output's largest magnitude (a mean-free
+    scale: B_theta and B_phi pass through zero), and the FP64 interpolation
+    is back to 1e-13 after switching to precision mode 0."""
+    from eprem_amd import mas_synth
+    r, t, ph = mas_synth.meshes(nr=40, nt=30, nphi=50, rmin=1.0, rmax=30.0)
+    lv0 = mas_synth.level(r, t, ph, cme_r=8.0, seed=3)
+    lv1 = mas_synth.level(r, t, ph, cme_r=12.0, seed=3)
+    nodes = workloads.parker_nodes(2, 2, 80, tdel_days=1.0 / 512, rscale=0.00465047, seed=13, jitter=0.3)
+    p = params(rScale=0.00465047)
+    dims = workloads.dims_of(nodes, 4, 4)
+    prev = mas_synth.prev_from_nodes(nodes, p.rScale)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_nodes(nodes)
+        sol.mas_set_level(0, *lv0)
+        sol.mas_set_level(1, *lv1)
+        sol.set_precision(1)
+        o32 = sol.interp_mhd(0.37, 0.8, 30.0, 50.0e5, prev)
+        sol.set_precision(0)
+        o64 = sol.interp_mhd(0.37, 0.8, 30.0, 50.0e5, prev)
+    finally:
+        sol.close()
+    ref = oracle.mas_interp(p.rScale, nodes["x"].ravel(), nodes["y"].ravel(), nodes["z"].ravel(),
+                            nodes["rmag"].ravel(), lv0, lv1, 0.37, 0.8, 30.0, 50.0e5,
+                            {k: v.ravel() for k, v in prev.items()})
+    inside = (nodes["rmag"].ravel() * p.rScale) <= 30.0 * (6.96e10 / 1.495978707e13)
+    assert inside.mean() > 0.5
+    moved = 0
+    for k in ("br", "btheta", "bphi", "vr", "vtheta", "vphi", "density"):
+        a, b = o32[k].ravel()[inside], ref[k][inside]
+        scale = np.abs(b).max()
+        if scale == 0.0:  # the synthetic wind has no theta components and no V_phi
+            np.testing.assert_array_equal(a, b, err_msg=k)
+        else:
+            err = np.abs(a - b).max() / scale
+            assert err < 1e-5, (k, err)
+            moved += err > 0.0
+        np.testing.assert_allclose(o64[k].ravel(), ref[k], rtol=1e-13, atol=0, err_msg=k)
+    assert moved >= 3  # the FP32 arithmetic is really in use
+
+
 @pytest.mark.parametrize("case", [
     dict(N=64, E=12, M=6, steps=1, tiling=None),
     dict(N=300, E=20, M=30, steps=3, tiling=(128, 8)),
