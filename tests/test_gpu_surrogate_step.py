@@ -157,3 +157,39 @@ def test_no_tolerance_graph_equals_eager_and_counts_add_up():
     assert stg == ste
     # an untrained network moves f away from the plain result
     assert np.max(np.abs(fg - fp) / fp) > 1e-3
+
+
+def test_surrogate_form_at_the_workload_size():
+    """The surrogate form on a six-stream slice of the c5 workload at its
+    per-stream size (2000 nodes, 50 energies, 30 mu, K = 16, the bench's
+    nodes and parameters): with a zero tolerance every qualifying stream
+    falls back and the step equals the plain step bit for bit; with the
+    plain check off, every slab with 16 or more subcycles is advanced by
+    the network and the counters add up."""
+    c = workloads.WORKLOADS["c5"]
+    p = workloads.workload_params(c)
+    nodes = workloads.workload_nodes(c)
+    sub = {k: np.ascontiguousarray(v[:6]) for k, v in nodes.items()}
+    dims = workloads.dims_of(sub, c["E"], c["M"])
+    sol = runtime.EpSolver(dims, p)
+    net = _network(c["M"], scale=0.05)
+    try:
+        sol.set_nodes(sub)
+        sol.init_seed_population()
+        sol.set_precision(2)
+        sol.step()
+        f0 = sol.download_f()
+        fp, _ = _run(sol, f0, 1)
+        fs, st0 = _run(sol, f0, 1, K=c["surrogate"], tol=0.0)
+        fn, st1 = _run(sol, f0, 1, K=c["surrogate"], tol=1e30)
+    finally:
+        net.close()
+        sol.close()
+    np.testing.assert_array_equal(fs, fp)
+    assert st0[0]["network_slabs"] == 0 and st0[0]["fallback_slabs"] > 0
+    s1 = st1[0]
+    total = s1["network_slabs"] + s1["fallback_slabs"] + s1["plain_slabs"]
+    assert total == dims.num_streams * dims.num_energies * p.numEpSteps
+    assert s1["network_slabs"] + s1["fallback_slabs"] == st0[0]["fallback_slabs"]
+    assert s1["network_slabs"] > 0
+    assert np.all(np.isfinite(fn)) and np.all(fn > 0)
