@@ -198,10 +198,12 @@ OBSERVERS = dict(r_au=np.array([0.3, 0.5, 1.0]), theta=np.array([1.2, 1.571, 2.0
 
 def run_mixed(args, c, p, nodes, dims):
     """Mixed-precision form: the timed steps run with f stored as float
-    (FP32 streaming and focusing, FP64 adiabatic change); the same steps are
-    then repeated on the FP64 path (equal to the oracle to 1e-10) to state
-    the error of f, of the omnidirectional flux at three observers and of
-    its fluence (time integral over the run)."""
+    (FP32 streaming, focusing and interpolation; FP64 relaxation and
+    adiabatic change); the same steps are then repeated on the FP64 path to
+    state the error of f, of the omnidirectional flux at three observers and
+    of its fluence (time integral over the run), and on 16 streams against
+    the FP64 oracle itself; then the same for the log-scaled variant (the
+    float storage holding ln f)."""
     grids = None
     sol = runtime.EpSolver(dims, p)
     sol.set_nodes(nodes)
@@ -209,38 +211,53 @@ def run_mixed(args, c, p, nodes, dims):
     f0 = workloads.power_law_f(grids, nodes, c["M"], seed=20261203)
     tdel_days = 1.0 / 64
 
-    def run(mixed, timed):
-        sol.set_precision(False)
-        sol.upload_f(f0)
-        sol.set_precision(mixed)
+    def run_mode(sv, f_init, mode):
+        sv.set_precision(0)
+        sv.upload_f(f_init)
+        sv.set_precision(mode)
         fluence = np.zeros((3, c["E"]))
         wall, cells = 0.0, 0
         flux = None
         for i in range(args.warmup + args.steps):
             t0 = time.perf_counter()
-            _, dg = sol.step()
+            _, dg = sv.step()
             t1 = time.perf_counter()
             if i >= args.warmup:
                 wall += t1 - t0
                 cells += dg["cell_updates"]
-            _, flux = sol.observe(OBSERVERS["r_au"], OBSERVERS["theta"], OBSERVERS["phi"])
+            _, flux = sv.observe(OBSERVERS["r_au"], OBSERVERS["theta"], OBSERVERS["phi"])
             fluence += flux * tdel_days
-        return sol.download_f(), flux, fluence, wall, cells
+        return sv.download_f(), flux, fluence, wall, cells
+
+    def run(mixed, timed):
+        return run_mode(sol, f0, 1 if mixed else 0)
 
     if not (args.block or args.halo):
         sol.autotune()
     f32, flux32, flu32, wall, cells = run(True, True)
     f64, flux64, flu64, wall64, cells64 = run(False, False)
     sol.close()
-    oracle_cmp = mixed_vs_oracle(args, c, p, nodes, grids, f0, tdel_days)
-    log("mixed precision against the oracle done")
-    interp_cmp = None if args.no_mas else interp_fp32_vs_oracle(c, p)
     # float storage floors at FLT_MIN: the error of f is stated over the
     # cells inside the workload's range (f >= 1e-30)
     live = f64 >= 1e-30
     rel = np.abs(f32[live] - f64[live]) / f64[live]
     live_frac = float(live.mean())
-    del f32, f64, live
+    f64k, live64 = f64, live
+    del f32
+    oracle_cmp = mixed_vs_oracle(args, c, p, nodes, grids, f0, tdel_days)
+    log("mixed precision against the oracle done")
+    # the log-scaled variant (precision mode 3: the float storage holds ln f)
+    sol = runtime.EpSolver(dims, p)
+    sol.set_nodes(nodes)
+    if not (args.block or args.halo):
+        sol.autotune()
+    fl, fluxl, flul, wall_l, cells_l = run_mode(sol, f0, 3)
+    sol.close()
+    rel_l = np.abs(fl[live64] - f64k[live64]) / f64k[live64]
+    del fl, f64k, live64
+    log_cmp = mixed_vs_oracle(args, c, p, nodes, grids, f0, tdel_days, mode=3)
+    log("log-scaled variant done")
+    interp_cmp = None if args.no_mas else interp_fp32_vs_oracle(c, p)
 
     def err(a, b):
         r = np.abs(a - b) / np.abs(b)
@@ -261,10 +278,18 @@ def run_mixed(args, c, p, nodes, dims):
             "observer_flux": err(flux32, flux64), "observer_fluence": err(flu32, flu64)},
         "error_vs_fp64_oracle": oracle_cmp,
         "interpolation_fp32_vs_oracle": interp_cmp,
+        "log_scaled_variant": {
+            "storage": "float holding ln f (precision mode 3); the operators as in the mixed form",
+            "value": cells_l / wall_l, "ms_per_step": wall_l * 1e3 / args.steps,
+            "error_vs_fp64_device_all_streams": {"f": {"max": float(rel_l.max()),
+                                                       "median": float(np.median(rel_l))},
+                                                 "observer_flux": err(fluxl, flux64),
+                                                 "observer_fluence": err(flul, flu64)},
+            "error_vs_fp64_oracle": log_cmp},
     }
 
 
-def mixed_vs_oracle(args, c, p, nodes, grids, f0, tdel_days, nsub=16):
+def mixed_vs_oracle(args, c, p, nodes, grids, f0, tdel_days, nsub=16, mode=1):
     """The mixed-precision form against the FP64 oracle itself on the first
     nsub streams of the workload, over the same steps: relative error of f
     (cells >= 1e-30), of the omnidirectional flux at the three observers
@@ -290,7 +315,7 @@ def mixed_vs_oracle(args, c, p, nodes, grids, f0, tdel_days, nsub=16):
     try:
         ss.set_nodes(sub)
         ss.upload_f(f0s)
-        ss.set_precision(1)
+        ss.set_precision(mode)
         flu_m = np.zeros((3, c["E"]))
         flux_m = None
         for _ in range(nsteps):

@@ -280,6 +280,7 @@ Dev make_dev() {
   D.c_bc1 = g.c_bc1;
   D.diag = g.diag;
   D.fphi = nullptr;
+  D.logf = g.logf;
   return D;
 }
 
@@ -364,7 +365,7 @@ static hipError_t enqueue_substeps(const Dev &D, const StepShape &sh, hipStream_
   // stage focuses each row over the fraction of the substep it still needs
   // (Dn); the seed floor then runs as its own kernel, before the network
   const bool sur = sh.sur_levels > 0;
-  const int fuse = (!sur && sh.do_stream && sh.do_focus && D.stream_fuse && D.focus_alg == 1) ? 1 : 0;
+  const int fuse = (!sur && !D.logf && sh.do_stream && sh.do_focus && D.stream_fuse && D.focus_alg == 1) ? 1 : 0;
   const bool rk = D.focus_alg != 1 || D.change_alg != 1;
   const Dev Ds = sur ? sur_stream_dev(D) : D;
   const Dev Dn = sur ? sur_node_dev(D) : D;
@@ -515,6 +516,9 @@ int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes,
     return fail(EP_ERR_INVALID_ARG,
                 "stream tiling: lane_nodes 1, 2 or 3, stream_block a multiple of 64*lane_nodes with at most "
                 "512 lanes, 1 <= stream_halo < stream_block/2, fuse_focus 0 or 1");
+  if (g.logf && (lane_nodes != 2 || fuse_focus != 0))
+    return fail(EP_ERR_UNSUPPORTED, "the log-scaled storage (precision mode 3) streams two nodes per lane, "
+                                    "focusing in its own kernel (lane_nodes 2, fuse_focus 0)");
   g.stream_block = stream_block;
   g.stream_halo = stream_halo;
   g.stream_r = lane_nodes;
@@ -576,7 +580,7 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
   }
   // the node stage as ep_step would run it: focusing alone, or focusing and
   // change in one pass
-  const int tune_node_fused = (g.node_fuse && hd.max_sub_change <= 1) ? 1 : 0;
+  const int tune_node_fused = (g.node_fuse && !g.logf && hd.max_sub_change <= 1) ? 1 : 0;
   struct Cand {
     int W, H, R, F, S;  // S: streaming schedule
     double ms;
@@ -622,17 +626,20 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
   // phase 1 times one node per lane with a launch per pass and two or
   // three with the persistent slab launch; phase 2 the best few with
   // focusing in the last pass and with the other schedule
+  // (the log-scaled storage has the slab kernel with two nodes per lane
+  // only, without fused focusing)
   std::vector<Cand> cands;
   for (int R = 1; R <= 3; R++)
     for (int Lw : lanes)
       for (int H : halos) {
+        if (g.logf && R != 2) continue;
         Cand c{Lw * R, H, R, 0, R > 1 ? 1 : 0, 0.0};
         if (4 * H > c.W) continue;
         if ((rc = time_cand(c))) return rc;
         cands.push_back(c);
       }
   std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-  const size_t nbest = std::min<size_t>(cands.size(), 6);
+  const size_t nbest = g.logf ? 0 : std::min<size_t>(cands.size(), 6);
   for (size_t k = 0; k < nbest; k++) {
     Cand c = cands[k];
     c.F = 1;
@@ -670,24 +677,45 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
 
 // Storage and arithmetic of the hot path: 0 = FP64 in the reference's
 // operation order (the parity path), 1 = f stored as float, streaming and
-// focusing in FP32, adiabatic change (the ln f energy flux) in FP64,
-// 2 = FP64 with fused multiply-adds in the streaming kernel.  The
-// distribution on the device is converted when the storage type changes.
+// focusing in FP32, relaxation and adiabatic change (the ln f energy flux)
+// in FP64, 2 = FP64 with fused multiply-adds in the streaming kernel,
+// 3 = mode 1 with the float storage holding ln f (the log-scaled variant).
+// The distribution on the device is converted when the storage changes.
 int ep_set_precision(int32_t mode) {
   if (!g.init) return fail(EP_ERR_NOT_INIT, "ep_init first");
-  if (mode < 0 || mode > 2)
-    return fail(EP_ERR_INVALID_ARG, "precision mode must be 0 (FP64), 1 (mixed) or 2 (FP64, fused)");
-  const int fp32 = (mode == 1) ? 1 : 0;
+  if (mode < 0 || mode > 3)
+    return fail(EP_ERR_INVALID_ARG,
+                "precision mode must be 0 (FP64), 1 (mixed), 2 (FP64, fused) or 3 (mixed, log-scaled f)");
+  const int fp32 = (mode == 1 || mode == 3) ? 1 : 0;
+  const int logf = (mode == 3) ? 1 : 0;
   int rc = check_algs(g.p, g.d, fp32);
   if (rc) return rc;
-  if (fp32 != g.fp32) {
+  if (fp32 != g.fp32 || logf != g.logf) {
+    // through FP64 storage: decode with the old codec, encode with the new
     Dev D = make_dev();
     HIPCHK(launch_normalize(D, g.st), "normalize");
-    HIPCHK(launch_convert(D, fp32, g.st), "convert");
+    if (g.fp32) HIPCHK(launch_convert(D, 0, g.logf, g.st), "convert");
+    if (fp32) {
+      D.fp32 = 0;
+      HIPCHK(launch_convert(D, 1, logf, g.st), "convert");
+    }
     HIPCHK(hipStreamSynchronize(g.st), "convert sync");
   }
   g.fp32 = fp32;
+  g.logf = logf;
   g.fast = (mode != 0) ? 1 : 0;
+  if (logf) {
+    // the log-scaled storage's streaming kernel: the slab launch with two
+    // nodes per lane (same lane count), focusing in its own kernel
+    if (g.stream_r != 2) {
+      const int lanes = g.stream_block / g.stream_r;
+      g.stream_block = 2 * lanes;
+      g.stream_r = 2;
+      if (2 * g.stream_halo >= g.stream_block) g.stream_halo = g.stream_block / 4;
+    }
+    g.stream_fuse = 0;
+    if (g.stream_sched == 0) g.stream_sched = 1;
+  }
   g.drop_graph();
   return EP_OK;
 }
@@ -695,6 +723,8 @@ int ep_set_precision(int32_t mode) {
 int ep_set_schedule(int32_t schedule) {
   if (schedule < 0 || schedule > 2)
     return fail(EP_ERR_INVALID_ARG, "schedule must be 0 (a launch per pass), 1 (persistent) or 2 (per stream)");
+  if (g.logf && schedule == 0)
+    return fail(EP_ERR_UNSUPPORTED, "the log-scaled storage (precision mode 3) streams with the slab launch");
   g.stream_sched = schedule;
   g.drop_graph();
   return EP_OK;
@@ -844,6 +874,7 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   const int do_drift = (g.shell_drift && (op_flags & EP_OP_DRIFT)) ? 1 : 0;
   if (do_shell || do_drift) {
     if (!g.mhd_state) return fail(EP_ERR_INVALID_ARG, "shell operators need the device MHD state (ep_update_mhd)");
+    if (g.logf) return fail(EP_ERR_UNSUPPORTED, "shell operators do not take the log-scaled storage (precision mode 3)");
     if (do_drift && g.mhd_mode != 0)
       return fail(EP_ERR_UNSUPPORTED, "drift needs the analytic-wind field (ep_update_mhd mode 0)");
     for (int32_t n : g.h_nactive)
@@ -854,6 +885,9 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   if (sur && (do_shell || do_drift))
     return fail(EP_ERR_UNSUPPORTED, "surrogate: shell diffusion and drift are not supported with it");
   const int sched = effective_schedule();
+  if (g.logf && do_stream && (sched == 0 || g.stream_r != 2))
+    return fail(EP_ERR_UNSUPPORTED, "the log-scaled storage (precision mode 3) streams with the slab launch "
+                                    "(schedule 1 or 2) and two nodes per lane");
   // With the persistent streaming launch nothing the host launches depends
   // on the preparation's results: the whole global step (preparation and
   // substeps) is one sequence, captured as one graph, and the host waits
@@ -870,7 +904,8 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   int node_fused;
   StepShape shape;
   if (one_sequence) {
-    node_fused = (g.node_fuse && do_focus && do_change && p.adiabaticFocusAlg == 1 && p.adiabaticChangeAlg == 1) ? 2 : 0;
+    node_fused = (g.node_fuse && !g.logf && do_focus && do_change && p.adiabaticFocusAlg == 1 &&
+                  p.adiabaticChangeAlg == 1) ? 2 : 0;
     shape = StepShape{0, 0, nsub, do_stream, do_focus, do_change, do_seed, do_shell | (do_drift << 1), g.generation,
                       node_fused, sched};
   } else {
@@ -889,8 +924,8 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
     if (sur) HIPCHK(launch_sur_prep(D, st), "surrogate scalars");
     // the surrogate form focuses each row over its own fraction of the
     // substep, which the two-kernel node stage (k_focus) carries
-    node_fused = (g.node_fuse && do_focus && do_change && hd.max_sub_change <= 1 && p.adiabaticFocusAlg == 1 &&
-                  p.adiabaticChangeAlg == 1 && !sur) ? 1 : 0;
+    node_fused = (g.node_fuse && !g.logf && do_focus && do_change && hd.max_sub_change <= 1 &&
+                  p.adiabaticFocusAlg == 1 && p.adiabaticChangeAlg == 1 && !sur) ? 1 : 0;
     shape = StepShape{hd.max_passes, hd.max_chunks, nsub, do_stream, do_focus, do_change, do_seed,
                       do_shell | (do_drift << 1), g.generation, node_fused, sched,
                       sur ? 1 + hd.max_nsteps / g.sur.K : 0};

@@ -126,7 +126,24 @@ struct Dev {
   // focusing kernels that take it are k_focus and k_focus_heavy (the
   // surrogate form runs the two-kernel node stage).
   const double *fphi;
+  int logf;  // f stored as FP32 ln f (precision mode 3); see f_dec / f_enc
 };
+
+// Storage codec of f.  FP64 and plain FP32 storage hold f itself; in
+// precision mode 3 the FP32 storage holds ln f (the log-scaled variant of
+// the mixed-precision form): kernels decode after loading and encode
+// before storing, and compute on f in FP32 as in mode 1.  The hot kernels
+// take the codec as a template argument (LOG: their mode-3 instantiations,
+// so that mode 1 does not carry it); kernels off the hot path test the
+// flag at run time.
+template <bool LOG> __device__ __forceinline__ double f_decL(double v) { return v; }
+template <bool LOG> __device__ __forceinline__ float f_decL(float v) { return LOG ? expf(v) : v; }
+template <bool LOG> __device__ __forceinline__ double f_encL(double v) { return v; }
+template <bool LOG> __device__ __forceinline__ float f_encL(float v) { return LOG ? logf(v) : v; }
+__device__ __forceinline__ double f_dec(const Dev &, double v) { return v; }
+__device__ __forceinline__ float f_dec(const Dev &D, float v) { return D.logf ? expf(v) : v; }
+__device__ __forceinline__ double f_enc(const Dev &, double v) { return v; }
+__device__ __forceinline__ float f_enc(const Dev &D, float v) { return D.logf ? logf(v) : v; }
 
 // Temporal-blocking tiling of the streaming passes of one (stream, energy)
 // slab with `size` kept nodes and `nsteps` subcycles: halo h (subcycles
@@ -257,6 +274,11 @@ template <>
 struct FloorT<float> {
   static constexpr float v = 1.17549435e-38f;
 };
+// ln FLT_MAX and ln of the FP32 floor: the bounds of ln f in the
+// log-scaled storage (f decodes to a finite FP32 value at or above the
+// floor).
+constexpr double kLnFltMax = 88.72283905206835;
+constexpr double kLnFloorF = -87.33654475055310;
 
 // Raw buffer access to one wave-uniform slab: the base address lives in
 // scalar registers (the buffer descriptor) and the per-mu offset in a

@@ -371,8 +371,8 @@ __global__ __launch_bounds__(256) void k_seed_check(Dev D) {
   const T sd = (T)seed_value(D, D.radial[(size_t)s * D.Npad + k], e);
   T *f = fslab<T>(D, D.loc[t], t) + k;
   for (int m = 0; m < D.M; m++) {
-    const T v = f[(size_t)m * D.Npad];
-    if (v < sd) f[(size_t)m * D.Npad] = sd;
+    const T v = f_dec(D, f[(size_t)m * D.Npad]);
+    if (v < sd) f[(size_t)m * D.Npad] = f_enc(D, sd);
   }
 }
 
@@ -409,13 +409,13 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
     const int m = idx / n, shell = idx % n;
     const int r = D.ref[on + shell];
     if (r != shell) {
-      T v = f[(size_t)m * D.Npad + r];
+      T v = f_dec(D, f[(size_t)m * D.Npad + r]);
       if (isnan(v))
         note_error(D, err_key(substep, 0, s, 1, e, m, shell, 0));
       else if (isinf(v))
         note_error(D, err_key(substep, 0, s, 1, e, m, shell, 1));
       if (v < FloorT<T>::v) v = FloorT<T>::v;
-      f[(size_t)m * D.Npad + shell] = v;
+      f[(size_t)m * D.Npad + shell] = f_enc(D, v);
     }
   }
 }
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_stream_finish(Dev D, int substep) {
 // v*f_upwind exactly for finite f, and the zero-flux walls reproduce the
 // operator's boundary rows.
 // ------------------------------------------------------------------------
-template <int M, typename T, bool FASTF>
+template <int M, typename T, bool FASTF, bool LOG = false>
 __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, int spare) {
   if (substep_off(D)) return;
   // blockIdx.x = ((stream, energy), chunk of 256 nodes), chunk fastest:
@@ -462,13 +462,13 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
   const double a = -v * dlnBds - (two / v) * DuParDt;
   T f[M];
 #pragma unroll
-  for (int m = 0; m < M; ++m) f[m] = buf_ld(slab, voff, m * row, T());
+  for (int m = 0; m < M; ++m) f[m] = f_decL<LOG>(buf_ld(slab, voff, m * row, T()));
   int bad = -1, bad_inf = 0;
   EP_FOCUS_ROW(T, M, D, a, b, f, nsub, focus_dtsub(D, task, dtsub), bad, bad_inf, FASTF);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const T val = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
-    if (nsub > 0 || val != f[m]) buf_st(val, out, voff, m * row);
+    if (nsub > 0 || val != f[m]) buf_st(f_encL<LOG>(val), out, voff, m * row);
   }
   if (bad >= 0 && !spare) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
 }
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
 // The row and its split face velocities take ~250 VGPRs at M = 30; capped
 // at 256 (two waves per SIMD, a few spilled scalars) the walk hides more of
 // its load latency than with one wave per SIMD (4% faster on C2).
-template <int M, typename T, bool FASTF>
+template <int M, typename T, bool FASTF, bool LOG = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_focus_heavy(Dev D, int substep) {
   if (substep_off(D)) return;
   const int count = D.diag->focus_heavy;
@@ -509,11 +509,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const double a = -v * dlnBds - (two / v) * DuParDt;
     T f[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) f[m] = fp[(size_t)m * D.Npad];
+    for (int m = 0; m < M; ++m) f[m] = f_decL<LOG>(fp[(size_t)m * D.Npad]);
     int bad = -1, bad_inf = 0;
     EP_FOCUS_ROW(T, M, D, a, b, f, nsub, focus_dtsub(D, task, dtsub), bad, bad_inf, FASTF);
 #pragma unroll
-    for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = (f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m];
+    for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = f_encL<LOG>((f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m]);
     if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
   }
 }
@@ -528,8 +528,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // first sweep takes the log and the last one writes exp back.  A lane
 // needs a handful of registers, so the kernel runs at full occupancy.
 // ------------------------------------------------------------------------
-// FM: ln f and exp through fm_log / fm_exp (tables in LDS).
-template <typename T, bool FM>
+// FM: ln f and exp through fm_log / fm_exp (tables in LDS).  LOG: the
+// float storage holds ln f, which the sweep reads and writes as it is.
+template <typename T, bool FM, bool LOG = false>
 __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
   if (substep_off(D)) return;
   // blockIdx.x = ((stream, mu), chunk of 256 nodes), chunk fastest
@@ -592,7 +593,7 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
       }
       return (double)buf_ld(rs, voff, 0, T());
     };
-    auto cvt = [&](double x) { return first ? (FM ? fm_log(x, fmt) : log(x)) : x; };
+    auto cvt = [&](double x) { return (first && !LOG) ? (FM ? fm_log(x, fmt) : log(x)) : x; };
     double r0 = cvt(ld(0));
     double r1 = cvt(ld(1));
     // read-ahead: kPf energies beyond the window are in flight
@@ -630,7 +631,14 @@ __global__ __launch_bounds__(256) void k_change(Dev D, int substep) {
       }
       const double nv = upd ? cur - dtsub * f1 : cur;
       const rsrc_t rs = slab_of(e);
-      if (fin) {
+      if (fin && LOG) {
+        // f = exp(nv) is finite in FP32 below ln FLT_MAX; the floor in ln f
+        if (bad < 0 && !(nv < kLnFltMax)) {
+          bad = e;
+          bad_inf = isnan(nv) ? 0 : 1;
+        }
+        buf_st((T)fmax(nv, kLnFloorF), rs, voff, 0);
+      } else if (fin) {
         T val = (T)(FM ? fm_exp(nv, fmt) : exp(nv));
         if (bad < 0 && !isfinite(val)) {
           bad = e;
@@ -672,7 +680,8 @@ __global__ __launch_bounds__(256) void k_from_reference(Dev D, const double *sta
   const int e = blockIdx.y, s = s0 + blockIdx.z;
   if (node >= D.N) return;
   const size_t src = (((size_t)blockIdx.z * D.N + node) * D.E + e) * D.M;
-  for (int m = 0; m < D.M; m++) dstbuf[fidx(D, s, e, m, node)] = (T)stage[src + m];
+  for (int m = 0; m < D.M; m++)
+    dstbuf[fidx(D, s, e, m, node)] = (sizeof(T) == 4 && D.logf) ? (T)log(stage[src + m]) : (T)stage[src + m];
 }
 
 template <typename T>
@@ -681,7 +690,11 @@ __global__ __launch_bounds__(256) void k_to_reference(Dev D, const T *srcbuf, do
   const int e = blockIdx.y, s = s0 + blockIdx.z;
   if (node >= D.N) return;
   const size_t dst = (((size_t)blockIdx.z * D.N + node) * D.E + e) * D.M;
-  for (int m = 0; m < D.M; m++) stage[dst + m] = (double)srcbuf[fidx(D, s, e, m, node)];
+  // decoded in FP64: ln f in mode 3 may stand for values below FLT_MIN
+  for (int m = 0; m < D.M; m++) {
+    const T raw = srcbuf[fidx(D, s, e, m, node)];
+    stage[dst + m] = (sizeof(T) == 4 && D.logf) ? exp((double)raw) : (double)raw;
+  }
 }
 
 // Move every slab whose current copy sits in buffer B back to A.
@@ -713,20 +726,26 @@ __global__ __launch_bounds__(256) void k_init_seed(Dev D) {
   const T sd = (T)seed_value(D, rad, e);
   const int t = s * D.E + e;
   T *f = fslab<T>(D, D.loc[t], t) + k;
-  for (int m = 0; m < D.M; m++) f[(size_t)m * D.Npad] = sd;
+  for (int m = 0; m < D.M; m++) f[(size_t)m * D.Npad] = f_enc(D, sd);
 }
 
 // Storage-type change of every slab: the current copy in buffer loc is
 // converted into the other buffer, which becomes current.
+// logf: the float side holds ln f (precision mode 3); the FP64 side
+// always holds f.
 template <typename TI, typename TO>
-__global__ __launch_bounds__(256) void k_convert(Dev D) {
+__global__ __launch_bounds__(256) void k_convert(Dev D, int logf) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y, s = blockIdx.z;
   if (k >= D.Npad) return;
   const int t = s * D.E + e;
   const TI *src = fbufT<TI>(D, D.loc[t]) + (size_t)t * D.M * D.Npad;
   TO *dst = fbufT<TO>(D, D.loc[t] ^ 1) + (size_t)t * D.M * D.Npad;
-  for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad + k] = (TO)src[(size_t)m * D.Npad + k];
+  for (int m = 0; m < D.M; m++) {
+    double v = (double)src[(size_t)m * D.Npad + k];
+    if (logf) v = (sizeof(TI) == 4) ? exp(v) : log(v);
+    dst[(size_t)m * D.Npad + k] = (TO)v;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_flip_all(Dev D) {
@@ -808,7 +827,12 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
 
 #define EP_FOCUS_CASE(MM)                                                                     \
   case MM:                                                                                    \
-    if (D.fp32) {                                                                             \
+    if (D.logf) {                                                                             \
+      hipLaunchKernelGGL((k_focus<MM, float, true, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), \
+                         0, st, D, sub, fused, spare);                                        \
+      if (!spare && heavy)                                                                    \
+        hipLaunchKernelGGL((k_focus_heavy<MM, float, true, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+    } else if (D.fp32) {                                                                      \
       hipLaunchKernelGGL((k_focus<MM, float, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
       if (!spare && heavy)                                                                    \
@@ -849,7 +873,9 @@ hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int sp
 
 #define EP_HEAVY_CASE(MM)                                                                     \
   case MM:                                                                                    \
-    if (D.fp32)                                                                               \
+    if (D.logf)                                                                               \
+      hipLaunchKernelGGL((k_focus_heavy<MM, float, true, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+    else if (D.fp32)                                                                          \
       hipLaunchKernelGGL((k_focus_heavy<MM, float, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
     else if (D.fast)                                                                          \
       hipLaunchKernelGGL((k_focus_heavy<MM, double, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
@@ -877,7 +903,9 @@ hipError_t launch_focus_heavy(const Dev &D, int sub, hipStream_t st) {
 // modes (fast, or float storage) use the table-driven exp/log.
 hipError_t launch_change(const Dev &D, int sub, hipStream_t st) {
   const dim3 grid((D.N + 255) / 256 * D.S * D.M), block(256);
-  if (D.fp32)
+  if (D.logf)
+    hipLaunchKernelGGL((k_change<float, true, true>), grid, block, 0, st, D, sub);
+  else if (D.fp32)
     hipLaunchKernelGGL((k_change<float, true>), grid, block, 0, st, D, sub);
   else if (D.fast)
     hipLaunchKernelGGL((k_change<double, true>), grid, block, 0, st, D, sub);
@@ -926,12 +954,12 @@ hipError_t launch_flip_all(const Dev &D, hipStream_t st) {
   return hipSuccess;
 }
 
-hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st) {
+hipError_t launch_convert(const Dev &D, int to_fp32, int logf, hipStream_t st) {
   const dim3 grid((D.Npad + 255) / 256, D.E, D.S);
   if (to_fp32)
-    hipLaunchKernelGGL((k_convert<double, float>), grid, dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_convert<double, float>), grid, dim3(256), 0, st, D, logf);
   else
-    hipLaunchKernelGGL((k_convert<float, double>), grid, dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_convert<float, double>), grid, dim3(256), 0, st, D, logf);
   EP_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_flip_all, dim3((D.S * D.E + 255) / 256), dim3(256), 0, st, D);
   EP_CHECK_LAUNCH();

@@ -51,7 +51,8 @@ hipError_t launch_from_reference(const Dev &D, const double *stage, double *dstb
 hipError_t launch_to_reference(const Dev &D, const double *srcbuf, double *stage, hipStream_t st, int s0 = 0,
                                int count = -1);
 hipError_t launch_normalize(const Dev &D, hipStream_t st);
-hipError_t launch_convert(const Dev &D, int to_fp32, hipStream_t st);
+// to_fp32: FP64 -> float storage, else back; logf: the float side holds ln f
+hipError_t launch_convert(const Dev &D, int to_fp32, int logf, hipStream_t st);
 hipError_t launch_flip_all(const Dev &D, hipStream_t st);
 hipError_t launch_shell_links(const Dev &D, hipStream_t st);
 hipError_t launch_shell_diffuse(const Dev &D, int sub, hipStream_t st);

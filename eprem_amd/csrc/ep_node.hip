@@ -567,6 +567,7 @@ static void focus_change_t(const Dev &D, int sub, int fused, int spare, hipStrea
     break;
 
 hipError_t launch_focus_change(const Dev &D, int sub, int fused, int spare, hipStream_t st) {
+  if (D.logf) return hipErrorInvalidValue;  // the log-scaled storage runs the two-kernel node stage
   switch (D.M) {
     EP_FC_CASE(2) EP_FC_CASE(3) EP_FC_CASE(4) EP_FC_CASE(5) EP_FC_CASE(6)
     EP_FC_CASE(7) EP_FC_CASE(8) EP_FC_CASE(9) EP_FC_CASE(10) EP_FC_CASE(11)

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_spawn_f(Dev D, const double *spawn_f, i
   const int e = q / D.M, m = q % D.M;
   const int task = s * D.E + e;
   T *row = fbufT<T>(D, D.loc[task]) + ((size_t)task * D.M + m) * D.Npad;
-  row[new_head] = spawn_value<T>(D, spawn_f, s, e, m);
+  row[new_head] = f_enc(D, spawn_value<T>(D, spawn_f, s, e, m));
 }
 
 // Head at slot 0: dst[m][new_head + k] = src[m][k - 1] (old head 0) into
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_ripple_f(Dev D, const double *spawn_f, 
   if (k > 0) {
     for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad + k] = src[(size_t)m * D.Npad + k - 1];
   } else {
-    for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad] = spawn_value<T>(D, spawn_f, s, e, m);
+    for (int m = 0; m < D.M; m++) dst[(size_t)m * D.Npad] = f_enc(D, spawn_value<T>(D, spawn_f, s, e, m));
   }
 }
 

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_obs_project(Dev D, ObsDev O) {
     const int e = c / D.M, m = c % D.M;
     const int task = s * D.E + e;
     const T *fb = fslab<T>(D, D.loc[task], task) + (size_t)m * D.Npad;
-    const double fi = (double)fb[in], fo = (double)fb[out];
+    const double fi = (double)f_dec(D, fb[in]), fo = (double)f_dec(D, fb[out]);
     double v;
     if ((sf >= 0.0) && (sf <= 1.0))
       v = fi * (1.0 - sf) + fo * sf;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_stream_flux(Dev D, double *flux) {
   const int task = s * D.E + e;
   const T *fb = fslab<T>(D, D.loc[task], task) + k;
   double iso = 0.0;
-  for (int m = 0; m < D.M; m++) iso += (double)fb[(size_t)m * D.Npad];
+  for (int m = 0; m < D.M; m++) iso += (double)f_dec(D, fb[(size_t)m * D.Npad]);
   iso /= D.M;
   flux[((size_t)s * D.N + k) * D.E + e] = 2.0 * D.egrid[e] * iso;
 }

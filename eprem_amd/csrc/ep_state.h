@@ -77,6 +77,7 @@ struct State {
   bool use_graph = true;
   int fp32 = 0;  // f stored as float (mixed-precision form)
   int fast = 0;  // streaming with fused multiply-adds
+  int logf = 0;  // float storage holds ln f (precision mode 3)
   unsigned generation = 0;
   hipGraphExec_t graph_exec = nullptr;
   StepShape graph_shape{};

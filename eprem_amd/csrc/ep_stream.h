@@ -542,7 +542,7 @@ __global__ __launch_bounds__(512) void k_stream_pass2(Dev D, int pass, int flip,
 // stream_chunk: the work of one block on chunk c of slab (s, e) in pass
 // `pass`, reading buffer src_buf and writing buffer dst_buf; shared by the
 // one-launch-per-pass kernel and the persistent slab kernel below.
-template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
+template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R, bool LOG = false>
 __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, int pass, int src_buf, int dst_buf,
                                              int do_seed, int substep) {
   const int task = s * D.E + e;
@@ -584,7 +584,7 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       T a = 0;
-      if (vv[r]) a = buf_ld(src, vo[r], m * row, T());
+      if (vv[r]) a = f_decL<LOG>(buf_ld(src, vo[r], m * row, T()));
       if (do_seed && a < sd[r]) a = sd[r];
       f[r][m] = a;
     }
@@ -732,7 +732,7 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
   for (int m = 0; m < M; ++m)
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (st[r]) buf_st(f[r][m], dst, vo[r], m * row);
+      if (st[r]) buf_st(f_encL<LOG>(f[r][m]), dst, vo[r], m * row);
 }
 
 template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(512) void k_stream_passR(Dev D, int pass, int flip,
 // left as it was.  one_stream >= 0: the unbatched form, one launch per
 // stream whose block b takes energy E-1-b of that stream.
 // ------------------------------------------------------------------------
-template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
+template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R, bool LOG = false>
 __global__ __launch_bounds__(512) void k_stream_slab(Dev D, int do_seed, int substep, int tune, int one_stream) {
   if (substep_off(D)) return;
   __shared__ int item_sh;
@@ -784,7 +784,7 @@ __global__ __launch_bounds__(512) void k_stream_slab(Dev D, int do_seed, int sub
       const int src = tune ? (p == 0 ? loc : loc ^ 1) : (loc ^ (p & 1));
       const int dst = tune ? (loc ^ 1) : (src ^ 1);
       for (int c = 0; c < chunks; ++c) {
-        stream_chunk<M, MPOS, T, FAST, FUSE, R>(D, c, e, s, p, src, dst, do_seed && p == 0, substep);
+        stream_chunk<M, MPOS, T, FAST, FUSE, R, LOG>(D, c, e, s, p, src, dst, do_seed && p == 0, substep);
         // the next chunk reuses the edge rows in LDS, and the next pass
         // reads what this one stored (the block's own stores)
         __syncthreads();
@@ -810,10 +810,10 @@ inline int resident_blocks(K kernel, int block, int *cache) {
   return cache[slot];
 }
 
-template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R>
+template <int M, int MPOS, typename T, bool FAST, bool FUSE, int R, bool LOG = false>
 inline void launch_slab_kernel(const Dev &D, int do_seed, int substep, int tune, int one_stream, hipStream_t st) {
   const int L = D.stream_block / R;
-  auto k = k_stream_slab<M, MPOS, T, FAST, FUSE, R>;
+  auto k = k_stream_slab<M, MPOS, T, FAST, FUSE, R, LOG>;
   static int cache[9] = {0};
   int grid;
   if (one_stream >= 0) {
@@ -828,6 +828,12 @@ inline void launch_slab_kernel(const Dev &D, int do_seed, int substep, int tune,
 
 template <int M, int MPOS, bool FUSE, int R>
 inline void stream_slab_tfr(const Dev &D, int do_seed, int substep, int tune, int one_stream, hipStream_t st) {
+  if constexpr (R == 2 && !FUSE) {
+    if (D.logf) {
+      launch_slab_kernel<M, MPOS, float, true, false, 2, true>(D, do_seed, substep, tune, one_stream, st);
+      return;
+    }
+  }
   if (D.fp32)
     launch_slab_kernel<M, MPOS, float, true, FUSE, R>(D, do_seed, substep, tune, one_stream, st);
   else if (D.fast)
@@ -836,11 +842,13 @@ inline void stream_slab_tfr(const Dev &D, int do_seed, int substep, int tune, in
     launch_slab_kernel<M, MPOS, double, false, FUSE, R>(D, do_seed, substep, tune, one_stream, st);
 }
 
-// Slab kernels exist for two and three nodes per lane.
+// Slab kernels exist for two and three nodes per lane; with the
+// log-scaled storage, for two nodes per lane without fused focusing.
 template <int M, int MPOS>
 inline hipError_t stream_slab_t(const Dev &D, int do_seed, int substep, int fuse, int tune, int one_stream,
                                 hipStream_t st) {
   if (D.stream_r != 2 && D.stream_r != 3) return hipErrorInvalidValue;
+  if (D.logf && (D.stream_r != 2 || fuse)) return hipErrorInvalidValue;
   if (D.stream_r == 3) {
     if (fuse)
       stream_slab_tfr<M, MPOS, true, 3>(D, do_seed, substep, tune, one_stream, st);
@@ -866,6 +874,7 @@ inline hipError_t stream_slab_t(const Dev &D, int do_seed, int substep, int fuse
 template <int M, int MPOS, bool FUSE>
 inline hipError_t stream_pass_tf(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                                  hipStream_t st) {
+  if (D.logf) return hipErrorInvalidValue;  // the log-scaled storage runs the slab kernel only
   const dim3 grid(chunks, D.E, D.S), block(D.stream_block / D.stream_r);
   if (D.stream_r == 3) {
     if (D.fp32)

@@ -151,8 +151,11 @@ class EpSolver:
 
     def set_precision(self, mode):
         """0 / False: FP64 in the reference's operation order; 1 / True: FP32
-        streaming and focusing with FP64 adiabatic change; 2: FP64 with fused
-        multiply-adds in streaming."""
+        storage, streaming, focusing and interpolation with FP64 relaxation
+        and adiabatic change; 2: FP64 with fused multiply-adds in streaming;
+        3: mode 1 with ln f in the float storage (streams with the slab
+        launch, two nodes per lane, focusing in its own kernel; setting it
+        moves the tiling there)."""
         _check(lib().ep_set_precision(int(mode)))
 
     def set_graph(self, enable):

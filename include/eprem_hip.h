@@ -472,14 +472,21 @@ int ep_set_graph(int32_t enable);
 int ep_set_node_fusion(int32_t fused);
 
 /* Precision of the hot path: 0 = FP64 in the reference's operation order
- * (default; the parity path), 1 = mixed: f stored as float with streaming,
- * relaxation and focusing in FP32 (FP32 accumulation) and the adiabatic
- * energy change (ln f flux) in FP64, 2 = FP64 with fused multiply-adds
- * in streaming and relaxation (same results to ~1e-15 relative, 6 instead
- * of 9 vector operations per cell and subcycle).  The distribution on the
- * device is converted when the storage type changes; the float floor is
- * FLT_MIN where the reference floors at DBL_MIN.  No counterpart in the
- * reference. */
+ * (default; the parity path), 1 = mixed: f stored as float, streaming
+ * (upwind step) and focusing in FP32 (FP32 accumulation), MAS
+ * interpolation in FP32, the relaxation (pitch-angle part) and the
+ * adiabatic energy change (ln f flux) in FP64, 2 = FP64 with fused
+ * multiply-adds in streaming and relaxation (same results to ~1e-15
+ * relative, 6 instead of 9 vector operations per cell and subcycle), 3 =
+ * mode 1 with the float storage holding ln f instead of f (the log-scaled
+ * variant; no shell operators).  Mode 3 streams with the slab launch, two
+ * nodes per lane, and runs focusing and change as two kernels: setting it
+ * moves the tiling to two nodes per lane (same lane count), switches
+ * fused focusing off and the schedule to 1 if it was 0, and ep_set_tuning
+ * and ep_set_schedule reject other choices while it is set.  The
+ * distribution on the device is
+ * converted when the storage changes; the float floor is FLT_MIN where the
+ * reference floors at DBL_MIN.  No counterpart in the reference. */
 int ep_set_precision(int32_t mode);
 
 /* Device self-test of the cross-lane shift the streaming kernel uses: for

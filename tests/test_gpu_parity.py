@@ -1000,6 +1000,71 @@ def test_mixed_precision_error_bounds():
     assert rel_flux.max() < 1e-4, rel_flux.max()
 
 
+def test_log_scaled_storage():
+    """Precision mode 3 (mode 1 with the float storage holding ln f): the
+    same operators on f decoded from FP32 ln f.  Against the FP64 oracle on
+    the 35-decade power law, f and the observer flux stay within the bounds
+    FP32 ln f can hold (2^-24 |ln f| per rounding, ~4e-6 relative at
+    |ln f| = 70, several roundings per step); upload and download keep
+    values far below FLT_MIN; switching modes converts through FP64."""
+    p = params(rScale=0.00465047, numEpSteps=3)
+    nodes = workloads.parker_nodes(1, 1, 200, rscale=0.00465047, seed=41, perturb=0.1, smooth=20,
+                                   jitter=0.02)
+    E, M = 20, 30
+    dims = workloads.dims_of(nodes, E, M)
+    g = oracle.build_grids(p, E, M)
+    f0 = workloads.power_law_f(g, nodes, M, seed=3)
+    r_au = np.array([0.3, 0.5, 0.8])
+    th = np.array([1.0, 1.6, 2.2])
+    ph = np.array([0.5, 2.0, 4.0])
+    tiny = f0.copy()
+    tiny[0, 5, 3, :] = 1e-300
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_nodes(nodes)
+        sol.set_precision(3)
+        sol.upload_f(tiny)
+        back = sol.download_f()
+        sol.set_precision(1)          # ln f -> f (float), through FP64
+        lin = sol.download_f()
+        sol.set_precision(3)
+        # mode 3 has the slab launch with two nodes per lane, focusing in
+        # its own kernel: other tilings and the per-pass schedule are refused
+        assert sol.schedule() in (1, 2)
+        with pytest.raises(runtime.EpError):
+            sol.set_schedule(0)
+        with pytest.raises(runtime.EpError):
+            sol.set_tuning(256, 16, lane_nodes=1, fuse_focus=0)
+        with pytest.raises(runtime.EpError):
+            sol.set_tuning(512, 16, lane_nodes=2, fuse_focus=1)
+        sol.upload_f(f0)
+        for _ in range(2):
+            sol.step()
+        f3 = sol.download_f()
+        _, flux3 = sol.observe(r_au, th, ph)
+        sol.set_precision(0)
+        back64 = sol.download_f()
+    finally:
+        sol.close()
+    np.testing.assert_allclose(back, tiny, rtol=1e-4)  # 1e-300 survives the log storage
+    keep = tiny > 1e-30  # linear float storage cannot hold the 1e-300 row
+    np.testing.assert_allclose(lin[keep], tiny[keep], rtol=1e-5)
+    assert np.all(lin[~keep] == 0.0)
+    np.testing.assert_allclose(back64, f3, rtol=0, atol=0)
+    fo = f0.copy()
+    for _ in range(2):
+        rc, _, fo, _, _, _ = oracle.update_ep(dims, p, nodes, fo)
+        assert rc == 0
+    _, flux64 = oracle.observers(dims, nodes, fo, g["egrid"], p.rScale, r_au, th, ph, 3.0)
+    live = fo >= 1e-30
+    rel_f = np.abs(f3[live] - fo[live]) / fo[live]
+    rel_flux = np.abs(flux3 - flux64) / flux64
+    print("log-scaled storage vs the FP64 oracle: f median %.2e max %.2e, flux max %.2e"
+          % (np.median(rel_f), rel_f.max(), rel_flux.max()))
+    assert np.median(rel_f) < 1e-4 and rel_f.max() < 1e-2, (np.median(rel_f), rel_f.max())
+    assert rel_flux.max() < 1e-3, rel_flux.max()
+
+
 def test_mixed_precision_interpolation():
     """In the mixed-precision form the MAS trilinear interpolation runs in
     FP32 (search fractions, stencil, time blend; unit conversions in FP64):
