@@ -146,19 +146,22 @@ static int check_algs(const ep_params &p, const ep_dims &d, int fp32) {
   return EP_OK;
 }
 
-// Tables of fm_exp / fm_log: 2^(j/64), 1/F and ln F (F = 1 + j/128) as
-// double hi + lo pairs, formed in x87 extended precision.
+// Tables of fm_exp / fm_log: 2^(j/64) and ln F (F = 1 + j/128) as double
+// hi + lo pairs, and 1/F, formed in x87 extended precision.  ln F's high
+// part is ln F rounded to a multiple of 2^-42 (fm_log adds it to m ln2_hi
+// exactly).
 static void build_fm_table(double *t) {
   for (int j = 0; j < kExpN; j++) {
     const long double v = exp2l((long double)j / kExpN);
     t[kFmExpHi + j] = (double)v;
     t[kFmExpLo + j] = (double)(v - (long double)t[kFmExpHi + j]);
   }
+  const long double grid = ldexpl(1.0L, 42);
   for (int j = 0; j <= kLogN; j++) {
     const long double F = 1.0L + (long double)j / kLogN;
     const long double l = logl(F);
     t[kFmLogInv + j] = (double)(1.0L / F);
-    t[kFmLogHi + j] = (double)l;
+    t[kFmLogHi + j] = (double)(roundl(l * grid) / grid);
     t[kFmLogLo + j] = (double)(l - (long double)t[kFmLogHi + j]);
   }
 }

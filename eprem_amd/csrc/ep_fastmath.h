@@ -8,7 +8,9 @@
 //            (truncation < 4e-18);
 //   log(x) = m ln2 + ln F + ln(1 + u),  x = 2^m Y, Y in [1, 2),
 //            F = 1 + j/128 nearest to Y, u = (Y - F)/F, |u| <= 1/256,
-//            ln(1 + u) by a degree-6 polynomial (truncation < 3e-18).
+//            ln(1 + u) by a degree-6 polynomial (truncation < 3e-18),
+//            ln F and ln 2 each split into a high part on a 2^-42 grid
+//            and the rest.
 //
 // The functions are __host__ __device__ and use only correctly rounded
 // operations (fma, rint, frexp, ldexp, integer ops), so the host build
@@ -35,17 +37,20 @@ __host__ __device__ inline uint32_t fm_hi_word(double x) {
   return (uint32_t)(__builtin_bit_cast(uint64_t, x) >> 32);
 }
 
-// exp(x); t = the table
+// exp(x); t = the table.  k = round(64 x / ln 2) is clamped before the
+// integer conversion (defined for every x, NaN included); with the
+// clamped k, x = +inf and every finite x above 709.79 reach infinity
+// through the polynomial or ldexp, x below -745.2 (where the clamped
+// reduction can overflow the polynomial) and -inf give 0, and NaN stays NaN.
 __host__ __device__ inline double fm_exp(double x, const double *t) {
   const double inv = 92.332482616893656768;          // 64 / ln 2
   const double l1 = 0.010830424696223417;            // ln2/64 to 36 bits: kd * l1 exact
   const double l2 = 2.572804622327669e-14;           // ln2/64 - l1
-  const double kd = rint(x * inv);
-  const double kc = fmin(fmax(kd, -80000.0), 80000.0);  // keeps the int conversion defined
-  const int k = (int)kc;
+  const double kd = fmin(fmax(rint(x * inv), -80000.0), 80000.0);
+  const int k = (int)kd;
   const double r = fma(-kd, l2, fma(-kd, l1, x));
   const int j = k & (kExpN - 1);
-  const int m = (k - j) / kExpN;
+  const int m = k >> 6;  // floor(k / 64): k = 64 m + j
   double q = fma(r, 1.0 / 720.0, 1.0 / 120.0);
   q = fma(r, q, 1.0 / 24.0);
   q = fma(r, q, 1.0 / 6.0);
@@ -54,14 +59,15 @@ __host__ __device__ inline double fm_exp(double x, const double *t) {
   const double th = t[kFmExpHi + j], tl = t[kFmExpLo + j];
   const double s = th + fma(th, p, tl);
   double y = ldexp(s, m);  // exact scaling, or correctly rounded into the subnormals / inf
-  if (x > 709.8) y = INFINITY;
   if (x < -745.2) y = 0.0;
   return y;
 }
 
-// log(x) for x > 0 (normal or not)
+// log(x) for x > 0 (normal or not).  ln F's high part is a multiple of
+// 2^-42 and so is ln2_hi, so m ln2_hi + ln F_hi is exact for every
+// exponent m (Tang's split) and needs no error-free sum.
 __host__ __device__ inline double fm_log(double x, const double *t) {
-  const double ln2_hi = 0.6931471805598903;           // ln 2 to 42 bits: m * ln2_hi exact
+  const double ln2_hi = 0.6931471805598903;           // ln 2 to a multiple of 2^-42
   const double ln2_lo = 5.497923018708371e-14;        // ln 2 - ln2_hi
   int e;
   const double y = frexp(x, &e);  // x = y 2^e, y in [0.5, 1)
@@ -78,15 +84,10 @@ __host__ __device__ inline double fm_log(double x, const double *t) {
   q = fma(u, q, -0.5);
   const double p = fma(u * u, q, u);  // ln(1 + u)
   const double md = (double)m;
-  // hi = m ln2_hi + ln F_hi with its rounding error recovered (two-sum)
-  const double a = md * ln2_hi, b = t[kFmLogHi + j];
-  const double hi = a + b;
-  const double bv = hi - a;
-  const double err = (a - (hi - bv)) + (b - bv);
-  const double lo = fma(md, ln2_lo, t[kFmLogLo + j]) + (p + err);
+  const double hi = fma(md, ln2_hi, t[kFmLogHi + j]);  // exact
+  const double lo = fma(md, ln2_lo, t[kFmLogLo + j]) + p;
   double r = hi + lo;
-  if (x == INFINITY) r = INFINITY;
-  if (x != x) r = x;
+  if (!(x <= 1.7976931348623157e308)) r = x;  // inf -> inf, NaN -> NaN
   return r;
 }
 

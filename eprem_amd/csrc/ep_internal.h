@@ -274,6 +274,31 @@ template <>
 struct FloorT<float> {
   static constexpr float v = 1.17549435e-38f;
 };
+// max(x, fl) as a single v_max: fmax adds a NaN-quieting v_max on every
+// value the compiler cannot prove quiet (a loaded one, say).  A NaN x
+// gives fl, as fmax does; the kernels report non-finite values before
+// they floor them.
+__device__ __forceinline__ double floor_max(double x, double fl) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(x), "s"(fl));
+  return r;
+}
+__device__ __forceinline__ float floor_max(float x, float fl) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(x), "s"(fl));
+  return r;
+}
+
+// A face velocity split by the side its focusing flux is taken from:
+// vh = min(va, 0), vl = va - vh, so vl = va for va >= 0 and +0 otherwise,
+// and vl + vh = va exactly.  Two instructions where sign-bit selects take
+// six.  A face then carries fma(vh, f_above, vl * f_below).
+template <typename T>
+__device__ __forceinline__ void split_face(T va, T &vl, T &vh) {
+  vh = fmin(va, (T)0);
+  vl = va - vh;
+}
+
 // ln FLT_MAX and ln of the FP32 floor: the bounds of ln f in the
 // log-scaled storage (f decodes to a finite FP32 value at or above the
 // floor).

@@ -67,7 +67,14 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
   // per mu (padded to 2 MH + 1 with zeros): mu, mu^2, 0.5 (1 - mu^2)
   __shared__ double smu[2 * MH + 2], smu2[2 * MH + 2], shc[2 * MH + 2];
   __shared__ double fmt[FM ? kFmTable : 1];
-  if (threadIdx.x < E) sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+  // per energy: v and 2/v (the same correctly rounded quotient each lane
+  // would form)
+  __shared__ double sv[kMaxE], s2v[kMaxE];
+  if (threadIdx.x < E) {
+    sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+    sv[threadIdx.x] = D.vgrid[threadIdx.x];
+    s2v[threadIdx.x] = 2.0 / D.vgrid[threadIdx.x];
+  }
   if (threadIdx.x < 2 * MH + 2) {
     const int m = threadIdx.x;
     const double mu = (m < M) ? D.mugrid[m] : 0.0;
@@ -95,7 +102,7 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
   const double nmb = DlnNDt - DlnBDt;
   const bool upd = nsub_c > 0;
   const double dlnp_i = 1.0 / D.dlnp;
-  const double half = 0.5, two = 2.0;
+  const double half = 0.5;
   const double *__restrict__ vgi = D.vgrid_i;
   constexpr uint32_t B = sizeof(T);
   const uint32_t slab_bytes = (uint32_t)((size_t)M * D.Npad * B);
@@ -147,7 +154,7 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
     }
 #pragma unroll
     for (int j = 0; j < MH; ++j)
-      if (live(j)) buf_st(fmax(val[j], FloorT<T>::v), out, voff, j * row);
+      if (live(j)) buf_st(floor_max(val[j], FloorT<T>::v), out, voff, j * row);
   };
   for (int e = 0; e < E; ++e) {
     const rsrc_t in = slab_of(e, 0);
@@ -157,8 +164,8 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
     if (foc) {
       // ---- AdiabaticFocusing of (node, e).  Half-mesh velocity va at the
       // interface below slot j (j = MH: above the last slot), split by the
-      // side the flux is taken from: vl = va when va >= 0 (+0 included),
-      // else 0; vh = va when va < 0 (or -0), else 0.  An interior face then
+      // side the flux is taken from (split_face: vh = min(va, 0), vl = va -
+      // vh, one of them an exact zero).  An interior face then
       // carries fma(vh, f_above, vl * f_below): one product plus an exact
       // zero, which is the reference's v 0.5 ((1 - c) f_above + (1 + c)
       // f_below) (src/energeticParticles.c:1830-1836): equal to v * f_upwind
@@ -170,11 +177,12 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
       constexpr bool LW = MH >= 2;
       constexpr bool UW = M - 1 > MH;
       constexpr int JWU = UW ? M - 2 - MH : 0;
-      const double v = D.vgrid[e];
-      const double a = -v * dlnBds - (two / v) * fDuPar;
+      const double v = sv[e];
+      const double a = -v * dlnBds - s2v[e] * fDuPar;
       const T dmu_i = (T)(1.0 / D.dmu);
       const T dts = (T)dtsub_f;
       const T sc = dts * dmu_i;
+      const T hsc = (T)half * sc;
       T vl[MH + 1], vh[MH + 1];
       T vw = 0;  // this lane's wall face velocity
       double vp = (m0 > 0) ? shc[m0 - 1] * (a + smu[m0 - 1] * fb) : 0.0;
@@ -182,11 +190,11 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
       for (int j = 0; j <= MH; ++j) {
         const int m = m0 + j;
         const double vel = shc[m] * (a + smu[m] * fb);
-        T va = (T)(half * (vel + vp));
-        if (FM) va = va * sc;  // fused modes: velocities in units of dmu / dt_sub
+        // fused modes: velocities in units of dmu / dt_sub; (vel + vp) (sc/2)
+        // rounds to the same value as ((vel + vp)/2) sc
+        const T va = FM ? (T)(vel + vp) * hsc : (T)(half * (vel + vp));
         vp = vel;
-        vl[j] = signbit(va) ? (T)0 : va;
-        vh[j] = signbit(va) ? va : (T)0;
+        split_face(va, vl[j], vh[j]);
         if (LW && j == 1) vw = hf ? vw : va;
         if (UW && j == JWU + 1) vw = hf ? va : vw;
       }
@@ -241,7 +249,7 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
           }
       }
 #pragma unroll
-      for (int j = 0; j < MH; ++j) x[j] = fmax(x[j], FloorT<T>::v);
+      for (int j = 0; j < MH; ++j) x[j] = floor_max(x[j], FloorT<T>::v);
     }
     // ---- AdiabaticChange, one sweep: finish energy e-1 (and e = E-1).
     // The energy flux takes the upwind cell alone: the reference's form
@@ -330,7 +338,14 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
   // per mu (padded with zeros): mu, mu^2, 0.5 (1 - mu^2)
   __shared__ double smu[G * MG + 2], smu2[G * MG + 2], shc[G * MG + 2];
   __shared__ double fmt[FM ? kFmTable : 1];
-  if (threadIdx.x < E) sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+  // per energy: v and 2/v (the same correctly rounded quotient each lane
+  // would form)
+  __shared__ double sv[kMaxE], s2v[kMaxE];
+  if (threadIdx.x < E) {
+    sloc[threadIdx.x] = D.loc[s * E + threadIdx.x];
+    sv[threadIdx.x] = D.vgrid[threadIdx.x];
+    s2v[threadIdx.x] = 2.0 / D.vgrid[threadIdx.x];
+  }
   if (threadIdx.x < G * MG + 2) {
     const int m = threadIdx.x;
     const double mu = (m < M) ? D.mugrid[m] : 0.0;
@@ -356,7 +371,7 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
   const double nmb = DlnNDt - DlnBDt;
   const bool upd = nsub_c > 0;
   const double dlnp_i = 1.0 / D.dlnp;
-  const double half = 0.5, two = 2.0;
+  const double half = 0.5;
   const double *__restrict__ vgi = D.vgrid_i;
   constexpr uint32_t B = sizeof(T);
   const uint32_t slab_bytes = (uint32_t)((size_t)M * D.Npad * B);
@@ -401,7 +416,7 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
     }
 #pragma unroll
     for (int j = 0; j < MG; ++j)
-      if (live(j)) buf_st(fmax(val[j], FloorT<T>::v), out, voff, j * row);
+      if (live(j)) buf_st(floor_max(val[j], FloorT<T>::v), out, voff, j * row);
   };
   for (int e = 0; e < E; ++e) {
     const rsrc_t in = slab_of(e, 0);
@@ -413,22 +428,23 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
       // velocities split by upwind side (vl, vh), interior faces
       // fma(vh, f_above, vl * f_below), the wall rows' faces (interface 1
       // above cell 0, interface M-1 below cell M-1) the upwind cell alone.
-      const double v = D.vgrid[e];
-      const double a = -v * dlnBds - (two / v) * fDuPar;
+      const double v = sv[e];
+      const double a = -v * dlnBds - s2v[e] * fDuPar;
       const T dmu_i = (T)(1.0 / D.dmu);
       const T dts = (T)dtsub_f;
       const T sc = dts * dmu_i;
+      const T hsc = (T)half * sc;
       T vl[MG + 1], vh[MG + 1];  // face below slot j (j = MG: above the last slot)
       double vp = (m0 > 0) ? shc[m0 - 1] * (a + smu[m0 - 1] * fb) : 0.0;
 #pragma unroll
       for (int j = 0; j <= MG; ++j) {
         const int m = m0 + j;
         const double vel = shc[m] * (a + smu[m] * fb);
-        T va = (T)(half * (vel + vp));
-        if (FM) va = va * sc;  // fused modes: velocities in units of dmu / dt_sub
+        // fused modes: velocities in units of dmu / dt_sub; (vel + vp) (sc/2)
+        // rounds to the same value as ((vel + vp)/2) sc
+        const T va = FM ? (T)(vel + vp) * hsc : (T)(half * (vel + vp));
         vp = vel;
-        vl[j] = signbit(va) ? (T)0 : va;
-        vh[j] = signbit(va) ? va : (T)0;
+        split_face(va, vl[j], vh[j]);
       }
       for (int it = 0; it < nsub_f; ++it) {
         // cell m0-1 (last of the group below), cell m0+MG (first above)
@@ -479,7 +495,7 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
           }
       }
 #pragma unroll
-      for (int j = 0; j < MG; ++j) x[j] = fmax(x[j], FloorT<T>::v);
+      for (int j = 0; j < MG; ++j) x[j] = floor_max(x[j], FloorT<T>::v);
     }
     // ---- AdiabaticChange, one sweep: finish energy e-1 (and e = E-1),
     // as in k_focus_change

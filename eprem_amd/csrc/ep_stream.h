@@ -76,9 +76,9 @@ __device__ __forceinline__ float dpp_shl1(float v, float old) {
 
 // The subcycles of AdiabaticFocusing on one mu row f[M] (the upwind
 // operator, src/energeticParticles.c:1806-1896).  The half-mesh velocity at
-// the face below cell m is split by the side the flux is taken from: vl_ =
-// va when va >= 0 (+0 included), else 0; vh_ = va when va < 0 (or -0), else
-// 0.  An interior face then carries fma(vh_, f[m], vl_ * f[m-1]): one
+// the face below cell m is split by the side the flux is taken from
+// (split_face: vh_ = min(va, 0), vl_ = va - vh_, so one of them is an
+// exact zero).  An interior face then carries fma(vh_, f[m], vl_ * f[m-1]): one
 // product plus an exact zero, which is the reference's v 0.5 ((1 - c) f[m]
 // + (1 + c) f[m-1]) (:1830-1836): equal to v * f_upwind for finite f and
 // NaN when either cell is NaN or Inf, with no select.  The wall rows
@@ -100,8 +100,7 @@ __device__ __forceinline__ float dpp_shl1(float v, float old) {
       if (m > 0) {                                                                 \
         T va_ = (T)(0.5 * (vel + vprev_));                                         \
         if (FASTF) va_ = va_ * sc_;                                                \
-        vl_[m] = signbit(va_) ? (T)0 : va_;                                        \
-        vh_[m] = signbit(va_) ? va_ : (T)0;                                        \
+        split_face(va_, vl_[m], vh_[m]);                                           \
         if (m == 1) vw0_ = va_;                                                    \
         if (m == (M) - 1) vw1_ = va_;                                              \
       }                                                                            \
