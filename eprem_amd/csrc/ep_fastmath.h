@@ -87,7 +87,8 @@ __host__ __device__ inline double fm_log(double x, const double *t) {
   const double hi = fma(md, ln2_hi, t[kFmLogHi + j]);  // exact
   const double lo = fma(md, ln2_lo, t[kFmLogLo + j]) + p;
   double r = hi + lo;
-  if (!(x <= 1.7976931348623157e308)) r = x;  // inf -> inf, NaN -> NaN
+  if (x == INFINITY) r = INFINITY;
+  if (x != x) r = x;
   return r;
 }
 
