@@ -473,6 +473,16 @@ __global__ __launch_bounds__(256) void k_focus(Dev D, int substep, int fused, in
   if (bad >= 0 && !spare) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
 }
 
+// Blocks of the heavy-focusing walk: enough waves to fill the chip, a
+// multiple of the 8 XCDs (k_focus_heavy gives each XCD a range of the list).
+constexpr int kHeavyGrid = 4096;
+static_assert(kHeavyGrid % 8 == 0, "k_focus_heavy splits its grid over 8 XCDs");
+// Longest heavy-node list k_focus_heavy4 takes (longer lists go to
+// k_focus_heavy), and its grid: up to 4 waves per node.
+constexpr int kHeavy4Max = 2048;
+constexpr int kHeavy4Grid = 2048;
+static_assert(kHeavy4Grid % 8 == 0, "k_focus_heavy4 splits its grid over 8 XCDs");
+
 // k_focus_heavy: the nodes k_focus leaves out (nsub_f >= kHeavyFocus), one
 // wave per node and one lane per energy, so every lane runs the node's
 // subcycle count.  A fixed grid walks the list built by k_node_prep, whose
@@ -489,6 +499,7 @@ template <int M, typename T, bool FASTF, bool LOG = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_focus_heavy(Dev D, int substep) {
   if (substep_off(D)) return;
   const int count = D.diag->focus_heavy;
+  if (M >= 12 && count <= kHeavy4Max) return;  // k_focus_heavy4
   const int e = threadIdx.x;
   const double two = 2.0;
   constexpr int kXcd = 8;
@@ -516,6 +527,148 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     for (int m = 0; m < M; ++m) fp[(size_t)m * D.Npad] = f_encL<LOG>((f[m] < FloorT<T>::v) ? FloorT<T>::v : f[m]);
     if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
   }
+}
+
+// k_focus_heavy4: the same walk with four lanes per (node, energy) row
+// (for M >= 12): lanes l, l+16, l+32, l+48 of a wave hold mu cells q MG ..
+// q MG + MG - 1 (q = l / 16, MG = ceil(M/4), the last group padded) of
+// energy eg 16 + (l mod 16), and the wave takes the node's energies 16 at a
+// time.  A subcycle's chain is then MG cells and two cross-lane shifts
+// instead of M cells, which is what bounds a node with hundreds of
+// subcycles; a lane needs about a quarter of the registers.  The
+// arithmetic is k_focus_change4's (face velocities split by split_face,
+// interior faces fma(vh, f_above, vl * f_below), the wall rows' faces the
+// upwind cell alone), so the values are those of k_focus_heavy and k_focus
+// bit for bit.  Units (node, group of 16 energies) are walked in list
+// order, one contiguous range per XCD, as in k_focus_heavy.
+template <int M, typename T, bool FASTF, bool LOG = false>
+__global__ __launch_bounds__(64) void k_focus_heavy4(Dev D, int substep) {
+  if (substep_off(D)) return;
+  constexpr int G = 4, W = 64 / G;
+  constexpr int MG = (M + G - 1) / G;
+  static_assert(M >= 12, "four lanes per row from 12 mu cells");
+  constexpr int QL = (M - 1) / MG;       // group of cell M-1
+  constexpr int JL = (M - 1) - QL * MG;  // its slot
+  static_assert(QL == G - 1, "cell M-1 in the last group");
+  __shared__ double smu[G * MG + 2], shc[G * MG + 2], s2v[kMaxE], sv[kMaxE];
+  const int E = D.E;
+  if (threadIdx.x < G * MG + 2) {
+    const int m = threadIdx.x;
+    const double mu = (m < M) ? D.mugrid[m] : 0.0;
+    smu[m] = mu;
+    shc[m] = 0.5 * (1.0 - mu * mu);
+  }
+  if (threadIdx.x < E) {
+    sv[threadIdx.x] = D.vgrid[threadIdx.x];
+    s2v[threadIdx.x] = 2.0 / D.vgrid[threadIdx.x];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const int q = lane / W, m0 = q * MG;
+  const int EW = (E + W - 1) / W;  // waves of 16 rows per node
+  if (D.diag->focus_heavy > kHeavy4Max) return;  // k_focus_heavy
+  const int count = D.diag->focus_heavy * EW;
+  constexpr int kXcd = 8;
+  const int per = gridDim.x / kXcd;  // the grid is a multiple of 8 blocks
+  const int xcd = blockIdx.x % kXcd, local = blockIdx.x / kXcd;
+  const int seg = (count + kXcd - 1) / kXcd;
+  const int end = min(count, (xcd + 1) * seg);
+  auto live = [&](int j) { return (j <= JL) || (q < G - 1); };
+  const double half = 0.5;
+  const T dmu_i = (T)(1.0 / D.dmu);
+  for (int u = xcd * seg + local; u < end; u += per) {
+    const size_t i = (size_t)D.heavy[u / EW];
+    const int e = (u % EW) * W + (lane % W);
+    const bool ok = e < E;  // the four lanes of a row agree
+    const int s = (int)(i / D.Npad), node = (int)(i % D.Npad);
+    const int nsub = D.nsub_f[i];  // the node's: the same in every lane
+    const double dlnBds = D.f_dlnbds[i], DuParDt = D.f_dupar[i], fb = D.f_b[i];
+    const int ec = ok ? e : 0;
+    const int task = s * E + ec;
+    T *fp = fslab<T>(D, D.loc[task], task) + node + (size_t)m0 * D.Npad;
+    T x[MG];
+#pragma unroll
+    for (int j = 0; j < MG; ++j) x[j] = (ok && live(j)) ? f_decL<LOG>(fp[(size_t)j * D.Npad]) : T(1);
+    const double v = sv[ec];
+    const double a = -v * dlnBds - s2v[ec] * DuParDt;
+    const T dts = (T)focus_dtsub(D, task, D.dtsub_f[i]);
+    const T hsc = (T)half * (dts * dmu_i);
+    T vl[MG + 1], vh[MG + 1];  // face below slot j (j = MG: above the last slot)
+    double vp = (m0 > 0) ? shc[m0 - 1] * (a + smu[m0 - 1] * fb) : 0.0;
+#pragma unroll
+    for (int j = 0; j <= MG; ++j) {
+      const int m = m0 + j;
+      const double vel = shc[m] * (a + smu[m] * fb);
+      // fused modes: velocities in units of dmu / dt_sub; (vel + vp) (sc/2)
+      // rounds to the same value as ((vel + vp)/2) sc
+      const T va = FASTF ? (T)(vel + vp) * hsc : (T)(half * (vel + vp));
+      vp = vel;
+      split_face(va, vl[j], vh[j]);
+    }
+    for (int it = 0; it < nsub; ++it) {
+      // cell m0-1 (last of the group below), cell m0+MG (first above)
+      const T below = __shfl_up(x[MG - 1], W, 64);
+      const T above = __shfl_down(x[0], W, 64);
+      // face below slot 0: zero-flux wall under cell 0, else interior (or,
+      // when cell M-1 is slot 0, that row's upwind-only face)
+      T flux_lo = fma(vh[0], x[0], vl[0] * below);
+      if (JL == 0) {
+        const T ub = (vl[0] + vh[0]) * (signbit(vh[0]) ? x[0] : below);
+        flux_lo = (q == QL) ? ub : flux_lo;
+      }
+      flux_lo = (q == 0) ? (T)0 : flux_lo;
+      const T ptop = fma(vh[MG], above, vl[MG] * x[MG - 1]);
+#pragma unroll
+      for (int j = 0; j < MG; ++j) {
+        T flux_hi, next_lo;
+        if (j + 1 < MG) {
+          const T pf = fma(vh[j + 1], x[j + 1], vl[j + 1] * x[j]);
+          flux_hi = pf;
+          next_lo = pf;
+          if (j == 0 || j + 1 == JL) {
+            const T uf = (vl[j + 1] + vh[j + 1]) * (signbit(vh[j + 1]) ? x[j + 1] : x[j]);
+            if (j == 0) flux_hi = (q == 0) ? uf : pf;         // row 0's upper face
+            if (j + 1 == JL) next_lo = (q == QL) ? uf : pf;   // row M-1's lower face
+          }
+        } else {
+          flux_hi = ptop;
+          next_lo = (T)0;
+        }
+        if (j >= JL) flux_hi = (q == QL) ? (T)0 : flux_hi;  // zero-flux wall above cell M-1
+        if (FASTF) {
+          x[j] = x[j] - (flux_hi - flux_lo);
+        } else {
+          const T dfl = (flux_hi - flux_lo) * dmu_i;
+          x[j] = x[j] - dts * dfl;
+        }
+        flux_lo = next_lo;
+      }
+    }
+    if (!ok) continue;
+    int bad = -1, bad_inf = 0;
+#pragma unroll
+    for (int j = 0; j < MG; ++j)
+      if (live(j)) {
+        if (bad < 0 && !isfinite(x[j])) {
+          bad = m0 + j;
+          bad_inf = isinf(x[j]) ? 1 : 0;
+        }
+        fp[(size_t)j * D.Npad] = f_encL<LOG>((x[j] < FloorT<T>::v) ? FloorT<T>::v : x[j]);
+      }
+    if (bad >= 0) note_error(D, err_key(substep, 1, s, 0, node, e, bad, bad_inf));
+  }
+}
+
+// The heavy-node walk.  From 12 mu cells both kernels are launched and the
+// list length, known only on the device, picks one: a short list (the
+// step's time is then the chain of its slowest node) runs four lanes per
+// row, a long one (throughput: four lanes per row issue about twice the
+// instructions per cell) a lane per row.
+template <int M, typename T, bool FASTF, bool LOG = false>
+inline void launch_heavy_t(const Dev &D, int sub, hipStream_t st) {
+  if constexpr (M >= 12)
+    hipLaunchKernelGGL((k_focus_heavy4<M, T, FASTF, LOG>), dim3(kHeavy4Grid), dim3(64), 0, st, D, sub);
+  hipLaunchKernelGGL((k_focus_heavy<M, T, FASTF, LOG>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub);
 }
 
 // ------------------------------------------------------------------------
@@ -831,29 +984,25 @@ hipError_t launch_stream_finish(const Dev &D, int sub, hipStream_t st) {
       hipLaunchKernelGGL((k_focus<MM, float, true, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), \
                          0, st, D, sub, fused, spare);                                        \
       if (!spare && heavy)                                                                    \
-        hipLaunchKernelGGL((k_focus_heavy<MM, float, true, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+        launch_heavy_t<MM, float, true, true>(D, sub, st); \
     } else if (D.fp32) {                                                                      \
       hipLaunchKernelGGL((k_focus<MM, float, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
       if (!spare && heavy)                                                                    \
-        hipLaunchKernelGGL((k_focus_heavy<MM, float, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+        launch_heavy_t<MM, float, true>(D, sub, st); \
     } else if (D.fast) {                                                                      \
       hipLaunchKernelGGL((k_focus<MM, double, true>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
       if (!spare && heavy)                                                                    \
-        hipLaunchKernelGGL((k_focus_heavy<MM, double, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+        launch_heavy_t<MM, double, true>(D, sub, st); \
     } else {                                                                                  \
       hipLaunchKernelGGL((k_focus<MM, double, false>), dim3((D.N + 255) / 256 * D.S * D.E), dim3(256), 0, \
                          st, D, sub, fused, spare);                                           \
       if (!spare && heavy)                                                                    \
-        hipLaunchKernelGGL((k_focus_heavy<MM, double, false>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+        launch_heavy_t<MM, double, false>(D, sub, st); \
     }                                                                                         \
     break;
 
-// Blocks of the heavy-focusing walk: enough waves to fill the chip, a
-// multiple of the 8 XCDs (k_focus_heavy gives each XCD a range of the list).
-constexpr int kHeavyGrid = 4096;
-static_assert(kHeavyGrid % 8 == 0, "k_focus_heavy splits its grid over 8 XCDs");
 
 hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int spare, int heavy) {
   switch (D.M) {
@@ -874,13 +1023,13 @@ hipError_t launch_focus(const Dev &D, int sub, int fused, hipStream_t st, int sp
 #define EP_HEAVY_CASE(MM)                                                                     \
   case MM:                                                                                    \
     if (D.logf)                                                                               \
-      hipLaunchKernelGGL((k_focus_heavy<MM, float, true, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+      launch_heavy_t<MM, float, true, true>(D, sub, st); \
     else if (D.fp32)                                                                          \
-      hipLaunchKernelGGL((k_focus_heavy<MM, float, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+      launch_heavy_t<MM, float, true>(D, sub, st); \
     else if (D.fast)                                                                          \
-      hipLaunchKernelGGL((k_focus_heavy<MM, double, true>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+      launch_heavy_t<MM, double, true>(D, sub, st); \
     else                                                                                      \
-      hipLaunchKernelGGL((k_focus_heavy<MM, double, false>), dim3(kHeavyGrid), dim3(64), 0, st, D, sub); \
+      launch_heavy_t<MM, double, false>(D, sub, st); \
     break;
 
 hipError_t launch_focus_heavy(const Dev &D, int sub, hipStream_t st) {

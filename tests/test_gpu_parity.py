@@ -1037,6 +1037,9 @@ def test_log_scaled_storage():
             sol.set_tuning(256, 16, lane_nodes=1, fuse_focus=0)
         with pytest.raises(runtime.EpError):
             sol.set_tuning(512, 16, lane_nodes=2, fuse_focus=1)
+        # the tile-size search keeps to what mode 3 has kernels for
+        _, _, lane_nodes, fuse, _ = sol.autotune()
+        assert lane_nodes == 2 and fuse == 0 and sol.schedule() in (1, 2)
         sol.upload_f(f0)
         for _ in range(2):
             sol.step()
