@@ -642,14 +642,25 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
         cands.push_back(c);
       }
   std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-  const size_t nbest = g.logf ? 0 : std::min<size_t>(cands.size(), 6);
+  // phase 2 takes the two best of each lane-node count: timings of the
+  // forms without fused focusing can sit within noise of each other (a
+  // stream that fits one chunk runs every subcycle in one pass whatever
+  // the halo), and fusing can reorder them
+  std::vector<Cand> top;
+  int per_r[4] = {0, 0, 0, 0};
+  for (const Cand &c : cands)
+    if (per_r[c.R] < 2) {
+      ++per_r[c.R];
+      top.push_back(c);
+    }
+  const size_t nbest = g.logf ? 0 : top.size();
   for (size_t k = 0; k < nbest; k++) {
-    Cand c = cands[k];
+    Cand c = top[k];
     c.F = 1;
     if ((rc = time_cand(c))) return rc;
     cands.push_back(c);
-    if (c.R > 1 && k < 3) {
-      Cand o = cands[k];
+    if (c.R > 1) {
+      Cand o = top[k];
       o.S = 1 - o.S;
       if ((rc = time_cand(o))) return rc;
       cands.push_back(o);

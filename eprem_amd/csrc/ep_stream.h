@@ -614,6 +614,21 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
   constexpr int NP = M - MPOS;
   __shared__ T up[2][9][NP > 0 ? NP : 1];
   __shared__ T dn[2][9][MPOS > 0 ? MPOS : 1];
+  // The streaming coefficients ds_i_multiplier * del_fac[m] of the lane's
+  // nodes are formed once per pass where they fit beside the rows: up to 40
+  // values (two nodes of up to 20 mu cells), and only in the form
+  // with focusing in the last pass, whose focusing sets its register count
+  // anyway (without it the copy would cost an occupancy step).  Else they
+  // are formed each subcycle; the same product either way.
+  constexpr bool kCoefRegs = FUSE && R * M <= 40;
+  T coef[kCoefRegs ? R : 1][kCoefRegs ? M : 1];
+  if constexpr (kCoefRegs) {
+    const double *delg = D.del_fac + (size_t)task * M;
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+      for (int r = 0; r < R; ++r) coef[r][m] = (T)(((m >= MPOS) ? pos[r] : neg[r]) * delg[m]);
+  }
   for (int jj = 0; jj < nsub; ++jj) {
     const cdouble *delj = opaque(del);
 #pragma unroll
@@ -647,7 +662,10 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         a[r] = f[r][m];
-        cf[r] = (T)(((m >= MPOS) ? pos[r] : neg[r]) * delj[m]);
+        if constexpr (kCoefRegs)
+          cf[r] = coef[r][m];
+        else
+          cf[r] = (T)(((m >= MPOS) ? pos[r] : neg[r]) * delj[m]);
       }
       if (m >= MPOS) {
         const T nb = dpp_shr1(a[R - 1], up[b][w][m - MPOS]);  // last node of the lane to the left
