@@ -697,7 +697,7 @@ def main():
         "node_stage_roofline": node_roof,
         "breakdown_ms_per_step": {k: acc[k] / args.steps for k in
                                   ("device_ms", "prep_ms", "stream_ms", "focus_ms", "change_ms")},
-        "node_stage": ("focus_ms: heavy-node focusing (k_focus_heavy); change_ms: focusing and adiabatic "
+        "node_stage": ("focus_ms: heavy-node focusing (k_focus_heavy4 for up to 2048 heavy nodes, else k_focus_heavy); change_ms: focusing and adiabatic "
                        "change in one pass (k_focus_change4 from 12 mu cells, k_focus_change below)") if dg["max_subcycles_change"] <= 1 else
                       "focus_ms: k_focus + k_focus_heavy; change_ms: k_change",
         "sweeps_per_step": {k: acc[k] // args.steps for k in ("stream_sweeps", "focus_sweeps",

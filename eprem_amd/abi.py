@@ -112,7 +112,7 @@ class EpDiag(C.Structure):
         ("focus_sweeps", C.c_int64),
         ("change_sweeps", C.c_int64),
         ("stream_launches", C.c_int32),
-        ("pad_", C.c_int32),
+        ("heavy_nodes", C.c_int32),
     ]
 
     def as_dict(self):

@@ -1010,6 +1010,7 @@ int ep_step(uint32_t op_flags, ep_diag *out) {
   dg.focus_sweeps = do_focus ? (int64_t)hd.focus_sweeps * nsub : 0;
   dg.change_sweeps = do_change ? (int64_t)hd.change_sweeps * nsub : 0;
   dg.stream_launches = !do_stream ? 0 : sched == 0 ? hd.max_passes * nsub : sched == 1 ? nsub : nsub * g.d.num_streams;
+  dg.heavy_nodes = do_focus ? (int32_t)hd.focus_heavy : 0;
   int code = EP_OK;
   if (hd.err_key != ~0ull) code = decode_err(hd.err_key, &dg);
   if (out) *out = dg;

@@ -140,7 +140,9 @@ typedef struct ep_diag {
    * operator), summed over the step's EP substeps. */
   int64_t stream_sweeps, focus_sweeps, change_sweeps;
   int32_t stream_launches;         /* streaming pass launches in the step  */
-  int32_t pad_;
+  int32_t heavy_nodes;             /* nodes whose focusing needs 32 or more
+                                      subcycles (focused by the heavy-node
+                                      walk; 0 without focusing)            */
 } ep_diag;
 
 /* ------------------------------------------------------------------ */

@@ -13,7 +13,10 @@ ABI, against the FP64 oracle at 1e-10 relative on every cell of f):
   oracle run on those streams alone (with the cross-stream operators off,
   streams are independent);
 * 24 x 400 x 20 x 20 in full for one simulated day (64 global steps of 10
-  EP substeps), with the per-step subcycle maxima.
+  EP substeps), with the per-step subcycle maxima;
+* the two heavy-node focusing kernels (four lanes per row for a short
+  heavy list, a lane per row for a long one) against each other, bit for
+  bit, on the 384-stream nodes and a slice of them.
 """
 import time
 
@@ -171,3 +174,37 @@ def test_c1_one_simulated_day(precision):
     log("c1 day, mode %d: oracle %.1f s, max rel %.3e" % (precision, time.time() - t0, max_rel(fg, fo)))
     assert_diags(gd, od)
     assert max_rel(fg, fo) <= RTOL
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_heavy_focusing_kernels_give_the_same_bits(precision):
+    """The heavy-node focusing walk runs four lanes per row for a short
+    heavy list and a lane per row for a long one (k_focus_heavy4 /
+    k_focus_heavy; the list length picks one on the device).  Streams are
+    independent, so one step of the 384-stream workload at 8 energies x 16
+    mu (a long list) and of a two-stream slice of it (a short one) must give
+    the slice's streams bit for bit."""
+    c = workloads.WORKLOADS["c2"]
+    p = workloads.workload_params(c)
+    nodes = workloads.workload_nodes(c)
+    E, M = 8, 16
+    idx = np.array([5, 200])
+    out = []
+    for nd in (nodes, subset(nodes, idx)[0]):
+        dims = workloads.dims_of(nd, E, M)
+        sol = runtime.EpSolver(dims, p)
+        try:
+            sol.set_precision(precision)
+            sol.set_nodes(nd)
+            sol.init_seed_population()
+            _, dg = sol.step()
+            heavy = dg["heavy_nodes"]
+            if nd is nodes:
+                out.append((heavy, np.concatenate([sol.download_f_streams(int(s), 1) for s in idx])))
+            else:
+                out.append((heavy, sol.download_f()))
+        finally:
+            sol.close()
+    log("heavy nodes: %d (all streams), %d (slice)" % (out[0][0], out[1][0]))
+    assert out[0][0] > 2048 and 0 < out[1][0] <= 2048
+    np.testing.assert_array_equal(out[0][1], out[1][1])
