@@ -289,6 +289,22 @@ __device__ __forceinline__ float floor_max(float x, float fl) {
   return r;
 }
 
+// Cross-lane shifts within each 16-lane row of a wave (DPP row_shr:1,
+// CTRL 0x111: lane i takes lane i-1 of its row; row_shl:1, CTRL 0x101:
+// lane i takes lane i+1); the row's first (last) lane gets 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+constexpr int kRowShr1 = 0x111, kRowShl1 = 0x101;
+
 // A face velocity split by the side its focusing flux is taken from:
 // vh = min(va, 0), vl = va - vh, so vl = va for va >= 0 and +0 otherwise,
 // and vl + vh = va exactly.  Two instructions where sign-bit selects take

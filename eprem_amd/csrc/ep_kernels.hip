@@ -530,11 +530,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 }
 
 // k_focus_heavy4: the same walk with four lanes per (node, energy) row
-// (for M >= 12): lanes l, l+16, l+32, l+48 of a wave hold mu cells q MG ..
-// q MG + MG - 1 (q = l / 16, MG = ceil(M/4), the last group padded) of
-// energy eg 16 + (l mod 16), and the wave takes the node's energies 16 at a
-// time.  A subcycle's chain is then MG cells and two cross-lane shifts
-// instead of M cells, which is what bounds a node with hundreds of
+// (for M >= 12): lanes 4k .. 4k+3 of a wave hold mu cells q MG .. q MG +
+// MG - 1 (q = l mod 4, MG = ceil(M/4), the last group padded) of energy
+// eg 16 + k, and the wave takes the node's energies 16 at a time.  A
+// subcycle's chain is then MG cells and two DPP row shifts instead of M
+// cells, which is what bounds a node with hundreds of
 // subcycles; a lane needs about a quarter of the registers.  The
 // arithmetic is k_focus_change4's (face velocities split by split_face,
 // interior faces fma(vh, f_above, vl * f_below), the wall rows' faces the
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(64) void k_focus_heavy4(Dev D, int substep) {
   }
   __syncthreads();
   const int lane = threadIdx.x;
-  const int q = lane / W, m0 = q * MG;
+  const int q = lane % G, m0 = q * MG;
   const int EW = (E + W - 1) / W;  // waves of 16 rows per node
   if (D.diag->focus_heavy > kHeavy4Max) return;  // k_focus_heavy
   const int count = D.diag->focus_heavy * EW;
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(64) void k_focus_heavy4(Dev D, int substep) {
   const T dmu_i = (T)(1.0 / D.dmu);
   for (int u = xcd * seg + local; u < end; u += per) {
     const size_t i = (size_t)D.heavy[u / EW];
-    const int e = (u % EW) * W + (lane % W);
+    const int e = (u % EW) * W + lane / G;
     const bool ok = e < E;  // the four lanes of a row agree
     const int s = (int)(i / D.Npad), node = (int)(i % D.Npad);
     const int nsub = D.nsub_f[i];  // the node's: the same in every lane
@@ -607,8 +607,8 @@ __global__ __launch_bounds__(64) void k_focus_heavy4(Dev D, int substep) {
     }
     for (int it = 0; it < nsub; ++it) {
       // cell m0-1 (last of the group below), cell m0+MG (first above)
-      const T below = __shfl_up(x[MG - 1], W, 64);
-      const T above = __shfl_down(x[0], W, 64);
+      const T below = dpp_row<kRowShr1>(x[MG - 1]);
+      const T above = dpp_row<kRowShl1>(x[0]);
       // face below slot 0: zero-flux wall under cell 0, else interior (or,
       // when cell M-1 is slot 0, that row's upwind-only face)
       T flux_lo = fma(vh[0], x[0], vl[0] * below);
