@@ -309,24 +309,28 @@ __global__ __launch_bounds__(256) void k_focus_change(Dev D, int substep, int fu
 }
 
 // ------------------------------------------------------------------------
-// k_focus_change4: the same pass with four lanes per node (for M >= 12).
-// Lanes l, l+16, l+32, l+48 of a wave hold mu cells q MG .. q MG + MG - 1
-// (q = l / 16, MG = ceil(M/4), the last group padded) of node (l mod 16);
-// a 256-lane block covers 64 consecutive nodes.  Each focusing subcycle
-// moves the boundary cells of the neighbouring groups in by two
-// cross-lane shifts of 16.  A lane keeps about half the state of the
+// k_focus_change4: the same pass with G = 4 lanes per node (for M >= 12).
+// Lanes l, l+W, l+2W, .. (W = 64/G) of a wave hold mu cells q MG .. q MG +
+// MG - 1 (q = l / W, MG = ceil(M/G), the last group padded) of node
+// (l mod W); a 256-lane block covers 4W consecutive nodes.  Each focusing
+// subcycle moves the boundary cells of the neighbouring groups in by two
+// cross-lane shifts of W.  A lane keeps about half the state of the
 // pair layout, so the kernel runs at a higher occupancy; arithmetic and
 // results are those of k_focus_change.
 // ------------------------------------------------------------------------
-template <int M, typename T, bool FM>
+// G = 8 (eight lanes per node, 32 nodes per block) serves workloads with
+// too few nodes to give every SIMD a wave at four lanes per node: a wave's
+// chain over the energies is then about half as long.  Groups after the
+// one holding cell M-1 are padding.
+template <int M, typename T, bool FM, int G = 4>
 __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int fused, int spare) {
   if (substep_off(D)) return;
-  constexpr int G = 4, W = 64 / G, NB = 4 * W;
+  constexpr int W = 64 / G, NB = 4 * W;
   constexpr int MG = (M + G - 1) / G;
-  static_assert(M >= 12, "four lanes per node from 12 mu cells");
-  constexpr int QL = (M - 1) / MG;       // group of cell M-1 (the last one for M >= 12)
+  static_assert(G == 4 || G == 8, "four or eight lanes per node");
+  static_assert(MG >= 2, "at least two mu cells per lane");
+  constexpr int QL = (M - 1) / MG;       // group of cell M-1
   constexpr int JL = (M - 1) - QL * MG;  // its slot
-  static_assert(QL == G - 1, "cell M-1 in the last group");
   const int nch = (D.N + NB - 1) / NB;
   const int s = blockIdx.x / nch;
   const int chunk = (int)((blockIdx.x % nch + (unsigned)s) % nch);
@@ -382,9 +386,8 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
     const int lc = __builtin_amdgcn_readfirstlane((int)sloc[e]) ^ flip;
     return slab_rsrc((const double *)fslab<T>(D, lc, task), slab_bytes);
   };
-  // slot j is mu cell m0 + j; slots past cell M-1 in the last group are
-  // padding
-  auto live = [&](int j) { return (j <= JL) || (q < G - 1); };
+  // slot j is mu cell m0 + j; slots past cell M-1 are padding
+  auto live = [&](int j) { return (q < QL) || (q == QL && j <= JL); };
   auto cvel = [&](int e, int j) {
     const int m = m0 + j;
     const double a = -smu[m] * cDuPar;
@@ -555,17 +558,31 @@ __global__ __launch_bounds__(256) void k_focus_change4(Dev D, int substep, int f
   }
 }
 
+// Waves of the four-lanes-per-node kernel below which eight lanes per node
+// are used (fewer than two per SIMD of the 1024).
+constexpr int64_t kFc8Waves = 2048;
+
+template <int M, int G>
+static void focus_change_g(const Dev &D, int sub, int fused, int spare, hipStream_t st) {
+  const int nb = 4 * (64 / G);  // nodes per 256-lane block
+  const dim3 grid((D.N + nb - 1) / nb * D.S), block(256);
+  if (D.fp32)
+    hipLaunchKernelGGL((k_focus_change4<M, float, true, G>), grid, block, 0, st, D, sub, fused, spare);
+  else if (D.fast)
+    hipLaunchKernelGGL((k_focus_change4<M, double, true, G>), grid, block, 0, st, D, sub, fused, spare);
+  else
+    hipLaunchKernelGGL((k_focus_change4<M, double, false, G>), grid, block, 0, st, D, sub, fused, spare);
+}
+
 template <int M>
 static void focus_change_t(const Dev &D, int sub, int fused, int spare, hipStream_t st) {
   if constexpr (M >= 12) {
-    // four lanes per node
-    const dim3 grid((D.N + 63) / 64 * D.S), block(256);
-    if (D.fp32)
-      hipLaunchKernelGGL((k_focus_change4<M, float, true>), grid, block, 0, st, D, sub, fused, spare);
-    else if (D.fast)
-      hipLaunchKernelGGL((k_focus_change4<M, double, true>), grid, block, 0, st, D, sub, fused, spare);
+    // four lanes per node, or eight when that leaves SIMDs without a wave
+    const int64_t waves4 = (int64_t)D.S * ((D.N + 15) / 16);
+    if (waves4 < kFc8Waves)
+      focus_change_g<M, 8>(D, sub, fused, spare, st);
     else
-      hipLaunchKernelGGL((k_focus_change4<M, double, false>), grid, block, 0, st, D, sub, fused, spare);
+      focus_change_g<M, 4>(D, sub, fused, spare, st);
   } else {
     const dim3 grid((D.N + 127) / 128 * D.S), block(256);
     if (D.fp32)
