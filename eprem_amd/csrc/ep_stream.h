@@ -589,20 +589,23 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
     }
   const size_t ok = (size_t)task * D.Npad;
   double pos[R], neg[R];
-  double q[R], omq[R];  // the relaxation runs in FP64 also when f is stored as float
-  T bcv[R];
+  double q[R];  // the relaxation runs in FP64 also when f is stored as float
   bool bc[R];
+  // boundary values of the stream's first three kept nodes (the lanes that
+  // hold them read them each subcycle; LDS rather than a register that
+  // every lane would carry)
+  __shared__ T bcs[3];
+  if (tid < 3) bcs[tid] = D.useBC ? (T)D.seed3[(size_t)task * 3 + tid] : (T)0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const double dsm = vv[r] ? D.dsm[ok + kk[r]] : 0.0;
     q[r] = vv[r] ? D.q[ok + kk[r]] : 1.0;
-    omq[r] = vv[r] ? 1.0 - D.q[ok + kk[r]] : 0.0;
     // stream ends are held: kept node 0 for mu >= 0, the last one for mu < 0
     pos[r] = (kk[r] == 0) ? 0.0 : dsm;
     neg[r] = (kk[r] == nk - 1) ? 0.0 : dsm;
     bc[r] = D.useBC && vv[r] && (kk[r] < 3);
-    bcv[r] = bc[r] ? (T)D.seed3[(size_t)task * 3 + (bc[r] ? kk[r] : 0)] : (T)0;
   }
+  __syncthreads();
   const double invM = 1.0 / M;
   const cdouble *del = as_const_space(D.del_fac + (size_t)task * M);
   // edge rows between the waves of the block, double-buffered by subcycle
@@ -634,8 +637,9 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (bc[r]) {
+        const T bv = bcs[kk[r]];
 #pragma unroll
-        for (int m = 0; m < M; ++m) f[r][m] = bcv[r];
+        for (int m = 0; m < M; ++m) f[r][m] = bv;
       }
     const int b = jj & 1;
     if (lane == 63) {
@@ -692,7 +696,7 @@ __device__ __forceinline__ void stream_chunk(const Dev &D, int c, int e, int s, 
     for (int r = 0; r < R; ++r) {
       const double iso = invM * sum[r];
       if (FAST) {
-        const double rr = omq[r] * iso;
+        const double rr = (1.0 - q[r]) * iso;  // 1 - q: 0 where q = 1 (nodes outside the stream)
 #pragma unroll
         for (int m = 0; m < M; ++m) f[r][m] = (T)fma(q[r], (double)f[r][m], rr);
       } else {
