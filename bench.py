@@ -232,8 +232,13 @@ def run_mixed(args, c, p, nodes, dims):
     def run(mixed, timed):
         return run_mode(sol, f0, 1 if mixed else 0)
 
+    tiling = None
     if not (args.block or args.halo):
-        sol.autotune()
+        # tile search in the mode that is timed (float storage)
+        sol.upload_f(f0)
+        sol.set_precision(1)
+        blk, halo, r, fz, _ = sol.autotune()
+        tiling = {"block": blk, "halo": halo, "lane_nodes": r, "fuse_focus": fz, "schedule": sol.schedule()}
     f32, flux32, flu32, wall, cells = run(True, True)
     f64, flux64, flu64, wall64, cells64 = run(False, False)
     sol.close()
@@ -271,6 +276,7 @@ def run_mixed(args, c, p, nodes, dims):
         "config": {"workload": "c4: c2 shapes, mixed precision, power-law f spanning %.1e..%.1e"
                                % (f0.min(), f0.max()), "path": "mixed precision"},
         "fp64_same_steps": {"value": cells64 / wall64, "ms_per_step": wall64 * 1e3 / args.steps},
+        "tiling": tiling,
         "precision": "f stored as float; streaming (upwind), focusing and MAS interpolation in FP32; "
                      "relaxation (the pitch-angle part) and the adiabatic change in FP64",
         "error_vs_fp64_device_all_streams": {
