@@ -669,7 +669,17 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
       cands.push_back(o);
     }
   }
-  const Cand best_c = *std::min_element(cands.begin(), cands.end(),
+  // phase 3: the four fastest so far timed again (their own best of two
+  // or six more), so that candidates within the run-to-run spread of each
+  // other are ranked on more than one sample each
+  std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+  const size_t nfinal = std::min<size_t>(cands.size(), 4);
+  for (size_t k = 0; k < nfinal; k++) {
+    Cand c = cands[k];
+    if ((rc = time_cand(c))) return rc;
+    cands[k].ms = std::min(cands[k].ms, c.ms);
+  }
+  const Cand best_c = *std::min_element(cands.begin(), cands.begin() + (nfinal ? nfinal : cands.size()),
                                         [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
   const int bb = best_c.W, bh = best_c.H, br = best_c.R, bf = best_c.F;
   const double best = best_c.ms;
