@@ -343,12 +343,8 @@ static int ensure_events(int count) {
   return EP_OK;
 }
 
-// Streaming schedule in use: the persistent slab launch needs two or three
-// nodes per lane; with one, the passes are launched one by one.
-static int effective_schedule() {
-  if (g.stream_sched != 0 && g.stream_r == 1) return 0;
-  return g.stream_sched;
-}
+// Streaming schedule in use (every lane-node count has all three).
+static int effective_schedule() { return g.stream_sched; }
 
 // The EP substep loop of one global step (src/energeticParticles.c:
 // 102-199): streaming passes and bookkeeping, then focusing and adiabatic
@@ -627,11 +623,11 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
   };
   // phase 1: every tiling with the separate focusing kernel; phase 2: the
   // fused form of the best few (its extra registers can change the order)
-  static const int lanes[] = {128, 192, 256, 320, 384, 512};
+  static const int lanes[] = {128, 192, 256, 320, 384, 448, 512};
   static const int halos[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
   // phase 1 times one node per lane with a launch per pass and two or
   // three with the persistent slab launch; phase 2 the best few with
-  // focusing in the last pass and with the other schedule
+  // focusing in the last pass and each with the other schedule
   // (the log-scaled storage has the slab kernel with two nodes per lane
   // only, without fused focusing)
   std::vector<Cand> cands;
@@ -662,9 +658,11 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
     c.F = 1;
     if ((rc = time_cand(c))) return rc;
     cands.push_back(c);
-    if (c.R > 1) {
+    // the other schedule, without and with fused focusing
+    for (int F = 0; F <= 1; F++) {
       Cand o = top[k];
       o.S = 1 - o.S;
+      o.F = F;
       if ((rc = time_cand(o))) return rc;
       cands.push_back(o);
     }

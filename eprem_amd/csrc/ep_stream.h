@@ -863,18 +863,23 @@ inline void stream_slab_tfr(const Dev &D, int do_seed, int substep, int tune, in
     launch_slab_kernel<M, MPOS, double, false, FUSE, R>(D, do_seed, substep, tune, one_stream, st);
 }
 
-// Slab kernels exist for two and three nodes per lane; with the
+// Slab kernels exist for one, two and three nodes per lane; with the
 // log-scaled storage, for two nodes per lane without fused focusing.
 template <int M, int MPOS>
 inline hipError_t stream_slab_t(const Dev &D, int do_seed, int substep, int fuse, int tune, int one_stream,
                                 hipStream_t st) {
-  if (D.stream_r != 2 && D.stream_r != 3) return hipErrorInvalidValue;
+  if (D.stream_r < 1 || D.stream_r > 3) return hipErrorInvalidValue;
   if (D.logf && (D.stream_r != 2 || fuse)) return hipErrorInvalidValue;
   if (D.stream_r == 3) {
     if (fuse)
       stream_slab_tfr<M, MPOS, true, 3>(D, do_seed, substep, tune, one_stream, st);
     else
       stream_slab_tfr<M, MPOS, false, 3>(D, do_seed, substep, tune, one_stream, st);
+  } else if (D.stream_r == 1) {
+    if (fuse)
+      stream_slab_tfr<M, MPOS, true, 1>(D, do_seed, substep, tune, one_stream, st);
+    else
+      stream_slab_tfr<M, MPOS, false, 1>(D, do_seed, substep, tune, one_stream, st);
   } else {
     if (fuse)
       stream_slab_tfr<M, MPOS, true, 2>(D, do_seed, substep, tune, one_stream, st);

@@ -440,8 +440,7 @@ int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes,
 /* How the streaming passes of an EP substep are launched: 1 (default) =
  * one persistent launch per substep whose blocks take whole (stream,
  * energy) slabs from a device queue, highest energies first, and run all
- * their passes and chunks (needs lane_nodes 2 or 3; with 1 the passes are
- * launched one by one); 0 = one launch per pass over every slab; 2 = the
+ * their passes and chunks; 0 = one launch per pass over every slab; 2 = the
  * unbatched form of 1, one launch per stream (the reference's loop over
  * streams, src/energeticParticles.c:102-160), kept for comparison.
  * Results are identical.  ep_get_schedule reports the schedule in effect

@@ -610,7 +610,10 @@ def test_rk3_operator_limits():
     dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=0, tiling=(192, 8, 3, 0)),
     dict(N=300, E=10, M=17, thin=0.1, ragged=True, precision=2, tiling=(256, 12, 2, 1)),
     dict(N=700, E=8, M=30, thin=0.0, ragged=True, precision=2, tiling=(384, 40, 3, 1)),
+    dict(N=300, E=10, M=20, thin=0.1, ragged=True, precision=2, tiling=(256, 8, 1, 1)),
+    dict(N=300, E=10, M=30, thin=0.0, ragged=False, precision=0, tiling=(192, 12, 1, 0)),
     # every stream fits in one chunk: all subcycles in one pass, no halo
+    dict(N=250, E=12, M=20, thin=0.0, ragged=True, precision=2, tiling=(512, 16, 1, 1)),
     dict(N=120, E=12, M=20, thin=0.0, ragged=False, precision=0, tiling=(384, 16, 3, 1)),
     dict(N=250, E=9, M=7, thin=0.15, ragged=True, precision=0, tiling=(256, 4, 2, 0)),
 ])
