@@ -502,6 +502,13 @@ int ep_set_params(const ep_params *params) {
   return upload_grids();
 }
 
+// With FP32 storage, one node per lane and halos below this are switched
+// off: steps at 8 subcycles per pass on streams of ~1.5e5 subcycles ran
+// past their time limit there (one and three nodes per lane; the same
+// streams at halo 12, two nodes per lane, and in FP64 at halo 8 finish),
+// and the cause is not found.
+constexpr int kFp32MinHalo = 12;
+
 // A streaming tile is stream_block kept nodes (the chunk, halo included),
 // carried by stream_block / lane_nodes lanes.
 static bool tiling_ok(int W, int H, int R) {
@@ -518,6 +525,10 @@ int ep_set_tuning(int32_t stream_block, int32_t stream_halo, int32_t lane_nodes,
   if (g.logf && (lane_nodes != 2 || fuse_focus != 0))
     return fail(EP_ERR_UNSUPPORTED, "the log-scaled storage (precision mode 3) streams two nodes per lane, "
                                     "focusing in its own kernel (lane_nodes 2, fuse_focus 0)");
+  if (g.fp32 && (lane_nodes == 1 || stream_halo < kFp32MinHalo))
+    return fail(EP_ERR_UNSUPPORTED, "with FP32 storage (precision modes 1 and 3) one node per lane and halos "
+                                    "below %d are switched off: use lane_nodes 2 or 3 and stream_halo >= %d",
+                kFp32MinHalo, kFp32MinHalo);
   g.stream_block = stream_block;
   g.stream_halo = stream_halo;
   g.stream_r = lane_nodes;
@@ -635,6 +646,7 @@ static int autotune_impl(int32_t *best_block, int32_t *best_halo, int32_t *best_
     for (int Lw : lanes)
       for (int H : halos) {
         if (g.logf && R != 2) continue;
+        if (g.fp32 && (R == 1 || H < kFp32MinHalo)) continue;  // switched off with FP32 storage
         Cand c{Lw * R, H, R, 0, R > 1 ? 1 : 0, 0.0};
         if (4 * H > c.W) continue;
         if ((rc = time_cand(c))) return rc;
@@ -729,6 +741,13 @@ int ep_set_precision(int32_t mode) {
   g.fp32 = fp32;
   g.logf = logf;
   g.fast = (mode != 0) ? 1 : 0;
+  if (fp32 && !logf && g.stream_r == 1) {
+    // one node per lane is switched off with FP32 storage: the same lane
+    // count with two nodes each
+    g.stream_block *= 2;
+    g.stream_r = 2;
+  }
+  if (fp32 && g.stream_halo < kFp32MinHalo) g.stream_halo = kFp32MinHalo;
   if (logf) {
     // the log-scaled storage's streaming kernel: the slab launch with two
     // nodes per lane (same lane count), focusing in its own kernel

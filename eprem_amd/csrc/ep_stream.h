@@ -870,6 +870,7 @@ inline hipError_t stream_slab_t(const Dev &D, int do_seed, int substep, int fuse
                                 hipStream_t st) {
   if (D.stream_r < 1 || D.stream_r > 3) return hipErrorInvalidValue;
   if (D.logf && (D.stream_r != 2 || fuse)) return hipErrorInvalidValue;
+  if (D.fp32 && D.stream_r == 1) return hipErrorInvalidValue;  // switched off with FP32 storage
   if (D.stream_r == 3) {
     if (fuse)
       stream_slab_tfr<M, MPOS, true, 3>(D, do_seed, substep, tune, one_stream, st);
@@ -901,6 +902,7 @@ template <int M, int MPOS, bool FUSE>
 inline hipError_t stream_pass_tf(const Dev &D, int pass, int flip, int do_seed, int substep, int chunks,
                                  hipStream_t st) {
   if (D.logf) return hipErrorInvalidValue;  // the log-scaled storage runs the slab kernel only
+  if (D.fp32 && D.stream_r == 1) return hipErrorInvalidValue;  // switched off with FP32 storage
   const dim3 grid(chunks, D.E, D.S), block(D.stream_block / D.stream_r);
   if (D.stream_r == 3) {
     if (D.fp32)

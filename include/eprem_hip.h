@@ -427,7 +427,8 @@ int ep_finalize(void);
 /* Tiling of the streaming kernel: kept nodes per chunk (stream_block, halo
  * included), Euler subcycles per pass (stream_halo, 1..stream_block/2-1),
  * kept nodes per lane (lane_nodes, 1, 2 or 3; stream_block must be a multiple
- * of 64*lane_nodes with at most 512 lanes) and fuse_focus (1: the
+ * of 64*lane_nodes with at most 512 lanes; 1 is refused with FP32 storage,
+ * precision modes 1 and 3) and fuse_focus (1: the
  * focusing of each node of an unthinned stream is done by its last
  * streaming pass on the values still in registers; 0: by the separate
  * focusing kernel).  Results are the same either way.  No counterpart in
@@ -484,8 +485,10 @@ int ep_set_node_fusion(int32_t fused);
  * nodes per lane, and runs focusing and change as two kernels: setting it
  * moves the tiling to two nodes per lane (same lane count), switches
  * fused focusing off and the schedule to 1 if it was 0, and ep_set_tuning
- * and ep_set_schedule reject other choices while it is set.  The
- * distribution on the device is
+ * and ep_set_schedule reject other choices while it is set.  Modes 1 and
+ * 3 stream two or three nodes per lane: setting mode 1 moves a tiling of
+ * one node per lane to two (same lane count).  The distribution on the
+ * device is
  * converted when the storage changes; the float floor is FLT_MIN where the
  * reference floors at DBL_MIN.  No counterpart in the reference. */
 int ep_set_precision(int32_t mode);

@@ -651,6 +651,83 @@ def test_streaming_schedules_parity(sched, case):
         assert gd[0][1]["stream_launches"] == (2 if sched == 1 else 2 * dims.num_streams)
 
 
+def mixed_precision_schedules(tiling):
+    """Two steps in precision mode 1 under each streaming schedule; f of
+    the active nodes per schedule."""
+    p = params(numEpSteps=2)
+    nodes = workloads.parker_nodes(1, 1, 300, seed=65, perturb=0.2, jitter=0.2, thin_frac=0.1)
+    n_active = np.array([300, 37, 150, 3, 99, 128], np.int32)
+    dims = workloads.dims_of(nodes, 10, 20)
+    out = []
+    for sched in (0, 1, 2):
+        sol = runtime.EpSolver(dims, p)
+        try:
+            sol.set_precision(1)
+            sol.set_tuning(*tiling)
+            sol.set_schedule(sched)
+            assert sol.schedule() == sched
+            f0 = workloads.seeded_f(p, sol.grids(), nodes, noise=0.3, seed=9)
+            sol.set_nodes(nodes, n_active)
+            sol.upload_f(f0)
+            for _ in range(2):
+                sol.step()
+            out.append(sol.download_f())
+        finally:
+            sol.close()
+    return out, n_active
+
+
+@pytest.mark.parametrize("tiling", [(512, 12, 2, 1), (256, 12, 2, 0)])
+def test_mixed_precision_schedules_same_bits(tiling):
+    """FP32 storage (precision mode 1): the persistent slab launch, its
+    unbatched form and one launch per pass run the same float arithmetic
+    per chunk, so two steps give the same bits, on one-chunk and two-chunk
+    streams, thinned and ragged, one of them needing 147,956 streaming and
+    55,986 focusing subcycles per substep (the oracle's counts for these
+    nodes)."""
+    out, n_active = mixed_precision_schedules(tiling)
+    for s, n in enumerate(n_active):
+        np.testing.assert_array_equal(out[1][s, :n], out[0][s, :n])
+        np.testing.assert_array_equal(out[2][s, :n], out[0][s, :n])
+
+
+@pytest.mark.skip(reason="these steps (FP32 storage, 8 subcycles per pass, one or three nodes per lane, on "
+                         "the nodes above whose stream needs 147,956 subcycles per substep) ran past the "
+                         "test's time limit on the GPU and the cause is not found; the library refuses "
+                         "one node per lane and halos below 12 with FP32 storage")
+@pytest.mark.parametrize("tiling", [(256, 8, 1, 0), (192, 8, 3, 0)])
+def test_mixed_precision_small_halo_switched_off(tiling):
+    out, n_active = mixed_precision_schedules(tiling)
+    for s, n in enumerate(n_active):
+        np.testing.assert_array_equal(out[1][s, :n], out[0][s, :n])
+
+
+def test_fp32_refuses_one_node_per_lane():
+    """One node per lane and halos below 12 are switched off with FP32
+    storage: ep_set_tuning refuses them in precision modes 1 and 3 and
+    accepts them again in FP64."""
+    p = params(numEpSteps=1)
+    nodes = workloads.parker_nodes(1, 1, 100, seed=66)
+    dims = workloads.dims_of(nodes, 6, 8)
+    sol = runtime.EpSolver(dims, p)
+    try:
+        sol.set_tuning(256, 16, 1, 1)
+        sol.set_precision(1)  # moves the tiling to two nodes per lane
+        for mode in (1, 3):
+            sol.set_precision(mode)
+            for tiling in ((256, 16, 1, 0), (512, 8, 2, 0)):
+                with pytest.raises(runtime.EpError) as ei:
+                    sol.set_tuning(*tiling)
+                assert ei.value.code == abi.EP_ERR_UNSUPPORTED
+        sol.set_precision(1)
+        sol.set_tuning(512, 12, 2, 0)
+        sol.set_precision(0)
+        sol.set_tuning(256, 16, 1, 1)
+        sol.set_tuning(192, 8, 3, 0)
+    finally:
+        sol.close()
+
+
 @pytest.mark.parametrize("tiling", [(256, 16, 1, 1), (512, 32, 2, 1), (384, 24, 2, 1), (256, 16, 1, 0),
                                     (576, 24, 3, 1)])
 def test_fused_focusing_mixed_thinning(tiling):
